@@ -1,0 +1,401 @@
+"""ctypes front end of the oracle library (oracle/liboracle.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and the
+cpu_baseline leg of bench.py as the checker.  Never imported by self_amd/.
+
+Each wrapper returns numpy arrays in the reference's Fortran layouts (see
+oracle.h); the restated reference routine is cited in oracle.h beside each C
+entry point.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB = None
+
+GAUSS, GAUSS_LOBATTO, DG, CG = 1, -1, 2000, 2001
+NO_NORMAL_FLOW, RADIATION = -100, -101
+
+_dp = C.POINTER(C.c_double)
+_ip = C.POINTER(C.c_int)
+
+
+def build() -> str:
+    """Compile the oracle with its Makefile (gcc, plain C99)."""
+    subprocess.run(["make", "-s", "-C", _HERE], check=True)
+    return os.path.join(_HERE, "liboracle.so")
+
+
+def lib():
+    global _LIB
+    if _LIB is None:
+        path = os.path.join(_HERE, "liboracle.so")
+        if not os.path.exists(path):
+            build()
+        _LIB = C.CDLL(path)
+        _declare(_LIB)
+    return _LIB
+
+
+def _declare(L):
+    L.orc_generate_legendre_quadrature.argtypes = [C.c_int, _dp, _dp, C.c_int]
+    L.orc_generate_legendre_quadrature.restype = C.c_int
+    L.orc_legendre_polynomial.argtypes = [C.c_int, C.c_double, _dp, _dp]
+    L.orc_barycentric_weights.argtypes = [C.c_int, _dp, _dp]
+    L.orc_lagrange_polynomials_at.argtypes = [C.c_int, _dp, _dp, C.c_double, _dp]
+    L.orc_lagrange_interpolate.argtypes = [C.c_int, _dp, _dp, C.c_double, _dp]
+    L.orc_lagrange_interpolate.restype = C.c_double
+    L.orc_lagrange_derivative.argtypes = [C.c_int, _dp, _dp, C.c_double, _dp]
+    L.orc_lagrange_derivative.restype = C.c_double
+    L.orc_derivative_matrix.argtypes = [C.c_int, _dp, _dp, _dp]
+    L.orc_interpolation_matrix.argtypes = [C.c_int, _dp, _dp, C.c_int, _dp, _dp]
+    L.orc_coarse_to_fine_2d.argtypes = [C.c_int, C.c_int, _dp, C.c_int, C.c_int, _dp, _dp, _dp]
+    L.orc_coarse_to_fine_3d.argtypes = [C.c_int, _dp, C.c_int, _dp, _dp]
+    L.orc_nodal2d_build.argtypes = [C.POINTER(Nodal2D)]
+    L.orc_nodal2d_build.restype = C.c_int
+    L.orc_apply_derivative_2d.argtypes = [C.c_int, _dp, _dp, _dp, _dp, _dp]
+    L.orc_boundary_interp_2d.argtypes = [C.c_int, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp]
+    L.orc_apply_derivative_3d.argtypes = [C.c_int, _dp, _dp, _dp, _dp, _dp]
+    L.orc_boundary_interp_3d.argtypes = [C.c_int, _dp, _dp, _dp, _dp]
+    L.orc_quadmesh_counts.argtypes = [C.POINTER(QuadMeshDesc)]
+    L.orc_quadmesh_build.argtypes = [C.POINTER(QuadMeshDesc), _dp, _ip, _ip] + [_dp] * 9
+    L.orc_quadmesh_build.restype = C.c_int
+    L.orc_sw2d_riemann.argtypes = [_dp, _dp, _dp, C.c_double, _dp]
+    L.orc_sw2d_external_state.argtypes = [C.c_int, _dp, _dp, _dp]
+    for name in ("orc_sw2d_solution_at_boundaries", "orc_sw2d_bathymetry_gradient"):
+        getattr(L, name).argtypes = [C.POINTER(SW2D)]
+    for name in ("orc_sw2d_edge_flux", "orc_sw2d_mapped_time_derivative",
+                 "orc_sw2d_global_time_derivative", "orc_sw2d_forward_step_rk3"):
+        getattr(L, name).argtypes = [C.POINTER(SW2D), C.c_double]
+    L.orc_sw2d_integrate.argtypes = [C.POINTER(SW2D), C.c_double, C.c_int, C.c_int]
+    for name in ("orc_solve_cg", "orc_solve_pcg", "orc_solve_gmres", "orc_solve_pcgmres"):
+        getattr(L, name).argtypes = [C.POINTER(Krylov)]
+        getattr(L, name).restype = C.c_int
+    L.orc_elliptic1d_solve.argtypes = [C.c_int, C.c_int, C.c_int, C.c_double, _dp, _dp, _dp, _ip]
+    L.orc_elliptic1d_solve.restype = C.c_int
+    for extra in ("_declare_extra_adv2d", "_declare_extra_adv3d", "_declare_extra_cgsem2d"):
+        fn = globals().get(extra)
+        if fn is not None:
+            fn(L)
+
+
+def _p(a):
+    if a is None:
+        return None
+    return a.ctypes.data_as(_dp)
+
+
+def _pi(a):
+    return a.ctypes.data_as(_ip)
+
+
+class Nodal2D(C.Structure):
+    _fields_ = [("N", C.c_int), ("quadrature", C.c_int), ("form", C.c_int)] + [
+        (n, _dp) for n in ("nodes", "qWeight", "bWeights", "dMatS", "dMatP", "dMatCorrect",
+                           "lagWest", "lagEast", "lagSouth", "lagNorth")]
+
+
+class QuadMeshDesc(C.Structure):
+    _fields_ = [("nX", C.c_int), ("nY", C.c_int), ("N", C.c_int), ("periodicX", C.c_int),
+                ("periodicY", C.c_int), ("xScale", C.c_double), ("yScale", C.c_double),
+                ("warp", C.c_double), ("nElems", C.c_int), ("nEdges", C.c_int), ("nNodes", C.c_int)]
+
+
+class SW2D(C.Structure):
+    _fields_ = [("N", C.c_int), ("nElems", C.c_int), ("nEdges", C.c_int),
+                ("g", C.c_double), ("dt", C.c_double), ("linearDrag", C.c_double)] + [
+        (n, _dp) for n in ("dMatS", "dMatP", "qWeightS", "qWeightP", "lagWest", "lagEast",
+                           "lagSouth", "lagNorth")] + [("edges", _ip)] + [
+        (n, _dp) for n in ("dxds", "dxdp", "dyds", "dydp", "J", "nHat", "nLen", "sol", "tend",
+                           "bsol", "bflux", "bathy", "bbathy", "relax", "plv", "G")]
+
+
+VecOp = C.CFUNCTYPE(None, C.c_void_p, _dp, _dp)
+
+
+class Krylov(C.Structure):
+    _fields_ = [("nDOF", C.c_int), ("maxIters", C.c_int), ("mInnerIters", C.c_int),
+                ("tolerance", C.c_double), ("ctx", C.c_void_p), ("matrixAction", VecOp),
+                ("residual", VecOp), ("precondition", VecOp), ("sol", _dp), ("resi", _dp),
+                ("resiLen", C.c_int), ("iterations", C.c_int)]
+
+
+# --------------------------------------------------------------------------- L1/L2
+def quadrature(N: int, kind: int = GAUSS):
+    x = np.zeros(N + 1)
+    w = np.zeros(N + 1)
+    rc = lib().orc_generate_legendre_quadrature(N, _p(x), _p(w), kind)
+    if rc != 0:
+        raise ValueError("invalid quadrature type")
+    return x, w
+
+
+def legendre(nP: int, x: float):
+    l, d = C.c_double(), C.c_double()
+    lib().orc_legendre_polynomial(nP, x, C.byref(l), C.byref(d))
+    return l.value, d.value
+
+
+def barycentric_weights(nodes):
+    nodes = np.ascontiguousarray(nodes, dtype=np.float64)
+    bw = np.zeros_like(nodes)
+    lib().orc_barycentric_weights(len(nodes) - 1, _p(nodes), _p(bw))
+    return bw
+
+
+def lagrange_at(nodes, bw, x):
+    out = np.zeros(len(nodes))
+    lib().orc_lagrange_polynomials_at(len(nodes) - 1, _p(nodes), _p(bw), float(x), _p(out))
+    return out
+
+
+def lagrange_derivative(nodes, bw, x, f):
+    f = np.ascontiguousarray(f, dtype=np.float64)
+    return lib().orc_lagrange_derivative(len(nodes) - 1, _p(nodes), _p(bw), float(x), _p(f))
+
+
+def derivative_matrix(nodes, bw):
+    n1 = len(nodes)
+    D = np.zeros(n1 * n1)
+    lib().orc_derivative_matrix(n1 - 1, _p(nodes), _p(bw), _p(D))
+    return D.reshape(n1, n1, order="F")
+
+
+def interpolation_matrix(nodes, bw, znew):
+    znew = np.ascontiguousarray(znew, dtype=np.float64)
+    T = np.zeros(len(znew) * len(nodes))
+    lib().orc_interpolation_matrix(len(nodes) - 1, _p(nodes), _p(bw), len(znew) - 1, _p(znew), _p(T))
+    return T.reshape(len(znew), len(nodes), order="F")
+
+
+def coarse_to_fine_2d(fold, Ts, Tp):
+    fold = np.asfortranarray(fold, dtype=np.float64)
+    n = fold.shape[0] - 1
+    m = Ts.shape[0] - 1
+    Ts = np.asfortranarray(Ts)
+    Tp = np.asfortranarray(Tp)
+    out = np.zeros((m + 1) * (m + 1))
+    lib().orc_coarse_to_fine_2d(n, n, _p(fold.ravel(order="F")), m, m,
+                                _p(Ts.ravel(order="F")), _p(Tp.ravel(order="F")), _p(out))
+    return out.reshape(m + 1, m + 1, order="F")
+
+
+def coarse_to_fine_3d(fold, T):
+    fold = np.asfortranarray(fold, dtype=np.float64)
+    n = fold.shape[0] - 1
+    m = T.shape[0] - 1
+    T = np.asfortranarray(T)
+    out = np.zeros((m + 1) ** 3)
+    lib().orc_coarse_to_fine_3d(n, _p(fold.ravel(order="F")), m, _p(T.ravel(order="F")), _p(out))
+    return out.reshape(m + 1, m + 1, m + 1, order="F")
+
+
+class NodalStorage2D:
+    """NodalStorage_2D (src/nodal/NodalStorage_2D_Class.f90:41-230)."""
+
+    def __init__(self, N: int, quadrature: int = GAUSS, form: int = DG):
+        n1 = N + 1
+        self.N = N
+        self.nodes = np.zeros(n1)
+        self.qWeight = np.zeros(n1)
+        self.bWeights = np.zeros(n1)
+        self.dMatS = np.zeros(n1 * n1)
+        self.dMatP = np.zeros(n1 * n1)
+        self.dMatCorrect = np.zeros(n1 * n1)
+        self.lagWest = np.zeros(n1)
+        self.lagEast = np.zeros(n1)
+        self.lagSouth = np.zeros(n1)
+        self.lagNorth = np.zeros(n1)
+        s = Nodal2D(N, quadrature, form, *[_p(getattr(self, n)) for n in (
+            "nodes", "qWeight", "bWeights", "dMatS", "dMatP", "dMatCorrect",
+            "lagWest", "lagEast", "lagSouth", "lagNorth")])
+        if lib().orc_nodal2d_build(C.byref(s)) != 0:
+            raise ValueError("invalid nodal storage arguments")
+        for n in ("dMatS", "dMatP", "dMatCorrect"):
+            setattr(self, n, getattr(self, n).reshape(n1, n1, order="F"))
+
+
+def apply_derivative_2d(N, dMatS, dMatP, f):
+    n1 = N + 1
+    f = np.asfortranarray(f).ravel(order="F")
+    a = np.zeros(n1 * n1)
+    b = np.zeros(n1 * n1)
+    lib().orc_apply_derivative_2d(N, _p(np.asfortranarray(dMatS).ravel(order="F")),
+                                  _p(np.asfortranarray(dMatP).ravel(order="F")), _p(f), _p(a), _p(b))
+    return a.reshape(n1, n1, order="F"), b.reshape(n1, n1, order="F")
+
+
+def boundary_interp_2d(N, lW, lE, lS, lN, f):
+    n1 = N + 1
+    f = np.asfortranarray(f).ravel(order="F")
+    outs = [np.zeros(n1) for _ in range(4)]
+    lib().orc_boundary_interp_2d(N, _p(lW), _p(lE), _p(lS), _p(lN), _p(f), *[_p(o) for o in outs])
+    return outs  # west, east, south, north
+
+
+def apply_derivative_3d(N, dMat, f):
+    n1 = N + 1
+    f = np.asfortranarray(f).ravel(order="F")
+    outs = [np.zeros(n1 ** 3) for _ in range(3)]
+    lib().orc_apply_derivative_3d(N, _p(np.asfortranarray(dMat).ravel(order="F")), _p(f),
+                                  *[_p(o) for o in outs])
+    return [o.reshape(n1, n1, n1, order="F") for o in outs]
+
+
+def boundary_interp_3d(N, lL, lR, f):
+    n1 = N + 1
+    f = np.asfortranarray(f).ravel(order="F")
+    out = np.zeros(6 * n1 * n1)
+    lib().orc_boundary_interp_3d(N, _p(lL), _p(lR), _p(f), _p(out))
+    return out.reshape(6, n1, n1).transpose(0, 2, 1)  # [face][a][b] with a fastest in C
+
+
+# --------------------------------------------------------------------------- mesh
+class QuadMesh:
+    """Structured quad mesh with reference edge conventions (oracle.h)."""
+
+    def __init__(self, nX, nY, N, sNodes, periodicX=True, periodicY=True,
+                 xScale=1.0, yScale=1.0, warp=0.0):
+        d = QuadMeshDesc(nX, nY, N, int(periodicX), int(periodicY), xScale, yScale, warp, 0, 0, 0)
+        lib().orc_quadmesh_counts(C.byref(d))
+        n1 = N + 1
+        self.N, self.nElems, self.nEdges = N, d.nElems, d.nEdges
+        E = d.nElems
+        self.elemNodes = np.zeros(4 * E, dtype=np.int32)
+        self.edges = np.zeros(6 * d.nEdges, dtype=np.int32)
+        for n in ("x", "y", "dxds", "dxdp", "dyds", "dydp", "J"):
+            setattr(self, n, np.zeros(n1 * n1 * E))
+        self.nHat = np.zeros(2 * n1 * 4 * E)
+        self.nLen = np.zeros(n1 * 4 * E)
+        sn = np.ascontiguousarray(sNodes, dtype=np.float64)
+        rc = lib().orc_quadmesh_build(C.byref(d), _p(sn), _pi(self.elemNodes), _pi(self.edges),
+                                      *[_p(getattr(self, n)) for n in (
+                                          "x", "y", "dxds", "dxdp", "dyds", "dydp", "J",
+                                          "nHat", "nLen")])
+        if rc != 0:
+            raise ValueError(f"orc_quadmesh_build failed ({rc})")
+        self.edges = self.edges.reshape(-1, 6)
+
+
+# --------------------------------------------------------------------------- SW2D
+class ShallowWater2D:
+    """Conservative shallow water (ConservativeShallowWaterClass.f90) on the oracle.
+
+    Arrays are flat numpy buffers in the per-element Fortran layouts of oracle.h.
+    """
+
+    def __init__(self, nodal: NodalStorage2D, mesh: QuadMesh, g=9.81, dt=1.0,
+                 linearDrag=0.0, dMatP=None, with_relax=False):
+        N, E = nodal.N, mesh.nElems
+        n1 = N + 1
+        self.N, self.nElems = N, E
+        self._keep = []
+        dms = np.asfortranarray(nodal.dMatS).ravel(order="F").copy()
+        dmp = np.asfortranarray(nodal.dMatCorrect if dMatP is None else dMatP).ravel(order="F").copy()
+        self._keep += [dms, dmp]
+        self.sol = np.zeros(n1 * n1 * 3 * E)
+        self.tend = np.zeros_like(self.sol)
+        self.G = np.zeros_like(self.sol)
+        self.bsol = np.zeros(n1 * 3 * 4 * E)
+        self.bflux = np.zeros_like(self.bsol)
+        self.bathy = np.zeros(n1 * n1 * 3 * E)
+        self.bbathy = np.zeros(n1 * 3 * 4 * E)
+        self.relax = np.zeros(n1 * n1 * 4 * E) if with_relax else None
+        self.plv = np.zeros(n1 * n1 * E)
+        self.mesh = mesh
+        self.nodal = nodal
+        self.s = SW2D(N, E, mesh.nEdges, g, dt, linearDrag,
+                      _p(dms), _p(dmp), _p(nodal.qWeight), _p(nodal.qWeight),
+                      _p(nodal.lagWest), _p(nodal.lagEast), _p(nodal.lagSouth), _p(nodal.lagNorth),
+                      _pi(np.ascontiguousarray(mesh.edges.ravel())),
+                      *[_p(getattr(mesh, n)) for n in ("dxds", "dxdp", "dyds", "dydp", "J", "nHat", "nLen")],
+                      _p(self.sol), _p(self.tend), _p(self.bsol), _p(self.bflux), _p(self.bathy),
+                      _p(self.bbathy), _p(self.relax) if self.relax is not None else None,
+                      _p(self.plv), _p(self.G))
+        self._edges = np.ascontiguousarray(mesh.edges.ravel())
+        self.s.edges = _pi(self._edges)
+
+    def set_dt(self, dt):
+        self.s.dt = dt
+
+    def global_time_derivative(self, t=0.0):
+        lib().orc_sw2d_global_time_derivative(C.byref(self.s), t)
+
+    def forward_step_rk3(self, tn=0.0):
+        lib().orc_sw2d_forward_step_rk3(C.byref(self.s), tn)
+
+    def integrate(self, t0, nsteps, nthreads=1):
+        lib().orc_sw2d_integrate(C.byref(self.s), t0, nsteps, nthreads)
+
+    def bathymetry_gradient(self):
+        lib().orc_sw2d_bathymetry_gradient(C.byref(self.s))
+
+
+def riemann(inS, outS, nHat, g):
+    out = np.zeros(3)
+    a = np.ascontiguousarray(inS, dtype=np.float64)
+    b = np.ascontiguousarray(outS, dtype=np.float64)
+    n = np.ascontiguousarray(nHat, dtype=np.float64)
+    lib().orc_sw2d_riemann(_p(a), _p(b), _p(n), g, _p(out))
+    return out
+
+
+# --------------------------------------------------------------------------- Krylov
+def _dense_solver(fn_name, A, b, x0=None, maxIters=200, m=30, tol=1e-12, precond=False):
+    L = lib()
+    n = A.shape[0]
+    A = np.asfortranarray(A, dtype=np.float64)
+    b = np.ascontiguousarray(b, dtype=np.float64)
+    Af = A.ravel(order="F").copy()
+
+    def mv(ctx, i, o):
+        v = np.ctypeslib.as_array(i, shape=(n,))
+        np.ctypeslib.as_array(o, shape=(n,))[:] = A @ v
+
+    def res(ctx, i, o):
+        v = np.ctypeslib.as_array(i, shape=(n,))
+        np.ctypeslib.as_array(o, shape=(n,))[:] = b - A @ v
+
+    def pc(ctx, i, o):
+        v = np.ctypeslib.as_array(i, shape=(n,))
+        np.ctypeslib.as_array(o, shape=(n,))[:] = v / np.diag(A)
+
+    cb = (VecOp(mv), VecOp(res), VecOp(pc))
+    sol = np.zeros(n) if x0 is None else np.array(x0, dtype=np.float64)
+    resiLen = maxIters * (m if "gmres" in fn_name else 1) + 1
+    resi = np.zeros(resiLen)
+    k = Krylov(n, maxIters, m, tol, None, cb[0], cb[1], cb[2], _p(sol), _p(resi), resiLen, 0)
+    ioerr = getattr(L, fn_name)(C.byref(k))
+    del Af
+    return sol, resi, ioerr, k.iterations
+
+
+def solve_cg(A, b, **kw):
+    return _dense_solver("orc_solve_cg", A, b, **kw)
+
+
+def solve_pcg(A, b, **kw):
+    return _dense_solver("orc_solve_pcg", A, b, **kw)
+
+
+def solve_gmres(A, b, **kw):
+    return _dense_solver("orc_solve_gmres", A, b, **kw)
+
+
+def solve_pcgmres(A, b, **kw):
+    return _dense_solver("orc_solve_pcgmres", A, b, **kw)
+
+
+def elliptic1d(N=5, nElems=10, maxIters=1000, tol=1e-8):
+    n = (N + 1) * nElems
+    x = np.zeros(n)
+    u = np.zeros(n)
+    resi = np.zeros(maxIters + 1)
+    ioerr = C.c_int()
+    iters = lib().orc_elliptic1d_solve(N, nElems, maxIters, tol, _p(x), _p(u), _p(resi), C.byref(ioerr))
+    return x, u, resi, ioerr.value, iters
