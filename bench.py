@@ -1,0 +1,183 @@
+"""Benchmark of the shallow-water DGSEM hot path on one MI355X.
+
+Workload (the configuration the headline metric is quoted on): 2-D nonlinear
+shallow water, 64x64 quad elements on a doubly periodic 1e6 m x 1e6 m
+f-plane (f = 1e-4 1/s, g = 9.81 m/s^2), N = 7 Gauss nodes (262144 nodes x 3
+fields), seeded bathymetry (8 Fourier modes, |h - 1000 m| <= 50 m) and a
+geostrophically balanced 1 m Gaussian, RK3 at CFL 0.5.  One "step" is one
+low-storage RK3 step (3 stages) over the whole state; inputs are resident in
+HBM when the timed region starts.
+
+value = DOF-stage/s of the mixed-precision fused path (DOF = node x field,
+one RK3 stage), whole job on one GPU.  The FP64 fused path and the FP64
+plain drop-in (one kernel per reference phase) are timed beside it, and the
+mixed result is compared with the FP64 result after the same steps.
+
+roofline: algorithmic bytes per stage launch of the dominant kernel (the
+fused stage kernel) over its mean launch time measured with HIP events on
+the model's stream.  cpu_baseline: the FP64 oracle (C restatement of the
+reference, OpenMP over elements and edges) on this host, on a bounded
+sample of the same workload.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "DOF-stage/s, SW2D N=7 FP64 vs mixed @1e-6 rel L2; p50/p99 batched-solve latency"
+HBM_PEAK_GBS = 8000.0
+SEED = 20261202
+
+
+def sw_bytes_per_element(N, s_q, s_G, s_m, s_b, s_n):
+    """Compulsory bytes per element per RK stage: q and G read+write,
+    5 metric terms, 4 bathymetry/Coriolis fields per node, and
+    (n_x, n_y, |n|) per side node."""
+    np_ = (N + 1) ** 2
+    return np_ * (2 * 3 * s_q + 2 * 3 * s_G + 5 * s_m + 4 * s_b) + 4 * (N + 1) * 3 * s_n
+
+
+def load_traffic(name):
+    path = os.path.join(ROOT, "profiles", name)
+    if os.path.exists(path):
+        with open(path) as f:
+            return json.load(f)
+    return None
+
+
+def cpu_baseline(nX, N, target_s=12.0):
+    """The oracle (FP64, OpenMP) on a bounded sample of the workload."""
+    import numpy as np
+    from oracle import pyoracle as po
+    from self_amd.workloads import sw2d_cfl_dt, sw2d_geostrophic_fields
+
+    threads = max(1, min(16, os.cpu_count() or 1))
+    L = 1.0e6
+    nodal = po.NodalStorage2D(N)
+    mesh = po.QuadMesh(nX, nX, N, nodal.nodes, True, True, L, L)
+    n1 = N + 1
+    x = mesh.x.reshape(-1, n1, n1)
+    y = mesh.y.reshape(-1, n1, n1)
+    h, U, V, H = sw2d_geostrophic_fields(x, y, L=L, seed=SEED)
+    dt = sw2d_cfl_dt(U, V, H, 9.81, L / nX, N)
+    orc = po.ShallowWater2D(nodal, mesh, g=9.81, dt=dt)
+    orc.sol[:] = np.stack([U, V, H], axis=1).ravel()
+    orc.bathy[:] = np.stack([h, 0 * h, 0 * h], axis=1).ravel()
+    orc.plv[:] = 1.0e-4
+    orc.bathymetry_gradient()
+    t0 = time.perf_counter()
+    orc.integrate(0.0, 2, threads)
+    probe = (time.perf_counter() - t0) / 2
+    steps = int(max(2, min(500, target_s / max(probe, 1e-6))))
+    t0 = time.perf_counter()
+    orc.integrate(0.0, steps, threads)
+    el = time.perf_counter() - t0
+    dof = 3 * nX * nX * n1 * n1
+    return {"value": dof * 3 * steps / el, "unit": "DOF-stage/s", "cores": threads, "kind": "port",
+            "sample": f"oracle FP64 C restatement, {nX}x{nX} N={N} geostrophic case, {steps} RK3 steps, "
+                      f"{threads} OpenMP threads, {el:.2f} s"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=500)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); the shallow-water path "
+              "shards by element blocks with a face-trace halo, not run here.", file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    from self_amd import _lib as L
+    from self_amd.workloads import sw2d_geostrophic
+
+    L.check(L.lib().self_init(0))
+    nX, N = 64, 7
+    K, W = args.steps, args.warmup
+    E = nX * nX
+    dof = 3 * E * (N + 1) ** 2
+    results, sols = {}, {}
+    for name, prec, form in (("mixed_fused", "mixed", "fused"), ("fp64_fused", "fp64", "fused"),
+                             ("fp64_plain", "fp64", "plain")):
+        m, info = sw2d_geostrophic(nX, N, precision=prec, form=form, seed=SEED)
+        if W:
+            m.ForwardStepRK3(0.0, W)
+        m.Synchronize()
+        t0 = time.perf_counter()
+        ms = m.ForwardStepRK3Timed(0.0, K)
+        wall = time.perf_counter() - t0
+        sols[name] = m.GetSolution()
+        kt = m.KernelTimes(0.0, 20)
+        results[name] = {"ms_per_step": ms / K, "wall_ms_per_step": 1e3 * wall / K,
+                         "dof_stage_per_s": dof * 3 * K / (ms * 1e-3), "kernel_ms": kt, "dt": info["dt"]}
+        m.Trash()
+
+    def rel(a, b):
+        return float(np.linalg.norm((a - b).ravel()) / np.linalg.norm(b.ravel()))
+
+    acc = [rel(sols["mixed_fused"][:, f], sols["fp64_fused"][:, f]) for f in range(3)]
+    acc_plain = [rel(sols["fp64_fused"][:, f], sols["fp64_plain"][:, f]) for f in range(3)]
+
+    head = results["mixed_fused"]
+    # algorithmic bytes per stage launch (mixed: FP32 tendency register and
+    # bathymetry/Coriolis, FP64 state, metrics and normals)
+    b_mixed = sw_bytes_per_element(N, 8, 4, 8, 4, 8) * E
+    b_fp64 = sw_bytes_per_element(N, 8, 8, 8, 8, 8) * E
+    # mean stage-kernel duration: the timed region is 3K back-to-back launches
+    # of the fused stage kernel on the model's stream
+    stage_ms = head["ms_per_step"] / 3.0
+    achieved = b_mixed / (stage_ms * 1e-3) / 1e9
+    traffic = load_traffic("sw2d_stage_fused_mixed_64x64_N7_traffic.json")
+    roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(achieved / HBM_PEAK_GBS, 4),
+                "traffic": traffic.get("bytes_per_launch") if traffic else None,
+                "kernel": "k_sw_stage_fused<7,mixed>", "algorithmic_bytes_per_launch": b_mixed,
+                "mean_launch_ms": stage_ms, "event_pair_ms": head["kernel_ms"][0]}
+    fp64_stage_ms = results["fp64_fused"]["ms_per_step"] / 3.0
+    out = {
+        "metric": METRIC,
+        "value": head["dof_stage_per_s"],
+        "unit": "DOF-stage/s",
+        "n_gpus": 1,
+        "steps": K,
+        "warmup": W,
+        "ms_per_step": head["ms_per_step"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64 state/metrics + f32 contractions",
+        "data": "synthetic",
+        "config": {"workload": "SW2D geostrophic f-plane 64x64 elements N=7 periodic, RK3 CFL 0.5",
+                   "elements": E, "nodes": E * (N + 1) ** 2, "fields": 3, "N": N, "dt_s": head["dt"],
+                   "seed": SEED, "parallelism": "single GPU"},
+        "roofline": roofline,
+        "variants": {
+            "fp64_fused": {"dof_stage_per_s": results["fp64_fused"]["dof_stage_per_s"],
+                           "ms_per_step": results["fp64_fused"]["ms_per_step"],
+                           "stage_kernel_ms": fp64_stage_ms,
+                           "achieved_GBs": round(b_fp64 / (fp64_stage_ms * 1e-3) / 1e9, 1)},
+            "fp64_plain": {"dof_stage_per_s": results["fp64_plain"]["dof_stage_per_s"],
+                           "ms_per_step": results["fp64_plain"]["ms_per_step"],
+                           "kernel_ms_traces_edgeflux_mapped_rk": results["fp64_plain"]["kernel_ms"]},
+        },
+        "speedup_mixed_vs_fp64_plain": head["dof_stage_per_s"] / results["fp64_plain"]["dof_stage_per_s"],
+        "speedup_fp64_fused_vs_fp64_plain": results["fp64_fused"]["dof_stage_per_s"]
+        / results["fp64_plain"]["dof_stage_per_s"],
+        "accuracy": {"mixed_vs_fp64_rel_l2_UVH": acc, "fp64_fused_vs_plain_rel_l2_UVH": acc_plain,
+                     "steps_compared": K + W, "gate": 1e-6, "pass": max(acc) <= 1e-6},
+    }
+    if not args.no_cpu:
+        cb = cpu_baseline(nX, N)
+        out["cpu_baseline"] = cb
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

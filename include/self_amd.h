@@ -1,0 +1,182 @@
+/*
+ * self_amd.h -- C ABI of the MI355X-native SELF spectral-element hot path.
+ *
+ * Plain C types only (int, double, float, pointers, sizes): every entry
+ * point is callable from Fortran through iso_c_binding, from C/C++, or from
+ * Python through ctypes.  Each declaration names the reference routine it
+ * replaces (paths relative to the lileipku00/SELF repository root).
+ *
+ * Conventions
+ *  - Return value: SELF_OK (0) or a negative status.  -1 and -2 keep the
+ *    reference solvers' ioerr meaning (IterativeSolvers_Class.f90:337-340);
+ *    the rest are new.  self_last_error() gives a message for the calling
+ *    thread.  Nothing here aborts the process.
+ *  - Host arrays use the reference's Fortran column-major layouts with its
+ *    0-based node ranges.  For per-element storage the element index is the
+ *    slowest dimension, i.e. the reference's per-element arrays
+ *    sol(iEl)%solution(0:N,0:N,1:nEq) stacked for iEl = 1..nElems.
+ *  - Element ids in edge tables are 1-based and sides 1..4 are
+ *    south, east, north, west, as in QuadMeshClass.f90.
+ *  - A handle owns its device buffers and one HIP stream (default priority);
+ *    state stays resident between calls.  One host thread uses a handle at a
+ *    time.  Calls on a handle are asynchronous on its stream unless stated;
+ *    Get/Set calls and *_synchronize return after the stream has drained.
+ */
+#ifndef SELF_AMD_H
+#define SELF_AMD_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SELF_OK 0
+#define SELF_NOT_CONVERGED (-1)   /* reference ioerr -1 */
+#define SELF_ERR_OTHER (-2)       /* reference ioerr -2 */
+#define SELF_ERR_HIP (-3)
+#define SELF_ERR_OOM (-4)
+#define SELF_ERR_ARG (-5)
+#define SELF_ERR_STATE (-6)
+
+/* Flags with the values of src/common/ModelFlags.f90 */
+#define SELF_GAUSS 1
+#define SELF_GAUSS_LOBATTO (-1)
+#define SELF_DG 2000
+#define SELF_CG 2001
+#define SELF_NO_NORMAL_FLOW (-100)
+#define SELF_RADIATION (-101)
+
+/* Precision of a model's tendency arithmetic */
+#define SELF_FP64 0
+#define SELF_MIXED 1   /* FP32 volume-flux contractions, FP64 state/metrics/RK */
+
+/* Execution form of a model's stage */
+#define SELF_PLAIN 0   /* one kernel per reference phase, fields round-trip HBM */
+#define SELF_FUSED 1   /* traces + Riemann flux + volume derivative + lift + RK3 in one kernel */
+
+/* ------------------------------------------------------------------ runtime */
+const char *self_last_error(void);
+int self_version(void);
+/* Select the HIP device used by handles created afterwards on this thread. */
+int self_init(int device);
+int self_device_count(int *n);
+/* Free device memory in bytes, as used for admission decisions. */
+int self_mem_info(size_t *free_bytes, size_t *total_bytes);
+
+/* ------------------------------------------------------- host setup (CPU)
+ * Quadrature, interpolants and derivative matrices are computed on the host
+ * once, exactly as the reference's Build routines do, then uploaded. */
+
+/* GenerateLegendreQuadrature: src/interp/Legendre.f90:180-212
+ * (LegendreGauss :216-308, LegendreGaussLobatto :312-405). */
+int self_legendre_quadrature(int N, int quadType, double *x, double *w);
+/* CalculateBarycentricWeights_1D: src/interp/Lagrange_1D_Class.f90:349-392 */
+int self_barycentric_weights(int N, const double *x, double *bw);
+/* EvaluateLagrangePolynomial_1D: src/interp/Lagrange_1D_Class.f90:448-515 */
+int self_lagrange_polynomials(int N, const double *x, const double *bw, double xEval, double *l);
+/* CalculateDerivativeMatrix_1D: src/interp/Lagrange_1D_Class.f90:612-689; D(k,j) at k+(N+1)*j */
+int self_derivative_matrix(int N, const double *x, const double *bw, double *D);
+/* CalculateInterpolationMatrix_1D: src/interp/Lagrange_1D_Class.f90:693-764; T(k,j) at k+(nNew+1)*j */
+int self_interpolation_matrix(int N, const double *x, const double *bw, int nNew,
+                              const double *zNew, double *T);
+/* BuildNodalStorage_2D: src/nodal/NodalStorage_2D_Class.f90:101-230.
+ * quirk_dmatp_transposed = 1 stores dMatP exactly as the reference's DG
+ * branch does (:209-218, the transpose of the weak operator); 0 stores the
+ * weak operator -W^-1 D^T W in both directions, as the reference's 1-D and
+ * 3-D storages do.  Arrays: (N+1) or (N+1)^2 doubles. */
+int self_nodal2d_build(int N, int quadType, int form, int quirk_dmatp_transposed,
+                       double *nodes, double *qWeight, double *dMatS, double *dMatP,
+                       double *lagWest, double *lagEast, double *lagSouth, double *lagNorth);
+
+/* Structured quad mesh with the node/element numbering of
+ * LoadDefaultMesh_QuadMesh (src/geom/QuadMeshClass.f90:1718-1859), the edge
+ * table of ConstructEdges_QuadMesh (:1489-1627) and its start/inc
+ * convention (:1830-1846), optional periodic pairing per direction (new:
+ * the reference has no periodic boundaries), and the covariant metrics,
+ * Jacobian and boundary normals of MappedGeometry_2D (src/geom/
+ * MappedGeometryClass_2D.f90:611-710, scaled as :921-948) for the bilinear
+ * element map composed with the unit-square warp
+ *   X = a + warp sin(2 pi a) sin(2 pi b),  Y = b + warp sin(2 pi a) sin(2 pi b).
+ * edges: 6 ints per edge (e1, s1, e2, s2, start, inc); e2 = NO_NORMAL_FLOW on
+ * walls.  Per-element arrays (N+1)^2 per element; nHat 2*(N+1)*4 and nLen
+ * (N+1)*4 per element, ordered [node][side] with side 1..4. */
+int self_quadmesh_counts(int nX, int nY, int periodicX, int periodicY, int *nElems, int *nEdges);
+int self_quadmesh_structured(int nX, int nY, int N, int periodicX, int periodicY,
+                             double xScale, double yScale, double warp, const double *sNodes,
+                             int *edges, double *x, double *y, double *dxds, double *dxdp,
+                             double *dyds, double *dydp, double *J, double *nHat, double *nLen);
+
+/* ------------------------------------------- conservative shallow water 2-D
+ * Replaces the ShallowWater type of
+ * src/highend/shallowwater/ConservativeShallowWaterClass.f90 (fields :60-151).
+ */
+typedef struct self_sw2d_s *self_sw2d_t;
+
+typedef struct {
+    int N;            /* polyDeg (nS = nP = N), 1 <= N <= 11 */
+    int nElems;
+    int nEdges;
+    int precision;    /* SELF_FP64 | SELF_MIXED */
+    int form;         /* SELF_PLAIN | SELF_FUSED */
+    double g;         /* SWParams%g */
+    double dt;        /* SWParams%dt */
+    double linearDrag;/* SWParams%linearDrag */
+} self_sw2d_desc;
+
+/* Field selectors for Set/Get (host layout per element, element slowest). */
+#define SELF_SW_SOLUTION 0        /* (0:N,0:N,1:3)  U, V, H                    (sol%solution) */
+#define SELF_SW_TENDENCY 1        /* (0:N,0:N,1:3)                             (sol%tendency), get only */
+#define SELF_SW_BATHYMETRY 2      /* (0:N,0:N,1:3)  h, dh/dx, dh/dy            (bathymetry%solution) */
+#define SELF_SW_PLANETARY_VORTICITY 3 /* (0:N,0:N)  f                          (vorticity var 1) */
+#define SELF_SW_RELAXATION 4      /* (0:N,0:N,1:4)  U_r, V_r, H_r, 1/timescale (relax%solution) */
+#define SELF_SW_BOUNDARY_SOLUTION 5 /* (0:N,1:3,1:4)                           (sol%boundarySolution), get only */
+#define SELF_SW_BOUNDARY_FLUX 6   /* (0:N,1:3,1:4)                             (sol%boundaryFlux), get only */
+
+/* Build_ShallowWater (:163-245): allocates device state for the descriptor. */
+int self_sw2d_create(const self_sw2d_desc *desc, self_sw2d_t *out);
+/* Trash_ShallowWater (:249-268) */
+int self_sw2d_destroy(self_sw2d_t h);
+/* NodalStorage_2D arrays used by the model (GetDerivativeMatrix and friends,
+ * NodalStorage_2D_Class.f90:403-418). */
+int self_sw2d_set_nodal(self_sw2d_t h, const double *dMatS, const double *dMatP,
+                        const double *qWeightS, const double *qWeightP,
+                        const double *lagWest, const double *lagEast,
+                        const double *lagSouth, const double *lagNorth);
+/* Mesh inputs: edge table as self_quadmesh_structured, metrics per element,
+ * boundary normals [node][side] per element. */
+int self_sw2d_set_geometry(self_sw2d_t h, const int *edges, const double *dxds,
+                           const double *dxdp, const double *dyds, const double *dydp,
+                           const double *J, const double *nHat, const double *nLen);
+/* SetSolution/SetBathymetry/SetPlanetaryVorticity/SetRelaxationField
+ * (:300-900 accessors) and their Get counterparts, whole mesh at once. */
+int self_sw2d_set_field(self_sw2d_t h, int field, const double *data);
+int self_sw2d_get_field(self_sw2d_t h, int field, double *data);
+int self_sw2d_set_params(self_sw2d_t h, double g, double dt, double linearDrag);
+/* CalculateSolutionAtBoundaries for every element (:1146-1156) */
+int self_sw2d_calculate_solution_at_boundaries(self_sw2d_t h);
+/* EdgeFlux for every edge (:1266-1360) */
+int self_sw2d_edge_flux(self_sw2d_t h, double t);
+/* MappedTimeDerivative for every element (:1364-1522) */
+int self_sw2d_mapped_time_derivative(self_sw2d_t h, double t);
+/* GlobalTimeDerivative (:1222-1262): the three phases above. */
+int self_sw2d_global_time_derivative(self_sw2d_t h, double t);
+/* ForwardStepRK3 (:1173-1218) applied nSteps times from tn. */
+int self_sw2d_forward_step_rk3(self_sw2d_t h, double tn, int nSteps);
+/* Same, bracketed by HIP events on the handle's stream; *elapsed_ms is the
+ * device time of the nSteps steps. */
+int self_sw2d_forward_step_rk3_timed(self_sw2d_t h, double tn, int nSteps, double *elapsed_ms);
+/* Runs nSteps RK3 steps with one HIP event pair around every kernel launch
+ * and returns the mean device time per launch of each kernel of the stage:
+ * FUSED -> 1 kernel (the stage kernel); PLAIN -> 4 (traces, edge flux,
+ * mapped tendency, RK update).  Measurement support, no reference counterpart. */
+int self_sw2d_kernel_times(self_sw2d_t h, double tn, int nSteps, double *avg_ms, int *nKernels);
+/* CalculateBathymetryAtBoundaries + CalculateBathymetryGradient for every
+ * element (:1160-1168, :1733-1868): sets bathymetry vars 2 and 3. */
+int self_sw2d_calculate_bathymetry_gradient(self_sw2d_t h);
+int self_sw2d_synchronize(self_sw2d_t h);
+/* Bytes of HBM state the handle holds. */
+int self_sw2d_device_bytes(self_sw2d_t h, size_t *bytes);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

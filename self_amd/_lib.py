@@ -1,0 +1,126 @@
+"""Loader for libself_amd.so, the C ABI declared in include/self_amd.h.
+
+The library is built in-tree by ``make -C self_amd`` (see __graft_entry__.build).
+There is no fallback: if the library is missing or a call fails, an
+exception is raised.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libself_amd.so")
+
+SELF_OK = 0
+SELF_NOT_CONVERGED = -1
+SELF_ERR_OTHER = -2
+
+GAUSS, GAUSS_LOBATTO = 1, -1
+DG, CG = 2000, 2001
+NO_NORMAL_FLOW, RADIATION = -100, -101
+FP64, MIXED = 0, 1
+PLAIN, FUSED = 0, 1
+
+dp = C.POINTER(C.c_double)
+fp = C.POINTER(C.c_float)
+ip = C.POINTER(C.c_int)
+szp = C.POINTER(C.c_size_t)
+
+_lib = None
+
+
+class SelfError(RuntimeError):
+    """A C-ABI call returned a negative status."""
+
+    def __init__(self, code, msg):
+        super().__init__(f"[{code}] {msg}")
+        self.code = code
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError(
+                f"{LIB_PATH} is not built; run __graft_entry__.build() or make -C self_amd")
+        _lib = C.CDLL(LIB_PATH)
+        _declare(_lib)
+    return _lib
+
+
+def check(rc, allow=()):
+    if rc != SELF_OK and rc not in allow:
+        msg = lib().self_last_error().decode(errors="replace")
+        raise SelfError(rc, msg)
+    return rc
+
+
+def ptr(a, kind=dp):
+    if a is None:
+        return None
+    return a.ctypes.data_as(kind)
+
+
+class SWDesc(C.Structure):
+    _fields_ = [("N", C.c_int), ("nElems", C.c_int), ("nEdges", C.c_int), ("precision", C.c_int),
+                ("form", C.c_int), ("g", C.c_double), ("dt", C.c_double), ("linearDrag", C.c_double)]
+
+
+_SIGS = {
+    "self_last_error": ([], C.c_char_p),
+    "self_version": ([], C.c_int),
+    "self_init": ([C.c_int], C.c_int),
+    "self_device_count": ([ip], C.c_int),
+    "self_mem_info": ([szp, szp], C.c_int),
+    "self_legendre_quadrature": ([C.c_int, C.c_int, dp, dp], C.c_int),
+    "self_barycentric_weights": ([C.c_int, dp, dp], C.c_int),
+    "self_lagrange_polynomials": ([C.c_int, dp, dp, C.c_double, dp], C.c_int),
+    "self_derivative_matrix": ([C.c_int, dp, dp, dp], C.c_int),
+    "self_interpolation_matrix": ([C.c_int, dp, dp, C.c_int, dp, dp], C.c_int),
+    "self_nodal2d_build": ([C.c_int, C.c_int, C.c_int, C.c_int] + [dp] * 8, C.c_int),
+    "self_quadmesh_counts": ([C.c_int, C.c_int, C.c_int, C.c_int, ip, ip], C.c_int),
+    "self_quadmesh_structured": ([C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_double, C.c_double,
+                                  C.c_double, dp, ip] + [dp] * 9, C.c_int),
+    "self_sw2d_create": ([C.POINTER(SWDesc), C.POINTER(C.c_void_p)], C.c_int),
+    "self_sw2d_destroy": ([C.c_void_p], C.c_int),
+    "self_sw2d_set_nodal": ([C.c_void_p] + [dp] * 8, C.c_int),
+    "self_sw2d_set_geometry": ([C.c_void_p, ip] + [dp] * 7, C.c_int),
+    "self_sw2d_set_field": ([C.c_void_p, C.c_int, dp], C.c_int),
+    "self_sw2d_get_field": ([C.c_void_p, C.c_int, dp], C.c_int),
+    "self_sw2d_set_params": ([C.c_void_p, C.c_double, C.c_double, C.c_double], C.c_int),
+    "self_sw2d_calculate_solution_at_boundaries": ([C.c_void_p], C.c_int),
+    "self_sw2d_edge_flux": ([C.c_void_p, C.c_double], C.c_int),
+    "self_sw2d_mapped_time_derivative": ([C.c_void_p, C.c_double], C.c_int),
+    "self_sw2d_global_time_derivative": ([C.c_void_p, C.c_double], C.c_int),
+    "self_sw2d_forward_step_rk3": ([C.c_void_p, C.c_double, C.c_int], C.c_int),
+    "self_sw2d_forward_step_rk3_timed": ([C.c_void_p, C.c_double, C.c_int, dp], C.c_int),
+    "self_sw2d_calculate_bathymetry_gradient": ([C.c_void_p], C.c_int),
+    "self_sw2d_kernel_times": ([C.c_void_p, C.c_double, C.c_int, dp, ip], C.c_int),
+    "self_sw2d_synchronize": ([C.c_void_p], C.c_int),
+    "self_sw2d_device_bytes": ([C.c_void_p, szp], C.c_int),
+}
+
+EXTRA_SIGS: dict = {}
+
+
+def _declare(L):
+    for name, (args, res) in list(_SIGS.items()) + list(EXTRA_SIGS.items()):
+        fn = getattr(L, name)
+        fn.argtypes = args
+        fn.restype = res
+
+
+def exported_symbols():
+    """Names this module binds (used by tests against include/self_amd.h)."""
+    return sorted(list(_SIGS) + list(EXTRA_SIGS))
+
+
+def f64(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def i32(a):
+    return np.ascontiguousarray(a, dtype=np.int32)

@@ -1,0 +1,101 @@
+// dg2d_device.hpp -- device building blocks shared by the 2-D DGSEM kernels.
+//
+// Element storage on the device is structure-of-arrays: a nodal field f of
+// element e lives at  base[(f*nElems + e)*NP + node],  node = i + (N+1)*j,
+// so the 64 nodes of an N=7 element are one contiguous 512-byte line per
+// field and a wavefront reads them with one coalesced load.  Face traces
+// and face fluxes are kept per element as [e][side][field][node] so one
+// neighbouring side is a contiguous (N+1)*nEq run.
+//
+// Sums over nodes use explicit fused multiply-adds in a fixed order, so a
+// trace computed in two different kernels (or twice in one) is bitwise the
+// same value: neighbours read exactly what the owner used.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+namespace self {
+
+template <int N>
+struct Shape {
+    static constexpr int n1 = N + 1;
+    static constexpr int np = n1 * n1;
+    static constexpr int threads = 256;
+    static constexpr int epb = (threads / np) > 0 ? threads / np : 1;  // elements per block
+};
+
+// dot of a Lagrange vector with a strided line of nodal values
+template <int N, typename T>
+__device__ __forceinline__ double line_dot(const double *__restrict__ l, const T *v, int stride)
+{
+    double acc = l[0] * (double)v[0];
+    #pragma unroll
+    for (int m = 1; m <= N; ++m) acc = __builtin_fma(l[m], (double)v[m * stride], acc);
+    return acc;
+}
+
+// Rusanov-type flux of the conservative shallow-water equations,
+// RiemannSolver, ConservativeShallowWaterClass.f90:1555-1604 (with g in the
+// wave speed, quirk Q2 fixed).  in/out are (U, V, H) on the two sides.
+__device__ __forceinline__ void sw_riemann(const double in[3], const double out[3], double nx,
+                                           double ny, double g, double f[3])
+{
+    const double uL = in[0], vL = in[1], pL = in[2];
+    const double uR = out[0], vR = out[1], pR = out[2];
+    const double uOut = uR * nx + vR * ny;
+    const double uIn = uL * nx + vL * ny;
+    const double cR = sqrt(g * pR), cL = sqrt(g * pL);
+    const double wR = uOut / pR, wL = uIn / pL;
+    const double fac = fmax(fmax(fabs(wR + cR), fabs(wR - cR)), fmax(fabs(wL + cL), fabs(wL - cL)));
+    const double hgR = 0.5 * g * pR * pR, hgL = 0.5 * g * pL * pL;
+    const double a0 = wR * uR + hgR * nx + wL * uL + hgL * nx;
+    const double a1 = wR * vR + hgR * ny + wL * vL + hgL * ny;
+    const double a2 = uOut + uIn;
+    f[0] = 0.5 * (a0 - fac * (out[0] - in[0]));
+    f[1] = 0.5 * (a1 - fac * (out[1] - in[1]));
+    f[2] = 0.5 * (a2 - fac * (out[2] - in[2]));
+}
+
+// GetExternalState, ConservativeShallowWaterClass.f90:1698-1729
+__device__ __forceinline__ void sw_external(int bc, double nx, double ny, const double in[3], double ex[3])
+{
+    if (bc == -100) {  // NO_NORMAL_FLOW: reflect the momentum about the wall
+        ex[0] = (ny * ny - nx * nx) * in[0] - 2.0 * nx * ny * in[1];
+        ex[1] = (nx * nx - ny * ny) * in[1] - 2.0 * nx * ny * in[0];
+        ex[2] = in[2];
+    } else {           // RADIATION and unknown flags: zero state
+        ex[0] = 0.0; ex[1] = 0.0; ex[2] = 0.0;
+    }
+}
+
+// XFlux / YFlux, ConservativeShallowWaterClass.f90:1608-1658
+__device__ __forceinline__ void sw_fluxes(const double s[3], double g, double xF[3], double yF[3])
+{
+    // one reciprocal instead of four divisions (differs from U*U/H by at most 1 ulp)
+    const double invH = 1.0 / s[2];
+    const double hp = g * 0.5 * s[2] * s[2];
+    xF[0] = s[0] * s[0] * invH + hp;
+    xF[1] = s[0] * s[1] * invH;
+    xF[2] = s[0];
+    yF[0] = s[1] * s[0] * invH;
+    yF[1] = s[1] * s[1] * invH + hp;
+    yF[2] = s[1];
+}
+
+// Side descriptor packed per element side (int2):
+//   x >= 0 : neighbour element (0-based); x < 0 : boundary condition flag
+//   y bits : 0-1 neighbour side (0..3), 2 flip (k -> N-k), 3 this side is the edge's primary
+__device__ __forceinline__ int side_nbr_side(int y) { return y & 3; }
+__device__ __forceinline__ bool side_flip(int y) { return (y >> 2) & 1; }
+__device__ __forceinline__ bool side_primary(int y) { return (y >> 3) & 1; }
+
+// XCD-aware block order: the hardware hands consecutive workgroups to the 8
+// XCDs round-robin; remapping gives each XCD one contiguous run of elements,
+// so neighbours share an L2 and DRAM pages.
+__device__ __forceinline__ int xcd_block(int b, int nb)
+{
+    if ((nb & 7) != 0) return b;
+    return (b & 7) * (nb >> 3) + (b >> 3);
+}
+
+}  // namespace self

@@ -1,0 +1,1217 @@
+// sw2d.hip -- conservative 2-D shallow-water DGSEM on gfx950.
+//
+// Reference: src/highend/shallowwater/ConservativeShallowWaterClass.f90
+//   ForwardStepRK3 :1173-1218, GlobalTimeDerivative :1222-1262,
+//   EdgeFlux :1266-1360, MappedTimeDerivative :1364-1522,
+//   DGSystemDerivative :1526-1551, RiemannSolver :1555-1604,
+//   XFlux/YFlux :1608-1658, Source :1662-1694, GetExternalState :1698-1729,
+//   CalculateBathymetryGradient :1733-1868;
+//   CalculateSolutionAtBoundaries src/dgsem/DGSEMSolutionStorageClass_2D.f90:749-802.
+//
+// Two execution forms:
+//  PLAIN  one kernel per reference phase (traces, edge fluxes, mapped
+//         tendency, RK3 update); traces, fluxes and the tendency go through
+//         HBM as in the reference.  This is the FP64 drop-in.
+//  FUSED  one kernel per RK stage.  Each element owns a workgroup slice:
+//         it loads its state once, evaluates its own face traces from LDS,
+//         reads the neighbours' traces written by the previous stage,
+//         computes the Rusanov flux of each side node (taking the edge's
+//         primary orientation so the value is the one EdgeFlux would store),
+//         applies the tensor-product weak derivative and lift, adds the
+//         source, does the low-storage RK3 update in place and writes the
+//         traces of the new state for the next stage (ping-pong buffers).
+// The MIXED precision form is FUSED with FP32 volume-flux contractions
+// (applied to the deviation of each flux line from its FP64 mean; the mean's
+// derivative is carried exactly by the FP64 lift, using
+// sum_k Dhat(i,k) = -(l_E(i) - l_W(i))/w_i) and FP32 storage of the RK
+// tendency register, bathymetry gradient and Coriolis parameter.  State,
+// traces, metrics, normals and the RK accumulation stay FP64.
+
+#include <cstring>
+#include <vector>
+
+#include "common.hpp"
+#include "dg2d_device.hpp"
+
+namespace self {
+
+namespace {
+
+constexpr double kRkA[3] = {0.0, -5.0 / 9.0, -153.0 / 128.0};
+constexpr double kRkG[3] = {1.0 / 3.0, 15.0 / 16.0, 8.0 / 15.0};
+
+struct SWKernelArgs {
+    int nE;
+    double g, drag, a, gdt;
+    int hasRelax;
+    // nodal storage
+    const double *dS, *dP;     // (N+1)^2 column-major
+    const float *dS32, *dP32;  // FP32 copies (mixed)
+    const double *qwS, *qwP;   // (N+1)
+    const double *lag;         // [4][N+1] west, east, south, north
+    // state
+    double *q;                 // [3][nE][np]
+    double *G64;               // [3][nE][np] (FP64 form)
+    float *G32;                // [3][nE][np] (mixed form)
+    double *tend;              // [3][nE][np] (plain form)
+    const double *metric;      // [5][nE][np] dxds, dxdp, dyds, dydp, J
+    const double *bathy;       // [3][nE][np] h, hx, hy
+    const float *bathy32;      // [2][nE][np] hx, hy (mixed)
+    const double *plv;         // [nE][np]
+    const float *plv32;        // [nE][np] (mixed)
+    const double *relax;       // [4][nE][np]
+    const double *nrm;         // [nE][4][3][n1] own normals (nx, ny, |n|)
+    const double *fnrm;        // [nE][4][3][n1] edge-primary normals seen from this side
+    const int2 *side;          // [nE][4]
+    const double *trIn;        // [nE][4][3][n1]
+    double *trOut;
+    const double *bflux;       // [nE][4][3][n1] (plain form)
+};
+
+// ---------------------------------------------------------------- helpers
+template <int N>
+__device__ __forceinline__ void load_lag(const double *__restrict__ lag, double lW[N + 1], double lE[N + 1],
+                                         double lS[N + 1], double lN[N + 1])
+{
+    constexpr int n1 = N + 1;
+    #pragma unroll
+    for (int m = 0; m < n1; ++m) {
+        lW[m] = lag[m];
+        lE[m] = lag[n1 + m];
+        lS[m] = lag[2 * n1 + m];
+        lN[m] = lag[3 * n1 + m];
+    }
+}
+
+// traces of the 3 fields of one element held in LDS (qs[f*np + node]) on
+// side s at side node k
+template <int N>
+__device__ __forceinline__ void element_trace(const double *qs, int s, int k, const double *lW,
+                                              const double *lE, const double *lS, const double *lN,
+                                              double out[3])
+{
+    constexpr int n1 = N + 1, np = n1 * n1;
+    #pragma unroll
+    for (int f = 0; f < 3; ++f) {
+        const double *base = qs + f * np;
+        switch (s) {
+        case 0: out[f] = line_dot<N>(lS, base + k, n1); break;       // south: along p at s = k
+        case 1: out[f] = line_dot<N>(lE, base + n1 * k, 1); break;   // east:  along s at p = k
+        case 2: out[f] = line_dot<N>(lN, base + k, n1); break;       // north
+        default: out[f] = line_dot<N>(lW, base + n1 * k, 1); break;  // west
+        }
+    }
+}
+
+// Source, ConservativeShallowWaterClass.f90:1662-1694
+__device__ __forceinline__ void sw_source(const double s[3], double vort, double g, double hx, double hy,
+                                          double drag, double h, bool relax, const double r[4],
+                                          double q[3])
+{
+    q[0] = vort * s[1] + g * s[2] * hx;
+    q[1] = -vort * s[0] + g * s[2] * hy;
+    q[2] = 0.0;
+    if (drag != 0.0) {
+        q[0] = q[0] - drag * s[0] / h;
+        q[1] = q[1] - drag * s[1] / h;
+    }
+    if (relax) {
+        q[0] = q[0] - (s[0] - r[0]) * r[3];
+        q[1] = q[1] - (s[1] - r[1]) * r[3];
+        q[2] = -(s[2] - r[2]) * r[3];
+    }
+}
+
+// ------------------------------------------------------------ PLAIN form
+// CalculateSolutionAtBoundaries for all elements: out[e][side][f][k]
+template <int N>
+__global__ __launch_bounds__(256) void k_sw_traces(const double *__restrict__ q, const double *__restrict__ lag,
+                                                   double *__restrict__ out, int nE, int nf)
+{
+    using S = Shape<N>;
+    constexpr int n1 = S::n1, np = S::np, epb = S::epb;
+    __shared__ double qs[epb * 3 * np];
+    const int t = threadIdx.x, le = t / np, nd = t - le * np;
+    const int e = blockIdx.x * epb + le;
+    const bool act = le < epb && e < nE;
+    double lW[n1], lE[n1], lS[n1], lN[n1];
+    load_lag<N>(lag, lW, lE, lS, lN);
+    if (act)
+        for (int f = 0; f < nf; ++f) qs[(le * 3 + f) * np + nd] = q[((size_t)f * nE + e) * np + nd];
+    __syncthreads();
+    if (!act) return;
+    for (int w = nd; w < 4 * n1; w += np) {
+        const int s = w / n1, k = w - s * n1;
+        double tr[3];
+        element_trace<N>(qs + le * 3 * np, s, k, lW, lE, lS, lN, tr);
+        for (int f = 0; f < nf; ++f) out[(((size_t)e * 4 + s) * nf + f) * n1 + k] = tr[f];
+    }
+}
+
+// EdgeFlux for all edges; one thread per edge node.
+template <int N>
+__global__ __launch_bounds__(256) void k_sw_edge_flux(const int *__restrict__ edges, int nEdges,
+                                                      const double *__restrict__ bsol,
+                                                      const double *__restrict__ nrm, double g,
+                                                      double *__restrict__ bflux)
+{
+    constexpr int n1 = N + 1;
+    const int gid = blockIdx.x * blockDim.x + threadIdx.x;
+    if (gid >= nEdges * n1) return;
+    const int ie = gid / n1, iNode = gid - ie * n1;
+    const int *ed = edges + 6 * ie;
+    const int e1 = ed[0] - 1, s1 = ed[1] - 1, e2 = ed[2], s2 = abs(ed[3]) - 1;
+    const int start = ed[4], inc = ed[5];
+    double in[3], ex[3], fl[3];
+    #pragma unroll
+    for (int f = 0; f < 3; ++f) in[f] = bsol[(((size_t)e1 * 4 + s1) * 3 + f) * n1 + iNode];
+    const size_t nb = ((size_t)e1 * 4 + s1) * 3 * n1 + iNode;
+    const double nx = nrm[nb], ny = nrm[nb + n1], len = nrm[nb + 2 * n1];
+    if (e2 > 0) {
+        const int k = start - inc + iNode * inc;
+        #pragma unroll
+        for (int f = 0; f < 3; ++f) ex[f] = bsol[(((size_t)(e2 - 1) * 4 + s2) * 3 + f) * n1 + k];
+        sw_riemann(in, ex, nx, ny, g, fl);
+        #pragma unroll
+        for (int f = 0; f < 3; ++f) {
+            const double v = fl[f] * len;
+            bflux[(((size_t)e1 * 4 + s1) * 3 + f) * n1 + iNode] = v;
+            bflux[(((size_t)(e2 - 1) * 4 + s2) * 3 + f) * n1 + k] = -v;
+        }
+    } else {
+        sw_external(e2, nx, ny, in, ex);
+        sw_riemann(in, ex, nx, ny, g, fl);
+        #pragma unroll
+        for (int f = 0; f < 3; ++f) bflux[(((size_t)e1 * 4 + s1) * 3 + f) * n1 + iNode] = fl[f] * len;
+    }
+}
+
+// Nodal constants staged in LDS once per workgroup: weak derivative
+// matrices (column-major, row i col m at i + n1*m), reciprocal quadrature
+// weights and the four boundary Lagrange vectors.  Reading them from LDS
+// (mostly broadcast reads) keeps them out of the VGPR budget.
+template <int N>
+struct NodalLDS {
+    static constexpr int n1 = N + 1;
+    double dS[n1 * n1], dP[n1 * n1];
+    double iwS[n1], iwP[n1];
+    double lW[n1], lE[n1], lS[n1], lN[n1];
+    float dS32[n1 * n1], dP32[n1 * n1];
+};
+
+// global nodal buffer: dS, dP, qwS, qwP, lagW, lagE, lagS, lagN (doubles)
+template <int N, bool F32>
+__device__ __forceinline__ void stage_nodal(NodalLDS<N> &L, const double *__restrict__ g,
+                                            const float *__restrict__ dS32, const float *__restrict__ dP32)
+{
+    constexpr int n1 = N + 1, nd = 2 * n1 * n1 + 6 * n1;
+    double *dst = L.dS;  // members up to lN are contiguous doubles
+    for (int k = threadIdx.x; k < nd; k += blockDim.x) {
+        const double v = g[k];
+        const bool w = (k >= 2 * n1 * n1) && (k < 2 * n1 * n1 + 2 * n1);
+        dst[k] = w ? 1.0 / v : v;
+    }
+    if constexpr (F32) {
+        for (int k = threadIdx.x; k < n1 * n1; k += blockDim.x) {
+            L.dS32[k] = dS32[k];
+            L.dP32[k] = dP32[k];
+        }
+    }
+}
+
+template <int N>
+__device__ __forceinline__ void trace3(const NodalLDS<N> &L, const double *qs, int s, int k, double out[3])
+{
+    constexpr int n1 = N + 1, np = n1 * n1;
+    const double *l = (s == 0) ? L.lS : (s == 1) ? L.lE : (s == 2) ? L.lN : L.lW;
+    const int off = (s == 0 || s == 2) ? k : n1 * k;      // south/north: p-line at s = k
+    const int stride = (s == 0 || s == 2) ? n1 : 1;       // east/west:   s-line at p = k
+    #pragma unroll
+    for (int f = 0; f < 3; ++f) {
+        const double *v = qs + f * np + off;
+        double acc = l[0] * v[0];
+        #pragma unroll
+        for (int m = 1; m < n1; ++m) acc = __builtin_fma(l[m], v[m * stride], acc);
+        out[f] = acc;
+    }
+}
+
+// Weak divergence + lift of one node (DGSystemDerivative in s then in p,
+// MappedTimeDerivative :1364-1522): returns -(T1 + T2)/J per field.
+// bf: [side][field][node] face fluxes (south, east, north, west).
+template <int N>
+__device__ __forceinline__ void dg_divergence64(const NodalLDS<N> &L, const double *sC, const double *pC,
+                                                const double *bf, int i, int j, double invJ, double tend[3])
+{
+    constexpr int n1 = N + 1, np = n1 * n1;
+    const double lEi = L.lE[i], lWi = L.lW[i], lNj = L.lN[j], lSj = L.lS[j];
+    const double iwS = L.iwS[i], iwP = L.iwP[j];
+    #pragma unroll 1
+    for (int f = 0; f < 3; ++f) {
+        const double *sl = sC + f * np + n1 * j;  // s-line at p = j
+        const double *pl = pC + f * np + i;       // p-line at s = i
+        double t1 = L.dS[i] * sl[0];
+        double t2 = L.dP[j] * pl[0];
+        #pragma unroll
+        for (int m = 1; m < n1; ++m) {
+            t1 = __builtin_fma(L.dS[i + n1 * m], sl[m], t1);
+            t2 = __builtin_fma(L.dP[j + n1 * m], pl[n1 * m], t2);
+        }
+        const double a = t1 + (bf[(1 * 3 + f) * n1 + j] * lEi + bf[(3 * 3 + f) * n1 + j] * lWi) * iwS;
+        const double b = t2 + (bf[(2 * 3 + f) * n1 + i] * lNj + bf[(0 * 3 + f) * n1 + i] * lSj) * iwP;
+        tend[f] = (-a - b) * invJ;
+    }
+}
+
+// Same with FP32 contractions applied to the deviation of each flux line
+// from its FP64 mean r; the mean's derivative is carried by the FP64 lift
+// through sum_m Dhat(i,m) = -(l_E(i) - l_W(i))/w_i.
+template <int N>
+__device__ __forceinline__ void dg_divergence_mixed(const NodalLDS<N> &L, const double *sC, const double *pC,
+                                                    const double *bf, int i, int j, double invJ, double tend[3])
+{
+    constexpr int n1 = N + 1, np = n1 * n1;
+    constexpr double inv_n1 = 1.0 / n1;
+    const double lEi = L.lE[i], lWi = L.lW[i], lNj = L.lN[j], lSj = L.lS[j];
+    const double iwS = L.iwS[i], iwP = L.iwP[j];
+    #pragma unroll 1
+    for (int f = 0; f < 3; ++f) {
+        const double *sl = sC + f * np + n1 * j;
+        const double *pl = pC + f * np + i;
+        double a = 0.0, b = 0.0;
+        #pragma unroll
+        for (int m = 0; m < n1; ++m) { a += sl[m]; b += pl[n1 * m]; }
+        const double rs = a * inv_n1, rp = b * inv_n1;
+        float t1 = 0.f, t2 = 0.f;
+        #pragma unroll
+        for (int m = 0; m < n1; ++m) {
+            t1 = __builtin_fmaf(L.dS32[i + n1 * m], (float)(sl[m] - rs), t1);
+            t2 = __builtin_fmaf(L.dP32[j + n1 * m], (float)(pl[n1 * m] - rp), t2);
+        }
+        const double ta = (double)t1 + ((bf[(1 * 3 + f) * n1 + j] - rs) * lEi +
+                                        (bf[(3 * 3 + f) * n1 + j] + rs) * lWi) * iwS;
+        const double tb = (double)t2 + ((bf[(2 * 3 + f) * n1 + i] - rp) * lNj +
+                                        (bf[(0 * 3 + f) * n1 + i] + rp) * lSj) * iwP;
+        tend[f] = (-ta - tb) * invJ;
+    }
+}
+
+// contravariant volume fluxes of one node into LDS (MappedTimeDerivative
+// :1383-1398 and :1434-1449)
+__device__ __forceinline__ void contravariant(const double s[3], const double met[4], double g, double *sC,
+                                              double *pC, int np)
+{
+    double xF[3], yF[3];
+    sw_fluxes(s, g, xF, yF);
+    #pragma unroll
+    for (int f = 0; f < 3; ++f) {
+        sC[f * np] = met[3] * xF[f] - met[1] * yF[f];   //  dydp*f - dxdp*g
+        pC[f * np] = -met[2] * xF[f] + met[0] * yF[f];  // -dyds*f + dxds*g
+    }
+}
+
+// MappedTimeDerivative for all elements (PLAIN form, FP64): writes tend.
+template <int N>
+__global__ __launch_bounds__(256, 4) void k_sw_mapped(SWKernelArgs A)
+{
+    using S = Shape<N>;
+    constexpr int n1 = S::n1, np = S::np, epb = S::epb;
+    __shared__ NodalLDS<N> L;
+    __shared__ double sC[epb * 3 * np], pC[epb * 3 * np], bfs[epb * 12 * n1];
+    const int t = threadIdx.x, le = t / np, nd = t - le * np;
+    const int e = xcd_block(blockIdx.x, gridDim.x) * epb + le;
+    const bool act = le < epb && e < A.nE;
+    const int i = nd % n1, j = nd / n1;
+    const size_t mstride = (size_t)A.nE * np, base = (size_t)e * np + nd;
+    stage_nodal<N, false>(L, A.dS, nullptr, nullptr);
+    double s[3], met[4], invJ = 0, hx = 0, hy = 0, vort = 0, h = 1.0, r[4] = {0, 0, 0, 0};
+    if (act) {
+        #pragma unroll
+        for (int f = 0; f < 3; ++f) s[f] = A.q[f * mstride + base];
+        #pragma unroll
+        for (int k = 0; k < 4; ++k) met[k] = A.metric[k * mstride + base];
+        invJ = 1.0 / A.metric[4 * mstride + base];
+        hx = A.bathy[mstride + base];
+        hy = A.bathy[2 * mstride + base];
+        vort = A.plv[base];
+        if (A.drag != 0.0) h = A.bathy[base];
+        if (A.hasRelax) {
+            #pragma unroll
+            for (int k = 0; k < 4; ++k) r[k] = A.relax[k * mstride + base];
+        }
+        for (int w = nd; w < 12 * n1; w += np) bfs[le * 12 * n1 + w] = A.bflux[(size_t)e * 12 * n1 + w];
+        contravariant(s, met, A.g, sC + le * 3 * np + nd, pC + le * 3 * np + nd, np);
+    }
+    __syncthreads();
+    if (!act) return;
+    double tend[3], src[3];
+    dg_divergence64<N>(L, sC + le * 3 * np, pC + le * 3 * np, bfs + le * 12 * n1, i, j, invJ, tend);
+    sw_source(s, vort, A.g, hx, hy, A.drag, h, A.hasRelax, r, src);
+    #pragma unroll
+    for (int f = 0; f < 3; ++f) A.tend[f * mstride + base] = tend[f] + src[f];
+}
+
+// RK3 low-storage update of ForwardStepRK3 (:1203-1211), element-wise.
+__global__ __launch_bounds__(256) void k_sw_rk_update(double *__restrict__ q, double *__restrict__ G,
+                                                      const double *__restrict__ tend, size_t n, double a,
+                                                      double gdt)
+{
+    const size_t gid = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (gid >= n) return;
+    const double gold = (a != 0.0) ? G[gid] : 0.0;  // first stage: G was zeroed, a = 0
+    const double gv = a * gold + tend[gid];
+    G[gid] = gv;
+    q[gid] = q[gid] + gdt * gv;
+}
+
+// ------------------------------------------------------------- FUSED form
+// Every global load of the stage is issued at the top (state, metrics,
+// sources, RK register, the side descriptor and the neighbour trace of the
+// thread's side node), so a workgroup waits for memory about once.
+template <int N, bool MIXED>
+__global__ __launch_bounds__(256, 4) void k_sw_stage_fused(SWKernelArgs A)
+{
+    using S = Shape<N>;
+    constexpr int n1 = S::n1, np = S::np, epb = S::epb;
+    __shared__ NodalLDS<N> L;
+    __shared__ double qs[epb * 3 * np], sC[epb * 3 * np], pC[epb * 3 * np], bfs[epb * 12 * n1];
+    const int t = threadIdx.x, le = t / np, nd = t - le * np;
+    const int e = xcd_block(blockIdx.x, gridDim.x) * epb + le;
+    const bool act = le < epb && e < A.nE;
+    const int i = nd % n1, j = nd / n1;
+    const size_t mstride = (size_t)A.nE * np, base = (size_t)e * np + nd;
+    double *qsE = qs + le * 3 * np, *sCE = sC + le * 3 * np, *pCE = pC + le * 3 * np, *bfE = bfs + le * 12 * n1;
+
+    // ---- loads
+    double s[3], met[4], invJ = 0, hx = 0, hy = 0, vort = 0, h = 1.0, gold[3] = {0, 0, 0}, r[4] = {0, 0, 0, 0};
+    // side node handled by this thread in phase B (w = nd; all of them for N >= 3)
+    const bool side0 = act && nd < 4 * n1;
+    int sd0 = 0, k0 = 0;
+    int2 info0 = make_int2(0, 0);
+    double fn0[3] = {0, 0, 0}, oth0[3] = {0, 0, 0};
+    if (act) {
+        #pragma unroll
+        for (int f = 0; f < 3; ++f) s[f] = A.q[f * mstride + base];
+        if (side0) {
+            sd0 = nd / n1;
+            k0 = nd - sd0 * n1;
+            info0 = A.side[(size_t)e * 4 + sd0];
+            const size_t nb = ((size_t)e * 4 + sd0) * 3 * n1 + k0;
+            fn0[0] = A.fnrm[nb]; fn0[1] = A.fnrm[nb + n1]; fn0[2] = A.fnrm[nb + 2 * n1];
+        }
+        #pragma unroll
+        for (int k = 0; k < 4; ++k) met[k] = A.metric[k * mstride + base];
+        const double Jv = A.metric[4 * mstride + base];
+        if constexpr (MIXED) {
+            hx = A.bathy32[base]; hy = A.bathy32[mstride + base]; vort = A.plv32[base];
+            if (A.a != 0.0) {
+                #pragma unroll
+                for (int f = 0; f < 3; ++f) gold[f] = A.G32[f * mstride + base];
+            }
+        } else {
+            hx = A.bathy[mstride + base]; hy = A.bathy[2 * mstride + base]; vort = A.plv[base];
+            if (A.a != 0.0) {
+                #pragma unroll
+                for (int f = 0; f < 3; ++f) gold[f] = A.G64[f * mstride + base];
+            }
+        }
+        if (A.drag != 0.0) h = A.bathy[base];
+        if (A.hasRelax) {
+            #pragma unroll
+            for (int k = 0; k < 4; ++k) r[k] = A.relax[k * mstride + base];
+        }
+        if (side0 && info0.x >= 0) {
+            const int kk = side_flip(info0.y) ? N - k0 : k0;
+            const double *tr = A.trIn + (((size_t)info0.x * 4 + side_nbr_side(info0.y)) * 3) * n1 + kk;
+            oth0[0] = tr[0]; oth0[1] = tr[n1]; oth0[2] = tr[2 * n1];
+        }
+        invJ = 1.0 / Jv;
+    }
+    stage_nodal<N, MIXED>(L, A.dS, A.dS32, A.dP32);
+
+    // ---- phase A: contravariant fluxes, state to LDS
+    if (act) {
+        #pragma unroll
+        for (int f = 0; f < 3; ++f) qsE[f * np + nd] = s[f];
+        contravariant(s, met, A.g, sCE + nd, pCE + nd, np);
+    }
+    __syncthreads();
+
+    // ---- phase B: Rusanov flux of each side node, in the edge's primary orientation
+    if (act) {
+        for (int w = nd; w < 4 * n1; w += np) {
+            int sd, k;
+            int2 info;
+            double fn[3], oth[3], own[3], fl[3];
+            if (w == nd) {
+                sd = sd0; k = k0; info = info0;
+                fn[0] = fn0[0]; fn[1] = fn0[1]; fn[2] = fn0[2];
+                oth[0] = oth0[0]; oth[1] = oth0[1]; oth[2] = oth0[2];
+            } else {  // N <= 2: more side nodes than element nodes
+                sd = w / n1; k = w - sd * n1;
+                info = A.side[(size_t)e * 4 + sd];
+                const size_t nb = ((size_t)e * 4 + sd) * 3 * n1 + k;
+                fn[0] = A.fnrm[nb]; fn[1] = A.fnrm[nb + n1]; fn[2] = A.fnrm[nb + 2 * n1];
+                if (info.x >= 0) {
+                    const int kk = side_flip(info.y) ? N - k : k;
+                    const double *tr = A.trIn + (((size_t)info.x * 4 + side_nbr_side(info.y)) * 3) * n1 + kk;
+                    oth[0] = tr[0]; oth[1] = tr[n1]; oth[2] = tr[2 * n1];
+                }
+            }
+            trace3<N>(L, qsE, sd, k, own);
+            double sign = 1.0;
+            if (info.x < 0) {
+                sw_external(info.x, fn[0], fn[1], own, oth);
+                sw_riemann(own, oth, fn[0], fn[1], A.g, fl);
+            } else if (side_primary(info.y)) {
+                sw_riemann(own, oth, fn[0], fn[1], A.g, fl);
+            } else {
+                sw_riemann(oth, own, fn[0], fn[1], A.g, fl);
+                sign = -1.0;
+            }
+            #pragma unroll
+            for (int f = 0; f < 3; ++f) bfE[(sd * 3 + f) * n1 + k] = sign * (fl[f] * fn[2]);
+        }
+    }
+    __syncthreads();
+
+    // ---- phase C: weak divergence + lift + source + RK3 update
+    if (act) {
+        double tend[3], src[3];
+        if constexpr (MIXED) dg_divergence_mixed<N>(L, sCE, pCE, bfE, i, j, invJ, tend);
+        else dg_divergence64<N>(L, sCE, pCE, bfE, i, j, invJ, tend);
+        sw_source(s, vort, A.g, hx, hy, A.drag, h, A.hasRelax, r, src);
+        #pragma unroll
+        for (int f = 0; f < 3; ++f) {
+            // G = a*G + L; the first stage has a = 0 and ForwardStepRK3 zeroes
+            // G before it, so that stage never reads G.
+            double gv = A.a * gold[f] + (tend[f] + src[f]);
+            if constexpr (MIXED) {
+                gv = (double)(float)gv;
+                A.G32[f * mstride + base] = (float)gv;
+            } else {
+                A.G64[f * mstride + base] = gv;
+            }
+            const double qn = s[f] + A.gdt * gv;
+            A.q[f * mstride + base] = qn;
+            qsE[f * np + nd] = qn;  // phase B finished reading qs before the barrier
+        }
+    }
+    __syncthreads();
+
+    // ---- phase D: traces of the updated state for the next stage
+    if (act) {
+        for (int w = nd; w < 4 * n1; w += np) {
+            const int sd = w / n1, k = w - sd * n1;
+            double tr[3];
+            trace3<N>(L, qsE, sd, k, tr);
+            double *o = A.trOut + (((size_t)e * 4 + sd) * 3) * n1 + k;
+            o[0] = tr[0]; o[n1] = tr[1]; o[2 * n1] = tr[2];
+        }
+    }
+}
+
+// CalculateBathymetryAtBoundaries + CalculateBathymetryGradient (setup)
+template <int N>
+__global__ __launch_bounds__(256) void k_sw_bathy_grad(SWKernelArgs A, double *bathy, float *bathy32)
+{
+    using S = Shape<N>;
+    constexpr int n1 = S::n1, np = S::np, epb = S::epb;
+    __shared__ double hs[epb * np], sC[epb * 2 * np], pC[epb * 2 * np], bfs[epb * 8 * n1];
+    const int t = threadIdx.x, le = t / np, nd = t - le * np;
+    const int e = blockIdx.x * epb + le;
+    const bool act = le < epb && e < A.nE;
+    const int i = nd % n1, j = nd / n1;
+    const size_t mstride = (size_t)A.nE * np, base = (size_t)e * np + nd;
+    double lW[n1], lE[n1], lS[n1], lN[n1];
+    load_lag<N>(A.lag, lW, lE, lS, lN);
+    if (act) {
+        const double h = bathy[base];
+        hs[le * np + nd] = h;
+        const double dxds = A.metric[base], dxdp = A.metric[mstride + base];
+        const double dyds = A.metric[2 * mstride + base], dydp = A.metric[3 * mstride + base];
+        // xF = (h, 0), yF = (0, h)
+        sC[(le * 2 + 0) * np + nd] = dydp * h - dxdp * 0.0;
+        sC[(le * 2 + 1) * np + nd] = dydp * 0.0 - dxdp * h;
+        pC[(le * 2 + 0) * np + nd] = -dyds * h + dxds * 0.0;
+        pC[(le * 2 + 1) * np + nd] = -dyds * 0.0 + dxds * h;
+    }
+    __syncthreads();
+    if (act) {
+        for (int w = nd; w < 4 * n1; w += np) {
+            const int sd = w / n1, k = w - sd * n1;
+            const double *hb = hs + le * np;
+            double tr;
+            switch (sd) {
+            case 0: tr = line_dot<N>(lS, hb + k, n1); break;
+            case 1: tr = line_dot<N>(lE, hb + n1 * k, 1); break;
+            case 2: tr = line_dot<N>(lN, hb + k, n1); break;
+            default: tr = line_dot<N>(lW, hb + n1 * k, 1); break;
+            }
+            const size_t nb = ((size_t)e * 4 + sd) * 3 * n1 + k;
+            const double nx = A.nrm[nb], ny = A.nrm[nb + n1], len = A.nrm[nb + 2 * n1];
+            bfs[le * 8 * n1 + (sd * 2 + 0) * n1 + k] = (tr * nx) * len;
+            bfs[le * 8 * n1 + (sd * 2 + 1) * n1 + k] = (tr * ny) * len;
+        }
+    }
+    __syncthreads();
+    if (!act) return;
+    const double *bf = bfs + le * 8 * n1;
+    const double Jv = A.metric[4 * mstride + base];
+    for (int qd = 0; qd < 2; ++qd) {
+        const double *sl = sC + (le * 2 + qd) * np + n1 * j;
+        double t1 = A.dS[i] * sl[0];
+        for (int m = 1; m < n1; ++m) t1 = __builtin_fma(A.dS[i + n1 * m], sl[m], t1);
+        t1 = t1 + (bf[(1 * 2 + qd) * n1 + j] * lE[i] + bf[(3 * 2 + qd) * n1 + j] * lW[i]) / A.qwS[i];
+        const double *pl = pC + (le * 2 + qd) * np + i;
+        double t2 = A.dP[j] * pl[0];
+        for (int m = 1; m < n1; ++m) t2 = __builtin_fma(A.dP[j + n1 * m], pl[m * n1], t2);
+        t2 = t2 + (bf[(2 * 2 + qd) * n1 + i] * lN[j] + bf[(0 * 2 + qd) * n1 + i] * lS[j]) / A.qwP[j];
+        const double gval = (t1 + t2) / Jv;
+        bathy[(1 + qd) * mstride + base] = gval;
+        if (bathy32) bathy32[qd * mstride + base] = (float)gval;
+    }
+}
+
+}  // namespace
+}  // namespace self
+
+// ==================================================================== host
+using namespace self;
+
+struct self_sw2d_s {
+    self_sw2d_desc d{};
+    int device = 0;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    int n1 = 0, np = 0;
+    size_t nE = 0;
+    // device buffers
+    double *q = nullptr, *G64 = nullptr, *tend = nullptr, *metric = nullptr, *bathy = nullptr;
+    float *G32 = nullptr, *bathy32 = nullptr, *plv32 = nullptr, *dS32 = nullptr, *dP32 = nullptr;
+    double *plv = nullptr, *relax = nullptr, *nrm = nullptr, *fnrm = nullptr;
+    double *bsol = nullptr, *bflux = nullptr, *tr[2] = {nullptr, nullptr};
+    double *nodal = nullptr;  // dS, dP, qwS, qwP, lag[4]
+    int2 *side = nullptr;
+    int *edges = nullptr;
+    int trCur = 0;
+    bool trValid = false, nodalSet = false, geomSet = false, hasRelax = false;
+    size_t bytes = 0;
+};
+
+namespace {
+
+template <typename T>
+int dalloc(self_sw2d_s *h, T **p, size_t n)
+{
+    if (n == 0) { *p = nullptr; return SELF_OK; }
+    SELF_HIP(hipMalloc(reinterpret_cast<void **>(p), n * sizeof(T)));
+    h->bytes += n * sizeof(T);
+    return SELF_OK;
+}
+
+void free_all(self_sw2d_s *h)
+{
+    void *ptrs[] = {h->q, h->G64, h->tend, h->metric, h->bathy, h->G32, h->bathy32, h->plv32,
+                    h->dS32, h->dP32, h->plv, h->relax, h->nrm, h->fnrm, h->bsol, h->bflux,
+                    h->tr[0], h->tr[1], h->nodal, h->side, h->edges};
+    for (void *p : ptrs)
+        if (p) (void)hipFree(p);
+}
+
+SWKernelArgs make_args(self_sw2d_s *h)
+{
+    SWKernelArgs A{};
+    const int n1 = h->n1;
+    A.nE = (int)h->nE;
+    A.g = h->d.g;
+    A.drag = h->d.linearDrag;
+    A.hasRelax = h->hasRelax ? 1 : 0;
+    A.dS = h->nodal;
+    A.dP = h->nodal + n1 * n1;
+    A.qwS = h->nodal + 2 * n1 * n1;
+    A.qwP = A.qwS + n1;
+    A.lag = A.qwP + n1;
+    A.dS32 = h->dS32;
+    A.dP32 = h->dP32;
+    A.q = h->q;
+    A.G64 = h->G64;
+    A.G32 = h->G32;
+    A.tend = h->tend;
+    A.metric = h->metric;
+    A.bathy = h->bathy;
+    A.bathy32 = h->bathy32;
+    A.plv = h->plv;
+    A.plv32 = h->plv32;
+    A.relax = h->relax;
+    A.nrm = h->nrm;
+    A.fnrm = h->fnrm;
+    A.side = h->side;
+    A.bflux = h->bflux;
+    return A;
+}
+
+#define SELF_DISPATCH_N(N_, CALL)                                                   \
+    switch (N_) {                                                                   \
+    case 1: { constexpr int NN = 1; CALL; } break;                                  \
+    case 2: { constexpr int NN = 2; CALL; } break;                                  \
+    case 3: { constexpr int NN = 3; CALL; } break;                                  \
+    case 4: { constexpr int NN = 4; CALL; } break;                                  \
+    case 5: { constexpr int NN = 5; CALL; } break;                                  \
+    case 6: { constexpr int NN = 6; CALL; } break;                                  \
+    case 7: { constexpr int NN = 7; CALL; } break;                                  \
+    case 8: { constexpr int NN = 8; CALL; } break;                                  \
+    case 9: { constexpr int NN = 9; CALL; } break;                                  \
+    case 10: { constexpr int NN = 10; CALL; } break;                                \
+    case 11: { constexpr int NN = 11; CALL; } break;                                \
+    default: break;                                                                 \
+    }
+
+template <int N>
+int grid_for(size_t nE)
+{
+    return (int)((nE + Shape<N>::epb - 1) / Shape<N>::epb);
+}
+
+int launch_traces(self_sw2d_s *h, const double *src, double *dst, int nf)
+{
+    const double *lag = h->nodal + 2 * h->n1 * h->n1 + 2 * h->n1;
+    SELF_DISPATCH_N(h->d.N, hipLaunchKernelGGL(k_sw_traces<NN>, dim3(grid_for<NN>(h->nE)), dim3(256), 0,
+                                               h->stream, src, lag, dst, (int)h->nE, nf));
+    SELF_HIP(hipGetLastError());
+    return SELF_OK;
+}
+
+int launch_edge_flux(self_sw2d_s *h)
+{
+    const int total = h->d.nEdges * h->n1;
+    const int grid = (total + 255) / 256;
+    SELF_DISPATCH_N(h->d.N, hipLaunchKernelGGL(k_sw_edge_flux<NN>, dim3(grid), dim3(256), 0, h->stream,
+                                               h->edges, h->d.nEdges, h->bsol, h->nrm, h->d.g, h->bflux));
+    SELF_HIP(hipGetLastError());
+    return SELF_OK;
+}
+
+int launch_mapped(self_sw2d_s *h)
+{
+    SWKernelArgs A = make_args(h);
+    SELF_DISPATCH_N(h->d.N, hipLaunchKernelGGL(k_sw_mapped<NN>, dim3(grid_for<NN>(h->nE)), dim3(256), 0,
+                                               h->stream, A));
+    SELF_HIP(hipGetLastError());
+    return SELF_OK;
+}
+
+int gtd_plain(self_sw2d_s *h)
+{
+    int rc = launch_traces(h, h->q, h->bsol, 3);
+    if (!rc) rc = launch_edge_flux(h);
+    if (!rc) rc = launch_mapped(h);
+    return rc;
+}
+
+int step_plain(self_sw2d_s *h)
+{
+    const size_t n = 3 * h->nE * h->np;
+    for (int m = 0; m < 3; ++m) {
+        int rc = gtd_plain(h);
+        if (rc) return rc;
+        hipLaunchKernelGGL(k_sw_rk_update, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, h->stream, h->q,
+                           h->G64, h->tend, n, kRkA[m], kRkG[m] * h->d.dt);
+        SELF_HIP(hipGetLastError());
+    }
+    return SELF_OK;
+}
+
+int step_fused(self_sw2d_s *h)
+{
+    if (!h->trValid) {
+        int rc = launch_traces(h, h->q, h->tr[h->trCur], 3);
+        if (rc) return rc;
+        h->trValid = true;
+    }
+    SWKernelArgs A = make_args(h);
+    for (int m = 0; m < 3; ++m) {
+        A.a = kRkA[m];
+        A.gdt = kRkG[m] * h->d.dt;
+        A.trIn = h->tr[h->trCur];
+        A.trOut = h->tr[1 - h->trCur];
+        if (h->d.precision == SELF_MIXED) {
+            SELF_DISPATCH_N(h->d.N, hipLaunchKernelGGL((k_sw_stage_fused<NN, true>), dim3(grid_for<NN>(h->nE)),
+                                                       dim3(256), 0, h->stream, A));
+        } else {
+            SELF_DISPATCH_N(h->d.N, hipLaunchKernelGGL((k_sw_stage_fused<NN, false>), dim3(grid_for<NN>(h->nE)),
+                                                       dim3(256), 0, h->stream, A));
+        }
+        SELF_HIP(hipGetLastError());
+        h->trCur = 1 - h->trCur;
+    }
+    return SELF_OK;
+}
+
+int check_ready(self_sw2d_s *h)
+{
+    SELF_REQUIRE(h, SELF_ERR_ARG, "null handle");
+    SELF_REQUIRE(h->nodalSet && h->geomSet, SELF_ERR_STATE,
+                 "shallow-water handle needs self_sw2d_set_nodal and self_sw2d_set_geometry first");
+    SELF_HIP(hipSetDevice(h->device));
+    return SELF_OK;
+}
+
+// host layout [e][f][node] <-> device [f][e][node]
+void to_device_layout(const double *src, double *dst, size_t nE, int nf, int np)
+{
+    for (size_t e = 0; e < nE; ++e)
+        for (int f = 0; f < nf; ++f)
+            std::memcpy(dst + ((size_t)f * nE + e) * np, src + (e * nf + f) * np, sizeof(double) * np);
+}
+
+void from_device_layout(const double *src, double *dst, size_t nE, int nf, int np)
+{
+    for (size_t e = 0; e < nE; ++e)
+        for (int f = 0; f < nf; ++f)
+            std::memcpy(dst + (e * nf + f) * np, src + ((size_t)f * nE + e) * np, sizeof(double) * np);
+}
+
+}  // namespace
+
+extern "C" {
+
+int self_sw2d_create(const self_sw2d_desc *desc, self_sw2d_t *out)
+try {
+    SELF_REQUIRE(desc && out, SELF_ERR_ARG, "self_sw2d_create: null argument");
+    SELF_REQUIRE(desc->N >= 1 && desc->N <= 11, SELF_ERR_ARG, "polyDeg %d outside 1..11", desc->N);
+    SELF_REQUIRE(desc->nElems >= 1 && desc->nEdges >= 1, SELF_ERR_ARG, "empty mesh");
+    SELF_REQUIRE(desc->precision == SELF_FP64 || desc->precision == SELF_MIXED, SELF_ERR_ARG,
+                 "unknown precision %d", desc->precision);
+    SELF_REQUIRE(desc->form == SELF_PLAIN || desc->form == SELF_FUSED, SELF_ERR_ARG, "unknown form %d",
+                 desc->form);
+    SELF_REQUIRE(!(desc->precision == SELF_MIXED && desc->form == SELF_PLAIN), SELF_ERR_ARG,
+                 "mixed precision runs in the fused form only");
+    auto *h = new self_sw2d_s();
+    h->d = *desc;
+    h->n1 = desc->N + 1;
+    h->np = h->n1 * h->n1;
+    h->nE = (size_t)desc->nElems;
+    int rc = SELF_OK;
+    auto fail = [&](int code) { free_all(h); if (h->stream) (void)hipStreamDestroy(h->stream); delete h; return code; };
+    hipError_t e = hipGetDevice(&h->device);
+    if (e != hipSuccess) return fail(hip_fail(e, "hipGetDevice", __FILE__, __LINE__));
+    e = hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking);
+    if (e != hipSuccess) return fail(hip_fail(e, "hipStreamCreate", __FILE__, __LINE__));
+    const size_t nE = h->nE, np = h->np, n1 = h->n1;
+    const size_t nfield = nE * np;
+    if ((rc = dalloc(h, &h->q, 3 * nfield))) return fail(rc);
+    if ((rc = dalloc(h, &h->tend, 3 * nfield))) return fail(rc);
+    if ((rc = dalloc(h, &h->G64, 3 * nfield))) return fail(rc);
+    if ((rc = dalloc(h, &h->metric, 5 * nfield))) return fail(rc);
+    if ((rc = dalloc(h, &h->bathy, 3 * nfield))) return fail(rc);
+    if ((rc = dalloc(h, &h->plv, nfield))) return fail(rc);
+    if ((rc = dalloc(h, &h->nrm, nE * 12 * n1))) return fail(rc);
+    if ((rc = dalloc(h, &h->fnrm, nE * 12 * n1))) return fail(rc);
+    if ((rc = dalloc(h, &h->side, nE * 4))) return fail(rc);
+    if ((rc = dalloc(h, &h->edges, (size_t)desc->nEdges * 6))) return fail(rc);
+    if ((rc = dalloc(h, &h->bsol, nE * 12 * n1))) return fail(rc);
+    if ((rc = dalloc(h, &h->bflux, nE * 12 * n1))) return fail(rc);
+    if ((rc = dalloc(h, &h->tr[0], nE * 12 * n1))) return fail(rc);
+    if ((rc = dalloc(h, &h->tr[1], nE * 12 * n1))) return fail(rc);
+    if ((rc = dalloc(h, &h->nodal, 2 * n1 * n1 + 2 * n1 + 4 * n1))) return fail(rc);
+    if (desc->precision == SELF_MIXED) {
+        if ((rc = dalloc(h, &h->G32, 3 * nfield))) return fail(rc);
+        if ((rc = dalloc(h, &h->bathy32, 2 * nfield))) return fail(rc);
+        if ((rc = dalloc(h, &h->plv32, nfield))) return fail(rc);
+        if ((rc = dalloc(h, &h->dS32, n1 * n1))) return fail(rc);
+        if ((rc = dalloc(h, &h->dP32, n1 * n1))) return fail(rc);
+    }
+    e = hipMemsetAsync(h->q, 0, 3 * nfield * sizeof(double), h->stream);
+    if (e == hipSuccess) e = hipMemsetAsync(h->bathy, 0, 3 * nfield * sizeof(double), h->stream);
+    if (e == hipSuccess) e = hipMemsetAsync(h->plv, 0, nfield * sizeof(double), h->stream);
+    if (e == hipSuccess) e = hipMemsetAsync(h->tend, 0, 3 * nfield * sizeof(double), h->stream);
+    if (e == hipSuccess && h->bathy32) e = hipMemsetAsync(h->bathy32, 0, 2 * nfield * sizeof(float), h->stream);
+    if (e == hipSuccess && h->plv32) e = hipMemsetAsync(h->plv32, 0, nfield * sizeof(float), h->stream);
+    if (e == hipSuccess) e = hipEventCreate(&h->ev0);
+    if (e == hipSuccess) e = hipEventCreate(&h->ev1);
+    if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+    if (e != hipSuccess) return fail(hip_fail(e, "create", __FILE__, __LINE__));
+    *out = h;
+    return SELF_OK;
+}
+SELF_CATCH_ALL
+
+int self_sw2d_destroy(self_sw2d_t h)
+{
+    if (!h) return SELF_OK;
+    (void)hipSetDevice(h->device);
+    if (h->stream) (void)hipStreamSynchronize(h->stream);
+    free_all(h);
+    if (h->ev0) (void)hipEventDestroy(h->ev0);
+    if (h->ev1) (void)hipEventDestroy(h->ev1);
+    if (h->stream) (void)hipStreamDestroy(h->stream);
+    delete h;
+    return SELF_OK;
+}
+
+int self_sw2d_set_nodal(self_sw2d_t h, const double *dMatS, const double *dMatP, const double *qwS,
+                        const double *qwP, const double *lW, const double *lE, const double *lS,
+                        const double *lN)
+try {
+    SELF_REQUIRE(h && dMatS && dMatP && qwS && qwP && lW && lE && lS && lN, SELF_ERR_ARG,
+                 "self_sw2d_set_nodal: null argument");
+    SELF_HIP(hipSetDevice(h->device));
+    const int n1 = h->n1;
+    std::vector<double> buf(2 * n1 * n1 + 6 * n1);
+    std::memcpy(buf.data(), dMatS, sizeof(double) * n1 * n1);
+    std::memcpy(buf.data() + n1 * n1, dMatP, sizeof(double) * n1 * n1);
+    double *p = buf.data() + 2 * n1 * n1;
+    std::memcpy(p, qwS, sizeof(double) * n1);
+    std::memcpy(p + n1, qwP, sizeof(double) * n1);
+    std::memcpy(p + 2 * n1, lW, sizeof(double) * n1);
+    std::memcpy(p + 3 * n1, lE, sizeof(double) * n1);
+    std::memcpy(p + 4 * n1, lS, sizeof(double) * n1);
+    std::memcpy(p + 5 * n1, lN, sizeof(double) * n1);
+    SELF_HIP(hipMemcpy(h->nodal, buf.data(), sizeof(double) * buf.size(), hipMemcpyHostToDevice));
+    if (h->dS32) {
+        std::vector<float> f(2 * n1 * n1);
+        for (int k = 0; k < 2 * n1 * n1; ++k) f[k] = (float)buf[k];
+        SELF_HIP(hipMemcpy(h->dS32, f.data(), sizeof(float) * n1 * n1, hipMemcpyHostToDevice));
+        SELF_HIP(hipMemcpy(h->dP32, f.data() + n1 * n1, sizeof(float) * n1 * n1, hipMemcpyHostToDevice));
+    }
+    h->nodalSet = true;
+    h->trValid = false;
+    return SELF_OK;
+}
+SELF_CATCH_ALL
+
+int self_sw2d_set_geometry(self_sw2d_t h, const int *edges, const double *dxds, const double *dxdp,
+                           const double *dyds, const double *dydp, const double *J, const double *nHat,
+                           const double *nLen)
+try {
+    SELF_REQUIRE(h && edges && dxds && dxdp && dyds && dydp && J && nHat && nLen, SELF_ERR_ARG,
+                 "self_sw2d_set_geometry: null argument");
+    SELF_HIP(hipSetDevice(h->device));
+    const size_t nE = h->nE, np = h->np;
+    const int n1 = h->n1, N = h->d.N, nEd = h->d.nEdges;
+    // metrics [5][e][np]
+    std::vector<double> met(5 * nE * np);
+    const double *src[5] = {dxds, dxdp, dyds, dydp, J};
+    for (int k = 0; k < 5; ++k) std::memcpy(met.data() + k * nE * np, src[k], sizeof(double) * nE * np);
+    for (size_t id = 0; id < nE * np; ++id)
+        SELF_REQUIRE(J[id] != 0.0, SELF_ERR_ARG, "zero Jacobian at element %zu", id / np + 1);
+    // own normals [e][side][c][k]
+    std::vector<double> nrm(nE * 12 * n1), fnrm(nE * 12 * n1);
+    for (size_t e = 0; e < nE; ++e)
+        for (int s = 0; s < 4; ++s)
+            for (int k = 0; k < n1; ++k) {
+                const size_t hb = k + (size_t)n1 * (s + 4 * e);
+                const size_t db = ((e * 4 + s) * 3) * n1 + k;
+                nrm[db] = nHat[2 * hb];
+                nrm[db + n1] = nHat[2 * hb + 1];
+                nrm[db + 2 * n1] = nLen[hb];
+            }
+    fnrm = nrm;
+    // side descriptors from the edge table
+    std::vector<int2> side(nE * 4);
+    std::vector<int> seen(nE * 4, 0);
+    for (int ie = 0; ie < nEd; ++ie) {
+        const int *ed = edges + 6 * ie;
+        const int e1 = ed[0] - 1, s1 = ed[1] - 1, e2 = ed[2], s2 = std::abs(ed[3]) - 1, inc = ed[5];
+        SELF_REQUIRE(e1 >= 0 && (size_t)e1 < nE && s1 >= 0 && s1 < 4, SELF_ERR_ARG, "edge %d: bad primary", ie + 1);
+        SELF_REQUIRE(inc == 1 || inc == -1, SELF_ERR_ARG, "edge %d: increment %d", ie + 1, inc);
+        if (e2 > 0) {
+            SELF_REQUIRE((size_t)(e2 - 1) < nE && s2 >= 0 && s2 < 4, SELF_ERR_ARG, "edge %d: bad secondary", ie + 1);
+            SELF_REQUIRE(ed[4] - inc == (inc > 0 ? 0 : N), SELF_ERR_ARG,
+                         "edge %d: start/inc must traverse the whole side", ie + 1);
+            const int flip = inc < 0 ? 1 : 0;
+            side[(size_t)e1 * 4 + s1] = make_int2(e2 - 1, s2 | (flip << 2) | (1 << 3));
+            side[(size_t)(e2 - 1) * 4 + s2] = make_int2(e1, s1 | (flip << 2));
+            seen[(size_t)e1 * 4 + s1]++;
+            seen[(size_t)(e2 - 1) * 4 + s2]++;
+            // the secondary side uses the primary's normal at the matching node
+            for (int k = 0; k < n1; ++k) {
+                const int kp = flip ? N - k : k;  // primary node seen from secondary node k
+                const size_t dsec = (((size_t)(e2 - 1) * 4 + s2) * 3) * n1 + k;
+                const size_t dpri = (((size_t)e1 * 4 + s1) * 3) * n1 + kp;
+                fnrm[dsec] = nrm[dpri];
+                fnrm[dsec + n1] = nrm[dpri + n1];
+                fnrm[dsec + 2 * n1] = nrm[dpri + 2 * n1];
+            }
+        } else {
+            side[(size_t)e1 * 4 + s1] = make_int2(e2, 1 << 3);
+            seen[(size_t)e1 * 4 + s1]++;
+        }
+    }
+    for (size_t k = 0; k < nE * 4; ++k)
+        SELF_REQUIRE(seen[k] == 1, SELF_ERR_ARG, "element %zu side %zu is on %d edges", k / 4 + 1, k % 4 + 1, seen[k]);
+    SELF_HIP(hipMemcpy(h->metric, met.data(), sizeof(double) * met.size(), hipMemcpyHostToDevice));
+    SELF_HIP(hipMemcpy(h->nrm, nrm.data(), sizeof(double) * nrm.size(), hipMemcpyHostToDevice));
+    SELF_HIP(hipMemcpy(h->fnrm, fnrm.data(), sizeof(double) * fnrm.size(), hipMemcpyHostToDevice));
+    SELF_HIP(hipMemcpy(h->side, side.data(), sizeof(int2) * side.size(), hipMemcpyHostToDevice));
+    SELF_HIP(hipMemcpy(h->edges, edges, sizeof(int) * 6 * nEd, hipMemcpyHostToDevice));
+    h->geomSet = true;
+    h->trValid = false;
+    return SELF_OK;
+}
+SELF_CATCH_ALL
+
+int self_sw2d_set_field(self_sw2d_t h, int field, const double *data)
+try {
+    SELF_REQUIRE(h && data, SELF_ERR_ARG, "self_sw2d_set_field: null argument");
+    SELF_HIP(hipSetDevice(h->device));
+    SELF_HIP(hipStreamSynchronize(h->stream));
+    const size_t nE = h->nE, np = h->np;
+    std::vector<double> buf;
+    switch (field) {
+    case SELF_SW_SOLUTION:
+        buf.resize(3 * nE * np);
+        to_device_layout(data, buf.data(), nE, 3, (int)np);
+        SELF_HIP(hipMemcpy(h->q, buf.data(), sizeof(double) * buf.size(), hipMemcpyHostToDevice));
+        h->trValid = false;
+        break;
+    case SELF_SW_BATHYMETRY: {
+        buf.resize(3 * nE * np);
+        to_device_layout(data, buf.data(), nE, 3, (int)np);
+        SELF_HIP(hipMemcpy(h->bathy, buf.data(), sizeof(double) * buf.size(), hipMemcpyHostToDevice));
+        if (h->bathy32) {
+            std::vector<float> f(2 * nE * np);
+            for (size_t k = 0; k < 2 * nE * np; ++k) f[k] = (float)buf[nE * np + k];
+            SELF_HIP(hipMemcpy(h->bathy32, f.data(), sizeof(float) * f.size(), hipMemcpyHostToDevice));
+        }
+        break;
+    }
+    case SELF_SW_PLANETARY_VORTICITY: {
+        SELF_HIP(hipMemcpy(h->plv, data, sizeof(double) * nE * np, hipMemcpyHostToDevice));
+        if (h->plv32) {
+            std::vector<float> f(nE * np);
+            for (size_t k = 0; k < nE * np; ++k) f[k] = (float)data[k];
+            SELF_HIP(hipMemcpy(h->plv32, f.data(), sizeof(float) * f.size(), hipMemcpyHostToDevice));
+        }
+        break;
+    }
+    case SELF_SW_RELAXATION: {
+        buf.resize(4 * nE * np);
+        to_device_layout(data, buf.data(), nE, 4, (int)np);
+        bool any = false;
+        for (size_t k = 0; k < nE * np; ++k) any |= buf[3 * nE * np + k] != 0.0;
+        if (!any) {  // every timescale zero: the relaxation term vanishes identically
+            h->hasRelax = false;
+            break;
+        }
+        if (!h->relax) {
+            int rc = dalloc(h, &h->relax, 4 * nE * np);
+            if (rc) return rc;
+        }
+        SELF_HIP(hipMemcpy(h->relax, buf.data(), sizeof(double) * buf.size(), hipMemcpyHostToDevice));
+        h->hasRelax = true;
+        break;
+    }
+    default:
+        SELF_REQUIRE(false, SELF_ERR_ARG, "field %d cannot be set", field);
+    }
+    return SELF_OK;
+}
+SELF_CATCH_ALL
+
+int self_sw2d_get_field(self_sw2d_t h, int field, double *data)
+try {
+    SELF_REQUIRE(h && data, SELF_ERR_ARG, "self_sw2d_get_field: null argument");
+    SELF_HIP(hipSetDevice(h->device));
+    SELF_HIP(hipStreamSynchronize(h->stream));
+    const size_t nE = h->nE, np = h->np, n1 = h->n1;
+    std::vector<double> buf;
+    auto get_nodal = [&](const double *dev, int nf) -> int {
+        buf.resize((size_t)nf * nE * np);
+        SELF_HIP(hipMemcpy(buf.data(), dev, sizeof(double) * buf.size(), hipMemcpyDeviceToHost));
+        from_device_layout(buf.data(), data, nE, nf, (int)np);
+        return SELF_OK;
+    };
+    switch (field) {
+    case SELF_SW_SOLUTION: return get_nodal(h->q, 3);
+    case SELF_SW_TENDENCY: return get_nodal(h->tend, 3);
+    case SELF_SW_BATHYMETRY: return get_nodal(h->bathy, 3);
+    case SELF_SW_PLANETARY_VORTICITY:
+        SELF_HIP(hipMemcpy(data, h->plv, sizeof(double) * nE * np, hipMemcpyDeviceToHost));
+        return SELF_OK;
+    case SELF_SW_RELAXATION:
+        if (!h->relax) { std::memset(data, 0, sizeof(double) * 4 * nE * np); return SELF_OK; }
+        return get_nodal(h->relax, 4);
+    case SELF_SW_BOUNDARY_SOLUTION:
+    case SELF_SW_BOUNDARY_FLUX: {
+        // device [e][side][f][k] -> host per element (0:N,1:3,1:4) = [e][side][f][k]: identical
+        const double *dev = field == SELF_SW_BOUNDARY_SOLUTION ? h->bsol : h->bflux;
+        SELF_HIP(hipMemcpy(data, dev, sizeof(double) * nE * 12 * n1, hipMemcpyDeviceToHost));
+        return SELF_OK;
+    }
+    default:
+        SELF_REQUIRE(false, SELF_ERR_ARG, "field %d cannot be read", field);
+    }
+    return SELF_OK;
+}
+SELF_CATCH_ALL
+
+int self_sw2d_set_params(self_sw2d_t h, double g, double dt, double linearDrag)
+{
+    SELF_REQUIRE(h, SELF_ERR_ARG, "null handle");
+    h->d.g = g;
+    h->d.dt = dt;
+    h->d.linearDrag = linearDrag;
+    return SELF_OK;
+}
+
+int self_sw2d_calculate_solution_at_boundaries(self_sw2d_t h)
+{
+    int rc = check_ready(h);
+    if (rc) return rc;
+    return launch_traces(h, h->q, h->bsol, 3);
+}
+
+int self_sw2d_edge_flux(self_sw2d_t h, double t)
+{
+    (void)t;  // no time-dependent boundary state in this model
+    int rc = check_ready(h);
+    if (rc) return rc;
+    return launch_edge_flux(h);
+}
+
+int self_sw2d_mapped_time_derivative(self_sw2d_t h, double t)
+{
+    (void)t;
+    int rc = check_ready(h);
+    if (rc) return rc;
+    return launch_mapped(h);
+}
+
+int self_sw2d_global_time_derivative(self_sw2d_t h, double t)
+{
+    (void)t;
+    int rc = check_ready(h);
+    if (rc) return rc;
+    return gtd_plain(h);
+}
+
+int self_sw2d_forward_step_rk3(self_sw2d_t h, double tn, int nSteps)
+{
+    (void)tn;
+    int rc = check_ready(h);
+    if (rc) return rc;
+    SELF_REQUIRE(nSteps >= 0, SELF_ERR_ARG, "negative step count");
+    for (int s = 0; s < nSteps && !rc; ++s) rc = (h->d.form == SELF_FUSED) ? step_fused(h) : step_plain(h);
+    return rc;
+}
+
+int self_sw2d_forward_step_rk3_timed(self_sw2d_t h, double tn, int nSteps, double *elapsed_ms)
+{
+    int rc = check_ready(h);
+    if (rc) return rc;
+    SELF_REQUIRE(elapsed_ms, SELF_ERR_ARG, "null elapsed_ms");
+    if (h->d.form == SELF_FUSED && !h->trValid) {  // keep the initial traces out of the timed region
+        rc = launch_traces(h, h->q, h->tr[h->trCur], 3);
+        if (rc) return rc;
+        h->trValid = true;
+    }
+    SELF_HIP(hipEventRecord(h->ev0, h->stream));
+    rc = self_sw2d_forward_step_rk3(h, tn, nSteps);
+    if (rc) return rc;
+    SELF_HIP(hipEventRecord(h->ev1, h->stream));
+    SELF_HIP(hipEventSynchronize(h->ev1));
+    float ms = 0.0f;
+    SELF_HIP(hipEventElapsedTime(&ms, h->ev0, h->ev1));
+    *elapsed_ms = ms;
+    return SELF_OK;
+}
+
+int self_sw2d_kernel_times(self_sw2d_t h, double tn, int nSteps, double *avg_ms, int *nKernels)
+try {
+    (void)tn;
+    int rc = check_ready(h);
+    if (rc) return rc;
+    SELF_REQUIRE(avg_ms && nKernels && nSteps > 0, SELF_ERR_ARG, "self_sw2d_kernel_times: bad arguments");
+    const bool fused = h->d.form == SELF_FUSED;
+    const int nk = fused ? 1 : 4;
+    *nKernels = nk;
+    if (fused && !h->trValid) {
+        rc = launch_traces(h, h->q, h->tr[h->trCur], 3);
+        if (rc) return rc;
+        h->trValid = true;
+    }
+    // one event pair per kernel launch, on the model's stream
+    const int nLaunch = 3 * nSteps * nk;
+    std::vector<hipEvent_t> ev(2 * nLaunch);
+    for (auto &e : ev) SELF_HIP(hipEventCreate(&e));
+    int li = 0;
+    auto mark = [&](int which) { return hipEventRecord(ev[2 * li + which], h->stream); };
+    const size_t n = 3 * h->nE * h->np;
+    for (int st = 0; st < nSteps && !rc; ++st) {
+        if (fused) {
+            SWKernelArgs A = make_args(h);
+            for (int m = 0; m < 3; ++m) {
+                A.a = kRkA[m];
+                A.gdt = kRkG[m] * h->d.dt;
+                A.trIn = h->tr[h->trCur];
+                A.trOut = h->tr[1 - h->trCur];
+                SELF_HIP(mark(0));
+                if (h->d.precision == SELF_MIXED) {
+                    SELF_DISPATCH_N(h->d.N, hipLaunchKernelGGL((k_sw_stage_fused<NN, true>), dim3(grid_for<NN>(h->nE)),
+                                                               dim3(256), 0, h->stream, A));
+                } else {
+                    SELF_DISPATCH_N(h->d.N, hipLaunchKernelGGL((k_sw_stage_fused<NN, false>), dim3(grid_for<NN>(h->nE)),
+                                                               dim3(256), 0, h->stream, A));
+                }
+                SELF_HIP(hipGetLastError());
+                SELF_HIP(mark(1));
+                ++li;
+                h->trCur = 1 - h->trCur;
+            }
+        } else {
+            for (int m = 0; m < 3; ++m) {
+                SELF_HIP(mark(0)); rc = launch_traces(h, h->q, h->bsol, 3); SELF_HIP(mark(1)); ++li;
+                if (rc) break;
+                SELF_HIP(mark(0)); rc = launch_edge_flux(h); SELF_HIP(mark(1)); ++li;
+                if (rc) break;
+                SELF_HIP(mark(0)); rc = launch_mapped(h); SELF_HIP(mark(1)); ++li;
+                if (rc) break;
+                SELF_HIP(mark(0));
+                hipLaunchKernelGGL(k_sw_rk_update, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, h->stream, h->q,
+                                   h->G64, h->tend, n, kRkA[m], kRkG[m] * h->d.dt);
+                SELF_HIP(hipGetLastError());
+                SELF_HIP(mark(1)); ++li;
+            }
+        }
+    }
+    if (!rc) SELF_HIP(hipStreamSynchronize(h->stream));
+    for (int k = 0; k < nk; ++k) avg_ms[k] = 0.0;
+    for (int i = 0; i < li && !rc; ++i) {
+        float ms = 0.0f;
+        SELF_HIP(hipEventElapsedTime(&ms, ev[2 * i], ev[2 * i + 1]));
+        avg_ms[i % nk] += ms;
+    }
+    for (int k = 0; k < nk; ++k) avg_ms[k] /= (double)(3 * nSteps);
+    for (auto &e : ev) (void)hipEventDestroy(e);
+    return rc;
+}
+SELF_CATCH_ALL
+
+int self_sw2d_calculate_bathymetry_gradient(self_sw2d_t h)
+{
+    int rc = check_ready(h);
+    if (rc) return rc;
+    SWKernelArgs A = make_args(h);
+    SELF_DISPATCH_N(h->d.N, hipLaunchKernelGGL(k_sw_bathy_grad<NN>, dim3(grid_for<NN>(h->nE)), dim3(256), 0,
+                                               h->stream, A, h->bathy, h->bathy32));
+    SELF_HIP(hipGetLastError());
+    return SELF_OK;
+}
+
+int self_sw2d_synchronize(self_sw2d_t h)
+{
+    SELF_REQUIRE(h, SELF_ERR_ARG, "null handle");
+    SELF_HIP(hipStreamSynchronize(h->stream));
+    return SELF_OK;
+}
+
+int self_sw2d_device_bytes(self_sw2d_t h, size_t *bytes)
+{
+    SELF_REQUIRE(h && bytes, SELF_ERR_ARG, "null argument");
+    *bytes = h->bytes;
+    return SELF_OK;
+}
+
+}  // extern "C"

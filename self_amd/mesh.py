@@ -1,0 +1,42 @@
+"""Structured quad meshes (host set-up) over the C ABI.
+
+Mirrors QuadMesh%LoadDefaultMesh / ConstructEdges (src/geom/QuadMeshClass.f90
+:1489-1627, :1718-1859) and MappedGeometry_2D metrics
+(src/geom/MappedGeometryClass_2D.f90:611-710), with optional periodic pairing.
+"""
+from __future__ import annotations
+
+import numpy as np
+
+from . import _lib as L
+
+
+class QuadMesh:
+    """Arrays per element (element slowest):
+    x, y, dxds, dxdp, dyds, dydp, J : (nElems, N+1, N+1) indexed [e, j, i]
+    nHat : (nElems, 4, N+1, 2) [e, side, node, component]; nLen (nElems, 4, N+1)
+    edges: (nEdges, 6) int32 = e1, s1, e2, s2, start, inc (1-based, reference convention)
+    """
+
+    def __init__(self, nXElem: int, nYElem: int, N: int, sNodes, periodicX=True, periodicY=True,
+                 xScale=1.0, yScale=1.0, warp=0.0):
+        lib = L.lib()
+        import ctypes as C
+        nE, nEd = C.c_int(), C.c_int()
+        L.check(lib.self_quadmesh_counts(nXElem, nYElem, int(periodicX), int(periodicY), C.byref(nE), C.byref(nEd)))
+        self.nX, self.nY, self.N = nXElem, nYElem, N
+        self.nElems, self.nEdges = nE.value, nEd.value
+        self.periodicX, self.periodicY = bool(periodicX), bool(periodicY)
+        self.xScale, self.yScale, self.warp = xScale, yScale, warp
+        n1 = N + 1
+        E = self.nElems
+        self.edges = np.zeros((self.nEdges, 6), dtype=np.int32)
+        shp = (E, n1, n1)
+        for name in ("x", "y", "dxds", "dxdp", "dyds", "dydp", "J"):
+            setattr(self, name, np.zeros(shp))
+        self.nHat = np.zeros((E, 4, n1, 2))
+        self.nLen = np.zeros((E, 4, n1))
+        L.check(lib.self_quadmesh_structured(
+            nXElem, nYElem, N, int(periodicX), int(periodicY), xScale, yScale, warp,
+            L.ptr(L.f64(sNodes)), L.ptr(self.edges, L.ip),
+            *[L.ptr(getattr(self, n)) for n in ("x", "y", "dxds", "dxdp", "dyds", "dydp", "J", "nHat", "nLen")]))

@@ -1,0 +1,83 @@
+"""Host-side interpolation and nodal storage (set-up), over the C ABI.
+
+Mirrors src/interp/Legendre.f90, src/interp/Lagrange_1D_Class.f90 and
+src/nodal/NodalStorage_2D_Class.f90 (BuildNodalStorage_2D :101-230).
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import _lib as L
+
+
+def legendre_quadrature(N: int, quadType: int = L.GAUSS):
+    """GenerateLegendreQuadrature (Legendre.f90:180-212): nodes, weights."""
+    x = np.zeros(N + 1)
+    w = np.zeros(N + 1)
+    L.check(L.lib().self_legendre_quadrature(N, quadType, L.ptr(x), L.ptr(w)))
+    return x, w
+
+
+def barycentric_weights(x):
+    x = L.f64(x)
+    bw = np.zeros_like(x)
+    L.check(L.lib().self_barycentric_weights(len(x) - 1, L.ptr(x), L.ptr(bw)))
+    return bw
+
+
+def lagrange_polynomials(x, bw, xEval):
+    x = L.f64(x)
+    out = np.zeros_like(x)
+    L.check(L.lib().self_lagrange_polynomials(len(x) - 1, L.ptr(x), L.ptr(L.f64(bw)), float(xEval), L.ptr(out)))
+    return out
+
+
+def derivative_matrix(x, bw):
+    x = L.f64(x)
+    n1 = len(x)
+    D = np.zeros(n1 * n1)
+    L.check(L.lib().self_derivative_matrix(n1 - 1, L.ptr(x), L.ptr(L.f64(bw)), L.ptr(D)))
+    return D.reshape(n1, n1, order="F")
+
+
+def interpolation_matrix(x, bw, znew):
+    x = L.f64(x)
+    znew = L.f64(znew)
+    T = np.zeros(len(znew) * len(x))
+    L.check(L.lib().self_interpolation_matrix(len(x) - 1, L.ptr(x), L.ptr(L.f64(bw)), len(znew) - 1,
+                                              L.ptr(znew), L.ptr(T)))
+    return T.reshape(len(znew), len(x), order="F")
+
+
+class NodalStorage2D:
+    """NodalStorage_2D: quadrature, derivative matrices, boundary interpolants.
+
+    quirk_dmatp_transposed reproduces the reference's DG dMatP (the transpose
+    of the weak operator, NodalStorage_2D_Class.f90:209-218); by default both
+    directions use the weak operator as the reference's 1-D/3-D storages do.
+    Matrices are returned as (N+1, N+1) numpy arrays, element (row, col).
+    """
+
+    def __init__(self, N: int, quadrature: int = L.GAUSS, form: int = L.DG,
+                 quirk_dmatp_transposed: bool = False):
+        n1 = N + 1
+        self.N = N
+        self.quadrature = quadrature
+        self.form = form
+        self.nodes = np.zeros(n1)
+        self.qWeight = np.zeros(n1)
+        dms = np.zeros(n1 * n1)
+        dmp = np.zeros(n1 * n1)
+        self.lagWest, self.lagEast = np.zeros(n1), np.zeros(n1)
+        self.lagSouth, self.lagNorth = np.zeros(n1), np.zeros(n1)
+        L.check(L.lib().self_nodal2d_build(N, quadrature, form, int(quirk_dmatp_transposed), L.ptr(self.nodes),
+                                           L.ptr(self.qWeight), L.ptr(dms), L.ptr(dmp), L.ptr(self.lagWest),
+                                           L.ptr(self.lagEast), L.ptr(self.lagSouth), L.ptr(self.lagNorth)))
+        self.dMatS = dms.reshape(n1, n1, order="F")
+        self.dMatP = dmp.reshape(n1, n1, order="F")
+
+    def flat(self, name):
+        """Column-major flat copy of a matrix, as the C ABI takes it."""
+        return np.ascontiguousarray(getattr(self, name).ravel(order="F"))

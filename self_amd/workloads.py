@@ -1,0 +1,112 @@
+"""Seeded synthetic workloads shared by the tests, the smoke check and the
+benchmark.
+
+Random numbers come from splitmix64 (a counter-based generator with a
+published constant set, Steele/Lea/Flood 2014), mapped to U[0,1) by the top
+53 bits, so the same inputs can be regenerated anywhere.
+
+sw2d_geostrophic(): 2-D nonlinear shallow water on a doubly periodic
+[0, L]^2 f-plane.  Depth h = H0 + sum_{k=1..8} a_k cos(2 pi (m_k x + n_k y)/L + phi_k)
+with a_k ~ U[0, amp/8] (so |h - H0| <= amp), (m_k, n_k) uniform on
+{-4..4}^2 minus the origin, phi_k ~ U[0, 2 pi).  Free surface
+eta = eta0 exp(-r^2 / width^2) about the domain centre, geostrophic
+velocity u = -(g/f) d(eta)/dy, v = (g/f) d(eta)/dx evaluated analytically,
+H = h + eta, U = H u, V = H v.  The time step follows the nodal-DG CFL
+estimate dt = cfl * dx_elem / ((N+1)^2 max(|u| + sqrt(g H))); with
+cfl = 0.5 this is 1.22 s for 64x64 elements at N = 7 (a (2N+1) denominator
+instead gives 5.2 s, which is above the RK3 stability limit of this
+Gauss-node DGSEM and blows up within 20 steps).
+"""
+from __future__ import annotations
+
+import math
+
+import numpy as np
+
+MASK64 = (1 << 64) - 1
+
+
+class SplitMix64:
+    def __init__(self, seed: int):
+        self.state = seed & MASK64
+
+    def next_u64(self) -> int:
+        self.state = (self.state + 0x9E3779B97F4A7C15) & MASK64
+        z = self.state
+        z = ((z ^ (z >> 30)) * 0xBF58476D1CE4E5B9) & MASK64
+        z = ((z ^ (z >> 27)) * 0x94D049BB133111EB) & MASK64
+        return z ^ (z >> 31)
+
+    def uniform(self) -> float:
+        return (self.next_u64() >> 11) * (1.0 / (1 << 53))
+
+    def randint(self, lo: int, hi: int) -> int:
+        """Uniform integer in [lo, hi]."""
+        return lo + int(self.uniform() * (hi - lo + 1))
+
+
+DEFAULT_SEED = 20261202
+
+
+def bathymetry_modes(seed=DEFAULT_SEED, nmodes=8, amp=50.0, kmax=4):
+    rng = SplitMix64(seed)
+    modes = []
+    for _ in range(nmodes):
+        a = rng.uniform() * amp / nmodes
+        while True:
+            m = rng.randint(-kmax, kmax)
+            n = rng.randint(-kmax, kmax)
+            if m != 0 or n != 0:
+                break
+        phi = 2.0 * math.pi * rng.uniform()
+        modes.append((a, m, n, phi))
+    return modes
+
+
+def sw2d_geostrophic_fields(x, y, L=1.0e6, H0=1000.0, g=9.81, f0=1.0e-4, eta0=1.0, width=1.0e5,
+                            seed=DEFAULT_SEED, nmodes=8, amp=50.0):
+    """Fields at node positions x, y (any matching shapes): returns
+    h (depth), U, V, H."""
+    h = np.full_like(x, H0)
+    for a, m, n, phi in bathymetry_modes(seed, nmodes, amp):
+        h = h + a * np.cos(2.0 * math.pi * (m * x + n * y) / L + phi)
+    xc = yc = 0.5 * L
+    eta = eta0 * np.exp(-((x - xc) ** 2 + (y - yc) ** 2) / width ** 2)
+    detadx = -2.0 * (x - xc) / width ** 2 * eta
+    detady = -2.0 * (y - yc) / width ** 2 * eta
+    u = -(g / f0) * detady
+    v = (g / f0) * detadx
+    H = h + eta
+    return h, H * u, H * v, H
+
+
+def sw2d_cfl_dt(U, V, H, g, dx_elem, N, cfl=0.5):
+    u = np.abs(U / H)
+    v = np.abs(V / H)
+    lam = np.max(np.maximum(u, v) + np.sqrt(g * H))
+    return cfl * dx_elem / ((N + 1) ** 2 * lam)
+
+
+def sw2d_geostrophic(nX=64, N=7, precision="fp64", form="fused", L=1.0e6, seed=DEFAULT_SEED,
+                     cfl=0.5, g=9.81, f0=1.0e-4, quirk_dmatp_transposed=False, compute_gradient=True):
+    """Build a ShallowWater model with the geostrophic case loaded.
+
+    Returns (model, info) with info holding the host arrays that were set.
+    """
+    from .shallow_water import ShallowWater, SWParams
+
+    p = SWParams(polyDeg=N, nXElem=nX, nYElem=nX, xScale=L, yScale=L, g=g, f0=f0,
+                 periodicX=True, periodicY=True)
+    model = ShallowWater(p, precision=precision, form=form, quirk_dmatp_transposed=quirk_dmatp_transposed)
+    m = model.mesh
+    h, U, V, H = sw2d_geostrophic_fields(m.x, m.y, L=L, g=g, f0=f0, seed=seed)
+    sol = np.stack([U, V, H], axis=1)
+    bathy = np.stack([h, np.zeros_like(h), np.zeros_like(h)], axis=1)
+    model.SetBathymetry(bathy)
+    model.SetPlanetaryVorticity(np.full_like(h, f0))
+    model.SetSolution(sol)
+    if compute_gradient:
+        model.CalculateBathymetryGradient()
+    dt = sw2d_cfl_dt(U, V, H, g, L / nX, N, cfl)
+    model.SetTimeStep(dt)
+    return model, {"sol": sol, "bathy": bathy, "dt": dt, "f0": f0, "g": g, "L": L}
