@@ -69,13 +69,13 @@ def cpu_baseline(nX, N, target_s=12.0):
     orc.bathy[:] = np.stack([h, 0 * h, 0 * h], axis=1).ravel()
     orc.plv[:] = 1.0e-4
     orc.bathymetry_gradient()
+    orc.integrate(0.0, 2, threads)  # warm the threads and caches
+    steps, el = 0, 0.0
     t0 = time.perf_counter()
-    orc.integrate(0.0, 2, threads)
-    probe = (time.perf_counter() - t0) / 2
-    steps = int(max(2, min(500, target_s / max(probe, 1e-6))))
-    t0 = time.perf_counter()
-    orc.integrate(0.0, steps, threads)
-    el = time.perf_counter() - t0
+    while el < target_s and steps < 5000:  # chunks of 20 steps until the sample is long enough
+        orc.integrate(0.0, 20, threads)
+        steps += 20
+        el = time.perf_counter() - t0
     dof = 3 * nX * nX * n1 * n1
     return {"value": dof * 3 * steps / el, "unit": "DOF-stage/s", "cores": threads, "kind": "port",
             "sample": f"oracle FP64 C restatement, {nX}x{nX} N={N} geostrophic case, {steps} RK3 steps, "

@@ -176,6 +176,85 @@ int self_sw2d_synchronize(self_sw2d_t h);
 /* Bytes of HBM state the handle holds. */
 int self_sw2d_device_bytes(self_sw2d_t h, size_t *bytes);
 
+/* ------------------------------------------------------------ Krylov solvers
+ * The reference's plug-in point is the abstract IterativeSolver type with
+ * deferred MatrixAction / Residual / PreConditionInvert procedures
+ * (src/iterativesolve/IterativeSolvers_Class.f90:55-186).  Here an operator
+ * handle plays that role: it owns device data and applies itself on the
+ * device, and the solver keeps every vector and scalar on the device.
+ * Methods keep the reference's control flow, residual history resi(0:) and
+ * stop tests:
+ *   SELF_KRYLOV_CG      Solve_ConjugateGradient    :328-405
+ *   SELF_KRYLOV_PCG     Solve_PCConjugateGradient  :409-489  (Jacobi)
+ *   SELF_KRYLOV_GMRES   Solve_GMRES                :493-652
+ *   SELF_KRYLOV_PCGMRES Solve_PCGMRES              :656-822  (Jacobi, flexible)
+ * ortho: 0 = modified Gram-Schmidt (the reference), 1 = classical
+ * Gram-Schmidt applied twice (two sweeps over the basis per step).
+ * ioerr: 0 converged, -1 not converged, -2 as the reference leaves it. */
+#define SELF_KRYLOV_CG 0
+#define SELF_KRYLOV_PCG 1
+#define SELF_KRYLOV_GMRES 2
+#define SELF_KRYLOV_PCGMRES 3
+#define SELF_KRYLOV_CGSEM_PCG 4  /* CGsemElliptic_2D's own PCG loop (:1114-1207) */
+
+typedef struct self_linop_s *self_linop_t;
+
+/* Dense n x n operator (column-major A), residual b - A x, Jacobi diag(A)^-1. */
+int self_linop_dense_create(int n, const double *A, const double *b, self_linop_t *out);
+/* Elliptic1D (examples/elliptic1d/parabola/localmods/Elliptic1D_Class.f90):
+ * Build :111-176 on nElems elements of degree N on [0,1], k = 1, source 1,
+ * homogeneous Dirichlet ends; flat vectors of (N+1)*nElems entries in the
+ * reference's masked element-local layout. */
+int self_linop_elliptic1d_create(int N, int nElems, double xScale, self_linop_t *out);
+/* UnMaskSolution (:268-276): the continuous solution at every local node,
+ * and the node positions x (nullable). */
+int self_linop_elliptic1d_unmask(self_linop_t op, const double *sol, double *u, double *x);
+int self_linop_size(self_linop_t op, int *n);
+int self_linop_destroy(self_linop_t op);
+/* IterativeSolver%Solve(ioerr): sol (host, nDOF) is the initial guess in and
+ * the solution out; resi (host, resiLen, nullable) receives resi(0:). */
+int self_krylov_solve(self_linop_t op, int method, double tol, int maxIters, int mInner, int ortho,
+                      double *sol, double *resi, int resiLen, int *iterations, int *ioerr);
+
+/* ---------------------------------------- CG spectral-element elliptic 2-D
+ * Replaces CGsemElliptic_2D (src/cgsem/CGsemElliptic_2D_Class.f90:57-93):
+ * div(k grad u) = s on GLL nodes of degree N, Dirichlet on every boundary
+ * edge.  Arrays use the reference's element-local layout (0:N,0:N,nElems);
+ * the device keeps global-DOF vectors (one entry per shared node). */
+typedef struct self_cgsem2d_s *self_cgsem2d_t;
+
+/* Build :99-154 with the mesh of self_quadmesh_structured (GLL sNodes);
+ * elemNodes: 4 corner node ids per element (SW, SE, NE, NW), 1-based;
+ * boundary edges (e2 <= 0) become Dirichlet (BuildQuadMesh :197-215). */
+int self_cgsem2d_create(int N, int nElems, int nEdges, int nNodes, const int *edges, const int *elemNodes,
+                        const double *dxds, const double *dxdp, const double *dyds, const double *dydp,
+                        const double *J, self_cgsem2d_t *out);
+int self_cgsem2d_destroy(self_cgsem2d_t h);
+int self_cgsem2d_global_dofs(self_cgsem2d_t h, int *nGlobal);
+/* fluxCoeff (:64), source (:63), Dirichlet values at boundary nodes (dirF of
+ * SetDirichletBoundaryCondition :234-300, sampled per local node) */
+int self_cgsem2d_set_flux_coefficient(self_cgsem2d_t h, const double *k);
+int self_cgsem2d_set_source(self_cgsem2d_t h, const double *s);
+int self_cgsem2d_set_dirichlet(self_cgsem2d_t h, const double *values);
+int self_cgsem2d_set_solution(self_cgsem2d_t h, const double *u);
+/* solution after UnMask + SetDirichletBoundaryCondition */
+int self_cgsem2d_get_solution(self_cgsem2d_t h, double *u);
+/* MatrixAction (:930-976) of a masked local array; Au returned masked */
+int self_cgsem2d_matrix_action(self_cgsem2d_t h, const double *u, double *Au);
+/* Residual (:980-1068) of the current solution, masked local layout */
+int self_cgsem2d_residual(self_cgsem2d_t h, double *r);
+/* Solve (:1114-1207 for SELF_KRYLOV_CGSEM_PCG; the generic methods above
+ * applied to the same operator otherwise).  precond 0 = identity (the
+ * reference's pcIterates = 0), 1 = Jacobi (assembled diagonal).
+ * elapsed_ms (nullable): device time of the solve. */
+int self_cgsem2d_solve(self_cgsem2d_t h, int method, int precond, double tol, int maxIters, int mInner,
+                       int ortho, int *ioerr, int *iterations, double *resi, int resiLen, double *elapsed_ms);
+/* Mixed precision: FP32 PCG inner solves to innerTol, FP64 residual and
+ * iterative refinement until |b - A x| <= tol |b - A x0|. */
+int self_cgsem2d_solve_mixed(self_cgsem2d_t h, int precond, double tol, double innerTol, int maxInner,
+                             int maxOuter, int *ioerr, int *outerIters, int *innerIters, double *relResidual,
+                             double *elapsed_ms);
+
 #ifdef __cplusplus
 }
 #endif

@@ -399,3 +399,102 @@ def elliptic1d(N=5, nElems=10, maxIters=1000, tol=1e-8):
     ioerr = C.c_int()
     iters = lib().orc_elliptic1d_solve(N, nElems, maxIters, tol, _p(x), _p(u), _p(resi), C.byref(ioerr))
     return x, u, resi, ioerr.value, iters
+
+
+# --------------------------------------------------------------------------- CGSEM 2-D
+class CGSEM2DStruct(C.Structure):
+    _fields_ = [("N", C.c_int), ("nElems", C.c_int), ("nEdges", C.c_int), ("nNodes", C.c_int),
+                ("edges", _ip), ("nodeElemStart", _ip), ("nodeElem", _ip), ("nodeLocal", _ip),
+                ("nodeType", _ip), ("D", _dp), ("w", _dp), ("dxds", _dp), ("dxdp", _dp), ("dyds", _dp),
+                ("dydp", _dp), ("J", _dp), ("fluxCoeff", _dp), ("solution", _dp), ("source", _dp),
+                ("dirichletValue", C.c_double), ("tolerance", C.c_double), ("maxIters", C.c_int),
+                ("resi", _dp), ("resiLen", C.c_int)]
+
+
+def _declare_extra_cgsem2d(L):
+    P = C.POINTER(CGSEM2DStruct)
+    L.orc_cgsem2d_connectivity.argtypes = [P, _ip, C.c_int]
+    for n in ("orc_cgsem2d_mask", "orc_cgsem2d_unmask", "orc_cgsem2d_global_sum", "orc_cgsem2d_residual",
+              "orc_cgsem2d_diagonal"):
+        getattr(L, n).argtypes = [P, _dp]
+    L.orc_cgsem2d_set_dirichlet.argtypes = [P]
+    L.orc_cgsem2d_matrix_action.argtypes = [P, _dp, _dp]
+    L.orc_cgsem2d_dot.argtypes = [P, _dp, _dp]
+    L.orc_cgsem2d_dot.restype = C.c_double
+    L.orc_cgsem2d_solve_pcg.argtypes = [P, C.c_int, _ip]
+    L.orc_cgsem2d_solve_pcg.restype = C.c_int
+
+
+class CGSEM2D:
+    """CGsemElliptic_2D on a structured walled mesh (all boundaries Dirichlet),
+    GLL nodes, CG derivative matrix.  Arrays element-local (nElems, N+1, N+1)
+    indexed [e, j(p), i(s)]."""
+
+    def __init__(self, N, nX, nY, Lx=1.0, Ly=1.0, warp=0.0, tolerance=1e-10, maxIters=2000):
+        self.N = N
+        n1 = N + 1
+        self.nodes, self.w = quadrature(N, GAUSS_LOBATTO)
+        bw = barycentric_weights(self.nodes)
+        self.D = np.ascontiguousarray(derivative_matrix(self.nodes, bw).ravel(order="F"))
+        d = QuadMeshDesc(nX, nY, N, 0, 0, Lx, Ly, warp, 0, 0, 0)
+        lib().orc_quadmesh_counts(C.byref(d))
+        self.mesh = QuadMesh(nX, nY, N, self.nodes, False, False, Lx, Ly, warp)
+        E = self.mesh.nElems
+        self.nElems = E
+        self.fluxCoeff = np.ones(n1 * n1 * E)
+        self.solution = np.zeros(n1 * n1 * E)
+        self.source = np.zeros(n1 * n1 * E)
+        self.resi = np.zeros(maxIters + 1)
+        nN = d.nNodes
+        self.edges = np.ascontiguousarray(self.mesh.edges.ravel().astype(np.int32))
+        self.nodeElemStart = np.zeros(nN + 1, dtype=np.int32)
+        self.nodeElem = np.zeros(4 * E, dtype=np.int32)
+        self.nodeLocal = np.zeros(4 * E, dtype=np.int32)
+        self.nodeType = np.zeros(nN, dtype=np.int32)
+        m = self.mesh
+        self.s = CGSEM2DStruct(N, E, m.nEdges, nN, _pi(self.edges), _pi(self.nodeElemStart), _pi(self.nodeElem),
+                               _pi(self.nodeLocal), _pi(self.nodeType), _p(self.D), _p(self.w), _p(m.dxds),
+                               _p(m.dxdp), _p(m.dyds), _p(m.dydp), _p(m.J), _p(self.fluxCoeff),
+                               _p(self.solution), _p(self.source), 0.0, tolerance, maxIters, _p(self.resi),
+                               maxIters + 1)
+        lib().orc_cgsem2d_connectivity(C.byref(self.s), _pi(np.ascontiguousarray(m.elemNodes)), nN)
+
+    def shape(self, a):
+        n1 = self.N + 1
+        return a.reshape(self.nElems, n1, n1)
+
+    def matrix_action(self, u):
+        u = np.ascontiguousarray(u, dtype=np.float64).ravel().copy()
+        out = np.zeros_like(u)
+        lib().orc_cgsem2d_matrix_action(C.byref(self.s), _p(u), _p(out))
+        return out, u
+
+    def residual(self):
+        r = np.zeros_like(self.solution)
+        lib().orc_cgsem2d_residual(C.byref(self.s), _p(r))
+        return r
+
+    def mask(self, u):
+        u = np.ascontiguousarray(u, dtype=np.float64).ravel().copy()
+        lib().orc_cgsem2d_mask(C.byref(self.s), _p(u))
+        return u
+
+    def unmask(self, u):
+        u = np.ascontiguousarray(u, dtype=np.float64).ravel().copy()
+        lib().orc_cgsem2d_unmask(C.byref(self.s), _p(u))
+        return u
+
+    def global_sum(self, u):
+        u = np.ascontiguousarray(u, dtype=np.float64).ravel().copy()
+        lib().orc_cgsem2d_global_sum(C.byref(self.s), _p(u))
+        return u
+
+    def diagonal(self):
+        d = np.zeros_like(self.solution)
+        lib().orc_cgsem2d_diagonal(C.byref(self.s), _p(d))
+        return d
+
+    def solve(self, precond=0):
+        it = C.c_int()
+        ioerr = lib().orc_cgsem2d_solve_pcg(C.byref(self.s), precond, C.byref(it))
+        return ioerr, it.value

@@ -1,0 +1,666 @@
+// cgsem2d.hip -- continuous-Galerkin spectral-element elliptic operator
+// div(k grad u) = s on GLL nodes, as src/cgsem/CGsemElliptic_2D_Class.f90:
+//   CalculateGradient :791-850, FluxDivergence :854-926,
+//   MatrixAction :930-976 (UnMask, element operator, GlobalSum, Mask),
+//   Residual :980-1068, DotProduct :1072-1110.
+//
+// Layout.  The reference keeps element-local arrays u(0:N,0:N,nEl) and a
+// mask convention (one "owner" copy per shared node, the others zero),
+// implemented with edge loops and node-to-element linked lists.  Here the
+// Krylov vectors are global-DOF vectors -- exactly the owner copies, in the
+// order the owners appear element by element -- so vector operations touch
+// each degree of freedom once (0.77x the local count at N = 7) and
+// DotProduct is a plain sum.  The operator gathers x through a local->global
+// map, applies the element operator with LDS-staged tensor contractions,
+// and assembles each global DOF as the sum of its copies in the reference's
+// GlobalSum order (edge: primary + secondary; corner: node-list order) --
+// a deterministic gather, no atomics.
+//
+// The element operator uses three precomputed geometric factors per node,
+//   G11 = k ws wp (dydp^2 + dxdp^2)/J,  G12 = -k ws wp (dyds dydp + dxds dxdp)/J,
+//   G22 = k ws wp (dxds^2 + dyds^2)/J,
+// the algebraic expansion of the reference's gradient/flux sequence:
+//   F1' = G11 du/ds + G12 du/dp,  F2' = G12 du/ds + G22 du/dp,
+//   Lu(i,j) = -( sum_m D(m,i) F1'(m,j) + sum_m D(m,j) F2'(i,m) ).
+// (3 doubles per node instead of the 6 metric/coefficient arrays.)
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "common.hpp"
+#include "dg2d_device.hpp"
+#include "krylov.hpp"
+
+namespace self {
+namespace {
+
+template <int N, typename T>
+struct CgLDS {
+    static constexpr int n1 = N + 1;
+    T D[n1 * n1];
+};
+
+// Element operator on gathered values -> Lbuf (local copies).
+// RESID: Dirichlet nodes take the prescribed values uD (Residual of the
+// solution); otherwise they are zero (search directions are masked there).
+template <int N, typename T, bool RESID>
+__global__ __launch_bounds__(256) void k_cg_local(const T *__restrict__ x, const int *__restrict__ gid,
+                                                  const double *__restrict__ uD, const T *__restrict__ G3,
+                                                  const T *__restrict__ Dg, T *__restrict__ Lbuf, int nE,
+                                                  const SolverState *st)
+{
+    using S = Shape<N>;
+    constexpr int n1 = S::n1, np = S::np, epb = S::epb;
+    if (st && (st->done || (!RESID && st->innerExit))) return;
+    __shared__ CgLDS<N, T> L;
+    __shared__ T us[epb * np], f1[epb * np], f2[epb * np];
+    for (int k = threadIdx.x; k < n1 * n1; k += blockDim.x) L.D[k] = Dg[k];
+    const int t = threadIdx.x, le = t / np, nd = t - le * np;
+    const int e = blockIdx.x * epb + le;
+    const bool act = le < epb && e < nE;
+    const int i = nd % n1, j = nd / n1;
+    const size_t lid = (size_t)e * np + nd, stride = (size_t)nE * np;
+    T g11 = 0, g12 = 0, g22 = 0;
+    if (act) {
+        const int g = gid[lid];
+        T u;
+        if (g >= 0) u = x[g];
+        else u = RESID ? (T)uD[lid] : (T)0;
+        us[le * np + nd] = u;
+        g11 = G3[lid];
+        g12 = G3[stride + lid];
+        g22 = G3[2 * stride + lid];
+    }
+    __syncthreads();
+    if (act) {
+        const T *ue = us + le * np;
+        // du/ds = D u along s (row j), du/dp = D u along p (column i)
+        T duds = L.D[i] * ue[n1 * j];
+        T dudp = L.D[j] * ue[i];
+        #pragma unroll
+        for (int m = 1; m < n1; ++m) {
+            duds = fma(L.D[i + n1 * m], ue[m + n1 * j], duds);
+            dudp = fma(L.D[j + n1 * m], ue[i + n1 * m], dudp);
+        }
+        f1[le * np + nd] = g11 * duds + g12 * dudp;
+        f2[le * np + nd] = g12 * duds + g22 * dudp;
+    }
+    __syncthreads();
+    if (act) {
+        const T *a = f1 + le * np, *b = f2 + le * np;
+        T s1 = L.D[n1 * i] * a[n1 * j];  // D(0,i) F1'(0,j)
+        T s2 = L.D[n1 * j] * b[i];       // D(0,j) F2'(i,0)
+        #pragma unroll
+        for (int m = 1; m < n1; ++m) {
+            s1 = fma(L.D[m + n1 * i], a[m + n1 * j], s1);
+            s2 = fma(L.D[m + n1 * j], b[i + n1 * m], s2);
+        }
+        Lbuf[lid] = -(s1 + s2);
+    }
+}
+
+// y[g] = sum over the copies of g (GlobalSum order); optional fused dot.
+// MODE 0: matvec.  MODE 1: residual  y = (b[c0] - sum L) + sum_{c>=1} b[c],
+// with |y|^2 reduced.
+template <typename T, int MODE>
+__global__ __launch_bounds__(kRedThreads) void k_cg_assemble(const T *__restrict__ Lbuf, const int *__restrict__ cs,
+                                                             const int *__restrict__ ci, const double *__restrict__ bl,
+                                                             size_t nG, T *__restrict__ y, const T *__restrict__ dotWith,
+                                                             SolverState *st, int fin, RedSlot slot)
+{
+    if (st && (st->done || (MODE == 0 && st->innerExit))) return;
+    double acc = 0.0;
+    for (size_t g = (size_t)blockIdx.x * blockDim.x + threadIdx.x; g < nG; g += (size_t)gridDim.x * blockDim.x) {
+        const int c0 = cs[g], c1 = cs[g + 1];
+        T sum = Lbuf[ci[c0]];
+        for (int c = c0 + 1; c < c1; ++c) sum = sum + Lbuf[ci[c]];
+        T out;
+        if (MODE == 1) {
+            double r = bl[ci[c0]] - (double)sum;
+            for (int c = c0 + 1; c < c1; ++c) r = r + bl[ci[c]];
+            out = (T)r;
+            acc += r * r;
+        } else {
+            out = sum;
+            if (dotWith) acc += (double)dotWith[g] * (double)out;
+        }
+        y[g] = out;
+    }
+    if (fin == kFinNone || !st) return;
+    __shared__ double lds4[4];
+    double part[1] = {block_sum(acc, lds4)};
+    if (publish_partials<1>(slot, part)) {
+        const double S = final_sum(slot, 0, lds4);
+        if (threadIdx.x == 0) {
+            finalize_scalar(st, fin, S);
+            *slot.ticket = 0u;
+        }
+    }
+}
+
+#define CG_DISPATCH_N(N_, CALL)                          \
+    switch (N_) {                                        \
+    case 1: { constexpr int NN = 1; CALL; } break;       \
+    case 2: { constexpr int NN = 2; CALL; } break;       \
+    case 3: { constexpr int NN = 3; CALL; } break;       \
+    case 4: { constexpr int NN = 4; CALL; } break;       \
+    case 5: { constexpr int NN = 5; CALL; } break;       \
+    case 6: { constexpr int NN = 6; CALL; } break;       \
+    case 7: { constexpr int NN = 7; CALL; } break;       \
+    case 8: { constexpr int NN = 8; CALL; } break;       \
+    case 9: { constexpr int NN = 9; CALL; } break;       \
+    case 10: { constexpr int NN = 10; CALL; } break;     \
+    case 11: { constexpr int NN = 11; CALL; } break;     \
+    default: break;                                      \
+    }
+
+template <int N>
+int cg_grid(size_t nE) { return (int)((nE + Shape<N>::epb - 1) / Shape<N>::epb); }
+
+}  // namespace
+
+// ------------------------------------------------------------ operator
+struct CgsemOp final : DevOp {
+    int N = 0, n1 = 0, np = 0;
+    size_t nE = 0, nG = 0, nLocal = 0;
+    hipStream_t stream = nullptr;
+    // host maps
+    std::vector<int> gidH, csH, ciH;
+    std::vector<double> wH, DH;
+    std::vector<double> met[5];  // dxds, dxdp, dyds, dydp, J (local)
+    std::vector<double> kH;      // flux coefficient (local)
+    std::vector<int> dirH;       // 1 at Dirichlet local nodes
+    // device
+    int *gid = nullptr, *cs = nullptr, *ci = nullptr;
+    double *G3 = nullptr, *D = nullptr, *bl = nullptr, *uD = nullptr, *dinv = nullptr, *Lbuf = nullptr;
+    float *G3f = nullptr, *Df = nullptr, *dinvf = nullptr, *Lbuf32 = nullptr;
+    double *xg = nullptr;  // global solution vector
+    size_t bytes = 0;
+
+    ~CgsemOp() override
+    {
+        void *p[] = {gid, cs, ci, G3, D, bl, uD, dinv, Lbuf, G3f, Df, dinvf, Lbuf32, xg};
+        for (void *q : p)
+            if (q) (void)hipFree(q);
+        if (stream) (void)hipStreamDestroy(stream);
+    }
+
+    template <typename T>
+    int dev(T **p, size_t cnt)
+    {
+        SELF_HIP(hipMalloc(reinterpret_cast<void **>(p), sizeof(T) * std::max<size_t>(cnt, 1)));
+        bytes += sizeof(T) * cnt;
+        return SELF_OK;
+    }
+
+    bool has_f32() const override { return true; }
+    const double *inv_diag() const override { return dinv; }
+    const float *inv_diag32() const override { return dinvf; }
+
+    int matvec(const double *x, double *y, const double *dotWith, SolverState *st, int fin, const RedSlot &slot,
+               hipStream_t s) override
+    {
+        CG_DISPATCH_N(N, hipLaunchKernelGGL((k_cg_local<NN, double, false>), dim3(cg_grid<NN>(nE)), dim3(256), 0, s,
+                                            x, gid, uD, G3, D, Lbuf, (int)nE, st));
+        hipLaunchKernelGGL((k_cg_assemble<double, 0>), dim3(kRedBlocks), dim3(kRedThreads), 0, s, Lbuf, cs, ci,
+                           (const double *)nullptr, nG, y, dotWith, st, fin, slot);
+        SELF_HIP(hipGetLastError());
+        return SELF_OK;
+    }
+
+    int matvec32(const float *x, float *y, const float *dotWith, SolverState *st, int fin, const RedSlot &slot,
+                 hipStream_t s) override
+    {
+        CG_DISPATCH_N(N, hipLaunchKernelGGL((k_cg_local<NN, float, false>), dim3(cg_grid<NN>(nE)), dim3(256), 0, s,
+                                            x, gid, uD, G3f, Df, Lbuf32, (int)nE, st));
+        hipLaunchKernelGGL((k_cg_assemble<float, 0>), dim3(kRedBlocks), dim3(kRedThreads), 0, s, Lbuf32, cs, ci,
+                           (const double *)nullptr, nG, y, dotWith, st, fin, slot);
+        SELF_HIP(hipGetLastError());
+        return SELF_OK;
+    }
+
+    int residual(const double *x, double *r, SolverState *st, int fin, const RedSlot &slot, hipStream_t s) override
+    {
+        CG_DISPATCH_N(N, hipLaunchKernelGGL((k_cg_local<NN, double, true>), dim3(cg_grid<NN>(nE)), dim3(256), 0, s,
+                                            x, gid, uD, G3, D, Lbuf, (int)nE, st));
+        hipLaunchKernelGGL((k_cg_assemble<double, 1>), dim3(kRedBlocks), dim3(kRedThreads), 0, s, Lbuf, cs, ci, bl,
+                           nG, r, (const double *)nullptr, st, fin, slot);
+        SELF_HIP(hipGetLastError());
+        return SELF_OK;
+    }
+};
+
+namespace {
+
+// side s (1..4) position k (0..N) -> local (i, j)
+inline void side_ij(int s, int k, int N, int &i, int &j)
+{
+    switch (s) {
+    case 1: i = k; j = 0; break;
+    case 2: i = N; j = k; break;
+    case 3: i = k; j = N; break;
+    default: i = 0; j = k; break;
+    }
+}
+
+constexpr int kEdgeMapC[2][4] = {{1, 2, 4, 1}, {2, 3, 3, 4}};
+constexpr int kCornerI[4] = {0, 1, 1, 0}, kCornerJ[4] = {0, 0, 1, 1};
+
+// Build gid/copy maps with the reference's ownership and GlobalSum order.
+int build_maps(CgsemOp &op, int nEdges, int nNodes, const int *edges, const int *elemNodes)
+{
+    const int N = op.N, n1 = op.n1, np = op.np;
+    const size_t nE = op.nE;
+    std::vector<std::vector<int>> copies;  // per global DOF: local ids in summation order
+    std::vector<int> owner(nE * np, -1);   // local id of the owner copy, or -1 (Dirichlet)
+    op.dirH.assign(nE * np, 0);
+    // node -> element list (increasing element order) and Dirichlet nodes
+    std::vector<std::vector<std::pair<int, int>>> nodeList(nNodes);
+    for (size_t e = 0; e < nE; ++e)
+        for (int k = 0; k < 4; ++k) {
+            const int nid = elemNodes[4 * e + k];
+            SELF_REQUIRE(nid >= 1 && nid <= nNodes, SELF_ERR_ARG, "element %zu corner %d: node %d", e + 1, k + 1, nid);
+            nodeList[nid - 1].push_back({(int)e, k});
+        }
+    std::vector<char> dirNode(nNodes, 0);
+    std::vector<int> sideEdge(nE * 4, -1);
+    for (int ie = 0; ie < nEdges; ++ie) {
+        const int *ed = edges + 6 * ie;
+        const int e1 = ed[0] - 1, s1 = ed[1];
+        SELF_REQUIRE(e1 >= 0 && (size_t)e1 < nE && s1 >= 1 && s1 <= 4, SELF_ERR_ARG, "edge %d: bad primary", ie + 1);
+        sideEdge[e1 * 4 + s1 - 1] = ie;
+        if (ed[2] > 0) {
+            SELF_REQUIRE((size_t)(ed[2] - 1) < nE && std::abs(ed[3]) >= 1 && std::abs(ed[3]) <= 4, SELF_ERR_ARG,
+                         "edge %d: bad secondary", ie + 1);
+            SELF_REQUIRE(ed[5] == 1 || ed[5] == -1, SELF_ERR_ARG, "edge %d: increment %d", ie + 1, ed[5]);
+            sideEdge[(ed[2] - 1) * 4 + std::abs(ed[3]) - 1] = ie;
+        } else {  // boundary edge: Dirichlet (BuildQuadMesh_CGsemElliptic_2D :207-214)
+            dirNode[elemNodes[4 * e1 + kEdgeMapC[0][s1 - 1] - 1] - 1] = 1;
+            dirNode[elemNodes[4 * e1 + kEdgeMapC[1][s1 - 1] - 1] - 1] = 1;
+        }
+    }
+    // interior nodes
+    std::vector<int> dofOfLocal(nE * np, -1);
+    std::vector<int> ownerOfDof;
+    auto new_dof = [&](std::vector<int> &&cp) {
+        const int d = (int)copies.size();
+        copies.push_back(std::move(cp));
+        return d;
+    };
+    // assign DOF identities (not yet numbered): edges
+    std::vector<int> dofKey(nE * np, -1);
+    for (size_t e = 0; e < nE; ++e)
+        for (int j = 1; j < N; ++j)
+            for (int i = 1; i < N; ++i) {
+                const int l = (int)e * np + i + n1 * j;
+                dofKey[l] = new_dof({l});
+            }
+    for (int ie = 0; ie < nEdges; ++ie) {
+        const int *ed = edges + 6 * ie;
+        const int e1 = ed[0] - 1, s1 = ed[1];
+        if (ed[2] <= 0) {
+            for (int k = 1; k < N; ++k) {
+                int i, j;
+                side_ij(s1, k, N, i, j);
+                op.dirH[e1 * np + i + n1 * j] = 1;
+            }
+            continue;
+        }
+        const int e2 = ed[2] - 1, s2 = std::abs(ed[3]);
+        int nn = ed[4];
+        for (int k = 1; k < N; ++k) {
+            int i1, j1, i2, j2;
+            side_ij(s1, k, N, i1, j1);
+            SELF_REQUIRE(nn >= 1 && nn <= N - 1, SELF_ERR_ARG, "edge %d: start/inc leave the side", ie + 1);
+            side_ij(s2, nn, N, i2, j2);
+            const int l1 = e1 * np + i1 + n1 * j1, l2 = e2 * np + i2 + n1 * j2;
+            const int d = new_dof({l1, l2});
+            dofKey[l1] = d;
+            dofKey[l2] = d;
+            nn += ed[5];
+        }
+    }
+    for (int nd = 0; nd < nNodes; ++nd) {
+        if (nodeList[nd].empty()) continue;
+        if (dirNode[nd]) {
+            for (auto &pe : nodeList[nd])
+                op.dirH[pe.first * np + kCornerI[pe.second] * N + n1 * (kCornerJ[pe.second] * N)] = 1;
+            continue;
+        }
+        std::vector<int> cp;
+        for (auto &pe : nodeList[nd]) cp.push_back(pe.first * np + kCornerI[pe.second] * N + n1 * (kCornerJ[pe.second] * N));
+        const int d = new_dof(std::move(cp));
+        for (int l : copies[d]) dofKey[l] = d;
+    }
+    for (size_t l = 0; l < nE * (size_t)np; ++l)
+        SELF_REQUIRE(dofKey[l] >= 0 || op.dirH[l], SELF_ERR_ARG,
+                     "local node %zu of element %zu is on no edge (mesh tables incomplete)", l % np, l / np + 1);
+    // number DOFs in the order their owner copies appear, element by element
+    std::vector<int> number(copies.size(), -1);
+    int next = 0;
+    for (size_t l = 0; l < nE * (size_t)np; ++l) {
+        const int d = dofKey[l];
+        if (d >= 0 && copies[d][0] == (int)l) number[d] = next++;
+    }
+    op.nG = (size_t)next;
+    op.gidH.assign(nE * np, -1);
+    for (size_t l = 0; l < nE * (size_t)np; ++l)
+        if (dofKey[l] >= 0) op.gidH[l] = number[dofKey[l]];
+    std::vector<int> order(copies.size());
+    for (size_t d = 0; d < copies.size(); ++d) order[number[d]] = (int)d;
+    op.csH.assign(op.nG + 1, 0);
+    op.ciH.clear();
+    for (size_t g = 0; g < op.nG; ++g) {
+        for (int l : copies[order[g]]) op.ciH.push_back(l);
+        op.csH[g + 1] = (int)op.ciH.size();
+    }
+    return SELF_OK;
+}
+
+}  // namespace
+}  // namespace self
+
+using namespace self;
+
+struct self_cgsem2d_s {
+    CgsemOp op;
+    KrylovWork wk;
+    int device = 0;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    std::vector<double> resi;
+};
+
+namespace {
+
+int upload_geometry(self_cgsem2d_s *h)
+{
+    CgsemOp &op = h->op;
+    const int n1 = op.n1, np = op.np;
+    const size_t nl = op.nLocal;
+    std::vector<double> G(3 * nl);
+    std::vector<float> Gf(3 * nl);
+    std::vector<double> diagLocal(nl);
+    for (size_t e = 0; e < op.nE; ++e)
+        for (int j = 0; j < n1; ++j)
+            for (int i = 0; i < n1; ++i) {
+                const size_t l = e * np + i + n1 * j;
+                const double dxds = op.met[0][l], dxdp = op.met[1][l], dyds = op.met[2][l], dydp = op.met[3][l];
+                const double J = op.met[4][l], k = op.kH[l];
+                const double c = k * op.wH[i] * op.wH[j] / J;
+                G[l] = c * (dydp * dydp + dxdp * dxdp);
+                G[nl + l] = -c * (dyds * dydp + dxds * dxdp);
+                G[2 * nl + l] = c * (dxds * dxds + dyds * dyds);
+            }
+    for (size_t q = 0; q < 3 * nl; ++q) Gf[q] = (float)G[q];
+    // element diagonal of the operator:
+    //  A(l,l) = -( sum_m G11(m,j) D(m,i)^2 + 2 G12(i,j) D(i,i) D(j,j) + sum_m G22(i,m) D(m,j)^2 )
+    const double *Dm = op.DH.data();
+    for (size_t e = 0; e < op.nE; ++e)
+        for (int j = 0; j < n1; ++j)
+            for (int i = 0; i < n1; ++i) {
+                double a = 0.0, b = 0.0;
+                for (int m = 0; m < n1; ++m) {
+                    const double dmi = Dm[m + n1 * i], dmj = Dm[m + n1 * j];
+                    a += G[e * np + m + n1 * j] * dmi * dmi;
+                    b += G[2 * nl + e * np + i + n1 * m] * dmj * dmj;
+                }
+                const size_t l = e * np + i + n1 * j;
+                diagLocal[l] = -(a + 2.0 * G[nl + l] * Dm[i + n1 * i] * Dm[j + n1 * j] + b);
+            }
+    std::vector<double> dinv(op.nG);
+    std::vector<float> dinvf(op.nG);
+    for (size_t g = 0; g < op.nG; ++g) {
+        double sum = diagLocal[op.ciH[op.csH[g]]];
+        for (int c = op.csH[g] + 1; c < op.csH[g + 1]; ++c) sum += diagLocal[op.ciH[c]];
+        dinv[g] = 1.0 / sum;
+        dinvf[g] = (float)dinv[g];
+    }
+    SELF_HIP(hipMemcpy(op.G3, G.data(), sizeof(double) * G.size(), hipMemcpyHostToDevice));
+    SELF_HIP(hipMemcpy(op.G3f, Gf.data(), sizeof(float) * Gf.size(), hipMemcpyHostToDevice));
+    SELF_HIP(hipMemcpy(op.dinv, dinv.data(), sizeof(double) * op.nG, hipMemcpyHostToDevice));
+    SELF_HIP(hipMemcpy(op.dinvf, dinvf.data(), sizeof(float) * op.nG, hipMemcpyHostToDevice));
+    return SELF_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int self_cgsem2d_create(int N, int nElems, int nEdges, int nNodes, const int *edges, const int *elemNodes,
+                        const double *dxds, const double *dxdp, const double *dyds, const double *dydp,
+                        const double *J, self_cgsem2d_t *out)
+try {
+    SELF_REQUIRE(out && edges && elemNodes && dxds && dxdp && dyds && dydp && J, SELF_ERR_ARG,
+                 "self_cgsem2d_create: null argument");
+    SELF_REQUIRE(N >= 2 && N <= 11 && nElems >= 1 && nEdges >= 1 && nNodes >= 1, SELF_ERR_ARG,
+                 "self_cgsem2d_create: N=%d nElems=%d", N, nElems);
+    auto *h = new self_cgsem2d_s();
+    CgsemOp &op = h->op;
+    op.N = N;
+    op.n1 = N + 1;
+    op.np = op.n1 * op.n1;
+    op.nE = (size_t)nElems;
+    op.nLocal = op.nE * op.np;
+    auto fail = [&](int rc) { delete h; return rc; };
+    hipError_t he = hipGetDevice(&h->device);
+    if (he != hipSuccess) return fail(hip_fail(he, "hipGetDevice", __FILE__, __LINE__));
+    he = hipStreamCreateWithFlags(&op.stream, hipStreamNonBlocking);
+    if (he != hipSuccess) return fail(hip_fail(he, "hipStreamCreate", __FILE__, __LINE__));
+    // GLL nodes/weights and the CG derivative matrix (NodalStorage_2D, CG form)
+    std::vector<double> x(op.n1), bw(op.n1);
+    op.wH.resize(op.n1);
+    op.DH.resize(op.np);
+    int rc = self_legendre_quadrature(N, SELF_GAUSS_LOBATTO, x.data(), op.wH.data());
+    if (!rc) rc = self_barycentric_weights(N, x.data(), bw.data());
+    if (!rc) rc = self_derivative_matrix(N, x.data(), bw.data(), op.DH.data());
+    if (rc) return fail(rc);
+    const double *src[5] = {dxds, dxdp, dyds, dydp, J};
+    for (int k = 0; k < 5; ++k) op.met[k].assign(src[k], src[k] + op.nLocal);
+    op.kH.assign(op.nLocal, 1.0);  // Build_CGsemElliptic_2D: fluxCoeff = ONE
+    if ((rc = build_maps(op, nEdges, nNodes, edges, elemNodes))) return fail(rc);
+    if ((rc = op.dev(&op.gid, op.nLocal)) || (rc = op.dev(&op.cs, op.nG + 1)) || (rc = op.dev(&op.ci, op.ciH.size())) ||
+        (rc = op.dev(&op.G3, 3 * op.nLocal)) || (rc = op.dev(&op.G3f, 3 * op.nLocal)) ||
+        (rc = op.dev(&op.D, op.np)) || (rc = op.dev(&op.Df, op.np)) || (rc = op.dev(&op.bl, op.nLocal)) ||
+        (rc = op.dev(&op.uD, op.nLocal)) || (rc = op.dev(&op.dinv, op.nG)) || (rc = op.dev(&op.dinvf, op.nG)) ||
+        (rc = op.dev(&op.Lbuf, op.nLocal)) || (rc = op.dev(&op.Lbuf32, op.nLocal)) || (rc = op.dev(&op.xg, op.nG)))
+        return fail(rc);
+    op.n = op.nG;
+    std::vector<float> Df(op.np);
+    for (int k = 0; k < op.np; ++k) Df[k] = (float)op.DH[k];
+    he = hipMemcpy(op.gid, op.gidH.data(), sizeof(int) * op.nLocal, hipMemcpyHostToDevice);
+    if (he == hipSuccess) he = hipMemcpy(op.cs, op.csH.data(), sizeof(int) * (op.nG + 1), hipMemcpyHostToDevice);
+    if (he == hipSuccess) he = hipMemcpy(op.ci, op.ciH.data(), sizeof(int) * op.ciH.size(), hipMemcpyHostToDevice);
+    if (he == hipSuccess) he = hipMemcpy(op.D, op.DH.data(), sizeof(double) * op.np, hipMemcpyHostToDevice);
+    if (he == hipSuccess) he = hipMemcpy(op.Df, Df.data(), sizeof(float) * op.np, hipMemcpyHostToDevice);
+    if (he == hipSuccess) he = hipMemset(op.bl, 0, sizeof(double) * op.nLocal);
+    if (he == hipSuccess) he = hipMemset(op.uD, 0, sizeof(double) * op.nLocal);
+    if (he == hipSuccess) he = hipMemset(op.xg, 0, sizeof(double) * op.nG);
+    if (he == hipSuccess) he = hipEventCreate(&h->ev0);
+    if (he == hipSuccess) he = hipEventCreate(&h->ev1);
+    if (he != hipSuccess) return fail(hip_fail(he, "cgsem2d upload", __FILE__, __LINE__));
+    if ((rc = upload_geometry(h))) return fail(rc);
+    *out = h;
+    return SELF_OK;
+}
+SELF_CATCH_ALL
+
+int self_cgsem2d_destroy(self_cgsem2d_t h)
+{
+    if (!h) return SELF_OK;
+    (void)hipSetDevice(h->device);
+    if (h->op.stream) (void)hipStreamSynchronize(h->op.stream);
+    h->wk.release();
+    if (h->ev0) (void)hipEventDestroy(h->ev0);
+    if (h->ev1) (void)hipEventDestroy(h->ev1);
+    delete h;
+    return SELF_OK;
+}
+
+int self_cgsem2d_global_dofs(self_cgsem2d_t h, int *nGlobal)
+{
+    SELF_REQUIRE(h && nGlobal, SELF_ERR_ARG, "null argument");
+    *nGlobal = (int)h->op.nG;
+    return SELF_OK;
+}
+
+int self_cgsem2d_set_flux_coefficient(self_cgsem2d_t h, const double *k)
+try {
+    SELF_REQUIRE(h && k, SELF_ERR_ARG, "null argument");
+    SELF_HIP(hipSetDevice(h->device));
+    h->op.kH.assign(k, k + h->op.nLocal);
+    return upload_geometry(h);
+}
+SELF_CATCH_ALL
+
+int self_cgsem2d_set_source(self_cgsem2d_t h, const double *s)
+try {
+    SELF_REQUIRE(h && s, SELF_ERR_ARG, "null argument");
+    SELF_HIP(hipSetDevice(h->device));
+    CgsemOp &op = h->op;
+    std::vector<double> b(op.nLocal);
+    for (size_t e = 0; e < op.nE; ++e)
+        for (int j = 0; j < op.n1; ++j)
+            for (int i = 0; i < op.n1; ++i) {
+                const size_t l = e * op.np + i + op.n1 * j;
+                b[l] = s[l] * op.met[4][l] * op.wH[i] * op.wH[j];  // source*J*ws*wp (:1052)
+            }
+    SELF_HIP(hipMemcpy(op.bl, b.data(), sizeof(double) * op.nLocal, hipMemcpyHostToDevice));
+    return SELF_OK;
+}
+SELF_CATCH_ALL
+
+int self_cgsem2d_set_dirichlet(self_cgsem2d_t h, const double *values)
+try {
+    SELF_REQUIRE(h && values, SELF_ERR_ARG, "null argument");
+    SELF_HIP(hipSetDevice(h->device));
+    CgsemOp &op = h->op;
+    std::vector<double> u(op.nLocal, 0.0);
+    for (size_t l = 0; l < op.nLocal; ++l)
+        if (op.dirH[l]) u[l] = values[l];
+    SELF_HIP(hipMemcpy(op.uD, u.data(), sizeof(double) * op.nLocal, hipMemcpyHostToDevice));
+    return SELF_OK;
+}
+SELF_CATCH_ALL
+
+// solution in the reference's local layout: owner copies are taken
+int self_cgsem2d_set_solution(self_cgsem2d_t h, const double *u)
+try {
+    SELF_REQUIRE(h && u, SELF_ERR_ARG, "null argument");
+    SELF_HIP(hipSetDevice(h->device));
+    CgsemOp &op = h->op;
+    std::vector<double> x(op.nG);
+    for (size_t g = 0; g < op.nG; ++g) x[g] = u[op.ciH[op.csH[g]]];
+    SELF_HIP(hipMemcpy(op.xg, x.data(), sizeof(double) * op.nG, hipMemcpyHostToDevice));
+    return SELF_OK;
+}
+SELF_CATCH_ALL
+
+// UnMask + SetDirichletBoundaryCondition: every copy holds its DOF's value
+int self_cgsem2d_get_solution(self_cgsem2d_t h, double *u)
+try {
+    SELF_REQUIRE(h && u, SELF_ERR_ARG, "null argument");
+    SELF_HIP(hipSetDevice(h->device));
+    CgsemOp &op = h->op;
+    SELF_HIP(hipStreamSynchronize(op.stream));
+    std::vector<double> x(op.nG), d(op.nLocal);
+    SELF_HIP(hipMemcpy(x.data(), op.xg, sizeof(double) * op.nG, hipMemcpyDeviceToHost));
+    SELF_HIP(hipMemcpy(d.data(), op.uD, sizeof(double) * op.nLocal, hipMemcpyDeviceToHost));
+    for (size_t l = 0; l < op.nLocal; ++l) u[l] = op.gidH[l] >= 0 ? x[op.gidH[l]] : d[l];
+    return SELF_OK;
+}
+SELF_CATCH_ALL
+
+// MatrixAction on a masked local array (host in, masked local out).
+int self_cgsem2d_matrix_action(self_cgsem2d_t h, const double *u, double *Au)
+try {
+    SELF_REQUIRE(h && u && Au, SELF_ERR_ARG, "null argument");
+    SELF_HIP(hipSetDevice(h->device));
+    CgsemOp &op = h->op;
+    int rc = h->wk.ensure(op.nG, 0, 1, false);
+    if (rc) return rc;
+    std::vector<double> x(op.nG), y(op.nG);
+    for (size_t g = 0; g < op.nG; ++g) x[g] = u[op.ciH[op.csH[g]]];
+    SELF_HIP(hipMemcpy(h->wk.v, x.data(), sizeof(double) * op.nG, hipMemcpyHostToDevice));
+    if ((rc = op.matvec(h->wk.v, h->wk.z, nullptr, nullptr, kFinNone, h->wk.slot, op.stream))) return rc;
+    SELF_HIP(hipStreamSynchronize(op.stream));
+    SELF_HIP(hipMemcpy(y.data(), h->wk.z, sizeof(double) * op.nG, hipMemcpyDeviceToHost));
+    std::memset(Au, 0, sizeof(double) * op.nLocal);
+    for (size_t g = 0; g < op.nG; ++g) Au[op.ciH[op.csH[g]]] = y[g];
+    return SELF_OK;
+}
+SELF_CATCH_ALL
+
+// Residual of the current solution, masked local layout.
+int self_cgsem2d_residual(self_cgsem2d_t h, double *r)
+try {
+    SELF_REQUIRE(h && r, SELF_ERR_ARG, "null argument");
+    SELF_HIP(hipSetDevice(h->device));
+    CgsemOp &op = h->op;
+    int rc = h->wk.ensure(op.nG, 0, 1, false);
+    if (rc) return rc;
+    if ((rc = op.residual(op.xg, h->wk.r, nullptr, kFinNone, h->wk.slot, op.stream))) return rc;
+    SELF_HIP(hipStreamSynchronize(op.stream));
+    std::vector<double> y(op.nG);
+    SELF_HIP(hipMemcpy(y.data(), h->wk.r, sizeof(double) * op.nG, hipMemcpyDeviceToHost));
+    std::memset(r, 0, sizeof(double) * op.nLocal);
+    for (size_t g = 0; g < op.nG; ++g) r[op.ciH[op.csH[g]]] = y[g];
+    return SELF_OK;
+}
+SELF_CATCH_ALL
+
+int self_cgsem2d_solve(self_cgsem2d_t h, int method, int precond, double tol, int maxIters, int mInner, int ortho,
+                       int *ioerr, int *iterations, double *resi, int resiLen, double *elapsed_ms)
+try {
+    SELF_REQUIRE(h && ioerr && iterations, SELF_ERR_ARG, "null argument");
+    SELF_REQUIRE(maxIters >= 1 && tol > 0, SELF_ERR_ARG, "maxIters/tolerance");
+    SELF_REQUIRE(method >= 0 && method <= 4, SELF_ERR_ARG, "unknown method %d", method);
+    SELF_REQUIRE(!(method == 2 || method == 3) || (mInner >= 1 && mInner <= 200), SELF_ERR_ARG, "GMRES m = %d", mInner);
+    SELF_HIP(hipSetDevice(h->device));
+    CgsemOp &op = h->op;
+    KrylovResult res;
+    SELF_HIP(hipEventRecord(h->ev0, op.stream));
+    int rc = krylov_solve(op, h->wk, method, precond, tol, maxIters, mInner, ortho, op.xg, resi, resiLen, res,
+                          op.stream);
+    if (rc) return rc;
+    SELF_HIP(hipEventRecord(h->ev1, op.stream));
+    SELF_HIP(hipEventSynchronize(h->ev1));
+    if (elapsed_ms) {
+        float ms = 0;
+        SELF_HIP(hipEventElapsedTime(&ms, h->ev0, h->ev1));
+        *elapsed_ms = ms;
+    }
+    *ioerr = res.ioerr;
+    *iterations = res.iterations;
+    return SELF_OK;
+}
+SELF_CATCH_ALL
+
+int self_cgsem2d_solve_mixed(self_cgsem2d_t h, int precond, double tol, double innerTol, int maxInner, int maxOuter,
+                             int *ioerr, int *outerIters, int *innerIters, double *relResidual, double *elapsed_ms)
+try {
+    SELF_REQUIRE(h && ioerr && outerIters && innerIters, SELF_ERR_ARG, "null argument");
+    SELF_HIP(hipSetDevice(h->device));
+    CgsemOp &op = h->op;
+    KrylovResult res;
+    int inner = 0;
+    SELF_HIP(hipEventRecord(h->ev0, op.stream));
+    int rc = krylov_solve_ir(op, h->wk, precond, tol, innerTol, maxInner, maxOuter, op.xg, res, inner, op.stream);
+    if (rc) return rc;
+    SELF_HIP(hipEventRecord(h->ev1, op.stream));
+    SELF_HIP(hipEventSynchronize(h->ev1));
+    if (elapsed_ms) {
+        float ms = 0;
+        SELF_HIP(hipEventElapsedTime(&ms, h->ev0, h->ev1));
+        *elapsed_ms = ms;
+    }
+    *ioerr = res.ioerr;
+    *outerIters = res.iterations;
+    *innerIters = inner;
+    if (relResidual) *relResidual = res.r0 > 0 ? res.rNorm / res.r0 : 0.0;
+    return SELF_OK;
+}
+SELF_CATCH_ALL
+
+}  // extern "C"

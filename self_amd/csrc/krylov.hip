@@ -1,0 +1,612 @@
+// krylov.hip -- device-resident Krylov solvers with the reference's control
+// flow and stop tests:
+//   Solve_ConjugateGradient   src/iterativesolve/IterativeSolvers_Class.f90:328-405
+//   Solve_PCConjugateGradient                                            :409-489
+//   Solve_GMRES (MGS, progressive Givens, back substitution)             :493-652
+//   Solve_PCGMRES (flexible, right preconditioned)                       :656-822
+//   CGsemElliptic_2D PCG       src/cgsem/CGsemElliptic_2D_Class.f90:1114-1207
+// Every vector and scalar lives on the device.  Dot products are fused into
+// the vector kernels that produce their operands and reduced
+// deterministically (reduce.hpp); the scalar recurrences (alpha, beta,
+// Givens rotations, the stop tests) run in the last workgroup of the
+// reduction or in a one-thread kernel.  A `done` flag in the device state
+// turns the kernels of iterations after convergence into no-ops, so the
+// host launches iterations in chunks and reads the flag between chunks.
+
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "krylov.hpp"
+
+namespace self {
+
+namespace {
+
+__device__ __forceinline__ bool st_done(const SolverState *st) { return st->done != 0; }
+
+template <int K>
+__device__ __forceinline__ void reduce_publish(const RedSlot &slot, double (&acc)[K], SolverState *st,
+                                               const int (&fin)[K])
+{
+    __shared__ double lds4[4];
+    double part[K];
+    #pragma unroll
+    for (int k = 0; k < K; ++k) part[k] = block_sum(acc[k], lds4);
+    if (publish_partials<K>(slot, part)) {
+        #pragma unroll
+        for (int k = 0; k < K; ++k) {
+            const double S = final_sum(slot, k, lds4);
+            if (threadIdx.x == 0) finalize_scalar(st, fin[k], S);
+        }
+        if (threadIdx.x == 0) *slot.ticket = 0u;
+    }
+}
+
+// a.b with finalize
+template <typename T>
+__global__ __launch_bounds__(kRedThreads) void k_dot(const T *__restrict__ a, const T *__restrict__ b, size_t n,
+                                                     SolverState *st, int fin, RedSlot slot)
+{
+    if (st_done(st)) return;
+    double acc[1] = {0.0};
+    for (size_t g = (size_t)blockIdx.x * blockDim.x + threadIdx.x; g < n; g += (size_t)gridDim.x * blockDim.x)
+        acc[0] += (double)a[g] * (double)b[g];
+    const int f[1] = {fin};
+    reduce_publish<1>(slot, acc, st, f);
+}
+
+// w = P r (Jacobi or identity), num = r.w
+template <typename T>
+__global__ __launch_bounds__(kRedThreads) void k_precond_dot(const T *__restrict__ r, const T *__restrict__ dinv,
+                                                             T *__restrict__ w, size_t n, SolverState *st, int fin,
+                                                             RedSlot slot)
+{
+    if (st_done(st)) return;
+    double acc[1] = {0.0};
+    for (size_t g = (size_t)blockIdx.x * blockDim.x + threadIdx.x; g < n; g += (size_t)gridDim.x * blockDim.x) {
+        const T rv = r[g];
+        const T wv = dinv ? dinv[g] * rv : rv;
+        w[g] = wv;
+        acc[0] += (double)rv * (double)wv;
+    }
+    const int f[1] = {fin};
+    reduce_publish<1>(slot, acc, st, f);
+}
+
+// v = w (first iteration) or w + beta v
+template <typename T>
+__global__ __launch_bounds__(256) void k_update_v(const T *__restrict__ w, T *__restrict__ v, size_t n,
+                                                  const SolverState *st)
+{
+    if (st_done(st)) return;
+    const bool first = st->iter == 1;
+    const T beta = (T)st->beta;
+    for (size_t g = (size_t)blockIdx.x * blockDim.x + threadIdx.x; g < n; g += (size_t)gridDim.x * blockDim.x)
+        v[g] = first ? w[g] : w[g] + beta * v[g];
+}
+
+// u += alpha v; r -= alpha z; |r|^2 -> rNorm, stop test
+template <typename T>
+__global__ __launch_bounds__(kRedThreads) void k_update_ur(T *__restrict__ u, const T *__restrict__ v,
+                                                           T *__restrict__ r, const T *__restrict__ z, size_t n,
+                                                           SolverState *st, RedSlot slot)
+{
+    if (st_done(st)) return;
+    const T alpha = (T)st->alpha;
+    double acc[1] = {0.0};
+    for (size_t g = (size_t)blockIdx.x * blockDim.x + threadIdx.x; g < n; g += (size_t)gridDim.x * blockDim.x) {
+        u[g] = u[g] + alpha * v[g];
+        const T rv = r[g] - alpha * z[g];
+        r[g] = rv;
+        acc[0] += (double)rv * (double)rv;
+    }
+    const int f[1] = {kFinRnorm};
+    reduce_publish<1>(slot, acc, st, f);
+}
+
+// ---------------------------------------------------------------- GMRES
+#define GH(st, a, b) (st)->H[((a)-1) + ((st)->m + 1) * ((b)-1)]
+#define GRHO(st, a, b) (st)->rho[((a)-1) + (st)->m * ((b)-1)]
+
+__device__ __forceinline__ bool almost_equal_dev(double a, double b)
+{
+    const double d = fabs(a - b), eps = 2.220446049250313e-16;
+    if (a == 0.0 || b == 0.0) return d <= eps;
+    return d <= eps * fabs(a) || d <= eps * fabs(b);
+}
+
+__device__ __forceinline__ bool inner_off(const SolverState *st) { return st->done != 0 || st->innerExit != 0; }
+
+// b = |r|, v1 = r / b happens in k_gmres_scale after this finalize
+__global__ void k_gmres_cycle_start(SolverState *st)
+{
+    if (threadIdx.x != 0 || st->done) return;
+    st->innerExit = 0;
+    st->b = st->bnorm;
+    st->bhat[0] = st->b;
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_scale(const T *__restrict__ src, T *__restrict__ dst, size_t n,
+                                               const SolverState *st, int which, int col)
+{
+    if (inner_off(st)) return;
+    // which 0: dst = src / b ; which 1: dst = src / H(col+1, col)
+    const double d = which == 0 ? st->b : GH(st, col + 1, col);
+    for (size_t g = (size_t)blockIdx.x * blockDim.x + threadIdx.x; g < n; g += (size_t)gridDim.x * blockDim.x)
+        dst[g] = (T)((double)src[g] / d);
+}
+
+__global__ void k_gmres_begin(SolverState *st, int i)
+{
+    if (threadIdx.x != 0 || inner_off(st)) return;
+    st->l += 1;
+    st->nr = i;
+}
+
+// H(k,i) = v_k . w
+__global__ __launch_bounds__(kRedThreads) void k_gmres_h(const double *__restrict__ vk, const double *__restrict__ w,
+                                                         size_t n, SolverState *st, int k, int i, RedSlot slot)
+{
+    if (inner_off(st)) return;
+    double acc = 0.0;
+    for (size_t g = (size_t)blockIdx.x * blockDim.x + threadIdx.x; g < n; g += (size_t)gridDim.x * blockDim.x)
+        acc += vk[g] * w[g];
+    __shared__ double lds4[4];
+    double part[1] = {block_sum(acc, lds4)};
+    if (publish_partials<1>(slot, part)) {
+        const double S = final_sum(slot, 0, lds4);
+        if (threadIdx.x == 0) {
+            if (k <= i) GH(st, k, i) = S;
+            else GH(st, i + 1, i) = sqrt(S);  // k = i+1: the norm of w
+            *slot.ticket = 0u;
+        }
+    }
+}
+
+// w -= H(k,i) v_k
+__global__ __launch_bounds__(256) void k_gmres_axpy(double *__restrict__ w, const double *__restrict__ vk, size_t n,
+                                                    const SolverState *st, int k, int i)
+{
+    if (inner_off(st)) return;
+    const double h = GH(st, k, i);
+    for (size_t g = (size_t)blockIdx.x * blockDim.x + threadIdx.x; g < n; g += (size_t)gridDim.x * blockDim.x)
+        w[g] = w[g] - h * vk[g];
+}
+
+// CGS2 pass: h_k = v_k . w for k = 1..i in one sweep (partials per k)
+__global__ __launch_bounds__(kRedThreads) void k_gmres_multidot(const double *__restrict__ V, const double *__restrict__ w,
+                                                                size_t n, int i, SolverState *st, double *hOut,
+                                                                RedSlot slot)
+{
+    if (inner_off(st)) return;
+    __shared__ double lds4[4];
+    __shared__ bool last;
+    for (int k = 0; k < i; ++k) {
+        double acc = 0.0;
+        const double *vk = V + (size_t)k * n;
+        for (size_t g = (size_t)blockIdx.x * blockDim.x + threadIdx.x; g < n; g += (size_t)gridDim.x * blockDim.x)
+            acc += vk[g] * w[g];
+        const double p = block_sum(acc, lds4);
+        if (threadIdx.x == 0) slot.partials[k * kRedBlocks + blockIdx.x] = p;
+    }
+    if (threadIdx.x == 0) {
+        __threadfence();
+        last = atomicAdd(slot.ticket, 1u) == gridDim.x - 1;
+    }
+    __syncthreads();
+    if (!last) return;
+    __threadfence();
+    for (int k = 0; k < i; ++k) {
+        const double S = final_sum(slot, k, lds4);
+        if (threadIdx.x == 0) hOut[k] = S;
+    }
+    if (threadIdx.x == 0) *slot.ticket = 0u;
+}
+
+// w -= V h  (CGS2), and accumulate h into H(:,i) when acc != 0
+__global__ __launch_bounds__(256) void k_gmres_multiaxpy(double *__restrict__ w, const double *__restrict__ V,
+                                                         size_t n, int i, SolverState *st, const double *h,
+                                                         int accumulate)
+{
+    if (inner_off(st)) return;
+    for (size_t g = (size_t)blockIdx.x * blockDim.x + threadIdx.x; g < n; g += (size_t)gridDim.x * blockDim.x) {
+        double acc = w[g];
+        for (int k = 0; k < i; ++k) acc -= h[k] * V[(size_t)k * n + g];
+        w[g] = acc;
+    }
+    if (accumulate && blockIdx.x == 0 && threadIdx.x < i) {
+        const int k = threadIdx.x;
+        GH(st, k + 1, i) = (accumulate == 1) ? h[k] : GH(st, k + 1, i) + h[k];
+    }
+}
+
+// breakdown test, progressive Givens rotation and the inner stop test
+__global__ void k_gmres_givens(SolverState *st, int i)
+{
+    if (threadIdx.x != 0 || inner_off(st)) return;
+    if (almost_equal_dev(GH(st, i + 1, i), 0.0)) {  // :568-570
+        st->innerExit = 2;
+        return;
+    }
+    GRHO(st, 1, i) = GH(st, 1, i);
+    for (int k = 2; k <= i; ++k) {
+        const double g = st->c[k - 2] * GRHO(st, k - 1, i) + st->s[k - 2] * GH(st, k, i);
+        GRHO(st, k, i) = -st->s[k - 2] * GRHO(st, k - 1, i) + st->c[k - 2] * GH(st, k, i);
+        GRHO(st, k - 1, i) = g;
+    }
+    const double d = sqrt(GRHO(st, i, i) * GRHO(st, i, i) + GH(st, i + 1, i) * GH(st, i + 1, i));
+    st->c[i - 1] = GRHO(st, i, i) / d;
+    st->s[i - 1] = GH(st, i + 1, i) / d;
+    GRHO(st, i, i) = st->c[i - 1] * GRHO(st, i, i) + st->s[i - 1] * GH(st, i + 1, i);
+    st->bhat[i] = -st->s[i - 1] * st->bhat[i - 1];
+    st->bhat[i - 1] = st->c[i - 1] * st->bhat[i - 1];
+    st->rc = fabs(st->bhat[i]);
+    if (st->l < st->resiLen) st->resi[st->l] = st->rc;
+    if (st->rc / st->r0 <= st->tol) st->innerExit = 1;
+}
+
+// v_{i+1} = w / H(i+1,i)
+__global__ __launch_bounds__(256) void k_gmres_newv(const double *__restrict__ w, double *__restrict__ vnext, size_t n,
+                                                    const SolverState *st, int i)
+{
+    if (st->done || st->innerExit == 2) return;  // breakdown exits before this (:568-572)
+    if (st->innerExit == 1) return;                // converged: v_{i+1} is never used
+    const double d = GH(st, i + 1, i);
+    for (size_t g = (size_t)blockIdx.x * blockDim.x + threadIdx.x; g < n; g += (size_t)gridDim.x * blockDim.x)
+        vnext[g] = w[g] / d;
+}
+
+// back substitution (:606-626)
+__global__ void k_gmres_backsub(SolverState *st)
+{
+    if (threadIdx.x != 0 || st->done) return;
+    if (st->rc / st->r0 > st->tol) st->nr = st->m;
+    const int nr = st->nr;
+    st->y[nr - 1] = st->bhat[nr - 1] / GRHO(st, nr, nr);
+    for (int k = nr - 1; k >= 1; --k) {
+        double yk = st->bhat[k - 1];
+        for (int i = k + 1; i <= nr; ++i) yk = yk - GRHO(st, k, i) * st->y[i - 1];
+        st->y[k - 1] = yk / GRHO(st, k, k);
+    }
+}
+
+// sol += basis(:,1:nr) y ; sets the convergence flag afterwards
+__global__ __launch_bounds__(256) void k_gmres_update(double *__restrict__ sol, const double *__restrict__ basis,
+                                                      size_t n, SolverState *st)
+{
+    if (st->done) return;
+    const int nr = st->nr;
+    for (size_t g = (size_t)blockIdx.x * blockDim.x + threadIdx.x; g < n; g += (size_t)gridDim.x * blockDim.x) {
+        double acc = 0.0;
+        for (int k = 0; k < nr; ++k) acc += basis[(size_t)k * n + g] * st->y[k];
+        sol[g] = sol[g] + acc;
+    }
+}
+
+__global__ void k_gmres_cycle_end(SolverState *st, int j, int nIt)
+{
+    if (threadIdx.x != 0 || st->done) return;
+    if (st->rc / st->r0 <= st->tol) { st->ioerr = 0; st->done = 1; return; }
+    if (j >= nIt) st->done = 2;
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_jacobi(const T *__restrict__ v, const T *__restrict__ dinv, T *__restrict__ z,
+                                                size_t n, const SolverState *st)
+{
+    if (inner_off(st)) return;
+    for (size_t g = (size_t)blockIdx.x * blockDim.x + threadIdx.x; g < n; g += (size_t)gridDim.x * blockDim.x)
+        z[g] = dinv ? dinv[g] * v[g] : v[g];
+}
+
+template <typename A, typename B>
+__global__ __launch_bounds__(256) void k_convert(const A *__restrict__ a, B *__restrict__ b, size_t n, double scale)
+{
+    for (size_t g = (size_t)blockIdx.x * blockDim.x + threadIdx.x; g < n; g += (size_t)gridDim.x * blockDim.x)
+        b[g] = (B)((double)a[g] * scale);
+}
+
+__global__ __launch_bounds__(256) void k_add_scaled32(double *__restrict__ x, const float *__restrict__ e, size_t n,
+                                                      double scale)
+{
+    for (size_t g = (size_t)blockIdx.x * blockDim.x + threadIdx.x; g < n; g += (size_t)gridDim.x * blockDim.x)
+        x[g] = x[g] + scale * (double)e[g];
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_fill(T *__restrict__ a, size_t n, T v)
+{
+    for (size_t g = (size_t)blockIdx.x * blockDim.x + threadIdx.x; g < n; g += (size_t)gridDim.x * blockDim.x) a[g] = v;
+}
+
+inline int vgrid(size_t n) { return (int)std::min<size_t>(4096, (n + 255) / 256 > 0 ? (n + 255) / 256 : 1); }
+
+}  // namespace
+
+// ------------------------------------------------------------- workspace
+int KrylovWork::ensure(size_t nn, int mm, int rl, bool f32)
+{
+    if (nn == n && mm <= m && rl <= resiLen && (!f32 || r32) && st) return SELF_OK;
+    release();
+    n = nn;
+    m = mm;
+    resiLen = rl;
+    auto A = [&](void **p, size_t bytesReq) -> int {
+        if (bytesReq == 0) { *p = nullptr; return SELF_OK; }
+        SELF_HIP(hipMalloc(p, bytesReq));
+        bytes += bytesReq;
+        return SELF_OK;
+    };
+    int rc = SELF_OK;
+    const size_t vb = sizeof(double) * n;
+    if ((rc = A((void **)&r, vb)) || (rc = A((void **)&v, vb)) || (rc = A((void **)&z, vb)) ||
+        (rc = A((void **)&w, vb)) || (rc = A((void **)&u, vb)))
+        return rc;
+    if (f32) {
+        const size_t fb = sizeof(float) * n;
+        if ((rc = A((void **)&r32, fb)) || (rc = A((void **)&v32, fb)) || (rc = A((void **)&z32, fb)) ||
+            (rc = A((void **)&w32, fb)) || (rc = A((void **)&u32, fb)))
+            return rc;
+    }
+    if (m > 0) {
+        if ((rc = A((void **)&V, vb * (m + 1))) || (rc = A((void **)&Z, vb * (m + 1)))) return rc;
+    }
+    const int mm1 = std::max(m, 1);
+    const size_t smallN = (size_t)(mm1 + 1) * mm1 + (size_t)mm1 * mm1 + 2 * mm1 + 2 * (mm1 + 1) + 64;
+    if ((rc = A((void **)&small, sizeof(double) * smallN))) return rc;
+    if ((rc = A((void **)&st, sizeof(SolverState)))) return rc;
+    if ((rc = A((void **)&resi, sizeof(double) * std::max(resiLen, 1)))) return rc;
+    if ((rc = A((void **)&slot.partials, sizeof(double) * kRedBlocks * std::max(32, m + 2)))) return rc;
+    if ((rc = A((void **)&slot.ticket, sizeof(unsigned)))) return rc;
+    SELF_HIP(hipMemset(slot.ticket, 0, sizeof(unsigned)));
+    SELF_HIP(hipMemset(small, 0, sizeof(double) * smallN));
+    return SELF_OK;
+}
+
+void KrylovWork::release()
+{
+    void *p[] = {r, v, z, w, u, r32, v32, z32, w32, u32, V, Z, small, st, resi, slot.partials, slot.ticket};
+    for (void *q : p)
+        if (q) (void)hipFree(q);
+    r = v = z = w = u = nullptr;
+    r32 = v32 = z32 = w32 = u32 = nullptr;
+    V = Z = small = nullptr;
+    st = nullptr;
+    resi = nullptr;
+    slot = RedSlot{};
+    n = 0;
+    m = 0;
+    resiLen = 0;
+    bytes = 0;
+}
+
+namespace {
+
+int init_state(KrylovWork &wk, double tol, int maxIters, int stopRule, int m, hipStream_t s)
+{
+    SolverState h{};
+    h.tol = tol;
+    h.iter = 1;
+    h.maxIters = maxIters;
+    h.stopRule = stopRule;
+    h.ioerr = -2;
+    h.m = m;
+    h.resi = wk.resi;
+    h.resiLen = wk.resiLen;
+    const int mm1 = std::max(m, 1);
+    double *p = wk.small;
+    h.H = p; p += (size_t)(mm1 + 1) * mm1;
+    h.rho = p; p += (size_t)mm1 * mm1;
+    h.c = p; p += mm1;
+    h.s = p; p += mm1;
+    h.bhat = p; p += mm1 + 1;
+    h.y = p;
+    SELF_HIP(hipMemcpyAsync(wk.st, &h, sizeof(SolverState), hipMemcpyHostToDevice, s));
+    SELF_HIP(hipMemsetAsync(wk.resi, 0, sizeof(double) * wk.resiLen, s));
+    return SELF_OK;
+}
+
+int read_state(KrylovWork &wk, SolverState &h, hipStream_t s)
+{
+    SELF_HIP(hipMemcpyAsync(&h, wk.st, sizeof(SolverState), hipMemcpyDeviceToHost, s));
+    SELF_HIP(hipStreamSynchronize(s));
+    return SELF_OK;
+}
+
+constexpr int kChunk = 32;  // iterations launched between flag reads
+
+// CG / PCG of IterativeSolvers (stopRule sqrt-norm) and the CGSEM PCG (norm rule)
+int run_cg(DevOp &op, KrylovWork &wk, bool precond, int stopRule, double tol, int maxIters, double *sol,
+           KrylovResult &res, hipStream_t s)
+{
+    const size_t n = op.n;
+    int rc = init_state(wk, tol, maxIters, stopRule, 0, s);
+    if (rc) return rc;
+    const int fin0 = stopRule == kStopNormRel ? kFinR0 : kFinR0Plain;
+    if ((rc = op.residual(sol, wk.r, wk.st, fin0, wk.slot, s))) return rc;
+    SELF_HIP(hipMemcpyAsync(wk.u, sol, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
+    const double *dinv = precond ? op.inv_diag() : nullptr;
+    const int gv = vgrid(n);
+    for (int it0 = 1; it0 <= maxIters; it0 += kChunk) {
+        for (int it = it0; it < it0 + kChunk && it <= maxIters; ++it) {
+            hipLaunchKernelGGL(k_precond_dot<double>, dim3(kRedBlocks), dim3(kRedThreads), 0, s, wk.r, dinv, wk.w, n,
+                               wk.st, (int)kFinNum, wk.slot);
+            hipLaunchKernelGGL(k_update_v<double>, dim3(gv), dim3(256), 0, s, wk.w, wk.v, n, wk.st);
+            if ((rc = op.matvec(wk.v, wk.z, wk.v, wk.st, kFinAlpha, wk.slot, s))) return rc;
+            hipLaunchKernelGGL(k_update_ur<double>, dim3(kRedBlocks), dim3(kRedThreads), 0, s, wk.u, wk.v, wk.r,
+                               wk.z, n, wk.st, wk.slot);
+        }
+        SELF_HIP(hipGetLastError());
+        SolverState h{};
+        if ((rc = read_state(wk, h, s))) return rc;
+        if (h.done) break;
+    }
+    SolverState h{};
+    if ((rc = read_state(wk, h, s))) return rc;
+    SELF_HIP(hipMemcpyAsync(sol, wk.u, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
+    res.r0 = h.r0;
+    res.rNorm = h.rNorm;
+    res.iterations = (h.done == 3) ? 0 : h.iter;
+    res.ioerr = -2;
+    if (stopRule == kStopSqrtNormRel) {
+        if (sqrt(h.rNorm) > tol) res.ioerr = -1;  // :399-403 (absolute test)
+    } else if (h.done != 3) {
+        if (h.rNorm / h.r0 > tol) res.ioerr = -1;  // CGsemElliptic_2D :1196-1200
+    }
+    return SELF_OK;
+}
+
+int run_gmres(DevOp &op, KrylovWork &wk, bool precond, double tol, int nIt, int m, int ortho, double *sol,
+              KrylovResult &res, hipStream_t s)
+{
+    const size_t n = op.n;
+    int rc = init_state(wk, tol, nIt, kStopNormRel, m, s);
+    if (rc) return rc;
+    if ((rc = op.residual(sol, wk.r, wk.st, kFinR0Plain, wk.slot, s))) return rc;
+    const double *dinv = precond ? op.inv_diag() : nullptr;
+    const int gv = vgrid(n);
+    double *hTmp = wk.small + (size_t)(m + 1) * m + (size_t)m * m + 2 * m + 2 * (m + 1);  // scratch for CGS2
+    for (int j = 1; j <= nIt; ++j) {
+        hipLaunchKernelGGL(k_dot<double>, dim3(kRedBlocks), dim3(kRedThreads), 0, s, wk.r, wk.r, n, wk.st,
+                           (int)kFinBnorm, wk.slot);
+        hipLaunchKernelGGL(k_gmres_cycle_start, dim3(1), dim3(64), 0, s, wk.st);
+        hipLaunchKernelGGL(k_scale<double>, dim3(gv), dim3(256), 0, s, wk.r, wk.V, n, wk.st, 0, 0);
+        for (int i = 1; i <= m; ++i) {
+            double *vi = wk.V + (size_t)(i - 1) * n;
+            hipLaunchKernelGGL(k_gmres_begin, dim3(1), dim3(64), 0, s, wk.st, i);
+            const double *xin = vi;
+            if (precond) {
+                double *zi = wk.Z + (size_t)(i - 1) * n;
+                hipLaunchKernelGGL(k_jacobi<double>, dim3(gv), dim3(256), 0, s, vi, dinv, zi, n, wk.st);
+                xin = zi;
+            }
+            // w = A x ; the matvec kernels honour done but not innerExit, so
+            // guard: they run and their result is ignored after an exit.
+            if ((rc = op.matvec(xin, wk.w, nullptr, wk.st, kFinNone, wk.slot, s))) return rc;
+            if (ortho == 0) {  // modified Gram-Schmidt, as the reference
+                for (int k = 1; k <= i; ++k) {
+                    hipLaunchKernelGGL(k_gmres_h, dim3(kRedBlocks), dim3(kRedThreads), 0, s,
+                                       (const double *)(wk.V + (size_t)(k - 1) * n), (const double *)wk.w, n, wk.st,
+                                       k, i, wk.slot);
+                    hipLaunchKernelGGL(k_gmres_axpy, dim3(gv), dim3(256), 0, s, wk.w,
+                                       (const double *)(wk.V + (size_t)(k - 1) * n), n, (const SolverState *)wk.st,
+                                       k, i);
+                }
+            } else {  // classical Gram-Schmidt applied twice: 2 sweeps over V
+                for (int pass = 0; pass < 2; ++pass) {
+                    hipLaunchKernelGGL(k_gmres_multidot, dim3(kRedBlocks), dim3(kRedThreads), 0, s,
+                                       (const double *)wk.V, (const double *)wk.w, n, i, wk.st, hTmp, wk.slot);
+                    hipLaunchKernelGGL(k_gmres_multiaxpy, dim3(gv), dim3(256), 0, s, wk.w, (const double *)wk.V, n,
+                                       i, wk.st, (const double *)hTmp, pass == 0 ? 1 : 2);
+                }
+            }
+            hipLaunchKernelGGL(k_gmres_h, dim3(kRedBlocks), dim3(kRedThreads), 0, s, (const double *)wk.w,
+                               (const double *)wk.w, n, wk.st, i + 1, i, wk.slot);
+            hipLaunchKernelGGL(k_gmres_givens, dim3(1), dim3(64), 0, s, wk.st, i);
+            if (i < m + 1)
+                hipLaunchKernelGGL(k_gmres_newv, dim3(gv), dim3(256), 0, s, (const double *)wk.w,
+                                   wk.V + (size_t)i * n, n, (const SolverState *)wk.st, i);
+        }
+        hipLaunchKernelGGL(k_gmres_backsub, dim3(1), dim3(64), 0, s, wk.st);
+        hipLaunchKernelGGL(k_gmres_update, dim3(gv), dim3(256), 0, s, sol, (const double *)(precond ? wk.Z : wk.V), n,
+                           wk.st);
+        hipLaunchKernelGGL(k_gmres_cycle_end, dim3(1), dim3(64), 0, s, wk.st, j, nIt);
+        SELF_HIP(hipGetLastError());
+        SolverState h{};
+        if ((rc = read_state(wk, h, s))) return rc;
+        if (h.done) break;
+        if ((rc = op.residual(sol, wk.r, wk.st, kFinNone, wk.slot, s))) return rc;
+    }
+    SolverState h{};
+    if ((rc = read_state(wk, h, s))) return rc;
+    res.r0 = h.r0;
+    res.rNorm = h.rc;
+    res.iterations = h.l;
+    res.ioerr = (h.rc / h.r0 <= tol) ? 0 : -1;
+    return SELF_OK;
+}
+
+}  // namespace
+
+int krylov_solve(DevOp &op, KrylovWork &wk, int method, int precond, double tol, int maxIters, int m, int ortho,
+                 double *sol, double *resiHost, int resiLen, KrylovResult &res, hipStream_t s)
+{
+    const bool gm = (method == kGMRES || method == kPCGMRES);
+    const int rl = gm ? maxIters * m + 1 : maxIters + 1;
+    int rc = wk.ensure(op.n, gm ? m : 0, std::max(rl, resiLen), false);
+    if (rc) return rc;
+    switch (method) {
+    case kCG: rc = run_cg(op, wk, false, kStopSqrtNormRel, tol, maxIters, sol, res, s); break;
+    case kPCG: rc = run_cg(op, wk, true, kStopSqrtNormRel, tol, maxIters, sol, res, s); break;
+    case kCGSEM_PCG: rc = run_cg(op, wk, precond != 0, kStopNormRel, tol, maxIters, sol, res, s); break;
+    case kGMRES: rc = run_gmres(op, wk, false, tol, maxIters, m, ortho, sol, res, s); break;
+    case kPCGMRES: rc = run_gmres(op, wk, true, tol, maxIters, m, ortho, sol, res, s); break;
+    default: set_error("unknown Krylov method %d", method); return SELF_ERR_ARG;
+    }
+    if (rc) return rc;
+    if (resiHost && resiLen > 0) {
+        SELF_HIP(hipMemcpyAsync(resiHost, wk.resi, sizeof(double) * std::min(resiLen, wk.resiLen),
+                                hipMemcpyDeviceToHost, s));
+        SELF_HIP(hipStreamSynchronize(s));
+    }
+    return SELF_OK;
+}
+
+// FP32 PCG inner solve of A e = rhs (e0 = 0) to innerTol relative, then
+// FP64 refinement x += |r| e.
+int krylov_solve_ir(DevOp &op, KrylovWork &wk, int precond, double tol, double innerTol, int maxInner,
+                    int maxOuter, double *x, KrylovResult &res, int &innerTotal, hipStream_t s)
+{
+    if (!op.has_f32()) { set_error("operator has no FP32 form"); return SELF_ERR_STATE; }
+    const size_t n = op.n;
+    int rc = wk.ensure(n, 0, maxInner + 1, true);
+    if (rc) return rc;
+    const int gv = vgrid(n);
+    innerTotal = 0;
+    const float *dinv = precond ? op.inv_diag32() : nullptr;
+    double r0 = -1.0;
+    for (int outer = 0; outer <= maxOuter; ++outer) {
+        // FP64 residual and its norm
+        if ((rc = init_state(wk, tol, 1, kStopNormRel, 0, s))) return rc;
+        if ((rc = op.residual(x, wk.r, wk.st, kFinBnorm, wk.slot, s))) return rc;
+        SolverState h{};
+        if ((rc = read_state(wk, h, s))) return rc;
+        const double rn = h.bnorm;
+        if (r0 < 0) r0 = rn;
+        res.r0 = r0;
+        res.rNorm = rn;
+        if (rn <= tol * r0 || rn == 0.0) { res.ioerr = 0; res.iterations = outer; return SELF_OK; }
+        if (outer == maxOuter) break;
+        // inner FP32 PCG on A e = r / |r|
+        hipLaunchKernelGGL((k_convert<double, float>), dim3(gv), dim3(256), 0, s, wk.r, wk.r32, n, 1.0 / rn);
+        hipLaunchKernelGGL(k_fill<float>, dim3(gv), dim3(256), 0, s, wk.u32, n, 0.0f);
+        if ((rc = init_state(wk, innerTol, maxInner, kStopNormRel, 0, s))) return rc;
+        // r0 of the inner problem is |r/|r|| = 1
+        hipLaunchKernelGGL(k_dot<float>, dim3(kRedBlocks), dim3(kRedThreads), 0, s, wk.r32, wk.r32, n, wk.st,
+                           (int)kFinR0Plain, wk.slot);
+        for (int it0 = 1; it0 <= maxInner; it0 += kChunk) {
+            for (int it = it0; it < it0 + kChunk && it <= maxInner; ++it) {
+                hipLaunchKernelGGL(k_precond_dot<float>, dim3(kRedBlocks), dim3(kRedThreads), 0, s, wk.r32, dinv,
+                                   wk.w32, n, wk.st, (int)kFinNum, wk.slot);
+                hipLaunchKernelGGL(k_update_v<float>, dim3(gv), dim3(256), 0, s, wk.w32, wk.v32, n, wk.st);
+                if ((rc = op.matvec32(wk.v32, wk.z32, wk.v32, wk.st, kFinAlpha, wk.slot, s))) return rc;
+                hipLaunchKernelGGL(k_update_ur<float>, dim3(kRedBlocks), dim3(kRedThreads), 0, s, wk.u32, wk.v32,
+                                   wk.r32, wk.z32, n, wk.st, wk.slot);
+            }
+            SELF_HIP(hipGetLastError());
+            if ((rc = read_state(wk, h, s))) return rc;
+            if (h.done) break;
+        }
+        if ((rc = read_state(wk, h, s))) return rc;
+        innerTotal += h.iter;
+        hipLaunchKernelGGL(k_add_scaled32, dim3(gv), dim3(256), 0, s, x, wk.u32, n, rn);
+        SELF_HIP(hipGetLastError());
+    }
+    res.ioerr = -1;
+    res.iterations = maxOuter;
+    return SELF_OK;
+}
+
+}  // namespace self

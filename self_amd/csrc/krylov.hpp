@@ -1,0 +1,151 @@
+// krylov.hpp -- device-resident Krylov solvers and the operators they drive.
+//
+// The reference's Krylov framework (src/iterativesolve/
+// IterativeSolvers_Class.f90:55-186) is an abstract type whose deferred
+// MatrixAction / Residual / PreConditionInvert procedures a model overrides.
+// Here the same plug-in point is the DevOp interface: an operator that owns
+// its device data and launches its own kernels on a stream.  The solvers
+// (CG, PCG, GMRES, FGMRES with the reference's control flow and stop tests,
+// the CG-SEM PCG of CGsemElliptic_2D_Class.f90:1114-1207, and a mixed
+// FP32-inner / FP64-refinement driver) keep every vector and scalar on the
+// device; the host only launches and, at chunk boundaries, reads a flag.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <vector>
+
+#include "common.hpp"
+#include "reduce.hpp"
+
+namespace self {
+
+// Scalars of one solve, in device memory.
+struct SolverState {
+    double num, den, alpha, beta, rNorm, r0, tol, vz, bnorm;
+    int iter, done, maxIters, stopRule, ioerr, l, nr, innerExit, m;
+    double *resi;
+    int resiLen;
+    // GMRES work (device pointers, sized for m)
+    double *H, *rho, *c, *s, *bhat, *y;
+    double rc, b;
+};
+
+enum StopRule {
+    kStopSqrtNormRel = 0,  // IterativeSolvers CG/PCG: sqrt(rNorm)/r0 < TOL (quirk Q5)
+    kStopNormRel = 1,      // CGsemElliptic_2D PCG and GMRES: rNorm/r0 < TOL
+};
+
+// Finalize actions a fused reduction can perform in its last block.
+enum Finalize {
+    kFinNone = 0,
+    kFinR0 = 1,      // r0 = rNorm = sqrt(S); resi[0] = r0; done if r0 < tol (CGSEM form)
+    kFinNum = 2,     // num = S; beta = (iter == 1) ? 0 : num / den
+    kFinAlpha = 3,   // vz = S; alpha = num / vz
+    kFinRnorm = 4,   // rNorm = sqrt(S); resi[iter] = rNorm; den = num; stop test; ++iter
+    kFinR0Plain = 5, // r0 = sqrt(S); resi[0] = r0 (IterativeSolvers form: no early return)
+    kFinNumCG = 6,   // num = S (r.r); beta = num/den unless iter == 1
+    kFinBnorm = 7,   // bnorm = sqrt(S)
+};
+
+// Scalar recurrences run by the last workgroup of a fused reduction.
+__device__ inline void finalize_scalar(SolverState *st, int kind, double S)
+{
+    switch (kind) {
+    case kFinR0:
+        st->r0 = sqrt(S);
+        st->rNorm = st->r0;
+        if (st->resiLen > 0) st->resi[0] = st->r0;
+        if (st->r0 < st->tol) st->done = 3;  // CGSEM PCG: IF( r0 < TOL ) RETURN
+        break;
+    case kFinR0Plain:
+        st->r0 = sqrt(S);
+        st->rNorm = 0.0;
+        if (st->resiLen > 0) st->resi[0] = st->r0;
+        break;
+    case kFinNum:
+    case kFinNumCG:
+        st->num = S;
+        st->beta = (st->iter == 1) ? 0.0 : S / st->den;
+        break;
+    case kFinAlpha:
+        st->vz = S;
+        st->alpha = st->num / S;
+        break;
+    case kFinRnorm: {
+        const double rn = sqrt(S);
+        st->rNorm = rn;
+        if (st->iter < st->resiLen) st->resi[st->iter] = rn;
+        st->den = st->num;
+        const bool stop = (st->stopRule == kStopSqrtNormRel) ? (sqrt(rn) / st->r0 < st->tol)
+                                                               : (rn / st->r0 < st->tol);
+        if (stop) st->done = 1;
+        else if (st->iter >= st->maxIters) st->done = 2;
+        else st->iter += 1;
+        break;
+    }
+    case kFinBnorm:
+        st->bnorm = sqrt(S);
+        break;
+    default:
+        break;
+    }
+}
+
+
+// Operator plug-in (the deferred procedures of IterativeSolver).
+struct DevOp {
+    size_t n = 0;                   // vector length the solver sees
+    virtual ~DevOp() {}
+    // y = A x.  When dotWith != nullptr the reduction dot(dotWith, y) is
+    // fused and finalized with `fin` into *st.
+    virtual int matvec(const double *x, double *y, const double *dotWith, SolverState *st, int fin,
+                       const RedSlot &slot, hipStream_t s) = 0;
+    // FP32 operator for mixed-precision inner solves (optional).
+    virtual bool has_f32() const { return false; }
+    virtual int matvec32(const float *x, float *y, const float *dotWith, SolverState *st, int fin,
+                         const RedSlot &slot, hipStream_t s) { return SELF_ERR_STATE; }
+    // r = b - A x (the model's Residual); |r|^2 fused and finalized with fin.
+    virtual int residual(const double *x, double *r, SolverState *st, int fin, const RedSlot &slot,
+                         hipStream_t s) = 0;
+    // inverse diagonal for Jacobi preconditioning (nullptr: identity)
+    virtual const double *inv_diag() const { return nullptr; }
+    virtual const float *inv_diag32() const { return nullptr; }
+};
+
+// Device workspace of a solver instance.
+struct KrylovWork {
+    size_t n = 0;
+    int m = 0;
+    double *r = nullptr, *v = nullptr, *z = nullptr, *w = nullptr, *u = nullptr;
+    float *r32 = nullptr, *v32 = nullptr, *z32 = nullptr, *w32 = nullptr, *u32 = nullptr;
+    double *V = nullptr, *Z = nullptr;  // GMRES bases (n x (m+1))
+    double *small = nullptr;            // GMRES small matrices
+    SolverState *st = nullptr;          // device
+    double *resi = nullptr;             // device residual history
+    int resiLen = 0;
+    RedSlot slot{};
+    size_t bytes = 0;
+    int ensure(size_t n, int m, int resiLen, bool f32);
+    void release();
+};
+
+enum KrylovMethod { kCG = 0, kPCG = 1, kGMRES = 2, kPCGMRES = 3, kCGSEM_PCG = 4 };
+
+struct KrylovResult {
+    int ioerr = -2;
+    int iterations = 0;
+    double r0 = 0, rNorm = 0;
+};
+
+// Solve with the reference's control flow; sol is a device vector (initial
+// guess in, solution out); resi (host, resiLen) receives the history.
+int krylov_solve(DevOp &op, KrylovWork &wk, int method, int precond, double tol, int maxIters, int m,
+                 int ortho, double *sol, double *resiHost, int resiLen, KrylovResult &res, hipStream_t s);
+
+// Mixed precision: FP32 PCG inner solves, FP64 residual and iterative
+// refinement to |b - A x| / |b - A x0| <= tol.
+int krylov_solve_ir(DevOp &op, KrylovWork &wk, int precond, double tol, double innerTol, int maxInner,
+                    int maxOuter, double *sol, KrylovResult &res, int &innerTotal, hipStream_t s);
+
+}  // namespace self
