@@ -25,6 +25,8 @@
 #ifndef SELF_AMD_H
 #define SELF_AMD_H
 
+#include <stddef.h>
+
 #ifdef __cplusplus
 extern "C" {
 #endif
@@ -44,6 +46,8 @@ extern "C" {
 #define SELF_CG 2001
 #define SELF_NO_NORMAL_FLOW (-100)
 #define SELF_RADIATION (-101)
+#define SELF_INFLOW_ONE (-103)
+#define SELF_INFLOW_TWO (-104)
 
 /* Precision of a model's tendency arithmetic */
 #define SELF_FP64 0
@@ -61,6 +65,12 @@ int self_init(int device);
 int self_device_count(int *n);
 /* Free device memory in bytes, as used for admission decisions. */
 int self_mem_info(size_t *free_bytes, size_t *total_bytes);
+
+/* STREAM-style copy of `bytes` between two device buffers, iters times:
+ * attainable HBM bandwidth (read + write) and mean ms per copy.  Measurement
+ * support (bandwidth ceiling and byte-counter calibration); no reference
+ * counterpart. */
+int self_stream_copy(size_t bytes, int iters, double *gbps, double *ms);
 
 /* ------------------------------------------------------- host setup (CPU)
  * Quadrature, interpolants and derivative matrices are computed on the host
@@ -215,6 +225,44 @@ int self_linop_destroy(self_linop_t op);
  * the solution out; resi (host, resiLen, nullable) receives resi(0:). */
 int self_krylov_solve(self_linop_t op, int method, double tol, int maxIters, int mInner, int ortho,
                       double *sol, double *resi, int resiLen, int *iterations, int *ioerr);
+
+/* ------------------------------------------- advection-diffusion 2-D (DGSEM)
+ * Replaces the Advection type of
+ * src/highend/advection-diffusion2d/AdvectionClass.f90:57-165: tracers
+ * c(0:N,0:N,1:nTracers) carried by a static velocity, diffusivity kappa,
+ * BR1-type central gradient, upwind advective flux, optional relaxation
+ * source.  Boundary sides take the external state of GetExternalState
+ * (:1851-1886): tanh(t) on tracer 1 at SELF_INFLOW_ONE, on tracer 2 at
+ * SELF_INFLOW_TWO, 0 elsewhere.  FP64 throughout. */
+typedef struct self_adv2d_s *self_adv2d_t;
+
+/* Build_Advection (:176-252) for a mesh from self_quadmesh_structured (or
+ * any edge table of the same form) and the nodal arrays of
+ * self_nodal2d_build; N 1..8, nTracers 1..2. */
+int self_adv2d_create(int N, int nTracers, int nElems, int nEdges, const int *edges, const double *dxds,
+                      const double *dxdp, const double *dyds, const double *dydp, const double *J,
+                      const double *nHat, const double *nLen, const double *dMatS, const double *dMatP,
+                      const double *qWeight, const double *lagWest, const double *lagEast,
+                      const double *lagSouth, const double *lagNorth, double kappa, double dt,
+                      self_adv2d_t *out);
+/* Trash_Advection (:256-288) */
+int self_adv2d_destroy(self_adv2d_t h);
+/* field 0: SetSolution (:362-375), (0:N,0:N,1:nTracers) per element;
+ * 1: SetVelocity (:434-449), (0:N,0:N,1:2) per element, also runs
+ * CalculateVelocityAtBoundaries (:1108-1119) once since velocity is static;
+ * 2: SetRelaxationField (:954-967), 3: SetRelaxationFactor (:1022-1035),
+ * tracer-shaped.  Element
+ * slowest in every host array. */
+int self_adv2d_set_field(self_adv2d_t h, int field, const double *data);
+/* field 0: solution, 1: tendency (after self_adv2d_global_time_derivative),
+ * 2: dc/dx, 3: dc/dy (CalculateConcentrationGradient :1123-1264) */
+int self_adv2d_get_field(self_adv2d_t h, int field, double *data);
+/* GlobalTimeDerivative (:1316-1378) at time t: tendency of the current state. */
+int self_adv2d_global_time_derivative(self_adv2d_t h, double t);
+/* ForwardStepRK3 (:1268-1312) applied nSteps times from tn; elapsed_ms
+ * (nullable) is the device time of the steps. */
+int self_adv2d_forward_step_rk3(self_adv2d_t h, double tn, int nSteps, double *elapsed_ms);
+int self_adv2d_synchronize(self_adv2d_t h);
 
 /* ---------------------------------------- CG spectral-element elliptic 2-D
  * Replaces CGsemElliptic_2D (src/cgsem/CGsemElliptic_2D_Class.f90:57-93):

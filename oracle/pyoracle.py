@@ -498,3 +498,65 @@ class CGSEM2D:
         it = C.c_int()
         ioerr = lib().orc_cgsem2d_solve_pcg(C.byref(self.s), precond, C.byref(it))
         return ioerr, it.value
+
+
+# --------------------------------------------------------------------------- advection-diffusion 2-D
+INFLOW_ONE, INFLOW_TWO = -103, -104
+
+
+class ADV2DStruct(C.Structure):
+    _fields_ = [("N", C.c_int), ("nq", C.c_int), ("nElems", C.c_int), ("nEdges", C.c_int),
+                ("kappa", C.c_double), ("dt", C.c_double)] + [
+        (n, _dp) for n in ("dMatS", "dMatP", "qWeight", "lagWest", "lagEast", "lagSouth", "lagNorth")] + [
+        ("edges", _ip)] + [
+        (n, _dp) for n in ("dxds", "dxdp", "dyds", "dydp", "J", "nHat", "nLen", "c", "tend", "G", "dcdx",
+                           "dcdy", "u", "v", "relax", "rFac", "bc", "bdx", "bdy", "fc", "fgx", "fgy", "bu", "bv")]
+
+
+def _declare_extra_adv2d(L):
+    P = C.POINTER(ADV2DStruct)
+    L.orc_adv2d_global_time_derivative.argtypes = [P, C.c_double]
+    L.orc_adv2d_forward_step_rk3.argtypes = [P, C.c_double]
+
+
+class Advection2D:
+    """Advection-diffusion (advection-diffusion2d/AdvectionClass.f90) on the oracle.
+
+    c, tend, dcdx, dcdy, relax, rFac: flat (nElems, nTracers, N+1, N+1) with
+    s fastest; u, v: flat (nElems, N+1, N+1).  dMatP defaults to the weak
+    operator (dMatCorrect); pass nodal.dMatP for the reference's transposed one.
+    """
+
+    def __init__(self, nodal: NodalStorage2D, mesh: QuadMesh, nTracers=1, kappa=0.0, dt=1.0, dMatP=None,
+                 with_relax=False, edges=None):
+        N, E = nodal.N, mesh.nElems
+        n1, nq = N + 1, nTracers
+        self.N, self.nElems, self.nq = N, E, nq
+        self.dms = np.asfortranarray(nodal.dMatS).ravel(order="F").copy()
+        self.dmp = np.asfortranarray(nodal.dMatCorrect if dMatP is None else dMatP).ravel(order="F").copy()
+        self.edges = np.ascontiguousarray((mesh.edges if edges is None else edges).ravel().astype(np.int32))
+        vol, tr = n1 * n1 * nq * E, n1 * nq * 4 * E
+        for n in ("c", "tend", "G", "dcdx", "dcdy"):
+            setattr(self, n, np.zeros(vol))
+        self.relax = np.zeros(vol) if with_relax else None
+        self.rFac = np.zeros(vol) if with_relax else None
+        self.u = np.zeros(n1 * n1 * E)
+        self.v = np.zeros(n1 * n1 * E)
+        for n in ("bc", "bdx", "bdy", "fc", "fgx", "fgy"):
+            setattr(self, n, np.zeros(tr))
+        self.bu = np.zeros(n1 * 4 * E)
+        self.bv = np.zeros(n1 * 4 * E)
+        self.mesh, self.nodal = mesh, nodal
+        self.s = ADV2DStruct(N, nq, E, mesh.nEdges, kappa, dt, _p(self.dms), _p(self.dmp), _p(nodal.qWeight),
+                             _p(nodal.lagWest), _p(nodal.lagEast), _p(nodal.lagSouth), _p(nodal.lagNorth),
+                             _pi(self.edges),
+                             *[_p(getattr(mesh, n)) for n in ("dxds", "dxdp", "dyds", "dydp", "J", "nHat", "nLen")],
+                             _p(self.c), _p(self.tend), _p(self.G), _p(self.dcdx), _p(self.dcdy), _p(self.u),
+                             _p(self.v), _p(self.relax), _p(self.rFac), _p(self.bc), _p(self.bdx), _p(self.bdy),
+                             _p(self.fc), _p(self.fgx), _p(self.fgy), _p(self.bu), _p(self.bv))
+
+    def global_time_derivative(self, t=0.0):
+        lib().orc_adv2d_global_time_derivative(C.byref(self.s), t)
+
+    def forward_step_rk3(self, tn=0.0):
+        lib().orc_adv2d_forward_step_rk3(C.byref(self.s), tn)

@@ -21,6 +21,7 @@ SELF_ERR_OTHER = -2
 GAUSS, GAUSS_LOBATTO = 1, -1
 DG, CG = 2000, 2001
 NO_NORMAL_FLOW, RADIATION = -100, -101
+INFLOW_ONE, INFLOW_TWO = -103, -104
 FP64, MIXED = 0, 1
 PLAIN, FUSED = 0, 1
 
@@ -75,6 +76,7 @@ _SIGS = {
     "self_init": ([C.c_int], C.c_int),
     "self_device_count": ([ip], C.c_int),
     "self_mem_info": ([szp, szp], C.c_int),
+    "self_stream_copy": ([C.c_size_t, C.c_int, dp, dp], C.c_int),
     "self_legendre_quadrature": ([C.c_int, C.c_int, dp, dp], C.c_int),
     "self_barycentric_weights": ([C.c_int, dp, dp], C.c_int),
     "self_lagrange_polynomials": ([C.c_int, dp, dp, C.c_double, dp], C.c_int),
@@ -101,6 +103,36 @@ _SIGS = {
     "self_sw2d_kernel_times": ([C.c_void_p, C.c_double, C.c_int, dp, ip], C.c_int),
     "self_sw2d_synchronize": ([C.c_void_p], C.c_int),
     "self_sw2d_device_bytes": ([C.c_void_p, szp], C.c_int),
+    "self_linop_dense_create": ([C.c_int, dp, dp, C.POINTER(C.c_void_p)], C.c_int),
+    "self_linop_elliptic1d_create": ([C.c_int, C.c_int, C.c_double, C.POINTER(C.c_void_p)], C.c_int),
+    "self_linop_elliptic1d_unmask": ([C.c_void_p, dp, dp, dp], C.c_int),
+    "self_linop_size": ([C.c_void_p, ip], C.c_int),
+    "self_linop_destroy": ([C.c_void_p], C.c_int),
+    "self_krylov_solve": ([C.c_void_p, C.c_int, C.c_double, C.c_int, C.c_int, C.c_int, dp, dp, C.c_int, ip, ip],
+                          C.c_int),
+    "self_cgsem2d_create": ([C.c_int, C.c_int, C.c_int, C.c_int, ip, ip] + [dp] * 5 + [C.POINTER(C.c_void_p)],
+                            C.c_int),
+    "self_cgsem2d_destroy": ([C.c_void_p], C.c_int),
+    "self_cgsem2d_global_dofs": ([C.c_void_p, ip], C.c_int),
+    "self_cgsem2d_set_flux_coefficient": ([C.c_void_p, dp], C.c_int),
+    "self_cgsem2d_set_source": ([C.c_void_p, dp], C.c_int),
+    "self_cgsem2d_set_dirichlet": ([C.c_void_p, dp], C.c_int),
+    "self_cgsem2d_set_solution": ([C.c_void_p, dp], C.c_int),
+    "self_cgsem2d_get_solution": ([C.c_void_p, dp], C.c_int),
+    "self_cgsem2d_matrix_action": ([C.c_void_p, dp, dp], C.c_int),
+    "self_cgsem2d_residual": ([C.c_void_p, dp], C.c_int),
+    "self_cgsem2d_solve": ([C.c_void_p, C.c_int, C.c_int, C.c_double, C.c_int, C.c_int, C.c_int, ip, ip, dp,
+                            C.c_int, dp], C.c_int),
+    "self_cgsem2d_solve_mixed": ([C.c_void_p, C.c_int, C.c_double, C.c_double, C.c_int, C.c_int, ip, ip, ip, dp, dp],
+                                 C.c_int),
+    "self_adv2d_create": ([C.c_int, C.c_int, C.c_int, C.c_int, ip] + [dp] * 14 + [C.c_double, C.c_double,
+                                                                          C.POINTER(C.c_void_p)], C.c_int),
+    "self_adv2d_destroy": ([C.c_void_p], C.c_int),
+    "self_adv2d_set_field": ([C.c_void_p, C.c_int, dp], C.c_int),
+    "self_adv2d_get_field": ([C.c_void_p, C.c_int, dp], C.c_int),
+    "self_adv2d_global_time_derivative": ([C.c_void_p, C.c_double], C.c_int),
+    "self_adv2d_forward_step_rk3": ([C.c_void_p, C.c_double, C.c_int, dp], C.c_int),
+    "self_adv2d_synchronize": ([C.c_void_p], C.c_int),
 }
 
 EXTRA_SIGS: dict = {}

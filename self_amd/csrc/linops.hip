@@ -213,8 +213,9 @@ SELF_CATCH_ALL
 // source = 1 and homogeneous Dirichlet ends (dirichletMask = 0).
 int self_linop_elliptic1d_create(int N, int nElems, double xScale, self_linop_t *out)
 try {
-    SELF_REQUIRE(N >= 1 && N <= 32 && nElems >= 1 && nElems <= 4096 && out, SELF_ERR_ARG,
-                 "self_linop_elliptic1d_create: bad arguments");
+    SELF_REQUIRE(N >= 1 && N <= 32 && nElems >= 1 && out, SELF_ERR_ARG, "self_linop_elliptic1d_create: bad arguments");
+    // one workgroup holds the operand and the result in LDS (2 x 8 B per DOF)
+    SELF_REQUIRE((N + 1) * nElems <= 4096, SELF_ERR_ARG, "Elliptic1D operator limited to 4096 DOF");
     const int n1 = N + 1, nd = n1 * nElems;
     std::vector<double> s(n1), w(n1), bw(n1), D(n1 * n1);
     int rc = self_legendre_quadrature(N, SELF_GAUSS_LOBATTO, s.data(), w.data());

@@ -186,39 +186,6 @@ __global__ __launch_bounds__(256) void k_sw_edge_flux(const int *__restrict__ ed
     }
 }
 
-// Nodal constants staged in LDS once per workgroup: weak derivative
-// matrices (column-major, row i col m at i + n1*m), reciprocal quadrature
-// weights and the four boundary Lagrange vectors.  Reading them from LDS
-// (mostly broadcast reads) keeps them out of the VGPR budget.
-template <int N>
-struct NodalLDS {
-    static constexpr int n1 = N + 1;
-    double dS[n1 * n1], dP[n1 * n1];
-    double iwS[n1], iwP[n1];
-    double lW[n1], lE[n1], lS[n1], lN[n1];
-    float dS32[n1 * n1], dP32[n1 * n1];
-};
-
-// global nodal buffer: dS, dP, qwS, qwP, lagW, lagE, lagS, lagN (doubles)
-template <int N, bool F32>
-__device__ __forceinline__ void stage_nodal(NodalLDS<N> &L, const double *__restrict__ g,
-                                            const float *__restrict__ dS32, const float *__restrict__ dP32)
-{
-    constexpr int n1 = N + 1, nd = 2 * n1 * n1 + 6 * n1;
-    double *dst = L.dS;  // members up to lN are contiguous doubles
-    for (int k = threadIdx.x; k < nd; k += blockDim.x) {
-        const double v = g[k];
-        const bool w = (k >= 2 * n1 * n1) && (k < 2 * n1 * n1 + 2 * n1);
-        dst[k] = w ? 1.0 / v : v;
-    }
-    if constexpr (F32) {
-        for (int k = threadIdx.x; k < n1 * n1; k += blockDim.x) {
-            L.dS32[k] = dS32[k];
-            L.dP32[k] = dP32[k];
-        }
-    }
-}
-
 template <int N>
 __device__ __forceinline__ void trace3(const NodalLDS<N> &L, const double *qs, int s, int k, double out[3])
 {

@@ -110,3 +110,24 @@ def sw2d_geostrophic(nX=64, N=7, precision="fp64", form="fused", L=1.0e6, seed=D
     dt = sw2d_cfl_dt(U, V, H, g, L / nX, N, cfl)
     model.SetTimeStep(dt)
     return model, {"sol": sol, "bathy": bathy, "dt": dt, "f0": f0, "g": g, "L": L}
+
+
+def adv2d_blob_fields(x, y, x0=0.5, y0=0.5, width=0.1, amp=1.0):
+    """Gaussian tracer blob amp * exp(-((x-x0)^2 + (y-y0)^2) / width^2)."""
+    return amp * np.exp(-((x - x0) ** 2 + (y - y0) ** 2) / width ** 2)
+
+
+def adv2d_blob(nX=10, N=7, u=1.0, v=0.5, kappa=1.0e-3, dt=1.0e-4, nTracers=1, quirk_dmatp_transposed=False):
+    """Advection-diffusion of a Gaussian blob on the doubly periodic unit
+    square: nX x nX elements of degree N on Legendre-Gauss nodes, constant
+    velocity (u, v), diffusivity kappa, RK3 step dt.  Returns (model, info)."""
+    from .advection import Advection, AdvectionParams
+
+    p = AdvectionParams(polyDeg=N, nXElem=nX, nYElem=nX, nTracers=nTracers, dt=dt, kappa=kappa)
+    model = Advection(p, quirk_dmatp_transposed=quirk_dmatp_transposed)
+    m = model.mesh
+    c = np.zeros((m.nElems, nTracers, N + 1, N + 1))
+    c[:, 0] = adv2d_blob_fields(m.x, m.y)
+    model.SetSolution(c)
+    model.SetVelocity(np.full_like(m.x, u), np.full_like(m.x, v))
+    return model, {"c": c, "u": u, "v": v, "kappa": kappa, "dt": dt}
