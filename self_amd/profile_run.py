@@ -1,0 +1,59 @@
+"""Run one workload for a fixed number of steps, for use under a profiler.
+
+    python -m self_amd.profile_run copy
+    python -m self_amd.profile_run sw2d --precision mixed --form fused --steps 10
+    python -m self_amd.profile_run adv2d --steps 10
+    python -m self_amd.profile_run cgsem --nx 128 --precond 1
+
+`copy` runs the STREAM-style copy kernel on 64 MiB (21 launches) and 1 GiB
+(11 launches): the calibration for hardware byte counters (FETCH_SIZE /
+WRITE_SIZE of a copy of known size).  Every other mode builds the named
+workload with its seeded inputs and runs the HIP path only.
+"""
+from __future__ import annotations
+
+import argparse
+import ctypes as C
+import sys
+
+from . import _lib as L
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser(prog="python -m self_amd.profile_run")
+    ap.add_argument("mode", choices=["copy", "sw2d", "adv2d", "cgsem"])
+    ap.add_argument("--precision", default="mixed", choices=["fp64", "mixed"])
+    ap.add_argument("--form", default="fused", choices=["fused", "plain"])
+    ap.add_argument("--nx", type=int, default=None)
+    ap.add_argument("--N", type=int, default=7)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--precond", type=int, default=1)
+    a = ap.parse_args(argv)
+    L.check(L.lib().self_init(0))
+    if a.mode == "copy":
+        g, ms = C.c_double(), C.c_double()
+        for size, iters in ((64 << 20, 20), (1 << 30, 10)):
+            L.check(L.lib().self_stream_copy(size, iters, C.byref(g), C.byref(ms)))
+            print(f"copy {size} B: {g.value:.1f} GB/s, {ms.value * 1e3:.2f} us per copy", flush=True)
+    elif a.mode == "sw2d":
+        from .workloads import sw2d_geostrophic
+        m, _ = sw2d_geostrophic(a.nx or 64, a.N, precision=a.precision, form=a.form)
+        m.ForwardStepRK3(0.0, a.steps)
+        m.Synchronize()
+        print(f"sw2d {a.precision} {a.form}: {a.steps} RK3 steps", flush=True)
+    elif a.mode == "adv2d":
+        from .workloads import adv2d_blob
+        m, _ = adv2d_blob(nX=a.nx or 10, N=a.N)
+        m.ForwardStepRK3(0.0, a.steps)
+        m.Synchronize()
+        print(f"adv2d: {a.steps} RK3 steps", flush=True)
+    elif a.mode == "cgsem":
+        from .workloads import cgsem_helmholtz
+        m, _ = cgsem_helmholtz(nX=a.nx or 128, N=a.N)
+        r = m.Solve("cgsem_pcg", precond=("none", "jacobi")[a.precond], tolerance=1e-10, maxIters=a.steps)
+        print(f"cgsem: {r}", flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -113,8 +113,10 @@ def sw2d_geostrophic(nX=64, N=7, precision="fp64", form="fused", L=1.0e6, seed=D
 
 
 def adv2d_blob_fields(x, y, x0=0.5, y0=0.5, width=0.1, amp=1.0):
-    """Gaussian tracer blob amp * exp(-((x-x0)^2 + (y-y0)^2) / width^2)."""
-    return amp * np.exp(-((x - x0) ** 2 + (y - y0) ** 2) / width ** 2)
+    """Gaussian tracer blob amp * exp(-r^2 / (2 width^2)), r measured from
+    (x0, y0) (the 2 sigma^2 convention of the commented-out initial condition
+    at advection-diffusion2d/Advection_InitialConditions.f90:91)."""
+    return amp * np.exp(-((x - x0) ** 2 + (y - y0) ** 2) / (2.0 * width ** 2))
 
 
 def adv2d_blob(nX=10, N=7, u=1.0, v=0.5, kappa=1.0e-3, dt=1.0e-4, nTracers=1, quirk_dmatp_transposed=False):
@@ -131,3 +133,46 @@ def adv2d_blob(nX=10, N=7, u=1.0, v=0.5, kappa=1.0e-3, dt=1.0e-4, nTracers=1, qu
     model.SetSolution(c)
     model.SetVelocity(np.full_like(m.x, u), np.full_like(m.x, v))
     return model, {"c": c, "u": u, "v": v, "kappa": kappa, "dt": dt}
+
+
+def cgsem_rhs_modes(seed=DEFAULT_SEED, nmodes=16, kmax=8):
+    """16 seeded Fourier modes (m, n) in [1..kmax]^2 with a ~ N(0, 1)
+    (Box-Muller on splitmix64 uniforms), amplitude a / (m^2 + n^2)."""
+    rng = SplitMix64(seed + 4)
+    modes = []
+    for _ in range(nmodes):
+        m = rng.randint(1, kmax)
+        n = rng.randint(1, kmax)
+        u1 = max(rng.uniform(), 1e-300)
+        u2 = rng.uniform()
+        a = math.sqrt(-2.0 * math.log(u1)) * math.cos(2.0 * math.pi * u2)
+        modes.append((a / (m * m + n * n), m, n))
+    return modes
+
+
+def cgsem_poisson_fields(x, y, seed=DEFAULT_SEED):
+    """k = 1 + 0.5 sin(2 pi x) sin(2 pi y) and f = sum a_mn sin(m pi x) sin(n pi y)
+    on [0,1]^2, for -div(k grad u) = f."""
+    k = 1.0 + 0.5 * np.sin(2 * math.pi * x) * np.sin(2 * math.pi * y)
+    f = np.zeros_like(x)
+    for a, m, n in cgsem_rhs_modes(seed):
+        f += a * np.sin(m * math.pi * x) * np.sin(n * math.pi * y)
+    return k, f
+
+
+def cgsem_helmholtz(nX=128, N=7, seed=DEFAULT_SEED, tolerance=1e-10, maxIters=20000):
+    """CGsemElliptic_2D with the variable-coefficient Poisson problem
+    -div(k grad u) = f, u = 0 on the boundary of [0,1]^2, nX x nX elements of
+    degree N on Gauss-Lobatto nodes.  The reference's operator is
+    div(k grad u) = s, so the source is s = -f.  Returns (model, info)."""
+    from .krylov import CGsemElliptic2D, CGsemParams
+
+    p = CGsemParams(polyDeg=N, nXElem=nX, nYElem=nX, tolerance=tolerance, maximumIterates=maxIters)
+    model = CGsemElliptic2D(p)
+    m = model.mesh
+    k, f = cgsem_poisson_fields(m.x, m.y, seed)
+    model.SetFluxCoefficient(k)
+    model.SetSource(-f)
+    model.SetDirichletBoundaryCondition(np.zeros_like(k))
+    model.SetSolution(np.zeros_like(k))
+    return model, {"k": k, "f": f}
