@@ -264,6 +264,75 @@ int self_adv2d_global_time_derivative(self_adv2d_t h, double t);
 int self_adv2d_forward_step_rk3(self_adv2d_t h, double tn, int nSteps, double *elapsed_ms);
 int self_adv2d_synchronize(self_adv2d_t h);
 
+/* ------------------------------------------------------ 3-D set-up (CPU)
+ * BuildNodalStorage_3D (src/nodal/NodalStorage_3D_Class.f90:107-264) with
+ * equal degree in s, p, q: nodes, weights, derivative matrix (SELF_DG: the
+ * weak operator -W^-1 D^T W, :223-251; SELF_CG: D), Lagrange vectors at -1
+ * (west, south, bottom) and +1 (east, north, top). */
+int self_nodal3d_build(int N, int quadType, int form, double *nodes, double *qWeight, double *dMat,
+                       double *lagLeft, double *lagRight);
+/* Direction factor F1 ((N+1)^2, column-major) of RollOffFilter3D
+ * (src/filters/RollOffFilter3D_Class.f90:87-180): the reference's dense
+ * (N+1)^3 filter matrix equals F1 (x) F1 (x) F1, since the Legendre modes,
+ * the quadrature weights and exp(-ln2 (a^2+b^2+c^2)/nc^2) all factor over
+ * the three directions. */
+int self_rolloff_filter1d(int N, int nCutoff, const double *nodes, const double *w, double *F1);
+/* Structured hex mesh of the unit cube, elements iX fastest then iY, iZ;
+ * sides 1..6 = south, east, north, west, bottom, top (ModelFlags.f90:36-41);
+ * optional periodic pairing per direction (the reference's default mesh has
+ * none) and the warp X = a + A s1 S, Y = b + A s2 S, Z = c + A s3 S with
+ * S = sin(2 pi a) sin(2 pi b) sin(2 pi c), scaled by (xScale, yScale, zScale).
+ * faces: 9 ints per face (e1, s1, e2, s2, iStart, jStart, iInc, jInc, swap)
+ * as the reference's face loop uses them (Advection3DClass.f90:1286-1296);
+ * e2 = NO_NORMAL_FLOW on boundaries.  Per element: x, y, z, J ((N+1)^3),
+ * cov (9 per node: dxds dxdp dxdq dyds dydp dydq dzds dzdp dzdq), and per
+ * face node [side][b][a]: nHat (3), nLen, bpos (3, the node position);
+ * face node (a, b) is (p, q) on west/east, (s, q) on south/north, (s, p) on
+ * bottom/top. */
+int self_hexmesh_counts(int nX, int nY, int nZ, int periodicX, int periodicY, int periodicZ, int *nElems,
+                        int *nFaces);
+int self_hexmesh_structured(int nX, int nY, int nZ, int N, int periodicX, int periodicY, int periodicZ,
+                            double xScale, double yScale, double zScale, double warp, const int *warpSign,
+                            const double *nodes, int *faces, double *x, double *y, double *z, double *cov,
+                            double *J, double *nHat, double *nLen, double *bpos);
+
+/* ---------------------------------------------------- 3-D advection (DGSEM)
+ * Replaces the Advection type of src/highend/advection3d/Advection3DClass.f90
+ * (:57-160): tracers c(0:N,0:N,0:N,1:nTracers) carried by a static velocity,
+ * upwind flux, relaxation source, the adaptive roll-off filter
+ * (DoTheAdaptiveFiltering :1125-1233) applied at the start of every stage.
+ * External states as GetExternalState (:1698-1735): tanh(t) on tracer 1
+ * (SELF_INFLOW_ONE) or 2 (SELF_INFLOW_TWO), the moving Gaussian
+ * exp(-|x_b - t|^2 / 0.04) at SELF_NO_NORMAL_FLOW, 0 otherwise.  FP64. */
+typedef struct self_adv3d_s *self_adv3d_t;
+
+/* Build_Advection (:167-272) on a mesh of self_hexmesh_structured's form;
+ * nodal arrays from self_nodal3d_build; filter1d from self_rolloff_filter1d
+ * (nullable when useFilter = 0); N 1..7, nTracers 1..2. */
+int self_adv3d_create(int N, int nTracers, int nElems, int nFaces, const int *faces, const double *cov,
+                      const double *J, const double *nHat, const double *nLen, const double *bpos,
+                      const double *dMat, const double *qWeight, const double *lagLeft, const double *lagRight,
+                      const double *filter1d, int useFilter, double dt, self_adv3d_t *out);
+/* Trash_Advection (:276-312) */
+int self_adv3d_destroy(self_adv3d_t h);
+/* field 0: SetSolution (:369-382) (0:N,0:N,0:N,1:nTracers) per element;
+ * 1: SetVelocity (:477-494) (0:N,0:N,0:N,1:3) per element, with the face
+ * velocities of CalculateVelocityAtBoundaries computed once; 2: relaxation
+ * field, 3: relaxation factor (tracer-shaped). */
+int self_adv3d_set_field(self_adv3d_t h, int field, const double *data);
+/* field 0: solution, 1: tendency, 2: filter state E1 (nTracers x nElems),
+ * E2 (same), limiter (3 x nTracers x nElems), in that order. */
+int self_adv3d_get_field(self_adv3d_t h, int field, double *data);
+/* GlobalTimeDerivative(t, stage) (:1049-1121): filters the state in place
+ * (when enabled), then computes the tendency. */
+int self_adv3d_global_time_derivative(self_adv3d_t h, double t, int stage);
+/* ForwardStepRK3 (:1000-1045) nSteps times from tn; elapsed_ms nullable. */
+int self_adv3d_forward_step_rk3(self_adv3d_t h, double tn, int nSteps, double *elapsed_ms);
+/* Element-tracer pairs the filter replaced since the last reset
+ * (measurement support, no reference counterpart). */
+int self_adv3d_filter_count(self_adv3d_t h, int reset, unsigned *count);
+int self_adv3d_synchronize(self_adv3d_t h);
+
 /* ---------------------------------------- CG spectral-element elliptic 2-D
  * Replaces CGsemElliptic_2D (src/cgsem/CGsemElliptic_2D_Class.f90:57-93):
  * div(k grad u) = s on GLL nodes of degree N, Dirichlet on every boundary

@@ -560,3 +560,133 @@ class Advection2D:
 
     def forward_step_rk3(self, tn=0.0):
         lib().orc_adv2d_forward_step_rk3(C.byref(self.s), tn)
+
+
+# --------------------------------------------------------------------------- 3-D: mesh, filter, advection
+class HexMeshDesc(C.Structure):
+    _fields_ = [("nX", C.c_int), ("nY", C.c_int), ("nZ", C.c_int), ("N", C.c_int), ("periodicX", C.c_int),
+                ("periodicY", C.c_int), ("periodicZ", C.c_int), ("xScale", C.c_double), ("yScale", C.c_double),
+                ("zScale", C.c_double), ("warp", C.c_double), ("warpSign", C.c_int * 3), ("nElems", C.c_int),
+                ("nFaces", C.c_int)]
+
+
+class ADV3DStruct(C.Structure):
+    _fields_ = [("N", C.c_int), ("nq", C.c_int), ("nElems", C.c_int), ("nFaces", C.c_int), ("useFilter", C.c_int),
+                ("dt", C.c_double)] + [(n, _dp) for n in ("dMat", "qWeight", "lagLeft", "lagRight", "F")] + [
+        ("faces", _ip)] + [(n, _dp) for n in ("cov", "J", "nHat", "nLen", "bpos", "c", "tend", "G", "u", "v", "w",
+                                             "relax", "rFac", "bc", "fc", "bu", "bv", "bw", "E1", "E2", "lim")] + [
+        ("nFiltered", C.c_int)]
+
+
+def _declare_extra_adv3d(L):
+    L.orc_hexmesh_counts.argtypes = [C.POINTER(HexMeshDesc)]
+    L.orc_hexmesh_build.argtypes = [C.POINTER(HexMeshDesc), _dp, _ip] + [_dp] * 8
+    L.orc_hexmesh_build.restype = C.c_int
+    L.orc_rolloff_filter3d_build.argtypes = [C.c_int, C.c_int, _dp, _dp, _dp]
+    L.orc_rolloff_filter3d_apply.argtypes = [C.c_int, _dp, _dp, _dp]
+    P = C.POINTER(ADV3DStruct)
+    L.orc_adv3d_adaptive_filter.argtypes = [P, C.c_int]
+    L.orc_adv3d_global_time_derivative.argtypes = [P, C.c_double, C.c_int]
+    L.orc_adv3d_forward_step_rk3.argtypes = [P, C.c_double]
+
+
+class HexMesh:
+    """Structured hex mesh of the unit cube (orc_adv3d.h).  Volume arrays
+    flat per element (i fastest); faces (nFaces, 9)."""
+
+    def __init__(self, nX, nY, nZ, N, nodes, periodic=(True, True, True), scale=(1.0, 1.0, 1.0), warp=0.0,
+                 warpSign=(1, 1, 1)):
+        d = HexMeshDesc(nX, nY, nZ, N, int(periodic[0]), int(periodic[1]), int(periodic[2]), scale[0], scale[1],
+                        scale[2], warp, (C.c_int * 3)(*warpSign), 0, 0)
+        lib().orc_hexmesh_counts(C.byref(d))
+        n1 = N + 1
+        E, nF = d.nElems, d.nFaces
+        self.N, self.nElems, self.nFaces = N, E, nF
+        self.faces = np.zeros(9 * nF, dtype=np.int32)
+        for n in ("x", "y", "z", "J"):
+            setattr(self, n, np.zeros(n1 ** 3 * E))
+        self.cov = np.zeros(9 * n1 ** 3 * E)
+        self.nHat = np.zeros(3 * n1 * n1 * 6 * E)
+        self.nLen = np.zeros(n1 * n1 * 6 * E)
+        self.bpos = np.zeros(3 * n1 * n1 * 6 * E)
+        rc = lib().orc_hexmesh_build(C.byref(d), _p(np.ascontiguousarray(nodes, dtype=np.float64)),
+                                     _pi(self.faces), _p(self.x), _p(self.y), _p(self.z), _p(self.cov), _p(self.J),
+                                     _p(self.nHat), _p(self.nLen), _p(self.bpos))
+        if rc != 0:
+            raise ValueError(f"orc_hexmesh_build failed ({rc})")
+        self.faces = self.faces.reshape(-1, 9)
+
+
+def rolloff_filter3d(N, nc, nodes, w):
+    nD = (N + 1) ** 3
+    F = np.zeros(nD * nD)
+    lib().orc_rolloff_filter3d_build(N, nc, _p(np.ascontiguousarray(nodes, dtype=np.float64)),
+                                     _p(np.ascontiguousarray(w, dtype=np.float64)), _p(F))
+    return F.reshape(nD, nD, order="F")
+
+
+class Nodal3D:
+    """NodalStorage_3D DG (src/nodal/NodalStorage_3D_Class.f90:107-264) on Gauss nodes."""
+
+    def __init__(self, N):
+        self.N = N
+        self.nodes, self.qWeight = quadrature(N, GAUSS)
+        bw = barycentric_weights(self.nodes)
+        D = derivative_matrix(self.nodes, bw)
+        n1 = N + 1
+        self.dMat = np.zeros((n1, n1))
+        for j in range(n1):
+            for i in range(n1):
+                self.dMat[i, j] = -D[j, i] * self.qWeight[j] / self.qWeight[i]
+        self.lagLeft = lagrange_at(self.nodes, bw, -1.0)
+        self.lagRight = lagrange_at(self.nodes, bw, 1.0)
+
+
+class Advection3D:
+    """Advection3D (advection3d/Advection3DClass.f90) on the oracle.
+
+    c, tend, relax, rFac: flat (nElems, nTracers, N+1, N+1, N+1) i fastest;
+    u, v, w: flat (nElems, N+1, N+1, N+1)."""
+
+    def __init__(self, nodal: Nodal3D, mesh: HexMesh, nTracers=1, dt=1e-3, nCutoff=3, useFilter=True,
+                 with_relax=False, faces=None):
+        N, E = nodal.N, mesh.nElems
+        n1, nq = N + 1, nTracers
+        self.N, self.nElems, self.nq = N, E, nq
+        self.dm = np.ascontiguousarray(nodal.dMat.ravel(order="F"))
+        self.F = np.ascontiguousarray(rolloff_filter3d(N, nCutoff, nodal.nodes, nodal.qWeight).ravel(order="F"))
+        self.faces = np.ascontiguousarray((mesh.faces if faces is None else faces).ravel().astype(np.int32))
+        vol, tr = n1 ** 3 * nq * E, n1 * n1 * nq * 6 * E
+        for n in ("c", "tend", "G"):
+            setattr(self, n, np.zeros(vol))
+        self.relax = np.zeros(vol) if with_relax else None
+        self.rFac = np.zeros(vol) if with_relax else None
+        for n in ("u", "v", "w"):
+            setattr(self, n, np.zeros(n1 ** 3 * E))
+        self.bc = np.zeros(tr)
+        self.fc = np.zeros(tr)
+        for n in ("bu", "bv", "bw"):
+            setattr(self, n, np.zeros(n1 * n1 * 6 * E))
+        self.E1 = np.zeros(nq * E)
+        self.E2 = np.zeros(nq * E)
+        self.lim = np.zeros(3 * nq * E)
+        self.mesh, self.nodal = mesh, nodal
+        self.s = ADV3DStruct(N, nq, E, mesh.nFaces, int(useFilter), dt, _p(self.dm), _p(nodal.qWeight),
+                             _p(nodal.lagLeft), _p(nodal.lagRight), _p(self.F), _pi(self.faces), _p(mesh.cov),
+                             _p(mesh.J), _p(mesh.nHat), _p(mesh.nLen), _p(mesh.bpos), _p(self.c), _p(self.tend),
+                             _p(self.G), _p(self.u), _p(self.v), _p(self.w), _p(self.relax), _p(self.rFac),
+                             _p(self.bc), _p(self.fc), _p(self.bu), _p(self.bv), _p(self.bw), _p(self.E1),
+                             _p(self.E2), _p(self.lim), 0)
+
+    @property
+    def nFiltered(self):
+        return self.s.nFiltered
+
+    def adaptive_filter(self, stage):
+        lib().orc_adv3d_adaptive_filter(C.byref(self.s), stage)
+
+    def global_time_derivative(self, t=0.0, stage=1):
+        lib().orc_adv3d_global_time_derivative(C.byref(self.s), t, stage)
+
+    def forward_step_rk3(self, tn=0.0):
+        lib().orc_adv3d_forward_step_rk3(C.byref(self.s), tn)

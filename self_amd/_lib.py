@@ -133,6 +133,19 @@ _SIGS = {
     "self_adv2d_global_time_derivative": ([C.c_void_p, C.c_double], C.c_int),
     "self_adv2d_forward_step_rk3": ([C.c_void_p, C.c_double, C.c_int, dp], C.c_int),
     "self_adv2d_synchronize": ([C.c_void_p], C.c_int),
+    "self_nodal3d_build": ([C.c_int, C.c_int, C.c_int, dp, dp, dp, dp, dp], C.c_int),
+    "self_rolloff_filter1d": ([C.c_int, C.c_int, dp, dp, dp], C.c_int),
+    "self_hexmesh_counts": ([C.c_int] * 6 + [ip, ip], C.c_int),
+    "self_hexmesh_structured": ([C.c_int] * 7 + [C.c_double] * 4 + [ip, dp, ip] + [dp] * 8, C.c_int),
+    "self_adv3d_create": ([C.c_int] * 4 + [ip] + [dp] * 10 + [C.c_int, C.c_double, C.POINTER(C.c_void_p)],
+                          C.c_int),
+    "self_adv3d_destroy": ([C.c_void_p], C.c_int),
+    "self_adv3d_set_field": ([C.c_void_p, C.c_int, dp], C.c_int),
+    "self_adv3d_get_field": ([C.c_void_p, C.c_int, dp], C.c_int),
+    "self_adv3d_global_time_derivative": ([C.c_void_p, C.c_double, C.c_int], C.c_int),
+    "self_adv3d_forward_step_rk3": ([C.c_void_p, C.c_double, C.c_int, dp], C.c_int),
+    "self_adv3d_filter_count": ([C.c_void_p, C.c_int, C.POINTER(C.c_uint)], C.c_int),
+    "self_adv3d_synchronize": ([C.c_void_p], C.c_int),
 }
 
 EXTRA_SIGS: dict = {}

@@ -104,13 +104,19 @@ __global__ __launch_bounds__(256) void k_cg_local(const T *__restrict__ x, const
 // y[g] = sum over the copies of g (GlobalSum order); optional fused dot.
 // MODE 0: matvec.  MODE 1: residual  y = (b[c0] - sum L) + sum_{c>=1} b[c],
 // with |y|^2 reduced.
-template <typename T, int MODE>
+template <typename T, int MODE, bool FIN>
 __global__ __launch_bounds__(kRedThreads) void k_cg_assemble(const T *__restrict__ Lbuf, const int *__restrict__ cs,
                                                              const int *__restrict__ ci, const double *__restrict__ bl,
                                                              size_t nG, T *__restrict__ y, const T *__restrict__ dotWith,
                                                              SolverState *st, int fin, RedSlot slot)
 {
     if (st && (st->done || (MODE == 0 && st->innerExit))) return;
+    __shared__ double lds4[4];
+    if constexpr (FIN) {  // one workgroup after the partial launch
+        const double S = final_sum(slot, 0, lds4);
+        if (threadIdx.x == 0) finalize_scalar(st, fin, S);
+        return;
+    }
     double acc = 0.0;
     for (size_t g = (size_t)blockIdx.x * blockDim.x + threadIdx.x; g < nG; g += (size_t)gridDim.x * blockDim.x) {
         const int c0 = cs[g], c1 = cs[g + 1];
@@ -129,15 +135,7 @@ __global__ __launch_bounds__(kRedThreads) void k_cg_assemble(const T *__restrict
         y[g] = out;
     }
     if (fin == kFinNone || !st) return;
-    __shared__ double lds4[4];
-    double part[1] = {block_sum(acc, lds4)};
-    if (publish_partials<1>(slot, part)) {
-        const double S = final_sum(slot, 0, lds4);
-        if (threadIdx.x == 0) {
-            finalize_scalar(st, fin, S);
-            *slot.ticket = 0u;
-        }
-    }
+    store_partial(slot, 0, acc, lds4);
 }
 
 #define CG_DISPATCH_N(N_, CALL)                          \
@@ -204,8 +202,11 @@ struct CgsemOp final : DevOp {
     {
         CG_DISPATCH_N(N, hipLaunchKernelGGL((k_cg_local<NN, double, false>), dim3(cg_grid<NN>(nE)), dim3(256), 0, s,
                                             x, gid, uD, G3, D, Lbuf, (int)nE, st));
-        hipLaunchKernelGGL((k_cg_assemble<double, 0>), dim3(kRedBlocks), dim3(kRedThreads), 0, s, Lbuf, cs, ci,
-                           (const double *)nullptr, nG, y, dotWith, st, fin, slot);
+        hipLaunchKernelGGL((k_cg_assemble<double, 0, false>), dim3(kRedBlocks), dim3(kRedThreads), 0, s, Lbuf, cs,
+                           ci, (const double *)nullptr, nG, y, dotWith, st, fin, slot);
+        if (fin != kFinNone && st)
+            hipLaunchKernelGGL((k_cg_assemble<double, 0, true>), dim3(1), dim3(kRedThreads), 0, s, Lbuf, cs, ci,
+                               (const double *)nullptr, nG, y, dotWith, st, fin, slot);
         SELF_HIP(hipGetLastError());
         return SELF_OK;
     }
@@ -215,8 +216,11 @@ struct CgsemOp final : DevOp {
     {
         CG_DISPATCH_N(N, hipLaunchKernelGGL((k_cg_local<NN, float, false>), dim3(cg_grid<NN>(nE)), dim3(256), 0, s,
                                             x, gid, uD, G3f, Df, Lbuf32, (int)nE, st));
-        hipLaunchKernelGGL((k_cg_assemble<float, 0>), dim3(kRedBlocks), dim3(kRedThreads), 0, s, Lbuf32, cs, ci,
-                           (const double *)nullptr, nG, y, dotWith, st, fin, slot);
+        hipLaunchKernelGGL((k_cg_assemble<float, 0, false>), dim3(kRedBlocks), dim3(kRedThreads), 0, s, Lbuf32, cs,
+                           ci, (const double *)nullptr, nG, y, dotWith, st, fin, slot);
+        if (fin != kFinNone && st)
+            hipLaunchKernelGGL((k_cg_assemble<float, 0, true>), dim3(1), dim3(kRedThreads), 0, s, Lbuf32, cs, ci,
+                               (const double *)nullptr, nG, y, dotWith, st, fin, slot);
         SELF_HIP(hipGetLastError());
         return SELF_OK;
     }
@@ -225,8 +229,11 @@ struct CgsemOp final : DevOp {
     {
         CG_DISPATCH_N(N, hipLaunchKernelGGL((k_cg_local<NN, double, true>), dim3(cg_grid<NN>(nE)), dim3(256), 0, s,
                                             x, gid, uD, G3, D, Lbuf, (int)nE, st));
-        hipLaunchKernelGGL((k_cg_assemble<double, 1>), dim3(kRedBlocks), dim3(kRedThreads), 0, s, Lbuf, cs, ci, bl,
-                           nG, r, (const double *)nullptr, st, fin, slot);
+        hipLaunchKernelGGL((k_cg_assemble<double, 1, false>), dim3(kRedBlocks), dim3(kRedThreads), 0, s, Lbuf, cs,
+                           ci, bl, nG, r, (const double *)nullptr, st, fin, slot);
+        if (fin != kFinNone && st)
+            hipLaunchKernelGGL((k_cg_assemble<double, 1, true>), dim3(1), dim3(kRedThreads), 0, s, Lbuf, cs, ci, bl,
+                               nG, r, (const double *)nullptr, st, fin, slot);
         SELF_HIP(hipGetLastError());
         return SELF_OK;
     }

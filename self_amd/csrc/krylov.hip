@@ -25,53 +25,55 @@ namespace {
 
 __device__ __forceinline__ bool st_done(const SolverState *st) { return st->done != 0; }
 
-template <int K>
-__device__ __forceinline__ void reduce_publish(const RedSlot &slot, double (&acc)[K], SolverState *st,
-                                               const int (&fin)[K])
+// FIN = false: store this workgroup's partial sums; FIN = true (one
+// workgroup, after the partial launch): sum them and update the state
+template <int K, bool FIN>
+__device__ __forceinline__ void reduce_rows(const RedSlot &slot, double (&acc)[K], SolverState *st,
+                                            const int (&fin)[K])
 {
     __shared__ double lds4[4];
-    double part[K];
     #pragma unroll
-    for (int k = 0; k < K; ++k) part[k] = block_sum(acc[k], lds4);
-    if (publish_partials<K>(slot, part)) {
-        #pragma unroll
-        for (int k = 0; k < K; ++k) {
+    for (int k = 0; k < K; ++k) {
+        if constexpr (FIN) {
             const double S = final_sum(slot, k, lds4);
             if (threadIdx.x == 0) finalize_scalar(st, fin[k], S);
+        } else {
+            store_partial(slot, k, acc[k], lds4);
         }
-        if (threadIdx.x == 0) *slot.ticket = 0u;
     }
 }
 
 // a.b with finalize
-template <typename T>
+template <typename T, bool FIN>
 __global__ __launch_bounds__(kRedThreads) void k_dot(const T *__restrict__ a, const T *__restrict__ b, size_t n,
                                                      SolverState *st, int fin, RedSlot slot)
 {
     if (st_done(st)) return;
     double acc[1] = {0.0};
-    for (size_t g = (size_t)blockIdx.x * blockDim.x + threadIdx.x; g < n; g += (size_t)gridDim.x * blockDim.x)
-        acc[0] += (double)a[g] * (double)b[g];
+    if constexpr (!FIN)
+        for (size_t g = (size_t)blockIdx.x * blockDim.x + threadIdx.x; g < n; g += (size_t)gridDim.x * blockDim.x)
+            acc[0] += (double)a[g] * (double)b[g];
     const int f[1] = {fin};
-    reduce_publish<1>(slot, acc, st, f);
+    reduce_rows<1, FIN>(slot, acc, st, f);
 }
 
 // w = P r (Jacobi or identity), num = r.w
-template <typename T>
+template <typename T, bool FIN>
 __global__ __launch_bounds__(kRedThreads) void k_precond_dot(const T *__restrict__ r, const T *__restrict__ dinv,
                                                              T *__restrict__ w, size_t n, SolverState *st, int fin,
                                                              RedSlot slot)
 {
     if (st_done(st)) return;
     double acc[1] = {0.0};
-    for (size_t g = (size_t)blockIdx.x * blockDim.x + threadIdx.x; g < n; g += (size_t)gridDim.x * blockDim.x) {
-        const T rv = r[g];
-        const T wv = dinv ? dinv[g] * rv : rv;
-        w[g] = wv;
-        acc[0] += (double)rv * (double)wv;
-    }
+    if constexpr (!FIN)
+        for (size_t g = (size_t)blockIdx.x * blockDim.x + threadIdx.x; g < n; g += (size_t)gridDim.x * blockDim.x) {
+            const T rv = r[g];
+            const T wv = dinv ? dinv[g] * rv : rv;
+            w[g] = wv;
+            acc[0] += (double)rv * (double)wv;
+        }
     const int f[1] = {fin};
-    reduce_publish<1>(slot, acc, st, f);
+    reduce_rows<1, FIN>(slot, acc, st, f);
 }
 
 // v = w (first iteration) or w + beta v
@@ -87,22 +89,24 @@ __global__ __launch_bounds__(256) void k_update_v(const T *__restrict__ w, T *__
 }
 
 // u += alpha v; r -= alpha z; |r|^2 -> rNorm, stop test
-template <typename T>
+template <typename T, bool FIN>
 __global__ __launch_bounds__(kRedThreads) void k_update_ur(T *__restrict__ u, const T *__restrict__ v,
                                                            T *__restrict__ r, const T *__restrict__ z, size_t n,
                                                            SolverState *st, RedSlot slot)
 {
     if (st_done(st)) return;
-    const T alpha = (T)st->alpha;
     double acc[1] = {0.0};
-    for (size_t g = (size_t)blockIdx.x * blockDim.x + threadIdx.x; g < n; g += (size_t)gridDim.x * blockDim.x) {
-        u[g] = u[g] + alpha * v[g];
-        const T rv = r[g] - alpha * z[g];
-        r[g] = rv;
-        acc[0] += (double)rv * (double)rv;
+    if constexpr (!FIN) {
+        const T alpha = (T)st->alpha;
+        for (size_t g = (size_t)blockIdx.x * blockDim.x + threadIdx.x; g < n; g += (size_t)gridDim.x * blockDim.x) {
+            u[g] = u[g] + alpha * v[g];
+            const T rv = r[g] - alpha * z[g];
+            r[g] = rv;
+            acc[0] += (double)rv * (double)rv;
+        }
     }
     const int f[1] = {kFinRnorm};
-    reduce_publish<1>(slot, acc, st, f);
+    reduce_rows<1, FIN>(slot, acc, st, f);
 }
 
 // ---------------------------------------------------------------- GMRES
@@ -146,21 +150,22 @@ __global__ void k_gmres_begin(SolverState *st, int i)
 }
 
 // H(k,i) = v_k . w
+template <bool FIN>
 __global__ __launch_bounds__(kRedThreads) void k_gmres_h(const double *__restrict__ vk, const double *__restrict__ w,
                                                          size_t n, SolverState *st, int k, int i, RedSlot slot)
 {
     if (inner_off(st)) return;
-    double acc = 0.0;
-    for (size_t g = (size_t)blockIdx.x * blockDim.x + threadIdx.x; g < n; g += (size_t)gridDim.x * blockDim.x)
-        acc += vk[g] * w[g];
     __shared__ double lds4[4];
-    double part[1] = {block_sum(acc, lds4)};
-    if (publish_partials<1>(slot, part)) {
+    if constexpr (!FIN) {
+        double acc = 0.0;
+        for (size_t g = (size_t)blockIdx.x * blockDim.x + threadIdx.x; g < n; g += (size_t)gridDim.x * blockDim.x)
+            acc += vk[g] * w[g];
+        store_partial(slot, 0, acc, lds4);
+    } else {
         const double S = final_sum(slot, 0, lds4);
         if (threadIdx.x == 0) {
             if (k <= i) GH(st, k, i) = S;
             else GH(st, i + 1, i) = sqrt(S);  // k = i+1: the norm of w
-            *slot.ticket = 0u;
         }
     }
 }
@@ -176,33 +181,25 @@ __global__ __launch_bounds__(256) void k_gmres_axpy(double *__restrict__ w, cons
 }
 
 // CGS2 pass: h_k = v_k . w for k = 1..i in one sweep (partials per k)
+template <bool FIN>
 __global__ __launch_bounds__(kRedThreads) void k_gmres_multidot(const double *__restrict__ V, const double *__restrict__ w,
                                                                 size_t n, int i, SolverState *st, double *hOut,
                                                                 RedSlot slot)
 {
     if (inner_off(st)) return;
     __shared__ double lds4[4];
-    __shared__ bool last;
     for (int k = 0; k < i; ++k) {
-        double acc = 0.0;
-        const double *vk = V + (size_t)k * n;
-        for (size_t g = (size_t)blockIdx.x * blockDim.x + threadIdx.x; g < n; g += (size_t)gridDim.x * blockDim.x)
-            acc += vk[g] * w[g];
-        const double p = block_sum(acc, lds4);
-        if (threadIdx.x == 0) slot.partials[k * kRedBlocks + blockIdx.x] = p;
+        if constexpr (!FIN) {
+            double acc = 0.0;
+            const double *vk = V + (size_t)k * n;
+            for (size_t g = (size_t)blockIdx.x * blockDim.x + threadIdx.x; g < n; g += (size_t)gridDim.x * blockDim.x)
+                acc += vk[g] * w[g];
+            store_partial(slot, k, acc, lds4);
+        } else {
+            const double S = final_sum(slot, k, lds4);
+            if (threadIdx.x == 0) hOut[k] = S;
+        }
     }
-    if (threadIdx.x == 0) {
-        __threadfence();
-        last = atomicAdd(slot.ticket, 1u) == gridDim.x - 1;
-    }
-    __syncthreads();
-    if (!last) return;
-    __threadfence();
-    for (int k = 0; k < i; ++k) {
-        const double S = final_sum(slot, k, lds4);
-        if (threadIdx.x == 0) hOut[k] = S;
-    }
-    if (threadIdx.x == 0) *slot.ticket = 0u;
 }
 
 // w -= V h  (CGS2), and accumulate h into H(:,i) when acc != 0
@@ -359,15 +356,14 @@ int KrylovWork::ensure(size_t nn, int mm, int rl, bool f32)
     if ((rc = A((void **)&st, sizeof(SolverState)))) return rc;
     if ((rc = A((void **)&resi, sizeof(double) * std::max(resiLen, 1)))) return rc;
     if ((rc = A((void **)&slot.partials, sizeof(double) * kRedBlocks * std::max(32, m + 2)))) return rc;
-    if ((rc = A((void **)&slot.ticket, sizeof(unsigned)))) return rc;
-    SELF_HIP(hipMemset(slot.ticket, 0, sizeof(unsigned)));
+
     SELF_HIP(hipMemset(small, 0, sizeof(double) * smallN));
     return SELF_OK;
 }
 
 void KrylovWork::release()
 {
-    void *p[] = {r, v, z, w, u, r32, v32, z32, w32, u32, V, Z, small, st, resi, slot.partials, slot.ticket};
+    void *p[] = {r, v, z, w, u, r32, v32, z32, w32, u32, V, Z, small, st, resi, slot.partials};
     for (void *q : p)
         if (q) (void)hipFree(q);
     r = v = z = w = u = nullptr;
@@ -431,11 +427,11 @@ int run_cg(DevOp &op, KrylovWork &wk, bool precond, int stopRule, double tol, in
     const int gv = vgrid(n);
     for (int it0 = 1; it0 <= maxIters; it0 += kChunk) {
         for (int it = it0; it < it0 + kChunk && it <= maxIters; ++it) {
-            hipLaunchKernelGGL(k_precond_dot<double>, dim3(kRedBlocks), dim3(kRedThreads), 0, s, wk.r, dinv, wk.w, n,
+            SELF_RED_LAUNCH((k_precond_dot<double, false>), (k_precond_dot<double, true>), s, wk.r, dinv, wk.w, n,
                                wk.st, (int)kFinNum, wk.slot);
             hipLaunchKernelGGL(k_update_v<double>, dim3(gv), dim3(256), 0, s, wk.w, wk.v, n, wk.st);
             if ((rc = op.matvec(wk.v, wk.z, wk.v, wk.st, kFinAlpha, wk.slot, s))) return rc;
-            hipLaunchKernelGGL(k_update_ur<double>, dim3(kRedBlocks), dim3(kRedThreads), 0, s, wk.u, wk.v, wk.r,
+            SELF_RED_LAUNCH((k_update_ur<double, false>), (k_update_ur<double, true>), s, wk.u, wk.v, wk.r,
                                wk.z, n, wk.st, wk.slot);
         }
         SELF_HIP(hipGetLastError());
@@ -469,7 +465,7 @@ int run_gmres(DevOp &op, KrylovWork &wk, bool precond, double tol, int nIt, int 
     const int gv = vgrid(n);
     double *hTmp = wk.small + (size_t)(m + 1) * m + (size_t)m * m + 2 * m + 2 * (m + 1);  // scratch for CGS2
     for (int j = 1; j <= nIt; ++j) {
-        hipLaunchKernelGGL(k_dot<double>, dim3(kRedBlocks), dim3(kRedThreads), 0, s, wk.r, wk.r, n, wk.st,
+        SELF_RED_LAUNCH((k_dot<double, false>), (k_dot<double, true>), s, wk.r, wk.r, n, wk.st,
                            (int)kFinBnorm, wk.slot);
         hipLaunchKernelGGL(k_gmres_cycle_start, dim3(1), dim3(64), 0, s, wk.st);
         hipLaunchKernelGGL(k_scale<double>, dim3(gv), dim3(256), 0, s, wk.r, wk.V, n, wk.st, 0, 0);
@@ -487,7 +483,7 @@ int run_gmres(DevOp &op, KrylovWork &wk, bool precond, double tol, int nIt, int 
             if ((rc = op.matvec(xin, wk.w, nullptr, wk.st, kFinNone, wk.slot, s))) return rc;
             if (ortho == 0) {  // modified Gram-Schmidt, as the reference
                 for (int k = 1; k <= i; ++k) {
-                    hipLaunchKernelGGL(k_gmres_h, dim3(kRedBlocks), dim3(kRedThreads), 0, s,
+                    SELF_RED_LAUNCH((k_gmres_h<false>), (k_gmres_h<true>), s,
                                        (const double *)(wk.V + (size_t)(k - 1) * n), (const double *)wk.w, n, wk.st,
                                        k, i, wk.slot);
                     hipLaunchKernelGGL(k_gmres_axpy, dim3(gv), dim3(256), 0, s, wk.w,
@@ -496,13 +492,13 @@ int run_gmres(DevOp &op, KrylovWork &wk, bool precond, double tol, int nIt, int 
                 }
             } else {  // classical Gram-Schmidt applied twice: 2 sweeps over V
                 for (int pass = 0; pass < 2; ++pass) {
-                    hipLaunchKernelGGL(k_gmres_multidot, dim3(kRedBlocks), dim3(kRedThreads), 0, s,
+                    SELF_RED_LAUNCH((k_gmres_multidot<false>), (k_gmres_multidot<true>), s,
                                        (const double *)wk.V, (const double *)wk.w, n, i, wk.st, hTmp, wk.slot);
                     hipLaunchKernelGGL(k_gmres_multiaxpy, dim3(gv), dim3(256), 0, s, wk.w, (const double *)wk.V, n,
                                        i, wk.st, (const double *)hTmp, pass == 0 ? 1 : 2);
                 }
             }
-            hipLaunchKernelGGL(k_gmres_h, dim3(kRedBlocks), dim3(kRedThreads), 0, s, (const double *)wk.w,
+            SELF_RED_LAUNCH((k_gmres_h<false>), (k_gmres_h<true>), s, (const double *)wk.w,
                                (const double *)wk.w, n, wk.st, i + 1, i, wk.slot);
             hipLaunchKernelGGL(k_gmres_givens, dim3(1), dim3(64), 0, s, wk.st, i);
             if (i < m + 1)
@@ -584,15 +580,15 @@ int krylov_solve_ir(DevOp &op, KrylovWork &wk, int precond, double tol, double i
         hipLaunchKernelGGL(k_fill<float>, dim3(gv), dim3(256), 0, s, wk.u32, n, 0.0f);
         if ((rc = init_state(wk, innerTol, maxInner, kStopNormRel, 0, s))) return rc;
         // r0 of the inner problem is |r/|r|| = 1
-        hipLaunchKernelGGL(k_dot<float>, dim3(kRedBlocks), dim3(kRedThreads), 0, s, wk.r32, wk.r32, n, wk.st,
+        SELF_RED_LAUNCH((k_dot<float, false>), (k_dot<float, true>), s, wk.r32, wk.r32, n, wk.st,
                            (int)kFinR0Plain, wk.slot);
         for (int it0 = 1; it0 <= maxInner; it0 += kChunk) {
             for (int it = it0; it < it0 + kChunk && it <= maxInner; ++it) {
-                hipLaunchKernelGGL(k_precond_dot<float>, dim3(kRedBlocks), dim3(kRedThreads), 0, s, wk.r32, dinv,
+                SELF_RED_LAUNCH((k_precond_dot<float, false>), (k_precond_dot<float, true>), s, wk.r32, dinv,
                                    wk.w32, n, wk.st, (int)kFinNum, wk.slot);
                 hipLaunchKernelGGL(k_update_v<float>, dim3(gv), dim3(256), 0, s, wk.w32, wk.v32, n, wk.st);
                 if ((rc = op.matvec32(wk.v32, wk.z32, wk.v32, wk.st, kFinAlpha, wk.slot, s))) return rc;
-                hipLaunchKernelGGL(k_update_ur<float>, dim3(kRedBlocks), dim3(kRedThreads), 0, s, wk.u32, wk.v32,
+                SELF_RED_LAUNCH((k_update_ur<float, false>), (k_update_ur<float, true>), s, wk.u32, wk.v32,
                                    wk.r32, wk.z32, n, wk.st, wk.slot);
             }
             SELF_HIP(hipGetLastError());

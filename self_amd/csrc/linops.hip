@@ -79,22 +79,20 @@ __global__ __launch_bounds__(256) void k_e1d_apply(const double *__restrict__ s,
     for (int k = threadIdx.x; k < nd; k += blockDim.x) out[k] = Au[k];
 }
 
-template <typename T>
+template <typename T, bool FIN>
 __global__ __launch_bounds__(kRedThreads) void k_op_dot(const T *__restrict__ a, const T *__restrict__ b, size_t n,
                                                         SolverState *st, int fin, RedSlot slot, int residual)
 {
     if (st->done || (!residual && st->innerExit)) return;
-    double acc = 0.0;
-    for (size_t g = (size_t)blockIdx.x * blockDim.x + threadIdx.x; g < n; g += (size_t)gridDim.x * blockDim.x)
-        acc += (double)a[g] * (double)b[g];
     __shared__ double lds4[4];
-    double part[1] = {block_sum(acc, lds4)};
-    if (publish_partials<1>(slot, part)) {
+    if constexpr (FIN) {
         const double S = final_sum(slot, 0, lds4);
-        if (threadIdx.x == 0) {
-            finalize_scalar(st, fin, S);
-            *slot.ticket = 0u;
-        }
+        if (threadIdx.x == 0) finalize_scalar(st, fin, S);
+    } else {
+        double acc = 0.0;
+        for (size_t g = (size_t)blockIdx.x * blockDim.x + threadIdx.x; g < n; g += (size_t)gridDim.x * blockDim.x)
+            acc += (double)a[g] * (double)b[g];
+        store_partial(slot, 0, acc, lds4);
     }
 }
 
@@ -102,7 +100,7 @@ int fused_dot(const double *a, const double *b, size_t n, SolverState *st, int f
               hipStream_t s)
 {
     if (fin == kFinNone || !st || !a) return SELF_OK;
-    hipLaunchKernelGGL(k_op_dot<double>, dim3(kRedBlocks), dim3(kRedThreads), 0, s, a, b, n, st, fin, slot, residual);
+    SELF_RED_LAUNCH((k_op_dot<double, false>), (k_op_dot<double, true>), s, a, b, n, st, fin, slot, residual);
     SELF_HIP(hipGetLastError());
     return SELF_OK;
 }

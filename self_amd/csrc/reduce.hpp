@@ -1,11 +1,15 @@
 // reduce.hpp -- deterministic device reductions for the Krylov solvers.
 //
-// A reduction kernel runs on a fixed grid; each workgroup writes one partial
-// sum, and the last workgroup to finish (atomic ticket) adds the partials
-// in block order and publishes the result.  For a given vector length the
-// summation tree is fixed, so a solve gives bitwise the same scalars
-// whatever else runs on the device (batch invariance), and there is no
-// second "finalize" launch.
+// A reduction runs as two launches on one stream: the partial kernel
+// (kRedBlocks workgroups) writes one partial sum per workgroup and row,
+// then the same kernel compiled as its final form runs as a single
+// workgroup that adds the partials in a fixed tree and applies the
+// epilogue (scalar bookkeeping of the solver state).  The kernel boundary
+// orders the two, so no workgroup needs a device-scope fence or ticket: on
+// a multi-XCD part such a fence writes back the XCD's L2, and one per
+// workgroup costs more than the reduction.  For a given vector length the
+// summation order is fixed, so solves are bitwise reproducible whatever
+// else runs on the device.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -13,11 +17,10 @@
 namespace self {
 
 constexpr int kRedThreads = 256;
-constexpr int kRedBlocks = 512;  // fixed partial count
+constexpr int kRedBlocks = 2048;  // partial sums per row (8 workgroups per CU)
 
 struct RedSlot {
-    double *partials;    // [kRedBlocks * 4] (up to 4 sums per reduction)
-    unsigned *ticket;    // one counter, reset to 0 by the finishing block
+    double *partials;  // [rows][kRedBlocks]
 };
 
 // sum over the 64 lanes of a wavefront
@@ -41,31 +44,26 @@ __device__ __forceinline__ double block_sum(double v, double *lds4)
     return r;
 }
 
-// Publish this block's partials (K sums) and return true in the block that
-// finishes last; that block then sees every partial.
-template <int K>
-__device__ __forceinline__ bool publish_partials(const RedSlot &slot, const double (&part)[K])
+// partial kernel: store this workgroup's sum of row k (every thread calls)
+__device__ __forceinline__ void store_partial(const RedSlot &slot, int k, double acc, double *lds4)
 {
-    __shared__ bool last;
-    if (threadIdx.x == 0) {
-        #pragma unroll
-        for (int k = 0; k < K; ++k) slot.partials[k * kRedBlocks + blockIdx.x] = part[k];
-        __threadfence();
-        const unsigned t = atomicAdd(slot.ticket, 1u);
-        last = (t == gridDim.x - 1);
-    }
-    __syncthreads();
-    if (last) __threadfence();
-    return last;
+    const double p = block_sum(acc, lds4);
+    if (threadIdx.x == 0) slot.partials[(size_t)k * kRedBlocks + blockIdx.x] = p;
 }
 
-// In the last block: fixed-order sum of partial k over gridDim.x blocks.
+// final kernel (one workgroup): fixed-order sum of row k, valid in thread 0
 __device__ __forceinline__ double final_sum(const RedSlot &slot, int k, double *lds4)
 {
     double v = 0.0;
-    for (int b = threadIdx.x; b < (int)gridDim.x; b += blockDim.x)
-        v += __hip_atomic_load(&slot.partials[k * kRedBlocks + b], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    for (int b = threadIdx.x; b < kRedBlocks; b += blockDim.x) v += slot.partials[(size_t)k * kRedBlocks + b];
     return block_sum(v, lds4);
 }
+
+// the two launches of a reduction kernel KERNEL<..., FIN>: partial then final
+#define SELF_RED_LAUNCH(KPART, KFIN, stream, ...)                                                   \
+    do {                                                                                            \
+        hipLaunchKernelGGL(KPART, dim3(::self::kRedBlocks), dim3(::self::kRedThreads), 0, stream, __VA_ARGS__); \
+        hipLaunchKernelGGL(KFIN, dim3(1), dim3(::self::kRedThreads), 0, stream, __VA_ARGS__);      \
+    } while (0)
 
 }  // namespace self

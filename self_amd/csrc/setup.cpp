@@ -299,4 +299,61 @@ int self_nodal2d_build(int N, int quadType, int form, int quirk, double *nodes, 
     return SELF_OK;
 }
 
+// BuildNodalStorage_3D (src/nodal/NodalStorage_3D_Class.f90:107-264) with
+// equal degree in the three directions: nodes, weights, the derivative
+// matrix (weak DG operator -W^-1 D^T W for SELF_DG, :223-251) and the
+// Lagrange vectors at -1 (west/south/bottom) and +1 (east/north/top).
+int self_nodal3d_build(int N, int quadType, int form, double *nodes, double *qw, double *dMat, double *lagLeft,
+                       double *lagRight)
+{
+    SELF_REQUIRE(N >= 1 && nodes && qw && dMat && lagLeft && lagRight, SELF_ERR_ARG, "self_nodal3d_build: bad arguments");
+    SELF_REQUIRE(form == SELF_DG || form == SELF_CG, SELF_ERR_ARG, "BuildNodalStorage_3D: invalid SEM form %d", form);
+    int rc = legendre_quadrature(N, quadType, nodes, qw);
+    if (rc) return rc;
+    const int n1 = N + 1;
+    std::vector<double> bw(n1), D(n1 * n1);
+    barycentric(N, nodes, bw.data());
+    lagrange_at(N, nodes, bw.data(), -1.0, lagLeft);
+    lagrange_at(N, nodes, bw.data(), 1.0, lagRight);
+    derivative_matrix(N, nodes, bw.data(), D.data());
+    for (int j = 0; j <= N; ++j)
+        for (int i = 0; i <= N; ++i)
+            dMat[i + n1 * j] = (form == SELF_CG) ? D[i + n1 * j] : -D[j + n1 * i] * qw[j] / qw[i];
+    return SELF_OK;
+}
+
+// One-direction factor of RollOffFilter3D (src/filters/RollOffFilter3D_Class.f90
+// :87-180).  The reference's dense filter is VInv diag(exp(-alpha r)) V with
+// r = a^2 + b^2 + c^2 over Legendre tensor modes, alpha = ln 2 / nc^2; the
+// modes, the weights and exp(-alpha r) all factor over the three directions,
+// so the dense matrix equals F1 (x) F1 (x) F1 with
+//   F1(i, m) = sum_a L_a(x_i) exp(-alpha a^2) L_a(x_m) w_m / n_a,
+//   n_a = sum_i L_a(x_i)^2 w_i,
+// which is applied as three (N+1) x (N+1) passes.  Column-major F1[i + (N+1) m].
+int self_rolloff_filter1d(int N, int nCutoff, const double *nodes, const double *w, double *F1)
+{
+    SELF_REQUIRE(N >= 1 && nCutoff >= 1 && nodes && w && F1, SELF_ERR_ARG, "self_rolloff_filter1d: bad arguments");
+    const int n1 = N + 1;
+    std::vector<double> L(n1 * n1), nrm(n1);
+    for (int a = 0; a <= N; ++a) {
+        double acc = 0.0;
+        for (int i = 0; i <= N; ++i) {
+            double p, dp;
+            legendre_p(a, nodes[i], p, dp);
+            L[a + n1 * i] = p;
+            acc += p * p * w[i];
+        }
+        nrm[a] = acc;
+    }
+    const double sc = (double)nCutoff, alpha = std::log(2.0) / (sc * sc);
+    for (int m = 0; m <= N; ++m)
+        for (int i = 0; i <= N; ++i) {
+            double acc = 0.0;
+            for (int a = 0; a <= N; ++a)
+                acc += L[a + n1 * i] * std::exp(-alpha * (double)(a * a)) * L[a + n1 * m] * w[m] / nrm[a];
+            F1[i + n1 * m] = acc;
+        }
+    return SELF_OK;
+}
+
 }  // extern "C"

@@ -40,3 +40,36 @@ class QuadMesh:
             nXElem, nYElem, N, int(periodicX), int(periodicY), xScale, yScale, warp,
             L.ptr(L.f64(sNodes)), L.ptr(self.edges, L.ip),
             *[L.ptr(getattr(self, n)) for n in ("x", "y", "dxds", "dxdp", "dyds", "dydp", "J", "nHat", "nLen")]))
+
+
+class HexMesh:
+    """Structured hex mesh of the unit cube (self_hexmesh_structured).
+
+    x, y, z, J: (nElems, N+1, N+1, N+1) indexed [e, k(q), j(p), i(s)];
+    cov: (nElems, N+1, N+1, N+1, 9); nHat, bpos: (nElems, 6, N+1, N+1, 3)
+    indexed [e, side, b, a, component]; nLen: (nElems, 6, N+1, N+1);
+    faces: (nFaces, 9) int32 = e1, s1, e2, s2, iStart, jStart, iInc, jInc, swap.
+    """
+
+    def __init__(self, nX, nY, nZ, N, nodes, periodic=(True, True, True), scale=(1.0, 1.0, 1.0), warp=0.0,
+                 warpSign=(1, 1, 1)):
+        import ctypes as C
+        lib = L.lib()
+        nE, nF = C.c_int(), C.c_int()
+        L.check(lib.self_hexmesh_counts(nX, nY, nZ, int(periodic[0]), int(periodic[1]), int(periodic[2]),
+                                        C.byref(nE), C.byref(nF)))
+        self.nX, self.nY, self.nZ, self.N = nX, nY, nZ, N
+        self.nElems, self.nFaces = nE.value, nF.value
+        n1, E = N + 1, self.nElems
+        self.faces = np.zeros((self.nFaces, 9), dtype=np.int32)
+        for name in ("x", "y", "z", "J"):
+            setattr(self, name, np.zeros((E, n1, n1, n1)))
+        self.cov = np.zeros((E, n1, n1, n1, 9))
+        self.nHat = np.zeros((E, 6, n1, n1, 3))
+        self.nLen = np.zeros((E, 6, n1, n1))
+        self.bpos = np.zeros((E, 6, n1, n1, 3))
+        sign = L.i32(warpSign)
+        L.check(lib.self_hexmesh_structured(
+            nX, nY, nZ, N, int(periodic[0]), int(periodic[1]), int(periodic[2]), scale[0], scale[1], scale[2],
+            warp, L.ptr(sign, L.ip), L.ptr(L.f64(nodes)), L.ptr(self.faces, L.ip),
+            *[L.ptr(getattr(self, n)) for n in ("x", "y", "z", "cov", "J", "nHat", "nLen", "bpos")]))

@@ -81,3 +81,27 @@ class NodalStorage2D:
     def flat(self, name):
         """Column-major flat copy of a matrix, as the C ABI takes it."""
         return np.ascontiguousarray(getattr(self, name).ravel(order="F"))
+
+
+class NodalStorage3D:
+    """NodalStorage_3D (src/nodal/NodalStorage_3D_Class.f90:107-264), equal
+    degree in s, p, q: nodes, qWeight, dMat (weak DG operator for L.DG),
+    lagLeft (at -1) and lagRight (at +1)."""
+
+    def __init__(self, N: int, quadrature: int = L.GAUSS, form: int = L.DG):
+        n1 = N + 1
+        self.N = N
+        self.nodes, self.qWeight = np.zeros(n1), np.zeros(n1)
+        dm = np.zeros(n1 * n1)
+        self.lagLeft, self.lagRight = np.zeros(n1), np.zeros(n1)
+        L.check(L.lib().self_nodal3d_build(N, quadrature, form, L.ptr(self.nodes), L.ptr(self.qWeight), L.ptr(dm),
+                                           L.ptr(self.lagLeft), L.ptr(self.lagRight)))
+        self.dMat = dm.reshape(n1, n1, order="F")
+
+
+def rolloff_filter1d(N, nCutoff, nodes, weights):
+    """Direction factor F1 of RollOffFilter3D ((N+1) x (N+1)); the dense
+    filter is kron(F1, kron(F1, F1))."""
+    F1 = np.zeros((N + 1) * (N + 1))
+    L.check(L.lib().self_rolloff_filter1d(N, nCutoff, L.ptr(L.f64(nodes)), L.ptr(L.f64(weights)), L.ptr(F1)))
+    return F1.reshape(N + 1, N + 1, order="F")

@@ -176,3 +176,44 @@ def cgsem_helmholtz(nX=128, N=7, seed=DEFAULT_SEED, tolerance=1e-10, maxIters=20
     model.SetDirichletBoundaryCondition(np.zeros_like(k))
     model.SetSolution(np.zeros_like(k))
     return model, {"k": k, "f": f}
+
+
+def adv3d_gaussians_fields(x, y, z, seed=DEFAULT_SEED, n=3):
+    """Sum of n seeded Gaussians: centres ~ U[0.2, 0.8]^3, widths ~ U[0.05, 0.1],
+    exp(-r^2 / (2 sigma^2))."""
+    rng = SplitMix64(seed + 3)
+    c = np.zeros_like(x)
+    for _ in range(n):
+        x0, y0, z0 = (0.2 + 0.6 * rng.uniform() for _ in range(3))
+        sig = 0.05 + 0.05 * rng.uniform()
+        c += np.exp(-((x - x0) ** 2 + (y - y0) ** 2 + (z - z0) ** 2) / (2.0 * sig * sig))
+    return c
+
+
+def adv3d_warp_signs(seed=DEFAULT_SEED):
+    rng = SplitMix64(seed + 5)
+    return tuple(1 if rng.uniform() < 0.5 else -1 for _ in range(3))
+
+
+def adv3d_cfl_dt(N, nX, nY, nZ, u=(1.0, 0.7, 0.3), cfl=0.5):
+    """dt = cfl / ((N+1)^2 sum_d |u_d| / h_d) on the unit cube."""
+    rate = abs(u[0]) * nX + abs(u[1]) * nY + abs(u[2]) * nZ
+    return cfl / ((N + 1) ** 2 * rate)
+
+
+def adv3d_gaussians(nX=32, nY=32, nZ=16, N=7, warp=0.05, u=(1.0, 0.7, 0.3), nTracers=1, useFilter=True,
+                    seed=DEFAULT_SEED, cfl=0.5, nCutoff=3):
+    """3-D linear advection on the periodic unit cube: nX x nY x nZ elements
+    of degree N, seeded warp of amplitude `warp`, uniform velocity u, tracer
+    = sum of 3 seeded Gaussians, RK3 dt from the CFL number.  Returns (model, info)."""
+    from .advection3d import Advection3D, Advection3DParams
+    dt = adv3d_cfl_dt(N, nX, nY, nZ, u, cfl)
+    p = Advection3DParams(polyDeg=N, nXElem=nX, nYElem=nY, nZElem=nZ, nTracers=nTracers, nCutoff=nCutoff, dt=dt,
+                          warp=warp, warpSign=adv3d_warp_signs(seed), useFilter=useFilter)
+    model = Advection3D(p)
+    m = model.mesh
+    c = np.zeros((m.nElems, nTracers, N + 1, N + 1, N + 1))
+    c[:, 0] = adv3d_gaussians_fields(m.x, m.y, m.z, seed)
+    model.SetSolution(c)
+    model.SetVelocity(np.full_like(m.x, u[0]), np.full_like(m.x, u[1]), np.full_like(m.x, u[2]))
+    return model, {"c": c, "dt": dt, "u": u}
