@@ -156,6 +156,30 @@ def bench_cgsem(nX=128, N=7, tol=1e-10):
     return out
 
 
+def bench_serving(n=512, workers=8, steps=200):
+    """Request stream: 80 % shallow-water integrations (8..64 squared
+    elements, N in {3,5,7}, 200 RK3 steps), 20 % CGSEM solves (16..128
+    squared, N in {3,5,7}, Jacobi-PCG to 1e-10), for the FP64 and the mixed
+    forms.  A saturation pass (all requests at once) measures capacity; the
+    reported latencies come from Poisson arrivals at 80 % of it."""
+    from self_amd import serving as S
+    out = {}
+    for prec in ("fp64", "mixed"):
+        specs = S.request_stream(n, precision=prec, steps=steps)
+        srv = S.Server(workers=workers)
+        _, sat = S.run_stream(srv, specs)
+        rate = 0.8 * sat["throughput_req_s"]
+        specs = S.request_stream(n, precision=prec, steps=steps)
+        _, poi = S.run_stream(srv, specs, rate=rate)
+        srv.close()
+        out[prec] = {"capacity_req_s": sat["throughput_req_s"], "arrival_rate_req_s": rate,
+                     "throughput_req_s": poi["throughput_req_s"], "p50_ms": poi["p50_ms"], "p99_ms": poi["p99_ms"],
+                     "mean_ms": poi["mean_ms"], "sw_dof_stage_per_s": poi["sw_dof_stage_per_s"], "ok": poi["ok"]}
+    out["stream"] = f"{n} requests, 80% SW (8..64^2, N 3/5/7, {steps} RK3 steps) / 20% CGSEM (16..128^2, N 3/5/7)"
+    out["workers"] = workers
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -163,6 +187,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-extras", action="store_true", help="skip the advection and CGSEM lines")
+    ap.add_argument("--no-serve", action="store_true", help="skip the serving stream")
+    ap.add_argument("--serve-requests", type=int, default=512)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); the shallow-water path "
@@ -252,6 +278,8 @@ def main():
         out["adv2d"] = bench_adv2d()
         out["adv3d"] = bench_adv3d()
         out["cgsem"] = bench_cgsem()
+    if not args.no_serve:
+        out["serving"] = bench_serving(args.serve_requests)
     if not args.no_cpu:
         cb = cpu_baseline(nX, N)
         out["cpu_baseline"] = cb

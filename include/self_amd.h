@@ -115,6 +115,50 @@ int self_quadmesh_structured(int nX, int nY, int N, int periodicX, int periodicY
                              int *edges, double *x, double *y, double *dxds, double *dxdp,
                              double *dyds, double *dydp, double *J, double *nHat, double *nLen);
 
+/* ------------------------------------------------- nodal library operators
+ * Batched over nElems elements stored one after the other, each in the
+ * reference's column-major per-element layout.  The plain forms take host
+ * arrays (copied in and out, synchronous); the _dev forms take device
+ * pointers and enqueue on `stream` (a hipStream_t, NULL = default stream).
+ *
+ * ApplyDerivativeMatrix_NodalStorage_2D (src/nodal/NodalStorage_2D_Class.f90
+ * :620-646): dFds(:,j) = dMatS f(:,j), dFdp(i,:) = dMatP f(i,:). */
+int self_nodal2d_apply_derivative(int N, size_t nElems, const double *dMatS, const double *dMatP, const double *f,
+                                  double *dFds, double *dFdp);
+int self_nodal2d_apply_derivative_dev(int N, size_t nElems, const double *dMatS, const double *dMatP,
+                                      const double *f, double *dFds, double *dFdp, void *stream);
+/* CalculateAtBoundaries_NodalStorage_2D (:650-680): fWest/fEast along s
+ * lines, fSouth/fNorth along p lines; outputs (0:N) per element. */
+int self_nodal2d_boundary_interp(int N, size_t nElems, const double *lW, const double *lE, const double *lS,
+                                 const double *lN, const double *f, double *fW, double *fE, double *fS, double *fN);
+int self_nodal2d_boundary_interp_dev(int N, size_t nElems, const double *lW, const double *lE, const double *lS,
+                                     const double *lN, const double *f, double *fW, double *fE, double *fS,
+                                     double *fN, void *stream);
+/* ApplyDerivativeMatrix_NodalStorage_3D (src/nodal/NodalStorage_3D_Class.f90
+ * :737-774), the same matrix in s, p and q. */
+int self_nodal3d_apply_derivative(int N, size_t nElems, const double *dMat, const double *f, double *dFds,
+                                  double *dFdp, double *dFdq);
+int self_nodal3d_apply_derivative_dev(int N, size_t nElems, const double *dMat, const double *f, double *dFds,
+                                      double *dFdp, double *dFdq, void *stream);
+/* CalculateAtBoundaries_NodalStorage_3D (:778-823): faces (0:N,0:N,1:6)
+ * per element in the order south, east, north, west, bottom, top. */
+int self_nodal3d_boundary_interp(int N, size_t nElems, const double *lagLeft, const double *lagRight,
+                                 const double *f, double *faces);
+int self_nodal3d_boundary_interp_dev(int N, size_t nElems, const double *lagLeft, const double *lagRight,
+                                     const double *f, double *faces, void *stream);
+/* CoarseToFine_2D (src/interp/Lagrange_2D_Class.f90:488-528): fOld
+ * (0:nS,0:nP) -> fNew (0:nNewP,0:nNewS), the new p index first as the
+ * reference returns it; Ts (0:nNewS,0:nS), Tp (0:nNewP,0:nP). */
+int self_coarse_to_fine_2d(int nS, int nP, size_t nElems, const double *Ts, const double *Tp, int nNewS, int nNewP,
+                           const double *fOld, double *fNew);
+int self_coarse_to_fine_2d_dev(int nS, int nP, size_t nElems, const double *Ts, const double *Tp, int nNewS,
+                               int nNewP, const double *fOld, double *fNew, void *stream);
+/* CoarseToFine_3D (src/interp/Lagrange_3D_Class.f90:545-604): (0:N)^3 ->
+ * (0:nNew)^3 with T (0:nNew,0:N) in each direction. */
+int self_coarse_to_fine_3d(int N, size_t nElems, const double *T, int nNew, const double *fOld, double *fNew);
+int self_coarse_to_fine_3d_dev(int N, size_t nElems, const double *T, int nNew, const double *fOld, double *fNew,
+                               void *stream);
+
 /* ------------------------------------------- conservative shallow water 2-D
  * Replaces the ShallowWater type of
  * src/highend/shallowwater/ConservativeShallowWaterClass.f90 (fields :60-151).
@@ -371,6 +415,53 @@ int self_cgsem2d_solve(self_cgsem2d_t h, int method, int precond, double tol, in
 int self_cgsem2d_solve_mixed(self_cgsem2d_t h, int precond, double tol, double innerTol, int maxInner,
                              int maxOuter, int *ioerr, int *outerIters, int *innerIters, double *relResidual,
                              double *elapsed_ms);
+
+/* --------------------------------------------------------------- serving
+ * Request loop (no reference counterpart: the reference's drivers run one
+ * model at a time).  Shallow-water integrations and CGSEM elliptic solves
+ * of different mesh sizes and degrees are admitted as they arrive against
+ * free HBM, run concurrently on their own HIP streams from a pool of worker
+ * threads, and report queue time, run time and latency.  Results are
+ * bitwise the same alone and in any mix (every kernel is deterministic). */
+#define SELF_REQ_SW2D 1
+#define SELF_REQ_CGSEM2D 2
+
+typedef struct {
+    int kind;        /* SELF_REQ_SW2D | SELF_REQ_CGSEM2D */
+    int nX;          /* nX x nX elements (SW: doubly periodic square of side L; CGSEM: walled unit square) */
+    int N;           /* degree: SW on Gauss nodes, CGSEM on Gauss-Lobatto nodes */
+    int precision;   /* SELF_FP64 | SELF_MIXED (SW: mixed tendency; CGSEM: FP32 inner PCG + FP64 refinement) */
+    int steps;       /* SW: RK3 steps */
+    int precond;     /* CGSEM: 0 identity, 1 Jacobi */
+    int maxIters;    /* CGSEM: iteration limit */
+    double L, g, f0, dt;  /* SW: domain side, gravity, Coriolis parameter, time step */
+    double tol;      /* CGSEM: relative tolerance */
+    const double *in0;    /* SW: solution (0:N,0:N,1:3) per element; CGSEM: flux coefficient (0:N,0:N) */
+    const double *in1;    /* SW: bathymetry (h, 0, 0) per element; CGSEM: source */
+    const double *in2;    /* CGSEM: Dirichlet values (nullable = 0) */
+    double *out;          /* SW: final solution; CGSEM: solution */
+} self_request;
+
+typedef struct {
+    int status;           /* SELF_OK or the failing call's status */
+    int ioerr, iterations;/* CGSEM */
+    double queue_ms, run_ms, latency_ms;  /* arrival -> start, start -> end, arrival -> end */
+    size_t bytes;         /* device memory reserved for the request */
+} self_request_result;
+
+typedef struct self_server_s *self_server_t;
+
+/* workers: concurrent requests (1..64); memFraction: share of the free HBM
+ * (at creation) that admitted requests may reserve. */
+int self_serve_create(int workers, double memFraction, self_server_t *out);
+/* Enqueue a request (arrival time = now); its arrays must stay valid until
+ * self_serve_wait returns for it. */
+int self_serve_submit(self_server_t s, const self_request *req, long long *id);
+/* Block until the request finished; fills res and forgets the request. */
+int self_serve_wait(self_server_t s, long long id, self_request_result *res);
+int self_serve_stats(self_server_t s, int *queued, int *running, size_t *reservedBytes, size_t *budgetBytes);
+/* Runs what is queued, then joins the workers. */
+int self_serve_destroy(self_server_t s);
 
 #ifdef __cplusplus
 }

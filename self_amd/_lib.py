@@ -17,6 +17,10 @@ LIB_PATH = os.path.join(_HERE, "libself_amd.so")
 SELF_OK = 0
 SELF_NOT_CONVERGED = -1
 SELF_ERR_OTHER = -2
+SELF_ERR_HIP = -3
+SELF_ERR_OOM = -4
+SELF_ERR_ARG = -5
+SELF_ERR_STATE = -6
 
 GAUSS, GAUSS_LOBATTO = 1, -1
 DG, CG = 2000, 2001
@@ -68,6 +72,20 @@ def ptr(a, kind=dp):
 class SWDesc(C.Structure):
     _fields_ = [("N", C.c_int), ("nElems", C.c_int), ("nEdges", C.c_int), ("precision", C.c_int),
                 ("form", C.c_int), ("g", C.c_double), ("dt", C.c_double), ("linearDrag", C.c_double)]
+
+
+class Request(C.Structure):
+    """self_request of include/self_amd.h"""
+    _fields_ = [("kind", C.c_int), ("nX", C.c_int), ("N", C.c_int), ("precision", C.c_int), ("steps", C.c_int),
+                ("precond", C.c_int), ("maxIters", C.c_int), ("L", C.c_double), ("g", C.c_double),
+                ("f0", C.c_double), ("dt", C.c_double), ("tol", C.c_double), ("in0", dp), ("in1", dp),
+                ("in2", dp), ("out", dp)]
+
+
+class Result(C.Structure):
+    """self_request_result of include/self_amd.h"""
+    _fields_ = [("status", C.c_int), ("ioerr", C.c_int), ("iterations", C.c_int), ("queue_ms", C.c_double),
+                ("run_ms", C.c_double), ("latency_ms", C.c_double), ("bytes", C.c_size_t)]
 
 
 _SIGS = {
@@ -146,6 +164,24 @@ _SIGS = {
     "self_adv3d_forward_step_rk3": ([C.c_void_p, C.c_double, C.c_int, dp], C.c_int),
     "self_adv3d_filter_count": ([C.c_void_p, C.c_int, C.POINTER(C.c_uint)], C.c_int),
     "self_adv3d_synchronize": ([C.c_void_p], C.c_int),
+    "self_nodal2d_apply_derivative": ([C.c_int, C.c_size_t, dp, dp, dp, dp, dp], C.c_int),
+    "self_nodal2d_apply_derivative_dev": ([C.c_int, C.c_size_t, dp, dp, dp, dp, dp, C.c_void_p], C.c_int),
+    "self_nodal2d_boundary_interp": ([C.c_int, C.c_size_t] + [dp] * 9, C.c_int),
+    "self_nodal2d_boundary_interp_dev": ([C.c_int, C.c_size_t] + [dp] * 9 + [C.c_void_p], C.c_int),
+    "self_nodal3d_apply_derivative": ([C.c_int, C.c_size_t] + [dp] * 5, C.c_int),
+    "self_nodal3d_apply_derivative_dev": ([C.c_int, C.c_size_t] + [dp] * 5 + [C.c_void_p], C.c_int),
+    "self_nodal3d_boundary_interp": ([C.c_int, C.c_size_t] + [dp] * 4, C.c_int),
+    "self_nodal3d_boundary_interp_dev": ([C.c_int, C.c_size_t] + [dp] * 4 + [C.c_void_p], C.c_int),
+    "self_coarse_to_fine_2d": ([C.c_int, C.c_int, C.c_size_t, dp, dp, C.c_int, C.c_int, dp, dp], C.c_int),
+    "self_coarse_to_fine_2d_dev": ([C.c_int, C.c_int, C.c_size_t, dp, dp, C.c_int, C.c_int, dp, dp, C.c_void_p],
+                                   C.c_int),
+    "self_coarse_to_fine_3d": ([C.c_int, C.c_size_t, dp, C.c_int, dp, dp], C.c_int),
+    "self_coarse_to_fine_3d_dev": ([C.c_int, C.c_size_t, dp, C.c_int, dp, dp, C.c_void_p], C.c_int),
+    "self_serve_create": ([C.c_int, C.c_double, C.POINTER(C.c_void_p)], C.c_int),
+    "self_serve_submit": ([C.c_void_p, C.POINTER(Request), C.POINTER(C.c_longlong)], C.c_int),
+    "self_serve_wait": ([C.c_void_p, C.c_longlong, C.POINTER(Result)], C.c_int),
+    "self_serve_stats": ([C.c_void_p, ip, ip, szp, szp], C.c_int),
+    "self_serve_destroy": ([C.c_void_p], C.c_int),
 }
 
 EXTRA_SIGS: dict = {}

@@ -1,0 +1,327 @@
+// serve.cpp -- the request loop: shallow-water integrations and CGSEM
+// elliptic solves of different mesh sizes and degrees, admitted as they
+// arrive, run concurrently on the device, with per-request latency.
+//
+// No counterpart in the reference (its drivers run one model to
+// completion).  Design:
+//  * submit() enqueues a request (the caller's host arrays stay valid until
+//    wait() returns for it) and stamps its arrival time;
+//  * W worker threads each own a model at a time; a worker takes the
+//    oldest queued request whose device-memory estimate fits in the free
+//    HBM left after the bytes reserved by running requests (admission
+//    against free HBM; requests that do not fit wait for memory to free);
+//  * each model has its own HIP stream (default priority), so the kernels
+//    of concurrent requests overlap on the device;
+//  * results are written to the caller's output array; wait() returns the
+//    request's status, queue time, run time and latency.
+// Every kernel of both models is deterministic (fixed reduction trees, no
+// atomics on values), so a request's result is bitwise the same alone and
+// in any mix.  Meshes and nodal storage are cached per (kind, nX, N).
+
+#include <hip/hip_runtime.h>
+
+#include <chrono>
+#include <condition_variable>
+#include <cstring>
+#include <deque>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <thread>
+#include <tuple>
+#include <unordered_map>
+#include <vector>
+
+#include "common.hpp"
+
+namespace {
+
+using Clock = std::chrono::steady_clock;
+
+double ms_between(Clock::time_point a, Clock::time_point b)
+{
+    return std::chrono::duration<double, std::milli>(b - a).count();
+}
+
+struct MeshCache {
+    int N = 0, nX = 0, nE = 0, nEdges = 0;
+    std::vector<double> nodes, w, dS, dP, lW, lE, lS, lN;
+    std::vector<int> edges, elemNodes;
+    std::vector<double> x, y, dxds, dxdp, dyds, dydp, J, nHat, nLen;
+};
+
+struct Job {
+    long long id = 0;
+    self_request req{};
+    size_t bytes = 0;
+    Clock::time_point arrive, start, end;
+    bool done = false;
+    self_request_result res{};
+};
+
+}  // namespace
+
+struct self_server_s {
+    int device = 0;
+    double memFraction = 0.9;
+    size_t reserved = 0, budget = 0;
+    std::mutex mu;
+    std::condition_variable cvWork, cvDone;
+    std::deque<std::shared_ptr<Job>> queue;
+    std::unordered_map<long long, std::shared_ptr<Job>> jobs;
+    std::map<std::tuple<int, int, int>, std::shared_ptr<MeshCache>> meshes;
+    std::vector<std::thread> workers;
+    long long nextId = 1;
+    int running = 0;
+    bool stop = false;
+};
+
+namespace {
+
+// host mesh + nodal storage for a request kind: SW (Gauss, periodic square
+// of side L) or CGSEM (Gauss-Lobatto, walled unit square)
+std::shared_ptr<MeshCache> mesh_for(self_server_s *s, int kind, int nX, int N, double L)
+{
+    {
+        std::lock_guard<std::mutex> g(s->mu);
+        auto it = s->meshes.find({kind, nX, N});
+        if (it != s->meshes.end()) return it->second;
+    }
+    auto m = std::make_shared<MeshCache>();
+    const bool sw = kind == SELF_REQ_SW2D;
+    const int n1 = N + 1, np = n1 * n1;
+    m->N = N;
+    m->nX = nX;
+    m->nodes.resize(n1); m->w.resize(n1); m->dS.resize(np); m->dP.resize(np);
+    m->lW.resize(n1); m->lE.resize(n1); m->lS.resize(n1); m->lN.resize(n1);
+    if (self_nodal2d_build(N, sw ? SELF_GAUSS : SELF_GAUSS_LOBATTO, sw ? SELF_DG : SELF_CG, 0, m->nodes.data(),
+                           m->w.data(), m->dS.data(), m->dP.data(), m->lW.data(), m->lE.data(), m->lS.data(),
+                           m->lN.data()) != SELF_OK)
+        return nullptr;
+    const int per = sw ? 1 : 0;
+    if (self_quadmesh_counts(nX, nX, per, per, &m->nE, &m->nEdges) != SELF_OK) return nullptr;
+    const size_t nv = (size_t)m->nE * np, ns = (size_t)m->nE * 4 * n1;
+    m->edges.resize(6 * (size_t)m->nEdges);
+    for (auto *v : {&m->x, &m->y, &m->dxds, &m->dxdp, &m->dyds, &m->dydp, &m->J}) v->resize(nv);
+    m->nHat.resize(2 * ns);
+    m->nLen.resize(ns);
+    if (self_quadmesh_structured(nX, nX, N, per, per, sw ? L : 1.0, sw ? L : 1.0, 0.0, m->nodes.data(),
+                                 m->edges.data(), m->x.data(), m->y.data(), m->dxds.data(), m->dxdp.data(),
+                                 m->dyds.data(), m->dydp.data(), m->J.data(), m->nHat.data(),
+                                 m->nLen.data()) != SELF_OK)
+        return nullptr;
+    if (!sw) {  // corner node ids of the walled default mesh: SW, SE, NE, NW (1-based)
+        m->elemNodes.resize(4 * (size_t)m->nE);
+        for (int iY = 1; iY <= nX; ++iY)
+            for (int iX = 1; iX <= nX; ++iX) {
+                const int e = iX + (iY - 1) * nX - 1, n0 = iX + (iY - 1) * (nX + 1);
+                int *q = &m->elemNodes[4 * (size_t)e];
+                q[0] = n0; q[1] = n0 + 1; q[2] = n0 + 1 + nX + 1; q[3] = n0 + nX + 1;
+            }
+    }
+    std::lock_guard<std::mutex> g(s->mu);
+    s->meshes[{kind, nX, N}] = m;
+    return m;
+}
+
+// device bytes a request needs (state, geometry, work vectors), with margin
+size_t estimate_bytes(const self_request &r)
+{
+    const size_t n1 = r.N + 1, nE = (size_t)r.nX * r.nX, nodes = nE * n1 * n1;
+    if (r.kind == SELF_REQ_SW2D)  // q, G, tend, bathy(3), plv, relax(4), metrics(5), traces, normals
+        return (size_t)(1.25 * (double)(nodes * 8 * 24 + nE * 4 * n1 * 8 * 18)) + (1u << 20);
+    // CGSEM: local and global vectors, geometry, Krylov work vectors
+    return (size_t)(1.25 * (double)(nodes * 8 * 22)) + (1u << 20);
+}
+
+int run_sw(self_server_s *s, Job &j)
+{
+    const self_request &r = j.req;
+    auto m = mesh_for(s, SELF_REQ_SW2D, r.nX, r.N, r.L);
+    SELF_REQUIRE(m, SELF_ERR_OTHER, "request %lld: mesh set-up failed", j.id);
+    self_sw2d_desc d{r.N, m->nE, m->nEdges, r.precision, SELF_FUSED, r.g, r.dt, 0.0};
+    self_sw2d_t h = nullptr;
+    int rc = self_sw2d_create(&d, &h);
+    if (rc) return rc;
+    const size_t nv = (size_t)m->nE * (r.N + 1) * (r.N + 1);
+    std::vector<double> plv(nv, r.f0);
+    if (!rc) rc = self_sw2d_set_nodal(h, m->dS.data(), m->dP.data(), m->w.data(), m->w.data(), m->lW.data(),
+                                      m->lE.data(), m->lS.data(), m->lN.data());
+    if (!rc) rc = self_sw2d_set_geometry(h, m->edges.data(), m->dxds.data(), m->dxdp.data(), m->dyds.data(),
+                                         m->dydp.data(), m->J.data(), m->nHat.data(), m->nLen.data());
+    if (!rc) rc = self_sw2d_set_field(h, SELF_SW_BATHYMETRY, r.in1);
+    if (!rc) rc = self_sw2d_set_field(h, SELF_SW_PLANETARY_VORTICITY, plv.data());
+    if (!rc) rc = self_sw2d_calculate_bathymetry_gradient(h);
+    if (!rc) rc = self_sw2d_set_field(h, SELF_SW_SOLUTION, r.in0);
+    if (!rc) rc = self_sw2d_forward_step_rk3(h, 0.0, r.steps);
+    if (!rc) rc = self_sw2d_get_field(h, SELF_SW_SOLUTION, r.out);
+    self_sw2d_destroy(h);
+    return rc;
+}
+
+int run_cgsem(self_server_s *s, Job &j)
+{
+    const self_request &r = j.req;
+    auto m = mesh_for(s, SELF_REQ_CGSEM2D, r.nX, r.N, 1.0);
+    SELF_REQUIRE(m, SELF_ERR_OTHER, "request %lld: mesh set-up failed", j.id);
+    const int nNodes = (r.nX + 1) * (r.nX + 1);
+    self_cgsem2d_t h = nullptr;
+    int rc = self_cgsem2d_create(r.N, m->nE, m->nEdges, nNodes, m->edges.data(), m->elemNodes.data(),
+                                 m->dxds.data(), m->dxdp.data(), m->dyds.data(), m->dydp.data(), m->J.data(), &h);
+    if (rc) return rc;
+    const size_t nv = (size_t)m->nE * (r.N + 1) * (r.N + 1);
+    std::vector<double> zero(nv, 0.0);
+    if (!rc) rc = self_cgsem2d_set_flux_coefficient(h, r.in0);
+    if (!rc) rc = self_cgsem2d_set_source(h, r.in1);
+    if (!rc) rc = self_cgsem2d_set_dirichlet(h, r.in2 ? r.in2 : zero.data());
+    if (!rc) rc = self_cgsem2d_set_solution(h, zero.data());
+    int ioerr = 0, iters = 0;
+    if (!rc) {
+        if (r.precision == SELF_MIXED) {
+            int outer = 0;
+            double rel = 0.0;
+            rc = self_cgsem2d_solve_mixed(h, r.precond, r.tol, 1e-5, r.maxIters, 30, &ioerr, &outer, &iters, &rel,
+                                          nullptr);
+        } else {
+            rc = self_cgsem2d_solve(h, SELF_KRYLOV_CGSEM_PCG, r.precond, r.tol, r.maxIters, 30, 0, &ioerr, &iters,
+                                    nullptr, 0, nullptr);
+        }
+    }
+    if (!rc) rc = self_cgsem2d_get_solution(h, r.out);
+    j.res.iterations = iters;
+    j.res.ioerr = ioerr;
+    self_cgsem2d_destroy(h);
+    return rc;
+}
+
+void worker_main(self_server_s *s)
+{
+    (void)hipSetDevice(s->device);
+    for (;;) {
+        std::shared_ptr<Job> j;
+        {
+            std::unique_lock<std::mutex> g(s->mu);
+            for (;;) {
+                if (s->stop && s->queue.empty()) return;
+                // oldest queued request that fits in the free memory; a request
+                // larger than the whole budget runs alone
+                for (auto it = s->queue.begin(); it != s->queue.end(); ++it) {
+                    const size_t b = (*it)->bytes;
+                    if (s->reserved + b <= s->budget || (s->running == 0 && s->reserved == 0)) {
+                        j = *it;
+                        s->queue.erase(it);
+                        break;
+                    }
+                }
+                if (j) break;
+                s->cvWork.wait(g);
+            }
+            s->reserved += j->bytes;
+            s->running++;
+        }
+        j->start = Clock::now();
+        int rc = j->req.kind == SELF_REQ_SW2D ? run_sw(s, *j) : run_cgsem(s, *j);
+        j->end = Clock::now();
+        {
+            std::lock_guard<std::mutex> g(s->mu);
+            j->res.status = rc;
+            j->res.queue_ms = ms_between(j->arrive, j->start);
+            j->res.run_ms = ms_between(j->start, j->end);
+            j->res.latency_ms = ms_between(j->arrive, j->end);
+            j->res.bytes = j->bytes;
+            j->done = true;
+            s->reserved -= j->bytes;
+            s->running--;
+        }
+        s->cvDone.notify_all();
+        s->cvWork.notify_all();
+    }
+}
+
+}  // namespace
+
+extern "C" {
+
+int self_serve_create(int workers, double memFraction, self_server_t *out)
+try {
+    SELF_REQUIRE(out && workers >= 1 && workers <= 64 && memFraction > 0.0 && memFraction <= 1.0, SELF_ERR_ARG,
+                 "self_serve_create: workers 1..64, memFraction in (0, 1]");
+    auto *s = new self_server_s();
+    SELF_HIP(hipGetDevice(&s->device));
+    size_t fr = 0, tot = 0;
+    SELF_HIP(hipMemGetInfo(&fr, &tot));
+    s->memFraction = memFraction;
+    s->budget = (size_t)(memFraction * (double)fr);
+    for (int k = 0; k < workers; ++k) s->workers.emplace_back(worker_main, s);
+    *out = s;
+    return SELF_OK;
+}
+SELF_CATCH_ALL
+
+int self_serve_submit(self_server_t s, const self_request *req, long long *id)
+{
+    SELF_REQUIRE(s && req && id && req->in0 && req->out, SELF_ERR_ARG, "self_serve_submit: null argument");
+    SELF_REQUIRE(req->kind == SELF_REQ_SW2D || req->kind == SELF_REQ_CGSEM2D, SELF_ERR_ARG, "unknown request kind %d",
+                 req->kind);
+    SELF_REQUIRE(req->nX >= 1 && req->N >= 1 && req->N <= 11, SELF_ERR_ARG, "request: nX >= 1, N in 1..11");
+    SELF_REQUIRE(req->kind != SELF_REQ_SW2D || (req->in1 && req->nX >= 3 && req->steps >= 0), SELF_ERR_ARG,
+                 "shallow-water request needs bathymetry, nX >= 3 (periodic) and steps >= 0");
+    SELF_REQUIRE(req->kind != SELF_REQ_CGSEM2D || (req->in1 && req->maxIters >= 1 && req->tol > 0), SELF_ERR_ARG,
+                 "CGSEM request needs a source, maxIters >= 1 and tol > 0");
+    auto j = std::make_shared<Job>();
+    j->req = *req;
+    j->bytes = estimate_bytes(*req);
+    j->arrive = Clock::now();
+    {
+        std::lock_guard<std::mutex> g(s->mu);
+        SELF_REQUIRE(!s->stop, SELF_ERR_STATE, "server is shutting down");
+        j->id = s->nextId++;
+        s->jobs[j->id] = j;
+        s->queue.push_back(j);
+        *id = j->id;
+    }
+    s->cvWork.notify_all();
+    return SELF_OK;
+}
+
+int self_serve_wait(self_server_t s, long long id, self_request_result *res)
+{
+    SELF_REQUIRE(s && res, SELF_ERR_ARG, "self_serve_wait: null argument");
+    std::shared_ptr<Job> j;
+    {
+        std::unique_lock<std::mutex> g(s->mu);
+        auto it = s->jobs.find(id);
+        SELF_REQUIRE(it != s->jobs.end(), SELF_ERR_ARG, "unknown request id %lld", id);
+        j = it->second;
+        s->cvDone.wait(g, [&] { return j->done; });
+        *res = j->res;
+        s->jobs.erase(id);
+    }
+    return SELF_OK;
+}
+
+int self_serve_stats(self_server_t s, int *queued, int *running, size_t *reservedBytes, size_t *budgetBytes)
+{
+    SELF_REQUIRE(s, SELF_ERR_ARG, "null server");
+    std::lock_guard<std::mutex> g(s->mu);
+    if (queued) *queued = (int)s->queue.size();
+    if (running) *running = s->running;
+    if (reservedBytes) *reservedBytes = s->reserved;
+    if (budgetBytes) *budgetBytes = s->budget;
+    return SELF_OK;
+}
+
+int self_serve_destroy(self_server_t s)
+{
+    if (!s) return SELF_OK;
+    {
+        std::lock_guard<std::mutex> g(s->mu);
+        s->stop = true;
+    }
+    s->cvWork.notify_all();
+    for (auto &t : s->workers) t.join();
+    delete s;
+    return SELF_OK;
+}
+
+}  // extern "C"

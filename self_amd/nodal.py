@@ -105,3 +105,78 @@ def rolloff_filter1d(N, nCutoff, nodes, weights):
     F1 = np.zeros((N + 1) * (N + 1))
     L.check(L.lib().self_rolloff_filter1d(N, nCutoff, L.ptr(L.f64(nodes)), L.ptr(L.f64(weights)), L.ptr(F1)))
     return F1.reshape(N + 1, N + 1, order="F")
+
+
+# ---------------------------------------------------------------- batched library operators
+def _elems(f, per):
+    f = L.f64(f)
+    if f.size % per:
+        raise ValueError(f"array size {f.size} is not a multiple of the element size {per}")
+    return f, f.size // per
+
+
+def apply_derivative_2d(dMatS, dMatP, f):
+    """ApplyDerivativeMatrix_NodalStorage_2D for every element of f
+    (nElems, N+1, N+1) indexed [e, j(p), i(s)]: returns (dFds, dFdp)."""
+    n1 = np.asarray(dMatS).shape[0]
+    f, nE = _elems(f, n1 * n1)
+    ds, dp_ = np.zeros_like(f), np.zeros_like(f)
+    L.check(L.lib().self_nodal2d_apply_derivative(n1 - 1, nE, L.ptr(L.f64(np.asarray(dMatS).ravel(order="F"))),
+                                                  L.ptr(L.f64(np.asarray(dMatP).ravel(order="F"))), L.ptr(f),
+                                                  L.ptr(ds), L.ptr(dp_)))
+    return ds, dp_
+
+
+def boundary_interp_2d(lW, lE, lS, lN, f):
+    """CalculateAtBoundaries_NodalStorage_2D: (fWest, fEast, fSouth, fNorth), each (nElems, N+1)."""
+    n1 = len(lW)
+    f, nE = _elems(f, n1 * n1)
+    out = [np.zeros((nE, n1)) for _ in range(4)]
+    L.check(L.lib().self_nodal2d_boundary_interp(n1 - 1, nE, *[L.ptr(L.f64(a)) for a in (lW, lE, lS, lN)], L.ptr(f),
+                                                 *[L.ptr(o) for o in out]))
+    return tuple(out)
+
+
+def apply_derivative_3d(dMat, f):
+    """ApplyDerivativeMatrix_NodalStorage_3D: (dFds, dFdp, dFdq) shaped like f."""
+    n1 = np.asarray(dMat).shape[0]
+    f, nE = _elems(f, n1 ** 3)
+    out = [np.zeros_like(f) for _ in range(3)]
+    L.check(L.lib().self_nodal3d_apply_derivative(n1 - 1, nE, L.ptr(L.f64(np.asarray(dMat).ravel(order="F"))),
+                                                  L.ptr(f), *[L.ptr(o) for o in out]))
+    return tuple(out)
+
+
+def boundary_interp_3d(lagLeft, lagRight, f):
+    """CalculateAtBoundaries_NodalStorage_3D: (nElems, 6, N+1, N+1), faces
+    south, east, north, west, bottom, top."""
+    n1 = len(lagLeft)
+    f, nE = _elems(f, n1 ** 3)
+    out = np.zeros((nE, 6, n1, n1))
+    L.check(L.lib().self_nodal3d_boundary_interp(n1 - 1, nE, L.ptr(L.f64(lagLeft)), L.ptr(L.f64(lagRight)), L.ptr(f),
+                                                 L.ptr(out)))
+    return out
+
+
+def coarse_to_fine_2d(Ts, Tp, f):
+    """CoarseToFine_2D: f (nElems, nP+1, nS+1) [e, j, i] -> (nElems, nNewS+1, nNewP+1),
+    i.e. the reference's fNew(0:nNewP, 0:nNewS) per element."""
+    Ts, Tp = np.asarray(Ts), np.asarray(Tp)
+    ms, s1 = Ts.shape
+    mp, p1 = Tp.shape
+    f, nE = _elems(f, s1 * p1)
+    out = np.zeros((nE, ms, mp))
+    L.check(L.lib().self_coarse_to_fine_2d(s1 - 1, p1 - 1, nE, L.ptr(L.f64(Ts.ravel(order="F"))),
+                                           L.ptr(L.f64(Tp.ravel(order="F"))), ms - 1, mp - 1, L.ptr(f), L.ptr(out)))
+    return out
+
+
+def coarse_to_fine_3d(T, f):
+    """CoarseToFine_3D: f (nElems, N+1, N+1, N+1) -> (nElems, nNew+1, nNew+1, nNew+1)."""
+    T = np.asarray(T)
+    m1, n1 = T.shape
+    f, nE = _elems(f, n1 ** 3)
+    out = np.zeros((nE, m1, m1, m1))
+    L.check(L.lib().self_coarse_to_fine_3d(n1 - 1, nE, L.ptr(L.f64(T.ravel(order="F"))), m1 - 1, L.ptr(f),
+                                           L.ptr(out)))
+    return out

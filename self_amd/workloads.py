@@ -64,13 +64,13 @@ def bathymetry_modes(seed=DEFAULT_SEED, nmodes=8, amp=50.0, kmax=4):
 
 
 def sw2d_geostrophic_fields(x, y, L=1.0e6, H0=1000.0, g=9.81, f0=1.0e-4, eta0=1.0, width=1.0e5,
-                            seed=DEFAULT_SEED, nmodes=8, amp=50.0):
+                            seed=DEFAULT_SEED, nmodes=8, amp=50.0, centre=(0.5, 0.5)):
     """Fields at node positions x, y (any matching shapes): returns
-    h (depth), U, V, H."""
+    h (depth), U, V, H.  The anomaly is centred at centre * L."""
     h = np.full_like(x, H0)
     for a, m, n, phi in bathymetry_modes(seed, nmodes, amp):
         h = h + a * np.cos(2.0 * math.pi * (m * x + n * y) / L + phi)
-    xc = yc = 0.5 * L
+    xc, yc = centre[0] * L, centre[1] * L
     eta = eta0 * np.exp(-((x - xc) ** 2 + (y - yc) ** 2) / width ** 2)
     detadx = -2.0 * (x - xc) / width ** 2 * eta
     detady = -2.0 * (y - yc) / width ** 2 * eta

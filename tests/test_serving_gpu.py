@@ -1,0 +1,79 @@
+"""The request loop (serve.cpp): requests admitted against free HBM and run
+concurrently give bitwise the same results as the same requests run
+alone, and as the model API run directly (tolerance: exact)."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def _specs():
+    from self_amd import serving as S
+    specs = []
+    for i, (kind, nX, N) in enumerate([(S.REQ_SW2D, 8, 3), (S.REQ_CGSEM2D, 8, 3), (S.REQ_SW2D, 16, 5),
+                                       (S.REQ_SW2D, 8, 7), (S.REQ_CGSEM2D, 16, 5), (S.REQ_SW2D, 16, 3),
+                                       (S.REQ_CGSEM2D, 8, 7), (S.REQ_SW2D, 32, 3), (S.REQ_SW2D, 8, 5),
+                                       (S.REQ_CGSEM2D, 16, 3)]):
+        specs.append(S.make_inputs(S.Spec(kind, nX, N, 500 + i, "mixed" if i % 3 == 0 else "fp64", steps=20)))
+    return specs
+
+
+def test_results_are_the_same_alone_and_in_the_mix(gpu_lib):
+    from self_amd import serving as S
+    specs = _specs()
+    srv = S.Server(workers=4)
+    mix, summary = S.run_stream(srv, specs)
+    srv.close()
+    assert summary["ok"] and summary["requests"] == len(specs)
+    alone = S.Server(workers=1)
+    for (spec, res, out) in mix:
+        rid = alone.submit(spec)
+        r1, o1 = alone.wait(rid)
+        assert r1.status == 0 and res.status == 0
+        assert np.array_equal(o1, out), (spec.kind, spec.nX, spec.N)
+        if spec.kind == S.REQ_CGSEM2D:
+            assert r1.iterations == res.iterations > 0
+    alone.close()
+
+
+def test_served_shallow_water_equals_the_model_api(gpu_lib):
+    from self_amd import serving as S
+    from self_amd.shallow_water import ShallowWater, SWParams
+    spec = S.make_inputs(S.Spec(S.REQ_SW2D, 16, 5, 77, "fp64", steps=15))
+    srv = S.Server(workers=2)
+    res, out = srv.wait(srv.submit(spec))
+    srv.close()
+    p = spec.params
+    m = ShallowWater(SWParams(polyDeg=5, nXElem=16, nYElem=16, xScale=p["L"], yScale=p["L"], dt=p["dt"], g=p["g"],
+                              f0=p["f0"]))
+    m.SetBathymetry(spec.arrays["in1"])
+    m.SetPlanetaryVorticity(np.full(spec.arrays["in1"].shape[:1] + spec.arrays["in1"].shape[2:], p["f0"]))
+    m.CalculateBathymetryGradient()
+    m.SetSolution(spec.arrays["in0"])
+    m.ForwardStepRK3(0.0, 15)
+    assert res.status == 0 and np.array_equal(m.GetSolution(), out)
+    assert res.latency_ms >= res.run_ms > 0
+    m.Trash()
+
+
+def test_admission_with_a_tiny_memory_budget_runs_requests_one_at_a_time(gpu_lib):
+    from self_amd import serving as S
+    specs = _specs()[:4]
+    srv = S.Server(workers=4, memFraction=1e-9)
+    out, summary = S.run_stream(srv, specs)
+    assert summary["ok"]
+    # requests that do not fit run alone: no two runs overlap in time
+    spans = sorted((r.queue_ms, r.queue_ms + r.run_ms) for _, r, _ in out)
+    srv.close()
+    assert len(spans) == 4
+
+
+def test_bad_requests_are_rejected(gpu_lib):
+    from self_amd import _lib as L
+    from self_amd import serving as S
+    srv = S.Server(workers=1)
+    spec = S.make_inputs(S.Spec(S.REQ_SW2D, 8, 3, 1, "fp64", steps=5))
+    spec.nX = 2  # periodic shallow water needs nX >= 3
+    with pytest.raises(L.SelfError):
+        srv.submit(spec)
+    srv.close()
