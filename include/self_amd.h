@@ -227,6 +227,14 @@ int self_sw2d_kernel_times(self_sw2d_t h, double tn, int nSteps, double *avg_ms,
  * element (:1160-1168, :1733-1868): sets bathymetry vars 2 and 3. */
 int self_sw2d_calculate_bathymetry_gradient(self_sw2d_t h);
 int self_sw2d_synchronize(self_sw2d_t h);
+/* Element groups per workgroup of the fused stage kernel: 1 (one round of
+ * waves), 2 (each workgroup prefetches its second group while computing the
+ * first), 0 = default (1).  No reference counterpart. */
+int self_sw2d_set_groups(self_sw2d_t h, int groups);
+/* Autotuner: times nSteps RK3 steps with each setting on a copy of the
+ * current state (the state is restored), keeps the faster one, returns it
+ * and its time per step. */
+int self_sw2d_autotune(self_sw2d_t h, int nSteps, int *groups, double *ms_per_step);
 /* Bytes of HBM state the handle holds. */
 int self_sw2d_device_bytes(self_sw2d_t h, size_t *bytes);
 

@@ -148,6 +148,40 @@ __global__ __launch_bounds__(256) void k_sw_traces(const double *__restrict__ q,
     }
 }
 
+// Traces of the fused form: each side node's trace is written into the
+// slot of the matching node of the neighbour's side ([e'][side'][f][k'],
+// k' flipped when the edge runs the other way), so a stage reads its
+// neighbours' traces from its own slots without first loading the side
+// descriptors.  Boundary sides write nothing.
+template <int N>
+__global__ __launch_bounds__(256) void k_sw_traces_push(const double *__restrict__ q, const double *__restrict__ lag,
+                                                        const int2 *__restrict__ side, double *__restrict__ out,
+                                                        int nE)
+{
+    using S = Shape<N>;
+    constexpr int n1 = S::n1, np = S::np, epb = S::epb;
+    __shared__ double qs[epb * 3 * np];
+    const int t = threadIdx.x, le = t / np, nd = t - le * np;
+    const int e = blockIdx.x * epb + le;
+    const bool act = le < epb && e < nE;
+    double lW[n1], lE[n1], lS[n1], lN[n1];
+    load_lag<N>(lag, lW, lE, lS, lN);
+    if (act)
+        for (int f = 0; f < 3; ++f) qs[(le * 3 + f) * np + nd] = q[((size_t)f * nE + e) * np + nd];
+    __syncthreads();
+    if (!act) return;
+    for (int w = nd; w < 4 * n1; w += np) {
+        const int sd = w / n1, k = w - sd * n1;
+        const int2 info = side[(size_t)e * 4 + sd];
+        if (info.x < 0) continue;
+        double tr[3];
+        element_trace<N>(qs + le * 3 * np, sd, k, lW, lE, lS, lN, tr);
+        const int kk = side_flip(info.y) ? N - k : k;
+        double *o = out + (((size_t)info.x * 4 + side_nbr_side(info.y)) * 3) * n1 + kk;
+        o[0] = tr[0]; o[n1] = tr[1]; o[2 * n1] = tr[2];
+    }
+}
+
 // EdgeFlux for all edges; one thread per edge node.
 template <int N>
 __global__ __launch_bounds__(256) void k_sw_edge_flux(const int *__restrict__ edges, int nEdges,
@@ -203,30 +237,57 @@ __device__ __forceinline__ void trace3(const NodalLDS<N> &L, const double *qs, i
     }
 }
 
+template <int N>
+__device__ __forceinline__ void dg_divergence64_field(const NodalLDS<N> &L, const double *sC, const double *pC,
+                                                      const double *bf, int i, int j, double invJ, int f,
+                                                      double &tend)
+{
+    constexpr int n1 = N + 1, np = n1 * n1;
+    const double *sl = sC + f * np + n1 * j;  // s-line at p = j
+    const double *pl = pC + f * np + i;       // p-line at s = i
+    double t1 = L.dS[i] * sl[0];
+    double t2 = L.dP[j] * pl[0];
+    #pragma unroll
+    for (int m = 1; m < n1; ++m) {
+        t1 = __builtin_fma(L.dS[i + n1 * m], sl[m], t1);
+        t2 = __builtin_fma(L.dP[j + n1 * m], pl[n1 * m], t2);
+    }
+    const double a = t1 + (bf[(1 * 3 + f) * n1 + j] * L.lE[i] + bf[(3 * 3 + f) * n1 + j] * L.lW[i]) * L.iwS[i];
+    const double b = t2 + (bf[(2 * 3 + f) * n1 + i] * L.lN[j] + bf[(0 * 3 + f) * n1 + i] * L.lS[j]) * L.iwP[j];
+    tend = (-a - b) * invJ;
+}
+
 // Weak divergence + lift of one node (DGSystemDerivative in s then in p,
 // MappedTimeDerivative :1364-1522): returns -(T1 + T2)/J per field.
-// bf: [side][field][node] face fluxes (south, east, north, west).
-template <int N>
+// bf: [side][field][node] face fluxes (south, east, north, west).  UNROLL
+// interleaves the three fields (more independent work per wave) where the
+// registers allow it.
+template <int N, bool UNROLL>
 __device__ __forceinline__ void dg_divergence64(const NodalLDS<N> &L, const double *sC, const double *pC,
                                                 const double *bf, int i, int j, double invJ, double tend[3])
 {
-    constexpr int n1 = N + 1, np = n1 * n1;
-    const double lEi = L.lE[i], lWi = L.lW[i], lNj = L.lN[j], lSj = L.lS[j];
-    const double iwS = L.iwS[i], iwP = L.iwP[j];
-    #pragma unroll 1
-    for (int f = 0; f < 3; ++f) {
-        const double *sl = sC + f * np + n1 * j;  // s-line at p = j
-        const double *pl = pC + f * np + i;       // p-line at s = i
-        double t1 = L.dS[i] * sl[0];
-        double t2 = L.dP[j] * pl[0];
+    if constexpr (UNROLL) {
         #pragma unroll
-        for (int m = 1; m < n1; ++m) {
-            t1 = __builtin_fma(L.dS[i + n1 * m], sl[m], t1);
-            t2 = __builtin_fma(L.dP[j + n1 * m], pl[n1 * m], t2);
+        for (int f = 0; f < 3; ++f) dg_divergence64_field<N>(L, sC, pC, bf, i, j, invJ, f, tend[f]);
+    } else {
+        constexpr int n1 = N + 1, np = n1 * n1;
+        const double lEi = L.lE[i], lWi = L.lW[i], lNj = L.lN[j], lSj = L.lS[j];
+        const double iwS = L.iwS[i], iwP = L.iwP[j];
+        #pragma unroll 1
+        for (int f = 0; f < 3; ++f) {
+            const double *sl = sC + f * np + n1 * j;
+            const double *pl = pC + f * np + i;
+            double t1 = L.dS[i] * sl[0];
+            double t2 = L.dP[j] * pl[0];
+            #pragma unroll
+            for (int m = 1; m < n1; ++m) {
+                t1 = __builtin_fma(L.dS[i + n1 * m], sl[m], t1);
+                t2 = __builtin_fma(L.dP[j + n1 * m], pl[n1 * m], t2);
+            }
+            const double a = t1 + (bf[(1 * 3 + f) * n1 + j] * lEi + bf[(3 * 3 + f) * n1 + j] * lWi) * iwS;
+            const double b = t2 + (bf[(2 * 3 + f) * n1 + i] * lNj + bf[(0 * 3 + f) * n1 + i] * lSj) * iwP;
+            tend[f] = (-a - b) * invJ;
         }
-        const double a = t1 + (bf[(1 * 3 + f) * n1 + j] * lEi + bf[(3 * 3 + f) * n1 + j] * lWi) * iwS;
-        const double b = t2 + (bf[(2 * 3 + f) * n1 + i] * lNj + bf[(0 * 3 + f) * n1 + i] * lSj) * iwP;
-        tend[f] = (-a - b) * invJ;
     }
 }
 
@@ -255,6 +316,37 @@ __device__ __forceinline__ void dg_divergence_mixed(const NodalLDS<N> &L, const 
             t1 = __builtin_fmaf(L.dS32[i + n1 * m], (float)(sl[m] - rs), t1);
             t2 = __builtin_fmaf(L.dP32[j + n1 * m], (float)(pl[n1 * m] - rp), t2);
         }
+        const double ta = (double)t1 + ((bf[(1 * 3 + f) * n1 + j] - rs) * lEi +
+                                        (bf[(3 * 3 + f) * n1 + j] + rs) * lWi) * iwS;
+        const double tb = (double)t2 + ((bf[(2 * 3 + f) * n1 + i] - rp) * lNj +
+                                        (bf[(0 * 3 + f) * n1 + i] + rp) * lSj) * iwP;
+        tend[f] = (-ta - tb) * invJ;
+    }
+}
+
+// Mixed form with the FP32 deviations prepared once per node in LDS:
+// sD(m, j) = sC(m, j) - sC(0, j) along s-lines, pD(i, m) = pC(i, m) - pC(i, 0)
+// along p-lines; the line's reference value re-enters through the FP64 lift
+// (sum_m Dhat(i,m) = -(l_E(i) - l_W(i))/w_i).
+template <int N>
+__device__ __forceinline__ void dg_divergence_dev32(const NodalLDS<N> &L, const double *sC, const double *pC,
+                                                    const float *sD, const float *pD, const double *bf, int i, int j,
+                                                    double invJ, double tend[3])
+{
+    constexpr int n1 = N + 1, np = n1 * n1;
+    const double lEi = L.lE[i], lWi = L.lW[i], lNj = L.lN[j], lSj = L.lS[j];
+    const double iwS = L.iwS[i], iwP = L.iwP[j];
+    #pragma unroll
+    for (int f = 0; f < 3; ++f) {
+        const float *sl = sD + f * np + n1 * j;
+        const float *pl = pD + f * np + i;
+        float t1 = 0.f, t2 = 0.f;
+        #pragma unroll
+        for (int m = 0; m < n1; ++m) {
+            t1 = __builtin_fmaf(L.dS32[i + n1 * m], sl[m], t1);
+            t2 = __builtin_fmaf(L.dP32[j + n1 * m], pl[n1 * m], t2);
+        }
+        const double rs = sC[f * np + n1 * j], rp = pC[f * np + i];
         const double ta = (double)t1 + ((bf[(1 * 3 + f) * n1 + j] - rs) * lEi +
                                         (bf[(3 * 3 + f) * n1 + j] + rs) * lWi) * iwS;
         const double tb = (double)t2 + ((bf[(2 * 3 + f) * n1 + i] - rp) * lNj +
@@ -312,7 +404,7 @@ __global__ __launch_bounds__(256, 4) void k_sw_mapped(SWKernelArgs A)
     __syncthreads();
     if (!act) return;
     double tend[3], src[3];
-    dg_divergence64<N>(L, sC + le * 3 * np, pC + le * 3 * np, bfs + le * 12 * n1, i, j, invJ, tend);
+    dg_divergence64<N, false>(L, sC + le * 3 * np, pC + le * 3 * np, bfs + le * 12 * n1, i, j, invJ, tend);
     sw_source(s, vort, A.g, hx, hy, A.drag, h, A.hasRelax, r, src);
     #pragma unroll
     for (int f = 0; f < 3; ++f) A.tend[f * mstride + base] = tend[f] + src[f];
@@ -332,98 +424,108 @@ __global__ __launch_bounds__(256) void k_sw_rk_update(double *__restrict__ q, do
 }
 
 // ------------------------------------------------------------- FUSED form
-// Every global load of the stage is issued at the top (state, metrics,
-// sources, RK register, the side descriptor and the neighbour trace of the
-// thread's side node), so a workgroup waits for memory about once.
+// One RK stage of one group of elements (epb elements, one per 64-node
+// slice of the workgroup).  sw_load issues every global load of the group,
+// in the order the phases consume them: state, metrics and the side node's
+// inputs (phases A, B) first, then the tendency-only inputs (phase C).  No
+// load depends on another: neighbour traces sit in the element's own slots,
+// written there by the neighbours in the previous stage.  sw_process runs
+// the four phases on the loaded values.
+struct SWPre {
+    double s[3], met[4], invJ, hx, hy, vort, h, gold[3], r[4], fn0[3], oth0[3];
+    int2 info0;
+};
+
 template <int N, bool MIXED>
-__global__ __launch_bounds__(256, 4) void k_sw_stage_fused(SWKernelArgs A)
+__device__ __forceinline__ void sw_load(const SWKernelArgs &A, int e, int nd, bool act, SWPre &P)
 {
-    using S = Shape<N>;
-    constexpr int n1 = S::n1, np = S::np, epb = S::epb;
-    __shared__ NodalLDS<N> L;
-    __shared__ double qs[epb * 3 * np], sC[epb * 3 * np], pC[epb * 3 * np], bfs[epb * 12 * n1];
-    const int t = threadIdx.x, le = t / np, nd = t - le * np;
-    const int e = xcd_block(blockIdx.x, gridDim.x) * epb + le;
-    const bool act = le < epb && e < A.nE;
+    constexpr int n1 = N + 1, np = n1 * n1;
+    const size_t mstride = (size_t)A.nE * np, base = (size_t)e * np + nd;
+    P.invJ = 0; P.hx = 0; P.hy = 0; P.vort = 0; P.h = 1.0;
+    #pragma unroll
+    for (int k = 0; k < 3; ++k) { P.gold[k] = 0; P.fn0[k] = 0; P.oth0[k] = 0; }
+    #pragma unroll
+    for (int k = 0; k < 4; ++k) P.r[k] = 0;
+    P.info0 = make_int2(0, 0);
+    if (!act) return;
+    #pragma unroll
+    for (int f = 0; f < 3; ++f) P.s[f] = A.q[f * mstride + base];
+    #pragma unroll
+    for (int k = 0; k < 4; ++k) P.met[k] = A.metric[k * mstride + base];
+    const double Jv = A.metric[4 * mstride + base];
+    if (nd < 4 * n1) {  // this thread's side node (all of them for N >= 3)
+        const int sd0 = nd / n1, k0 = nd - sd0 * n1;
+        P.info0 = A.side[(size_t)e * 4 + sd0];
+        const size_t nb = ((size_t)e * 4 + sd0) * 3 * n1 + k0;
+        P.fn0[0] = A.fnrm[nb]; P.fn0[1] = A.fnrm[nb + n1]; P.fn0[2] = A.fnrm[nb + 2 * n1];
+        const double *tr = A.trIn + (((size_t)e * 4 + sd0) * 3) * n1 + k0;
+        P.oth0[0] = tr[0]; P.oth0[1] = tr[n1]; P.oth0[2] = tr[2 * n1];
+    }
+    if constexpr (MIXED) {
+        P.hx = A.bathy32[base]; P.hy = A.bathy32[mstride + base]; P.vort = A.plv32[base];
+        if (A.a != 0.0) {
+            #pragma unroll
+            for (int f = 0; f < 3; ++f) P.gold[f] = A.G32[f * mstride + base];
+        }
+    } else {
+        P.hx = A.bathy[mstride + base]; P.hy = A.bathy[2 * mstride + base]; P.vort = A.plv[base];
+        if (A.a != 0.0) {
+            #pragma unroll
+            for (int f = 0; f < 3; ++f) P.gold[f] = A.G64[f * mstride + base];
+        }
+    }
+    if (A.drag != 0.0) P.h = A.bathy[base];
+    if (A.hasRelax) {
+        #pragma unroll
+        for (int k = 0; k < 4; ++k) P.r[k] = A.relax[k * mstride + base];
+    }
+    P.invJ = 1.0 / Jv;
+}
+
+// Every thread of the block calls this (it synchronizes the block).
+template <int N, bool MIXED>
+__device__ __forceinline__ void sw_process(const SWKernelArgs &A, const NodalLDS<N> &L, double *qsE, double *sCE,
+                                           double *pCE, float *sDE, float *pDE, double *bfE, int e, int nd, bool act,
+                                           const SWPre &P)
+{
+    constexpr int n1 = N + 1, np = n1 * n1;
     const int i = nd % n1, j = nd / n1;
     const size_t mstride = (size_t)A.nE * np, base = (size_t)e * np + nd;
-    double *qsE = qs + le * 3 * np, *sCE = sC + le * 3 * np, *pCE = pC + le * 3 * np, *bfE = bfs + le * 12 * n1;
-
-    // ---- loads
-    double s[3], met[4], invJ = 0, hx = 0, hy = 0, vort = 0, h = 1.0, gold[3] = {0, 0, 0}, r[4] = {0, 0, 0, 0};
-    // side node handled by this thread in phase B (w = nd; all of them for N >= 3)
-    const bool side0 = act && nd < 4 * n1;
-    int sd0 = 0, k0 = 0;
-    int2 info0 = make_int2(0, 0);
-    double fn0[3] = {0, 0, 0}, oth0[3] = {0, 0, 0};
-    if (act) {
-        #pragma unroll
-        for (int f = 0; f < 3; ++f) s[f] = A.q[f * mstride + base];
-        if (side0) {
-            sd0 = nd / n1;
-            k0 = nd - sd0 * n1;
-            info0 = A.side[(size_t)e * 4 + sd0];
-            const size_t nb = ((size_t)e * 4 + sd0) * 3 * n1 + k0;
-            fn0[0] = A.fnrm[nb]; fn0[1] = A.fnrm[nb + n1]; fn0[2] = A.fnrm[nb + 2 * n1];
-        }
-        #pragma unroll
-        for (int k = 0; k < 4; ++k) met[k] = A.metric[k * mstride + base];
-        const double Jv = A.metric[4 * mstride + base];
-        if constexpr (MIXED) {
-            hx = A.bathy32[base]; hy = A.bathy32[mstride + base]; vort = A.plv32[base];
-            if (A.a != 0.0) {
-                #pragma unroll
-                for (int f = 0; f < 3; ++f) gold[f] = A.G32[f * mstride + base];
-            }
-        } else {
-            hx = A.bathy[mstride + base]; hy = A.bathy[2 * mstride + base]; vort = A.plv[base];
-            if (A.a != 0.0) {
-                #pragma unroll
-                for (int f = 0; f < 3; ++f) gold[f] = A.G64[f * mstride + base];
-            }
-        }
-        if (A.drag != 0.0) h = A.bathy[base];
-        if (A.hasRelax) {
-            #pragma unroll
-            for (int k = 0; k < 4; ++k) r[k] = A.relax[k * mstride + base];
-        }
-        if (side0 && info0.x >= 0) {
-            const int kk = side_flip(info0.y) ? N - k0 : k0;
-            const double *tr = A.trIn + (((size_t)info0.x * 4 + side_nbr_side(info0.y)) * 3) * n1 + kk;
-            oth0[0] = tr[0]; oth0[1] = tr[n1]; oth0[2] = tr[2 * n1];
-        }
-        invJ = 1.0 / Jv;
-    }
-    stage_nodal<N, MIXED>(L, A.dS, A.dS32, A.dP32);
 
     // ---- phase A: contravariant fluxes, state to LDS
     if (act) {
         #pragma unroll
-        for (int f = 0; f < 3; ++f) qsE[f * np + nd] = s[f];
-        contravariant(s, met, A.g, sCE + nd, pCE + nd, np);
+        for (int f = 0; f < 3; ++f) qsE[f * np + nd] = P.s[f];
+        contravariant(P.s, P.met, A.g, sCE + nd, pCE + nd, np);
     }
     __syncthreads();
 
-    // ---- phase B: Rusanov flux of each side node, in the edge's primary orientation
+    // ---- phase B: FP32 deviations of the contravariant fluxes (mixed), and
+    // the Rusanov flux of each side node in the edge's primary orientation
+    if constexpr (MIXED) {
+        if (act) {
+            #pragma unroll
+            for (int f = 0; f < 3; ++f) {
+                sDE[f * np + nd] = (float)(sCE[f * np + nd] - sCE[f * np + n1 * j]);
+                pDE[f * np + nd] = (float)(pCE[f * np + nd] - pCE[f * np + i]);
+            }
+        }
+    }
     if (act) {
         for (int w = nd; w < 4 * n1; w += np) {
-            int sd, k;
+            const int sd = w / n1, k = w - sd * n1;
             int2 info;
             double fn[3], oth[3], own[3], fl[3];
             if (w == nd) {
-                sd = sd0; k = k0; info = info0;
-                fn[0] = fn0[0]; fn[1] = fn0[1]; fn[2] = fn0[2];
-                oth[0] = oth0[0]; oth[1] = oth0[1]; oth[2] = oth0[2];
+                info = P.info0;
+                fn[0] = P.fn0[0]; fn[1] = P.fn0[1]; fn[2] = P.fn0[2];
+                oth[0] = P.oth0[0]; oth[1] = P.oth0[1]; oth[2] = P.oth0[2];
             } else {  // N <= 2: more side nodes than element nodes
-                sd = w / n1; k = w - sd * n1;
                 info = A.side[(size_t)e * 4 + sd];
                 const size_t nb = ((size_t)e * 4 + sd) * 3 * n1 + k;
                 fn[0] = A.fnrm[nb]; fn[1] = A.fnrm[nb + n1]; fn[2] = A.fnrm[nb + 2 * n1];
-                if (info.x >= 0) {
-                    const int kk = side_flip(info.y) ? N - k : k;
-                    const double *tr = A.trIn + (((size_t)info.x * 4 + side_nbr_side(info.y)) * 3) * n1 + kk;
-                    oth[0] = tr[0]; oth[1] = tr[n1]; oth[2] = tr[2 * n1];
-                }
+                const double *tr = A.trIn + (((size_t)e * 4 + sd) * 3) * n1 + k;
+                oth[0] = tr[0]; oth[1] = tr[n1]; oth[2] = tr[2 * n1];
             }
             trace3<N>(L, qsE, sd, k, own);
             double sign = 1.0;
@@ -445,36 +547,74 @@ __global__ __launch_bounds__(256, 4) void k_sw_stage_fused(SWKernelArgs A)
     // ---- phase C: weak divergence + lift + source + RK3 update
     if (act) {
         double tend[3], src[3];
-        if constexpr (MIXED) dg_divergence_mixed<N>(L, sCE, pCE, bfE, i, j, invJ, tend);
-        else dg_divergence64<N>(L, sCE, pCE, bfE, i, j, invJ, tend);
-        sw_source(s, vort, A.g, hx, hy, A.drag, h, A.hasRelax, r, src);
+        if constexpr (MIXED) dg_divergence_dev32<N>(L, sCE, pCE, sDE, pDE, bfE, i, j, P.invJ, tend);
+        else dg_divergence64<N, (N <= 5 || N % 2 == 1)>(L, sCE, pCE, bfE, i, j, P.invJ, tend);
+        sw_source(P.s, P.vort, A.g, P.hx, P.hy, A.drag, P.h, A.hasRelax, P.r, src);
         #pragma unroll
         for (int f = 0; f < 3; ++f) {
             // G = a*G + L; the first stage has a = 0 and ForwardStepRK3 zeroes
             // G before it, so that stage never reads G.
-            double gv = A.a * gold[f] + (tend[f] + src[f]);
+            double gv = A.a * P.gold[f] + (tend[f] + src[f]);
             if constexpr (MIXED) {
                 gv = (double)(float)gv;
                 A.G32[f * mstride + base] = (float)gv;
             } else {
                 A.G64[f * mstride + base] = gv;
             }
-            const double qn = s[f] + A.gdt * gv;
+            const double qn = P.s[f] + A.gdt * gv;
             A.q[f * mstride + base] = qn;
             qsE[f * np + nd] = qn;  // phase B finished reading qs before the barrier
         }
     }
     __syncthreads();
 
-    // ---- phase D: traces of the updated state for the next stage
+    // ---- phase D: traces of the updated state, pushed into the
+    // neighbours' slots for the next stage
     if (act) {
         for (int w = nd; w < 4 * n1; w += np) {
             const int sd = w / n1, k = w - sd * n1;
+            const int2 info = (w == nd) ? P.info0 : A.side[(size_t)e * 4 + sd];
+            if (info.x < 0) continue;
             double tr[3];
             trace3<N>(L, qsE, sd, k, tr);
-            double *o = A.trOut + (((size_t)e * 4 + sd) * 3) * n1 + k;
+            const int kk = side_flip(info.y) ? N - k : k;
+            double *o = A.trOut + (((size_t)info.x * 4 + side_nbr_side(info.y)) * 3) * n1 + kk;
             o[0] = tr[0]; o[n1] = tr[1]; o[2 * n1] = tr[2];
         }
+    }
+}
+
+// One RK stage for G element groups per workgroup.  G = 2 issues the
+// second group's loads before the first group computes, so its memory
+// traffic overlaps the first group's arithmetic (the stage is otherwise
+// one round of waves that all load, then all compute).
+template <int N, bool MIXED, int G>
+__global__ __launch_bounds__(256, 4 / G) void k_sw_stage_fused(SWKernelArgs A)
+{
+    using S = Shape<N>;
+    constexpr int np = S::np, epb = S::epb, n1 = S::n1;
+    __shared__ NodalLDS<N> L;
+    __shared__ double qs[epb * 3 * np], sC[epb * 3 * np], pC[epb * 3 * np], bfs[epb * 12 * n1];
+    __shared__ float sD[MIXED ? epb * 3 * np : 1], pD[MIXED ? epb * 3 * np : 1];
+    const int t = threadIdx.x, le = t / np, nd = t - le * np;
+    const int wg = xcd_block(blockIdx.x, gridDim.x);
+    SWPre P[G];
+    int e[G];
+    bool act[G];
+    #pragma unroll
+    for (int g = 0; g < G; ++g) {
+        e[g] = (wg * G + g) * epb + le;
+        act[g] = le < epb && e[g] < A.nE;
+        sw_load<N, MIXED>(A, e[g], nd, act[g], P[g]);
+    }
+    stage_nodal<N, MIXED>(L, A.dS, A.dS32, A.dP32);
+    const int ls = le < epb ? le : 0;
+    double *qsE = qs + ls * 3 * np, *sCE = sC + ls * 3 * np, *pCE = pC + ls * 3 * np, *bfE = bfs + ls * 12 * n1;
+    float *sDE = MIXED ? sD + ls * 3 * np : sD, *pDE = MIXED ? pD + ls * 3 * np : pD;
+    #pragma unroll
+    for (int g = 0; g < G; ++g) {
+        if (g > 0) __syncthreads();  // the previous group's phase D read qs
+        sw_process<N, MIXED>(A, L, qsE, sCE, pCE, sDE, pDE, bfE, e[g], nd, act[g], P[g]);
     }
 }
 
@@ -563,6 +703,7 @@ struct self_sw2d_s {
     int *edges = nullptr;
     int trCur = 0;
     bool trValid = false, nodalSet = false, geomSet = false, hasRelax = false;
+    int groups = 0;  // element groups per workgroup of the fused kernel (0: heuristic)
     size_t bytes = 0;
 };
 
@@ -640,11 +781,38 @@ int grid_for(size_t nE)
     return (int)((nE + Shape<N>::epb - 1) / Shape<N>::epb);
 }
 
+// element groups per workgroup of the fused stage kernel: the handle's
+// setting (self_sw2d_set_groups / self_sw2d_autotune), else one (halving the
+// waves per SIMD to prefetch a second group costs more latency hiding than
+// the overlap gains at 64x64, N = 7)
+inline int fused_groups(int setting, size_t nE, int epb)
+{
+    (void)nE;
+    (void)epb;
+    return (setting == 2) ? 2 : 1;  // measured: one group is faster at 64x64, N = 7
+}
+
+template <int N>
+int grid_fused(size_t nE, int G)
+{
+    const size_t per = (size_t)Shape<N>::epb * G;
+    return (int)((nE + per - 1) / per);
+}
+
 int launch_traces(self_sw2d_s *h, const double *src, double *dst, int nf)
 {
     const double *lag = h->nodal + 2 * h->n1 * h->n1 + 2 * h->n1;
     SELF_DISPATCH_N(h->d.N, hipLaunchKernelGGL(k_sw_traces<NN>, dim3(grid_for<NN>(h->nE)), dim3(256), 0,
                                                h->stream, src, lag, dst, (int)h->nE, nf));
+    SELF_HIP(hipGetLastError());
+    return SELF_OK;
+}
+
+int launch_traces_push(self_sw2d_s *h, double *dst)
+{
+    const double *lag = h->nodal + 2 * h->n1 * h->n1 + 2 * h->n1;
+    SELF_DISPATCH_N(h->d.N, hipLaunchKernelGGL(k_sw_traces_push<NN>, dim3(grid_for<NN>(h->nE)), dim3(256), 0,
+                                               h->stream, h->q, lag, h->side, dst, (int)h->nE));
     SELF_HIP(hipGetLastError());
     return SELF_OK;
 }
@@ -692,7 +860,7 @@ int step_plain(self_sw2d_s *h)
 int step_fused(self_sw2d_s *h)
 {
     if (!h->trValid) {
-        int rc = launch_traces(h, h->q, h->tr[h->trCur], 3);
+        int rc = launch_traces_push(h, h->tr[h->trCur]);
         if (rc) return rc;
         h->trValid = true;
     }
@@ -703,11 +871,9 @@ int step_fused(self_sw2d_s *h)
         A.trIn = h->tr[h->trCur];
         A.trOut = h->tr[1 - h->trCur];
         if (h->d.precision == SELF_MIXED) {
-            SELF_DISPATCH_N(h->d.N, hipLaunchKernelGGL((k_sw_stage_fused<NN, true>), dim3(grid_for<NN>(h->nE)),
-                                                       dim3(256), 0, h->stream, A));
+            SELF_DISPATCH_N(h->d.N, if (fused_groups(h->groups, h->nE, Shape<NN>::epb) == 2) { hipLaunchKernelGGL((k_sw_stage_fused<NN, true, 2>), dim3(grid_fused<NN>(h->nE, 2)), dim3(256), 0, h->stream, A); } else { hipLaunchKernelGGL((k_sw_stage_fused<NN, true, 1>), dim3(grid_fused<NN>(h->nE, 1)), dim3(256), 0, h->stream, A); });
         } else {
-            SELF_DISPATCH_N(h->d.N, hipLaunchKernelGGL((k_sw_stage_fused<NN, false>), dim3(grid_for<NN>(h->nE)),
-                                                       dim3(256), 0, h->stream, A));
+            SELF_DISPATCH_N(h->d.N, if (fused_groups(h->groups, h->nE, Shape<NN>::epb) == 2) { hipLaunchKernelGGL((k_sw_stage_fused<NN, false, 2>), dim3(grid_fused<NN>(h->nE, 2)), dim3(256), 0, h->stream, A); } else { hipLaunchKernelGGL((k_sw_stage_fused<NN, false, 1>), dim3(grid_fused<NN>(h->nE, 1)), dim3(256), 0, h->stream, A); });
         }
         SELF_HIP(hipGetLastError());
         h->trCur = 1 - h->trCur;
@@ -793,6 +959,9 @@ try {
     if (e == hipSuccess) e = hipMemsetAsync(h->bathy, 0, 3 * nfield * sizeof(double), h->stream);
     if (e == hipSuccess) e = hipMemsetAsync(h->plv, 0, nfield * sizeof(double), h->stream);
     if (e == hipSuccess) e = hipMemsetAsync(h->tend, 0, 3 * nfield * sizeof(double), h->stream);
+    // boundary slots of the pushed traces are never written
+    if (e == hipSuccess) e = hipMemsetAsync(h->tr[0], 0, nE * 12 * n1 * sizeof(double), h->stream);
+    if (e == hipSuccess) e = hipMemsetAsync(h->tr[1], 0, nE * 12 * n1 * sizeof(double), h->stream);
     if (e == hipSuccess && h->bathy32) e = hipMemsetAsync(h->bathy32, 0, 2 * nfield * sizeof(float), h->stream);
     if (e == hipSuccess && h->plv32) e = hipMemsetAsync(h->plv32, 0, nfield * sizeof(float), h->stream);
     if (e == hipSuccess) e = hipEventCreate(&h->ev0);
@@ -1070,7 +1239,7 @@ int self_sw2d_forward_step_rk3_timed(self_sw2d_t h, double tn, int nSteps, doubl
     if (rc) return rc;
     SELF_REQUIRE(elapsed_ms, SELF_ERR_ARG, "null elapsed_ms");
     if (h->d.form == SELF_FUSED && !h->trValid) {  // keep the initial traces out of the timed region
-        rc = launch_traces(h, h->q, h->tr[h->trCur], 3);
+        rc = launch_traces_push(h, h->tr[h->trCur]);
         if (rc) return rc;
         h->trValid = true;
     }
@@ -1095,7 +1264,7 @@ try {
     const int nk = fused ? 1 : 4;
     *nKernels = nk;
     if (fused && !h->trValid) {
-        rc = launch_traces(h, h->q, h->tr[h->trCur], 3);
+        rc = launch_traces_push(h, h->tr[h->trCur]);
         if (rc) return rc;
         h->trValid = true;
     }
@@ -1116,11 +1285,9 @@ try {
                 A.trOut = h->tr[1 - h->trCur];
                 SELF_HIP(mark(0));
                 if (h->d.precision == SELF_MIXED) {
-                    SELF_DISPATCH_N(h->d.N, hipLaunchKernelGGL((k_sw_stage_fused<NN, true>), dim3(grid_for<NN>(h->nE)),
-                                                               dim3(256), 0, h->stream, A));
+                    SELF_DISPATCH_N(h->d.N, if (fused_groups(h->groups, h->nE, Shape<NN>::epb) == 2) { hipLaunchKernelGGL((k_sw_stage_fused<NN, true, 2>), dim3(grid_fused<NN>(h->nE, 2)), dim3(256), 0, h->stream, A); } else { hipLaunchKernelGGL((k_sw_stage_fused<NN, true, 1>), dim3(grid_fused<NN>(h->nE, 1)), dim3(256), 0, h->stream, A); });
                 } else {
-                    SELF_DISPATCH_N(h->d.N, hipLaunchKernelGGL((k_sw_stage_fused<NN, false>), dim3(grid_for<NN>(h->nE)),
-                                                               dim3(256), 0, h->stream, A));
+                    SELF_DISPATCH_N(h->d.N, if (fused_groups(h->groups, h->nE, Shape<NN>::epb) == 2) { hipLaunchKernelGGL((k_sw_stage_fused<NN, false, 2>), dim3(grid_fused<NN>(h->nE, 2)), dim3(256), 0, h->stream, A); } else { hipLaunchKernelGGL((k_sw_stage_fused<NN, false, 1>), dim3(grid_fused<NN>(h->nE, 1)), dim3(256), 0, h->stream, A); });
                 }
                 SELF_HIP(hipGetLastError());
                 SELF_HIP(mark(1));
@@ -1166,6 +1333,68 @@ int self_sw2d_calculate_bathymetry_gradient(self_sw2d_t h)
     SELF_HIP(hipGetLastError());
     return SELF_OK;
 }
+
+int self_sw2d_set_groups(self_sw2d_t h, int groups)
+{
+    SELF_REQUIRE(h && groups >= 0 && groups <= 2, SELF_ERR_ARG, "groups must be 0 (heuristic), 1 or 2");
+    h->groups = groups;
+    return SELF_OK;
+}
+
+int self_sw2d_autotune(self_sw2d_t h, int nSteps, int *groups, double *ms_per_step)
+try {
+    int rc = check_ready(h);
+    if (rc) return rc;
+    SELF_REQUIRE(nSteps >= 1 && groups, SELF_ERR_ARG, "self_sw2d_autotune: bad arguments");
+    if (h->d.form != SELF_FUSED) {
+        *groups = 1;
+        return SELF_OK;
+    }
+    // the trials advance a copy of the state: save and restore everything a step writes
+    const size_t nq = 3 * h->nE * h->np, nt = h->nE * 12 * h->n1;
+    const size_t gb = h->G32 ? sizeof(float) * nq : sizeof(double) * nq;
+    void *G = h->G32 ? (void *)h->G32 : (void *)h->G64;
+    double *sq = nullptr, *st0 = nullptr, *st1 = nullptr;
+    void *sg = nullptr;
+    SELF_HIP(hipStreamSynchronize(h->stream));
+    if (!h->trValid) {
+        if ((rc = launch_traces_push(h, h->tr[h->trCur]))) return rc;
+        h->trValid = true;
+    }
+    SELF_HIP(hipMalloc(&sq, sizeof(double) * nq));
+    hipError_t e = hipMalloc(&sg, gb);
+    if (e == hipSuccess) e = hipMalloc(&st0, sizeof(double) * nt);
+    if (e == hipSuccess) e = hipMalloc(&st1, sizeof(double) * nt);
+    if (e == hipSuccess) e = hipMemcpyAsync(sq, h->q, sizeof(double) * nq, hipMemcpyDeviceToDevice, h->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(sg, G, gb, hipMemcpyDeviceToDevice, h->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(st0, h->tr[0], sizeof(double) * nt, hipMemcpyDeviceToDevice, h->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(st1, h->tr[1], sizeof(double) * nt, hipMemcpyDeviceToDevice, h->stream);
+    const int cur = h->trCur, saved = h->groups;
+    double best = 1e300;
+    int bestG = 1;
+    for (int g = 1; g <= 2 && e == hipSuccess && !rc; ++g) {
+        h->groups = g;
+        double ms = 0.0;
+        rc = self_sw2d_forward_step_rk3(h, 0.0, 1);  // warm
+        if (!rc) rc = self_sw2d_forward_step_rk3_timed(h, 0.0, nSteps, &ms);
+        if (!rc && ms < best) { best = ms; bestG = g; }
+        e = hipMemcpyAsync(h->q, sq, sizeof(double) * nq, hipMemcpyDeviceToDevice, h->stream);
+        if (e == hipSuccess) e = hipMemcpyAsync(G, sg, gb, hipMemcpyDeviceToDevice, h->stream);
+        if (e == hipSuccess) e = hipMemcpyAsync(h->tr[0], st0, sizeof(double) * nt, hipMemcpyDeviceToDevice, h->stream);
+        if (e == hipSuccess) e = hipMemcpyAsync(h->tr[1], st1, sizeof(double) * nt, hipMemcpyDeviceToDevice, h->stream);
+        h->trCur = cur;
+        h->trValid = true;
+    }
+    if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+    (void)hipFree(sq); (void)hipFree(sg); (void)hipFree(st0); (void)hipFree(st1);
+    if (e != hipSuccess) { h->groups = saved; return hip_fail(e, "self_sw2d_autotune", __FILE__, __LINE__); }
+    if (rc) { h->groups = saved; return rc; }
+    h->groups = bestG;
+    *groups = bestG;
+    if (ms_per_step) *ms_per_step = best / nSteps;
+    return SELF_OK;
+}
+SELF_CATCH_ALL
 
 int self_sw2d_synchronize(self_sw2d_t h)
 {

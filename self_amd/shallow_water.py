@@ -195,6 +195,17 @@ class ShallowWater:
     def CalculateBathymetryGradient(self):
         L.check(L.lib().self_sw2d_calculate_bathymetry_gradient(self._h))
 
+    def SetGroups(self, groups):
+        """Element groups per workgroup of the fused kernel (0 = heuristic)."""
+        L.check(L.lib().self_sw2d_set_groups(self._h, groups))
+
+    def Autotune(self, nSteps=50):
+        """Time each fused-kernel setting on a copy of the state; keeps the
+        faster.  Returns (groups, ms_per_step)."""
+        g, ms = C.c_int(), C.c_double()
+        L.check(L.lib().self_sw2d_autotune(self._h, nSteps, C.byref(g), C.byref(ms)))
+        return g.value, ms.value
+
     def Synchronize(self):
         L.check(L.lib().self_sw2d_synchronize(self._h))
 

@@ -208,3 +208,46 @@ def test_geostrophic_500_steps_mixed_within_gate(gpu_lib):
         assert rel_l2(mixed[:, f], fused[:, f]) < 1e-6, (f, rel_l2(mixed[:, f], fused[:, f]))
     vol1 = float(np.sum(Jw * fused[:, 2]))
     assert abs(vol1 - vol0) / abs(vol0) < 1e-13
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+@pytest.mark.parametrize("case", [CASES[0], CASES[2], CASES[5]], ids=_ids)
+def test_pipelined_groups_match_the_one_group_schedule(gpu_lib, case, precision):
+    """The two-group (prefetching) schedule of the fused kernel performs the
+    same arithmetic as the one-group schedule; the compiler may contract
+    multiply-adds differently in the two instantiations, so the results agree
+    to round-off (1e-13), and the FP64 form matches the oracle (1e-12)."""
+    orc, info = build_oracle_case(case["N"], case["nX"], case["nY"], case.get("periodicX", True),
+                                  case.get("periodicY", True), warp=case.get("warp", 0.0))
+    orc.bathymetry_gradient()
+    out = []
+    for groups in (1, 2):
+        m = gpu_model(case, form="fused", precision=precision)
+        load(m, orc, info)
+        m.SetGroups(groups)
+        m.ForwardStepRK3(0.0, 3)
+        out.append(m.GetSolution())
+        m.Trash()
+    for f in range(3):
+        assert rel_l2(out[1][:, f], out[0][:, f]) < 1e-13
+    if precision == "fp64":
+        for s in range(3):
+            orc.forward_step_rk3(s * 1e-3)
+        ref = orc.sol.reshape(out[1].shape)
+        for f in range(3):
+            assert rel_l2(out[1][:, f], ref[:, f]) < TOL64
+
+
+def test_autotune_keeps_the_state(gpu_lib):
+    from self_amd.workloads import sw2d_geostrophic
+    m, info = sw2d_geostrophic(16, 7, precision="mixed", form="fused")
+    before = m.GetSolution()
+    g, ms = m.Autotune(5)
+    assert g in (1, 2) and ms > 0
+    assert np.array_equal(m.GetSolution(), before)
+    m.ForwardStepRK3(0.0, 2)
+    ref, _ = sw2d_geostrophic(16, 7, precision="mixed", form="fused")
+    ref.ForwardStepRK3(0.0, 2)
+    assert np.array_equal(m.GetSolution(), ref.GetSolution())
+    m.Trash()
+    ref.Trash()
