@@ -3,6 +3,7 @@
     python -m self_amd.profile_run copy
     python -m self_amd.profile_run sw2d --precision mixed --form fused --steps 10
     python -m self_amd.profile_run adv2d --steps 10
+    python -m self_amd.profile_run adv3d --steps 5
     python -m self_amd.profile_run cgsem --nx 128 --precond 1
 
 `copy` runs the STREAM-style copy kernel on 64 MiB (21 launches) and 1 GiB
@@ -21,7 +22,7 @@ from . import _lib as L
 
 def main(argv=None):
     ap = argparse.ArgumentParser(prog="python -m self_amd.profile_run")
-    ap.add_argument("mode", choices=["copy", "sw2d", "adv2d", "cgsem"])
+    ap.add_argument("mode", choices=["copy", "sw2d", "adv2d", "adv3d", "cgsem"])
     ap.add_argument("--precision", default="mixed", choices=["fp64", "mixed"])
     ap.add_argument("--form", default="fused", choices=["fused", "plain"])
     ap.add_argument("--nx", type=int, default=None)
@@ -47,6 +48,12 @@ def main(argv=None):
         m.ForwardStepRK3(0.0, a.steps)
         m.Synchronize()
         print(f"adv2d: {a.steps} RK3 steps", flush=True)
+    elif a.mode == "adv3d":
+        from .workloads import adv3d_gaussians
+        m, _ = adv3d_gaussians(nX=a.nx or 32, nY=a.nx or 32, nZ=(a.nx or 32) // 2, N=a.N)
+        m.ForwardStepRK3(0.0, a.steps)
+        m.Synchronize()
+        print(f"adv3d: {a.steps} RK3 steps", flush=True)
     elif a.mode == "cgsem":
         from .workloads import cgsem_helmholtz
         m, _ = cgsem_helmholtz(nX=a.nx or 128, N=a.N)
