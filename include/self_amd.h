@@ -290,7 +290,7 @@ typedef struct self_adv2d_s *self_adv2d_t;
 
 /* Build_Advection (:176-252) for a mesh from self_quadmesh_structured (or
  * any edge table of the same form) and the nodal arrays of
- * self_nodal2d_build; N 1..8, nTracers 1..2. */
+ * self_nodal2d_build; N 1..8, nTracers 1..4. */
 int self_adv2d_create(int N, int nTracers, int nElems, int nEdges, const int *edges, const double *dxds,
                       const double *dxdp, const double *dyds, const double *dydp, const double *J,
                       const double *nHat, const double *nLen, const double *dMatS, const double *dMatP,

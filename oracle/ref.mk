@@ -1,42 +1,65 @@
 # Builds oracle/_ref (test infrastructure only): the reference's own Fortran
-# modules for the 2-D advection-diffusion model, compiled unmodified from
-# where they lie under the reference tree (never copied into this
-# repository), linked with the checker driver ref_adv2d.f90.  All outputs
-# (objects, .mod files, the binary) go to oracle/_ref/.
+# modules for the 2-D advection-diffusion model and the 3-D advection model,
+# compiled unmodified from where they lie under the reference tree (never
+# copied into this repository), linked with the checker drivers
+# ref_adv2d.f90 and ref_adv3d.f90.  All outputs (objects, .mod files,
+# binaries) go to oracle/_ref/.
 #
-#   make -f oracle/ref.mk            # -> oracle/_ref/ref_adv2d
+#   make -f oracle/ref.mk            # -> oracle/_ref/ref_adv2d, oracle/_ref/ref_adv3d
 #
 # No OpenMP, as in the reference's own build/makefile (OPT=-O3, no -fopenmp):
 # its !$OMP directives stay comments (Advection3DClass.f90 has some that
-# flang rejects when they are read).  Module dependency order of
-# src/highend/advection-diffusion2d/AdvectionClass.f90.
+# flang rejects when they are read).  The two models each define a module
+# named AdvectionParamsClass, so each model's own modules get a directory of
+# their own; the shared modules live in oracle/_ref.
 REF      ?= /root/reference/src
 AMDFLANG ?= amdflang
 FFLAGS   ?= -O2
 HERE     := $(dir $(abspath $(lastword $(MAKEFILE_LIST))))
 OUT      := $(HERE)_ref
 
-ADV2D_SRC := common/ModelPrecision common/ConstantsDictionary common/ModelFlags common/CommonRoutines \
-             interp/Legendre interp/Lagrange_1D_Class interp/Lagrange_2D_Class \
-             nodal/NodalStorage_2D_Class dgsem/DGSEMSolutionStorageClass_2D \
-             common/LinkedListClass common/HashTableClass interp/Chebyshev interp/Lagrange_3D_Class \
-             geom/CurveClass_1D geom/CurveClass_2D geom/CurveClass_3D geom/VectorClass \
-             geom/GeometrySupportRoutines geom/MappedGeometryClass_2D geom/NodeClass_2D geom/EdgeClass \
-             geom/QuadElementClass geom/QuadMeshClass \
-             highend/advection-diffusion2d/AdvectionParamsClass highend/advection-diffusion2d/AdvectionClass
-ADV2D_OBJ := $(addprefix $(OUT)/,$(addsuffix .o,$(notdir $(ADV2D_SRC))))
+# Shared modules in dependency order.
+COMMON_SRC := common/ModelPrecision common/ConstantsDictionary common/ModelFlags common/CommonRoutines \
+              common/Timing common/LinkedListClass common/KeyRingClass common/HashTableClass \
+              interp/Legendre interp/Chebyshev interp/Lagrange_1D_Class interp/Lagrange_2D_Class \
+              interp/Lagrange_3D_Class \
+              nodal/NodalStorage_2D_Class nodal/NodalStorage_3D_Class \
+              dgsem/DGSEMSolutionStorageClass_2D dgsem/DGSEMSolutionStorageClass_3D \
+              filters/RollOffFilter3D_Class \
+              geom/CurveClass_1D geom/CurveClass_2D geom/CurveClass_3D geom/VectorClass \
+              geom/GeometrySupportRoutines geom/MappedGeometryClass_2D geom/NodeClass_2D geom/EdgeClass \
+              geom/QuadElementClass geom/QuadMeshClass \
+              geom/SurfaceClass_3D geom/MappedGeometryClass_3D geom/NodeClass_3D geom/FaceClass \
+              geom/HexElementClass geom/HexMeshClass
+COMMON_OBJ := $(addprefix $(OUT)/,$(addsuffix .o,$(notdir $(COMMON_SRC))))
 
-all: $(OUT)/ref_adv2d
+ADV2D_SRC := highend/advection-diffusion2d/AdvectionParamsClass highend/advection-diffusion2d/AdvectionClass
+ADV3D_SRC := highend/advection3d/Advection3DParamsClass highend/advection3d/Advection3DClass
 
-# One recipe compiles the chain in order: each module needs the .mod files of
-# the ones before it.
-$(OUT)/ref_adv2d: $(addprefix $(REF)/,$(addsuffix .f90,$(ADV2D_SRC))) $(HERE)ref_adv2d.f90 $(HERE)ref.mk
+all: $(OUT)/ref_adv2d $(OUT)/ref_adv3d
+
+# One recipe compiles the shared chain in order: each module needs the .mod
+# files of the ones before it.
+$(OUT)/common.stamp: $(addprefix $(REF)/,$(addsuffix .f90,$(COMMON_SRC))) $(HERE)ref.mk
 	@mkdir -p $(OUT)
-	@for f in $(ADV2D_SRC); do \
+	@for f in $(COMMON_SRC); do \
 	  $(AMDFLANG) $(FFLAGS) -J$(OUT) -c $(REF)/$$f.f90 -o $(OUT)/$$(basename $$f).o || exit 1; \
 	done
-	$(AMDFLANG) $(FFLAGS) -J$(OUT) -c $(HERE)ref_adv2d.f90 -o $(OUT)/ref_adv2d.o
-	$(AMDFLANG) $(FFLAGS) -o $@.tmp $(ADV2D_OBJ) $(OUT)/ref_adv2d.o && mv $@.tmp $@
+	@touch $@
+
+# $(call model,name,sources,driver)
+define model
+$(OUT)/ref_$(1): $(OUT)/common.stamp $(addprefix $(REF)/,$(addsuffix .f90,$(2))) $(HERE)ref_$(1).f90
+	@mkdir -p $(OUT)/$(1)
+	@for f in $(2); do \
+	  $(AMDFLANG) $(FFLAGS) -J$(OUT)/$(1) -I$(OUT) -c $(REF)/$$$$f.f90 -o $(OUT)/$(1)/$$$$(basename $$$$f).o || exit 1; \
+	done
+	$(AMDFLANG) $(FFLAGS) -J$(OUT)/$(1) -I$(OUT) -c $(HERE)ref_$(1).f90 -o $(OUT)/$(1)/ref_$(1).o
+	$(AMDFLANG) $(FFLAGS) -o $$@.tmp $(COMMON_OBJ) $(addprefix $(OUT)/$(1)/,$(addsuffix .o,$(notdir $(2)))) \
+	  $(OUT)/$(1)/ref_$(1).o && mv $$@.tmp $$@
+endef
+$(eval $(call model,adv2d,$(ADV2D_SRC)))
+$(eval $(call model,adv3d,$(ADV3D_SRC)))
 
 clean:
 	rm -rf $(OUT)

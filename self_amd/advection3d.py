@@ -43,10 +43,18 @@ class Advection3DParams:
 class Advection3D:
     """Build_Advection (:167-272) with the hex mesh, nodal storage, filter
     and state on the device.  faces: optional (nFaces, 9) face table
-    replacing the mesh's (for boundary flags)."""
+    replacing the mesh's (for boundary flags).
 
-    def __init__(self, params: Advection3DParams, mesh: HexMesh | None = None, faces=None):
+    quirk_relaxation_alias: the reference's SetRelaxationFactor and
+    GetRelaxationFactorAtNode (:915-945) use the relaxation-field storage,
+    so field and factor are one array and the last Set wins.  False (the
+    default) keeps them separate; True reproduces the reference (each Set
+    writes both device arrays)."""
+
+    def __init__(self, params: Advection3DParams, mesh: HexMesh | None = None, faces=None,
+                 quirk_relaxation_alias: bool = False):
         self.params = params
+        self.quirk_relaxation_alias = quirk_relaxation_alias
         N = params.polyDeg
         self.nS = self.nP = self.nQ = N
         self.nEq = params.nTracers
@@ -106,9 +114,13 @@ class Advection3D:
 
     def SetRelaxationField(self, r):
         self._set(2, r, self.nEq)
+        if self.quirk_relaxation_alias:
+            self._set(3, r, self.nEq)
 
     def SetRelaxationFactor(self, f):
         self._set(3, f, self.nEq)
+        if self.quirk_relaxation_alias:
+            self._set(2, f, self.nEq)
 
     def GetFilterState(self):
         """(E1, E2, lim): E1, E2 (nTracers, nElems); lim (3, nTracers, nElems)."""

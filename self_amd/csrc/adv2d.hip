@@ -361,6 +361,22 @@ namespace {
     case 62: { constexpr int NN = 6, QQ = 2; CALL; } break;                                 \
     case 72: { constexpr int NN = 7, QQ = 2; CALL; } break;                                 \
     case 82: { constexpr int NN = 8, QQ = 2; CALL; } break;                                 \
+    case 13: { constexpr int NN = 1, QQ = 3; CALL; } break;                                 \
+    case 23: { constexpr int NN = 2, QQ = 3; CALL; } break;                                 \
+    case 33: { constexpr int NN = 3, QQ = 3; CALL; } break;                                 \
+    case 43: { constexpr int NN = 4, QQ = 3; CALL; } break;                                 \
+    case 53: { constexpr int NN = 5, QQ = 3; CALL; } break;                                 \
+    case 63: { constexpr int NN = 6, QQ = 3; CALL; } break;                                 \
+    case 73: { constexpr int NN = 7, QQ = 3; CALL; } break;                                 \
+    case 83: { constexpr int NN = 8, QQ = 3; CALL; } break;                                 \
+    case 14: { constexpr int NN = 1, QQ = 4; CALL; } break;                                 \
+    case 24: { constexpr int NN = 2, QQ = 4; CALL; } break;                                 \
+    case 34: { constexpr int NN = 3, QQ = 4; CALL; } break;                                 \
+    case 44: { constexpr int NN = 4, QQ = 4; CALL; } break;                                 \
+    case 54: { constexpr int NN = 5, QQ = 4; CALL; } break;                                 \
+    case 64: { constexpr int NN = 6, QQ = 4; CALL; } break;                                 \
+    case 74: { constexpr int NN = 7, QQ = 4; CALL; } break;                                 \
+    case 84: { constexpr int NN = 8, QQ = 4; CALL; } break;                                 \
     default: break;                                                                         \
     }
 
@@ -442,8 +458,8 @@ try {
     SELF_REQUIRE(out && edges && dxds && dxdp && dyds && dydp && J && nHat && nLen && dMatS && dMatP && qWeight &&
                      lagWest && lagEast && lagSouth && lagNorth,
                  SELF_ERR_ARG, "self_adv2d_create: null argument");
-    SELF_REQUIRE(N >= 1 && N <= 8 && nTracers >= 1 && nTracers <= 2 && nElems >= 1, SELF_ERR_ARG,
-                 "self_adv2d_create: N=%d (1..8), nTracers=%d (1..2)", N, nTracers);
+    SELF_REQUIRE(N >= 1 && N <= 8 && nTracers >= 1 && nTracers <= 4 && nElems >= 1, SELF_ERR_ARG,
+                 "self_adv2d_create: N=%d (1..8), nTracers=%d (1..4)", N, nTracers);
     Geo2D G;
     int rc = build_geo2d(N, (size_t)nElems, nEdges, edges, dxds, dxdp, dyds, dydp, J, nHat, nLen, G);
     if (rc) return rc;

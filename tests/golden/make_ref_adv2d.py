@@ -18,7 +18,7 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.dirname(HERE))
 
-import ref_adv2d_run as R  # noqa: E402
+import ref_run as R  # noqa: E402
 
 # name: (N, nX, nY, nTracers, kappa, dt, t0, nSteps, xScale, yScale, seed)
 CASES = {

@@ -1,0 +1,191 @@
+"""Runs the reference's own models, built by oracle/ref.mk from the
+reference's sources into oracle/_ref, on one case and reads back what they
+computed:
+  ref_adv2d  src/highend/advection-diffusion2d/AdvectionClass.f90
+  ref_adv3d  src/highend/advection3d/Advection3DClass.f90
+The file formats are the ones documented at the top of oracle/ref_adv2d.f90
+and oracle/ref_adv3d.f90.  Inputs are seeded random fields: the comparison
+is of the discrete operator, which needs no smoothness.
+"""
+from __future__ import annotations
+
+import os
+import subprocess
+import tempfile
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+REF_DIR = os.path.join(ROOT, "oracle", "_ref")
+
+
+def available(model="adv2d") -> bool:
+    return os.access(os.path.join(REF_DIR, f"ref_{model}"), os.X_OK)
+
+
+def _execute(model, params_text, input_arrays, header, timeout):
+    """Writes advection.params and ref_in.bin into a fresh directory, runs
+    oracle/_ref/ref_<model> there and returns the bytes of ref_out.bin."""
+    exe = os.path.join(REF_DIR, f"ref_{model}")
+    if not available(model):
+        raise FileNotFoundError(exe)
+    with tempfile.TemporaryDirectory() as d:
+        with open(os.path.join(d, "advection.params"), "w") as f:
+            f.write(params_text)
+        with open(os.path.join(d, "ref_in.bin"), "wb") as f:
+            f.write(header)
+            for a in input_arrays:
+                f.write(np.asfortranarray(a, dtype=np.float64).tobytes(order="F"))
+        p = subprocess.run([exe], cwd=d, capture_output=True, text=True, timeout=timeout)
+        if p.returncode != 0:
+            raise RuntimeError(f"ref_{model} failed ({p.returncode}): {p.stderr[-2000:]}")
+        with open(os.path.join(d, "ref_out.bin"), "rb") as f:
+            return f.read()
+
+
+class _Reader:
+    def __init__(self, raw):
+        self.raw, self.pos = raw, 0
+
+    def take(self, dtype, shape):
+        cnt = int(np.prod(shape))
+        a = np.frombuffer(self.raw, dtype=dtype, count=cnt, offset=self.pos)
+        self.pos += cnt * np.dtype(dtype).itemsize
+        return a.reshape(shape, order="F").copy()
+
+    def done(self):
+        assert self.pos == len(self.raw), (self.pos, len(self.raw))
+
+
+def random_inputs(N, nX, nY, nTracers, seed):
+    """c ~ U[-1, 1], u, v ~ U[-1, 1] + (0.6, -0.3), relax ~ U[-1, 1],
+    rFac ~ U[0, 2]; arrays element-last (Fortran order of the reference)."""
+    rng = np.random.default_rng(seed)
+    n1, E = N + 1, nX * nY
+    c = rng.uniform(-1, 1, (n1, n1, nTracers, E))
+    u = rng.uniform(-1, 1, (n1, n1, E)) + 0.6
+    v = rng.uniform(-1, 1, (n1, n1, E)) - 0.3
+    rl = rng.uniform(-1, 1, (n1, n1, nTracers, E))
+    rf = rng.uniform(0, 2, (n1, n1, nTracers, E))
+    return {"c": c, "u": u, "v": v, "relax": rl, "rFac": rf}
+
+
+def _params_text(N, nX, nY, nTracers, kappa, dt, xScale, yScale):
+    return ("&TracerSetup\n"
+            f" nTracers = {nTracers},\n kappa = {kappa!r},\n/\n"
+            "&TimeManagement\n"
+            f" dt = {dt!r},\n iterInit = 0,\n nTimeSteps = 0,\n dumpFreq = 1,\n/\n"
+            "&SpaceManagement\n"
+            f" polyDeg = {N},\n nXElem = {nX},\n nYElem = {nY},\n nPlot = 4,\n"
+            f" xScale = {xScale!r},\n yScale = {yScale!r},\n/\n")
+
+
+def run(N, nX, nY, nTracers, kappa, dt, t0, nSteps, inputs, xScale=1.0, yScale=1.0, timeout=120):
+    """Returns a dict: the inputs, the reference's nodal arrays (dMatS,
+    dMatP (n1, n1), qWeightS, qWeightP, lagWest/East/South/North), its mesh
+    (edges (nEdges, 6) int32, x, y, dxds, dxdp, dyds, dydp, J (n1, n1, E),
+    nHat (2, n1, 4, E), nLen (n1, 4, E)), the tendency and gradients after
+    GlobalTimeDerivative(t0) and the solution after nSteps RK3 steps."""
+    n1, E = N + 1, nX * nY
+    raw = _execute("adv2d", _params_text(N, nX, nY, nTracers, kappa, dt, xScale, yScale),
+                   [inputs[k] for k in ("c", "u", "v", "relax", "rFac")],
+                   np.int32(nSteps).tobytes() + np.float64(t0).tobytes(), timeout)
+    r = _Reader(raw)
+    take = r.take
+    out = dict(inputs)
+    nE, nEd = take(np.int32, (2,))
+    assert nE == E
+    out["dMatS"] = take(np.float64, (n1, n1))
+    out["dMatP"] = take(np.float64, (n1, n1))
+    for k in ("qWeightS", "qWeightP", "lagWest", "lagEast", "lagSouth", "lagNorth"):
+        out[k] = take(np.float64, (n1,))
+    out["edges"] = take(np.int32, (6, int(nEd))).T.copy()
+    for k in ("x", "y", "dxds", "dxdp", "dyds", "dydp", "J"):
+        out[k] = take(np.float64, (n1, n1, E))
+    out["nHat"] = take(np.float64, (2, n1, 4, E))
+    out["nLen"] = take(np.float64, (n1, 4, E))
+    shp = (n1, n1, nTracers, E)
+    out["tend"] = take(np.float64, shp)
+    out["dcdx"] = take(np.float64, shp)
+    out["dcdy"] = take(np.float64, shp)
+    out["sol"] = take(np.float64, shp)
+    r.done()
+    out.update(N=N, nX=nX, nY=nY, nTracers=nTracers, kappa=kappa, dt=dt, t0=t0, nSteps=nSteps)
+    return out
+
+
+def random_inputs_3d(N, nX, nY, nZ, nTracers, seed, noise=1.0):
+    """c: a smooth function of the node indices plus noise whose amplitude
+    (noise x U[0,1]^3) varies per element (so the adaptive filter fires in some elements and
+    not in others); u, v, w ~ U[-0.5, 0.5] + (1, 0.7, 0.3); relax ~ U[-1, 1],
+    rFac ~ U[0, 2].  Arrays element-last (Fortran order)."""
+    rng = np.random.default_rng(seed)
+    n1, E = N + 1, nX * nY * nZ
+    i = np.arange(n1)
+    I, J, K = np.meshgrid(i, i, i, indexing="ij")
+    c = np.empty((n1, n1, n1, nTracers, E))
+    for e in range(E):
+        for f in range(nTracers):
+            ph = rng.uniform(0, 2 * np.pi, 3)
+            smooth = np.cos(0.4 * I + ph[0]) * np.cos(0.3 * J + ph[1]) * np.cos(0.35 * K + ph[2])
+            c[..., f, e] = smooth + noise * rng.uniform(0, 1) ** 3 * rng.uniform(-1, 1, (n1, n1, n1))
+    vel = [rng.uniform(-0.5, 0.5, (n1, n1, n1, E)) + m for m in (1.0, 0.7, 0.3)]
+    rl = rng.uniform(-1, 1, (n1, n1, n1, nTracers, E))
+    rf = rng.uniform(0, 2, (n1, n1, n1, nTracers, E))
+    return {"c": c, "u": vel[0], "v": vel[1], "w": vel[2], "relax": rl, "rFac": rf}
+
+
+def _params_text_3d(N, nX, nY, nZ, nTracers, nCutoff, dt, scale):
+    return ("&TracerSetup\n"
+            f" nTracers = {nTracers},\n/\n"
+            "&TimeManagement\n"
+            f" dt = {dt!r},\n iterInit = 0,\n nTimeSteps = 0,\n dumpFreq = 1,\n/\n"
+            "&SpaceManagement\n"
+            f" polyDeg = {N},\n nCutoff = {nCutoff},\n nXElem = {nX},\n nYElem = {nY},\n nZElem = {nZ},\n"
+            f" nPlot = 4,\n xScale = {scale[0]!r},\n yScale = {scale[1]!r},\n zScale = {scale[2]!r},\n/\n")
+
+
+def run_3d(N, nX, nY, nZ, nTracers, nCutoff, dt, t0, nSteps, inputs, scale=(1.0, 1.0, 1.0), timeout=300):
+    """Returns a dict: the inputs; the reference's nodal arrays (dMatS,
+    dMatP, dMatQ (n1, n1), qWeightS/P/Q, lagWest/East/South/North/Bottom/Top,
+    filterMat (n1^3, n1^3)); its mesh (faces (nFaces, 9) int32, x, y, z, J
+    (n1, n1, n1, E), cov (9, n1, n1, n1, E), nHat (3, n1, n1, 6, E), nLen
+    (n1, n1, 6, E), bpos (3, n1, n1, 6, E)); after GlobalTimeDerivative(t0, 1):
+    tend1, sol1, E1_1, E2_1 (E, nTracers), lim1 (nTracers, E, 3); after
+    nSteps RK3 steps that follow: sol, E1, E2, lim."""
+    n1, E = N + 1, nX * nY * nZ
+    raw = _execute("adv3d", _params_text_3d(N, nX, nY, nZ, nTracers, nCutoff, dt, scale),
+                   [inputs[k] for k in ("c", "u", "v", "w", "relax", "rFac")],
+                   np.int32(nSteps).tobytes() + np.float64(t0).tobytes(), timeout)
+    r = _Reader(raw)
+    take = r.take
+    out = dict(inputs)
+    nE, nF = take(np.int32, (2,))
+    assert nE == E
+    for k in ("dMatS", "dMatP", "dMatQ"):
+        out[k] = take(np.float64, (n1, n1))
+    for k in ("qWeightS", "qWeightP", "qWeightQ", "lagWest", "lagEast", "lagSouth", "lagNorth", "lagBottom",
+              "lagTop"):
+        out[k] = take(np.float64, (n1,))
+    out["filterMat"] = take(np.float64, (n1 ** 3, n1 ** 3))
+    out["faces"] = take(np.int32, (9, int(nF))).T.copy()
+    for k in ("x", "y", "z", "J"):
+        out[k] = take(np.float64, (n1, n1, n1, E))
+    out["cov"] = take(np.float64, (9, n1, n1, n1, E))
+    out["nHat"] = take(np.float64, (3, n1, n1, 6, E))
+    out["nLen"] = take(np.float64, (n1, n1, 6, E))
+    out["bpos"] = take(np.float64, (3, n1, n1, 6, E))
+    shp = (n1, n1, n1, nTracers, E)
+    out["tend1"] = take(np.float64, shp)
+    out["sol1"] = take(np.float64, shp)
+    out["E1_1"] = take(np.float64, (E, nTracers))
+    out["E2_1"] = take(np.float64, (E, nTracers))
+    out["lim1"] = take(np.float64, (nTracers, E, 3))
+    out["sol"] = take(np.float64, shp)
+    out["E1"] = take(np.float64, (E, nTracers))
+    out["E2"] = take(np.float64, (E, nTracers))
+    out["lim"] = take(np.float64, (nTracers, E, 3))
+    r.done()
+    out.update(N=N, nX=nX, nY=nY, nZ=nZ, nTracers=nTracers, nCutoff=nCutoff, dt=dt, t0=t0, nSteps=nSteps,
+               xScale=scale[0], yScale=scale[1], zScale=scale[2])
+    return out

@@ -24,6 +24,8 @@ CASES = [
     dict(N=8, nX=3, nY=3, nq=1, periodicX=False, periodicY=True, warp=0.05, relax=True),
     dict(N=1, nX=4, nY=4, nq=1),
     dict(N=2, nX=4, nY=3, nq=2, quirk=True),
+    dict(N=4, nX=3, nY=3, nq=3, periodicX=False, periodicY=False, inflow=True, relax=True),
+    dict(N=8, nX=2, nY=3, nq=4, periodicX=False, periodicY=True, warp=0.03, inflow=True),
 ]
 
 
@@ -131,7 +133,7 @@ def test_errors_are_reported(gpu_lib):
     with pytest.raises(L.SelfError):
         Advection(AdvectionParams(polyDeg=9, nXElem=3, nYElem=3))
     with pytest.raises(L.SelfError):
-        Advection(AdvectionParams(polyDeg=3, nXElem=3, nYElem=3, nTracers=3))
+        Advection(AdvectionParams(polyDeg=3, nXElem=3, nYElem=3, nTracers=5))
     m = Advection(AdvectionParams(polyDeg=3, nXElem=3, nYElem=3))
     with pytest.raises(ValueError):
         m.SetSolution(np.zeros((9, 2, 4, 4)))

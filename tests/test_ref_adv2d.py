@@ -25,7 +25,7 @@ import numpy as np
 import pytest
 
 from oracle import pyoracle as po
-import ref_adv2d_run as R
+import ref_run as R
 
 GOLDEN = sorted(glob.glob(os.path.join(os.path.dirname(__file__), "golden", "ref_adv2d_*.npz")))
 
