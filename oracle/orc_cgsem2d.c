@@ -10,8 +10,10 @@
  * Node-to-element lists are in increasing element order (LinkedList
  * AddToList appends, src/common/LinkedListClass.f90:239-312;
  * GetNodeToElementConnectivity_QuadMesh adds elements in order).
- * Quirk Q6 (Mask tests an uninitialised eID, :387) is resolved to the
- * intended behaviour: secondary sides of interior edges are masked.
+ * Quirk Q6 (Mask tests an uninitialised eID, :387): by default resolved to
+ * the intended behaviour (secondary sides of interior edges are masked);
+ * q6AsBuilt = 1 leaves them, as the reference compiled with amdflang -O2
+ * does.
  * The preconditioner is the identity (the reference's pcIterates = 0
  * branch, CGsemPreconditioner_2D_Class.f90:740-742) or, as an addition,
  * the inverse of the assembled diagonal (Jacobi).
@@ -48,7 +50,7 @@ void orc_cgsem2d_mask(const orc_cgsem2d *c, double *u)
     for (int ie = 0; ie < c->nEdges; ++ie) {
         const int *ed = c->edges + 6 * ie;
         if (ed[2] == ORC_DIRICHLET) mask_side(c, u, ed[0] - 1, ed[1]);
-        else if (ed[2] > 0) mask_side(c, u, ed[2] - 1, ed[3]);
+        else if (ed[2] > 0 && !c->q6AsBuilt) mask_side(c, u, ed[2] - 1, ed[3]);
     }
     for (int n = 0; n < c->nNodes; ++n) {
         int start = c->nodeType[n] == ORC_DIRICHLET ? 0 : 1;

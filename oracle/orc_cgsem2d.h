@@ -18,6 +18,11 @@ typedef struct {
     int maxIters;
     double *resi;
     int resiLen;
+    /* Q6 (Mask tests an uninitialised element ID, CGsemElliptic_2D_Class.f90:387):
+     * 0 masks the secondary sides of interior edges (the evident intent);
+     * 1 leaves them (what the reference does when compiled with amdflang -O2,
+     * observed on its own build: the test never succeeds). */
+    int q6AsBuilt;
 } orc_cgsem2d;
 
 int orc_cgsem2d_connectivity(orc_cgsem2d *c, const int *elemNodes, int nNodes);

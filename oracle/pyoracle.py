@@ -408,7 +408,7 @@ class CGSEM2DStruct(C.Structure):
                 ("nodeType", _ip), ("D", _dp), ("w", _dp), ("dxds", _dp), ("dxdp", _dp), ("dyds", _dp),
                 ("dydp", _dp), ("J", _dp), ("fluxCoeff", _dp), ("solution", _dp), ("source", _dp),
                 ("dirichletValue", C.c_double), ("tolerance", C.c_double), ("maxIters", C.c_int),
-                ("resi", _dp), ("resiLen", C.c_int)]
+                ("resi", _dp), ("resiLen", C.c_int), ("q6AsBuilt", C.c_int)]
 
 
 def _declare_extra_cgsem2d(L):
@@ -428,9 +428,10 @@ def _declare_extra_cgsem2d(L):
 class CGSEM2D:
     """CGsemElliptic_2D on a structured walled mesh (all boundaries Dirichlet),
     GLL nodes, CG derivative matrix.  Arrays element-local (nElems, N+1, N+1)
-    indexed [e, j(p), i(s)]."""
+    indexed [e, j(p), i(s)].  q6_as_built: Mask leaves the secondary sides of
+    interior edges, as the reference compiled with amdflang -O2 does."""
 
-    def __init__(self, N, nX, nY, Lx=1.0, Ly=1.0, warp=0.0, tolerance=1e-10, maxIters=2000):
+    def __init__(self, N, nX, nY, Lx=1.0, Ly=1.0, warp=0.0, tolerance=1e-10, maxIters=2000, q6_as_built=False):
         self.N = N
         n1 = N + 1
         self.nodes, self.w = quadrature(N, GAUSS_LOBATTO)
@@ -456,7 +457,7 @@ class CGSEM2D:
                                _pi(self.nodeLocal), _pi(self.nodeType), _p(self.D), _p(self.w), _p(m.dxds),
                                _p(m.dxdp), _p(m.dyds), _p(m.dydp), _p(m.J), _p(self.fluxCoeff),
                                _p(self.solution), _p(self.source), 0.0, tolerance, maxIters, _p(self.resi),
-                               maxIters + 1)
+                               maxIters + 1, int(q6_as_built))
         lib().orc_cgsem2d_connectivity(C.byref(self.s), _pi(np.ascontiguousarray(m.elemNodes)), nN)
 
     def shape(self, a):

@@ -3,8 +3,9 @@ reference's sources into oracle/_ref, on one case and reads back what they
 computed:
   ref_adv2d  src/highend/advection-diffusion2d/AdvectionClass.f90
   ref_adv3d  src/highend/advection3d/Advection3DClass.f90
-The file formats are the ones documented at the top of oracle/ref_adv2d.f90
-and oracle/ref_adv3d.f90.  Inputs are seeded random fields: the comparison
+  ref_cgsem2d  src/cgsem/CGsemElliptic_2D_Class.f90
+The file formats are the ones documented at the top of oracle/ref_adv2d.f90,
+oracle/ref_adv3d.f90 and oracle/ref_cgsem2d.f90.  Inputs are seeded random fields: the comparison
 is of the discrete operator, which needs no smoothness.
 """
 from __future__ import annotations
@@ -23,14 +24,15 @@ def available(model="adv2d") -> bool:
     return os.access(os.path.join(REF_DIR, f"ref_{model}"), os.X_OK)
 
 
-def _execute(model, params_text, input_arrays, header, timeout):
-    """Writes advection.params and ref_in.bin into a fresh directory, runs
-    oracle/_ref/ref_<model> there and returns the bytes of ref_out.bin."""
+def _execute(model, params_text, input_arrays, header, timeout, params_file="advection.params"):
+    """Writes the namelist file (params_file) and ref_in.bin into a fresh
+    directory, runs oracle/_ref/ref_<model> there and returns the bytes of
+    ref_out.bin."""
     exe = os.path.join(REF_DIR, f"ref_{model}")
     if not available(model):
         raise FileNotFoundError(exe)
     with tempfile.TemporaryDirectory() as d:
-        with open(os.path.join(d, "advection.params"), "w") as f:
+        with open(os.path.join(d, params_file), "w") as f:
             f.write(params_text)
         with open(os.path.join(d, "ref_in.bin"), "wb") as f:
             f.write(header)
@@ -188,4 +190,52 @@ def run_3d(N, nX, nY, nZ, nTracers, nCutoff, dt, t0, nSteps, inputs, scale=(1.0,
     r.done()
     out.update(N=N, nX=nX, nY=nY, nZ=nZ, nTracers=nTracers, nCutoff=nCutoff, dt=dt, t0=t0, nSteps=nSteps,
                xScale=scale[0], yScale=scale[1], zScale=scale[2])
+    return out
+
+
+def random_inputs_cgsem2d(N, nX, nY, seed):
+    """fluxCoeff ~ U[0.5, 1.5], source ~ U[-1, 1], solution ~ U[-0.5, 0.5],
+    test vectors u, v ~ U[-1, 1] (element-last, Fortran order)."""
+    rng = np.random.default_rng(seed)
+    shp = (N + 1, N + 1, nX * nY)
+    return {"fluxCoeff": rng.uniform(0.5, 1.5, shp), "source": rng.uniform(-1, 1, shp),
+            "solution": rng.uniform(-0.5, 0.5, shp), "u": rng.uniform(-1, 1, shp), "v": rng.uniform(-1, 1, shp)}
+
+
+def run_cgsem2d(N, nX, nY, maxIters, tolerance, dirValue, inputs, timeout=300):
+    """The reference's CGsemElliptic_2D with pcIterates = 0 (identity
+    preconditioner).  Returns a dict: the inputs; dMat, qWeight; edges
+    (nEdges, 6) int32, nodeType (nNodes,) int32; x, y, dxds, dxdp, dyds,
+    dydp, J (n1, n1, E); mask_v, unmask_mask_v, gsum_v; Au, u_masked (after
+    MatrixAction(u)); dot_uv; r, sol_after_residual; ioerr, resi
+    (maxIters + 1,), sol_after_solve."""
+    n1, E = N + 1, nX * nY
+    params = ("&SolverCriteria\n"
+              f" MaximumIterates = {maxIters},\n pcIterates = 0,\n tolerance = {tolerance!r},\n"
+              " pcTolerance = 1.0e-4,\n pcDegree = 1,\n/\n"
+              "&SpaceManagement\n"
+              f" polyDeg = {N},\n nXElem = {nX},\n nYElem = {nY},\n nPlot = 4,\n xScale = 1.0,\n/\n")
+    raw = _execute("cgsem2d", params, [inputs[k] for k in ("fluxCoeff", "source", "solution", "u", "v")],
+                   np.float64(dirValue).tobytes(), timeout, params_file="runtime.params")
+    r = _Reader(raw)
+    take = r.take
+    out = dict(inputs)
+    nE, nEd, nNo = take(np.int32, (3,))
+    assert nE == E
+    out["dMat"] = take(np.float64, (n1, n1))
+    out["qWeight"] = take(np.float64, (n1,))
+    out["edges"] = take(np.int32, (6, int(nEd))).T.copy()
+    out["nodeType"] = take(np.int32, (int(nNo),))
+    shp = (n1, n1, E)
+    for k in ("x", "y", "dxds", "dxdp", "dyds", "dydp", "J", "mask_v", "unmask_mask_v", "gsum_v", "Au",
+              "u_masked"):
+        out[k] = take(np.float64, shp)
+    out["dot_uv"] = take(np.float64, (1,))[0]
+    out["r"] = take(np.float64, shp)
+    out["sol_after_residual"] = take(np.float64, shp)
+    out["ioerr"] = int(take(np.int32, (1,))[0])
+    out["resi"] = take(np.float64, (maxIters + 1,))
+    out["sol_after_solve"] = take(np.float64, shp)
+    r.done()
+    out.update(N=N, nX=nX, nY=nY, maxIters=maxIters, tolerance=tolerance, dirValue=dirValue)
     return out
