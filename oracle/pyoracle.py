@@ -346,7 +346,14 @@ def riemann(inS, outS, nHat, g):
 
 
 # --------------------------------------------------------------------------- Krylov
-def _dense_solver(fn_name, A, b, x0=None, maxIters=200, m=30, tol=1e-12, precond=False):
+class DenseCtx(C.Structure):
+    _fields_ = [("n", C.c_int), ("A", _dp), ("b", _dp)]
+
+
+def _dense_solver(fn_name, A, b, x0=None, maxIters=200, m=30, tol=1e-12, precond=False, native=False):
+    """native=True applies the operator with the oracle's C helpers
+    (orc_dense_matvec/residual/jacobi: column sums in ascending order, b - As,
+    s / diag A) instead of numpy."""
     L = lib()
     n = A.shape[0]
     A = np.asfortranarray(A, dtype=np.float64)
@@ -365,11 +372,18 @@ def _dense_solver(fn_name, A, b, x0=None, maxIters=200, m=30, tol=1e-12, precond
         v = np.ctypeslib.as_array(i, shape=(n,))
         np.ctypeslib.as_array(o, shape=(n,))[:] = v / np.diag(A)
 
-    cb = (VecOp(mv), VecOp(res), VecOp(pc))
+    ctx = None
+    if native:
+        ctx = DenseCtx(n, _p(Af), _p(b))
+        cb = tuple(C.cast(getattr(L, f), VecOp) for f in ("orc_dense_matvec", "orc_dense_residual",
+                                                          "orc_dense_jacobi"))
+    else:
+        cb = (VecOp(mv), VecOp(res), VecOp(pc))
     sol = np.zeros(n) if x0 is None else np.array(x0, dtype=np.float64)
     resiLen = maxIters * (m if "gmres" in fn_name else 1) + 1
     resi = np.zeros(resiLen)
-    k = Krylov(n, maxIters, m, tol, None, cb[0], cb[1], cb[2], _p(sol), _p(resi), resiLen, 0)
+    k = Krylov(n, maxIters, m, tol, C.cast(C.pointer(ctx), C.c_void_p) if native else None, cb[0], cb[1], cb[2],
+               _p(sol), _p(resi), resiLen, 0)
     ioerr = getattr(L, fn_name)(C.byref(k))
     del Af
     return sol, resi, ioerr, k.iterations

@@ -1,12 +1,12 @@
 # Builds oracle/_ref (test infrastructure only): the reference's own Fortran
 # modules for the 2-D advection-diffusion model, the 3-D advection model and
-# the 2-D CG spectral-element elliptic solver, compiled unmodified from where
-# they lie under the reference tree (never copied into this repository),
-# linked with the checker drivers ref_adv2d.f90, ref_adv3d.f90 and
-# ref_cgsem2d.f90.  All outputs (objects, .mod files, binaries) go to
-# oracle/_ref/.
+# the 2-D CG spectral-element elliptic solver, and its Krylov loops, compiled
+# unmodified from where they lie under the reference tree (never copied into
+# this repository), linked with the checker drivers ref_adv2d.f90,
+# ref_adv3d.f90, ref_cgsem2d.f90 and ref_krylov.f90.  All outputs (objects,
+# .mod files, binaries) go to oracle/_ref/.
 #
-#   make -f oracle/ref.mk     # -> oracle/_ref/ref_adv2d, ref_adv3d, ref_cgsem2d
+#   make -f oracle/ref.mk   # -> oracle/_ref/ref_adv2d, ref_adv3d, ref_cgsem2d, ref_krylov
 #
 # No OpenMP, as in the reference's own build/makefile (OPT=-O3, no -fopenmp):
 # its !$OMP directives stay comments (Advection3DClass.f90 has some that
@@ -37,8 +37,9 @@ COMMON_OBJ := $(addprefix $(OUT)/,$(addsuffix .o,$(notdir $(COMMON_SRC))))
 ADV2D_SRC := highend/advection-diffusion2d/AdvectionParamsClass highend/advection-diffusion2d/AdvectionClass
 ADV3D_SRC := highend/advection3d/Advection3DParamsClass highend/advection3d/Advection3DClass
 CGSEM2D_SRC := cgsem/CGsemParams_Class cgsem/CGsemPreconditioner_2D_Class cgsem/CGsemElliptic_2D_Class
+KRYLOV_SRC := iterativesolve/IterativeSolversParams_Class iterativesolve/IterativeSolvers_Class
 
-all: $(OUT)/ref_adv2d $(OUT)/ref_adv3d $(OUT)/ref_cgsem2d
+all: $(OUT)/ref_adv2d $(OUT)/ref_adv3d $(OUT)/ref_cgsem2d $(OUT)/ref_krylov
 
 # One recipe compiles the shared chain in order: each module needs the .mod
 # files of the ones before it.
@@ -63,6 +64,7 @@ endef
 $(eval $(call model,adv2d,$(ADV2D_SRC)))
 $(eval $(call model,adv3d,$(ADV3D_SRC)))
 $(eval $(call model,cgsem2d,$(CGSEM2D_SRC)))
+$(eval $(call model,krylov,$(KRYLOV_SRC)))
 
 clean:
 	rm -rf $(OUT)

@@ -239,3 +239,22 @@ def run_cgsem2d(N, nX, nY, maxIters, tolerance, dirValue, inputs, timeout=300):
     r.done()
     out.update(N=N, nX=nX, nY=nY, maxIters=maxIters, tolerance=tolerance, dirValue=dirValue)
     return out
+
+
+KRYLOV_METHODS = {"cg": 1, "pcg": 2, "gmres": 3, "pcgmres": 4}
+
+
+def run_krylov(A, b, x0, method, maxIters, mInner, tolerance, timeout=120):
+    """The reference's Solve_<method> of IterativeSolvers_Class on the dense
+    system A x = b from x0 (oracle/ref_krylov.f90).  Returns ioerr, resi (the
+    whole history array) and x."""
+    n = A.shape[0]
+    header = (np.array([n, KRYLOV_METHODS[method], maxIters, mInner], dtype=np.int32).tobytes()
+              + np.float64(tolerance).tobytes())
+    raw = _execute("krylov", "", [A, b, x0], header, timeout, params_file="unused.params")
+    r = _Reader(raw)
+    ioerr, resiLen = (int(v) for v in r.take(np.int32, (2,)))
+    resi = r.take(np.float64, (resiLen,))
+    x = r.take(np.float64, (n,))
+    r.done()
+    return {"ioerr": ioerr, "resi": resi, "x": x}
