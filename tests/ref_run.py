@@ -258,3 +258,34 @@ def run_krylov(A, b, x0, method, maxIters, mInner, tolerance, timeout=120):
     x = r.take(np.float64, (n,))
     r.done()
     return {"ioerr": ioerr, "resi": resi, "x": x}
+
+
+def run_nodal(N, quadrature, form, nNew, f2, f3, timeout=60):
+    """The reference's quadrature, 2-D/3-D nodal storage, ApplyDerivativeMatrix,
+    CalculateAtBoundaries, interpolation matrix and CoarseToFine for one
+    degree (oracle/ref_nodal.f90).  f2 (N+1, N+1) and f3 (N+1, N+1, N+1)
+    indexed [i, j(, k)]."""
+    n1, m1 = N + 1, nNew + 1
+    header = np.array([N, quadrature, form, nNew], dtype=np.int32).tobytes()
+    raw = _execute("nodal", "", [f2, f3], header, timeout, params_file="unused.params")
+    r = _Reader(raw)
+    t = r.take
+    out = {"N": N, "quadrature": quadrature, "form": form, "nNew": nNew, "f2": f2, "f3": f3}
+    out["nodes"], out["weights"] = t(np.float64, (n1,)), t(np.float64, (n1,))
+    out["dMatS"], out["dMatP"] = t(np.float64, (n1, n1)), t(np.float64, (n1, n1))
+    out["qWeightS"] = t(np.float64, (n1,))
+    for k in ("lagWest", "lagEast", "lagSouth", "lagNorth"):
+        out[k] = t(np.float64, (n1,))
+    out["dFds2"], out["dFdp2"] = t(np.float64, (n1, n1)), t(np.float64, (n1, n1))
+    for k in ("fWest2", "fEast2", "fSouth2", "fNorth2"):
+        out[k] = t(np.float64, (n1,))
+    out["dMat3"] = t(np.float64, (n1, n1))
+    for k in ("dFds3", "dFdp3", "dFdq3"):
+        out[k] = t(np.float64, (n1, n1, n1))
+    for k in ("fWest3", "fEast3", "fSouth3", "fNorth3", "fBottom3", "fTop3"):
+        out[k] = t(np.float64, (n1, n1))
+    out["Ts"] = t(np.float64, (m1, n1))
+    out["fine2"] = t(np.float64, (m1, m1))
+    out["fine3"] = t(np.float64, (m1, m1, m1))
+    r.done()
+    return out
