@@ -484,7 +484,7 @@ try {
     hipError_t e = hipGetDevice(&h->device);
     if (e == hipSuccess) e = hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking);
     auto M = [&](double **p, size_t n) { if (e == hipSuccess) e = hipMalloc(p, sizeof(double) * n); };
-    M(&h->nodal, 2 * n1 * n1 + 6 * n1);
+    M(&h->nodal, nodal_buffer_size(n1));
     M(&h->c, nq * nE * np);
     M(&h->G, nq * nE * np);
     M(&h->tend, nq * nE * np);
@@ -500,7 +500,7 @@ try {
     if (e == hipSuccess) e = hipMalloc(&h->side, sizeof(int2) * nE * 4);
     if (e == hipSuccess) e = hipMalloc(&h->pnode, sizeof(int) * nE * 4 * n1);
     if (e != hipSuccess) return fail(e);
-    std::vector<double> nod(2 * n1 * n1 + 6 * n1);
+    std::vector<double> nod(nodal_buffer_size(n1));
     std::memcpy(nod.data(), dMatS, sizeof(double) * n1 * n1);
     std::memcpy(nod.data() + n1 * n1, dMatP, sizeof(double) * n1 * n1);
     double *p = nod.data() + 2 * n1 * n1;
@@ -510,6 +510,7 @@ try {
     std::memcpy(p + 3 * n1, lagEast, sizeof(double) * n1);
     std::memcpy(p + 4 * n1, lagSouth, sizeof(double) * n1);
     std::memcpy(p + 5 * n1, lagNorth, sizeof(double) * n1);
+    for (int k = 0; k < n1; ++k) p[6 * n1 + k] = p[7 * n1 + k] = 1.0 / qWeight[k];
     std::vector<int2> sd(nE * 4);
     for (size_t k = 0; k < nE * 4; ++k) sd[k] = make_int2(G.sideX[k], G.sideY[k]);
     e = hipMemcpy(h->nodal, nod.data(), sizeof(double) * nod.size(), hipMemcpyHostToDevice);

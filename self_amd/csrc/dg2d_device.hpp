@@ -56,6 +56,41 @@ __device__ __forceinline__ void sw_riemann(const double in[3], const double out[
     f[2] = 0.5 * (a2 - fac * (out[2] - in[2]));
 }
 
+// 1/x from the FP32 reciprocal refined by two FP64 Newton steps
+// (r <- r + r (1 - x r)): within an ulp or two of the IEEE quotient, for x
+// inside the FP32 range; used by the mixed-precision form only.
+__device__ __forceinline__ double rcp_refined(double x)
+{
+    double r = (double)__builtin_amdgcn_rcpf((float)x);
+    double e = __builtin_fma(-x, r, 1.0);
+    r = __builtin_fma(r, e, r);
+    e = __builtin_fma(-x, r, 1.0);
+    return __builtin_fma(r, e, r);
+}
+
+// Rusanov flux with the mixed form's arithmetic: the central part in FP64
+// (velocities from refined reciprocals), the dissipation coefficient
+// max |u_n +- c| in FP32 (it multiplies the state jump only).
+__device__ __forceinline__ void sw_riemann_mixed(const double in[3], const double out[3], double nx, double ny,
+                                                 double g, double f[3])
+{
+    const double uL = in[0], vL = in[1], pL = in[2];
+    const double uR = out[0], vR = out[1], pR = out[2];
+    const double uOut = uR * nx + vR * ny;
+    const double uIn = uL * nx + vL * ny;
+    const double wR = uOut * rcp_refined(pR), wL = uIn * rcp_refined(pL);
+    const float cR = __builtin_amdgcn_sqrtf((float)(g * pR)), cL = __builtin_amdgcn_sqrtf((float)(g * pL));
+    const float wRf = (float)wR, wLf = (float)wL;
+    const double fac = (double)fmaxf(fmaxf(fabsf(wRf + cR), fabsf(wRf - cR)), fmaxf(fabsf(wLf + cL), fabsf(wLf - cL)));
+    const double hgR = 0.5 * g * pR * pR, hgL = 0.5 * g * pL * pL;
+    const double a0 = wR * uR + hgR * nx + wL * uL + hgL * nx;
+    const double a1 = wR * vR + hgR * ny + wL * vL + hgL * ny;
+    const double a2 = uOut + uIn;
+    f[0] = 0.5 * (a0 - fac * (out[0] - in[0]));
+    f[1] = 0.5 * (a1 - fac * (out[1] - in[1]));
+    f[2] = 0.5 * (a2 - fac * (out[2] - in[2]));
+}
+
 // GetExternalState, ConservativeShallowWaterClass.f90:1698-1729
 __device__ __forceinline__ void sw_external(int bc, double nx, double ny, const double in[3], double ex[3])
 {
@@ -69,10 +104,11 @@ __device__ __forceinline__ void sw_external(int bc, double nx, double ny, const 
 }
 
 // XFlux / YFlux, ConservativeShallowWaterClass.f90:1608-1658
+template <bool FAST = false>
 __device__ __forceinline__ void sw_fluxes(const double s[3], double g, double xF[3], double yF[3])
 {
     // one reciprocal instead of four divisions (differs from U*U/H by at most 1 ulp)
-    const double invH = 1.0 / s[2];
+    const double invH = FAST ? rcp_refined(s[2]) : 1.0 / s[2];
     const double hp = g * 0.5 * s[2] * s[2];
     xF[0] = s[0] * s[0] * invH + hp;
     xF[1] = s[0] * s[1] * invH;
@@ -111,17 +147,19 @@ struct NodalLDS {
     float dS32[n1 * n1], dP32[n1 * n1];
 };
 
-// global nodal buffer: dS, dP, qwS, qwP, lagW, lagE, lagS, lagN (doubles)
+// global nodal buffer (doubles): dS, dP, qwS, qwP, lagW, lagE, lagS, lagN,
+// then 1/qwS, 1/qwP (computed on the host: the same IEEE quotients)
+constexpr int nodal_buffer_size(int n1) { return 2 * n1 * n1 + 8 * n1; }
+
 template <int N, bool F32>
 __device__ __forceinline__ void stage_nodal(NodalLDS<N> &L, const double *__restrict__ g,
                                             const float *__restrict__ dS32, const float *__restrict__ dP32)
 {
     constexpr int n1 = N + 1, nd = 2 * n1 * n1 + 6 * n1;
-    double *dst = L.dS;  // members up to lN are contiguous doubles
+    double *dst = L.dS;  // members up to lN are contiguous doubles: dS, dP, iwS, iwP, lags
     for (int k = threadIdx.x; k < nd; k += blockDim.x) {
-        const double v = g[k];
         const bool w = (k >= 2 * n1 * n1) && (k < 2 * n1 * n1 + 2 * n1);
-        dst[k] = w ? 1.0 / v : v;
+        dst[k] = g[w ? k + 6 * n1 : k];
     }
     if constexpr (F32) {
         for (int k = threadIdx.x; k < n1 * n1; k += blockDim.x) {

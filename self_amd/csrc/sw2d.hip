@@ -55,6 +55,8 @@ struct SWKernelArgs {
     float *G32;                // [3][nE][np] (mixed form)
     double *tend;              // [3][nE][np] (plain form)
     const double *metric;      // [5][nE][np] dxds, dxdp, dyds, dydp, J
+    const double *metM;        // [nE][5] per-element reference values (mixed)
+    const float *metD;         // [5][nE][np] FP32 deviations from them (mixed)
     const double *bathy;       // [3][nE][np] h, hx, hy
     const float *bathy32;      // [2][nE][np] hx, hy (mixed)
     const double *plv;         // [nE][np]
@@ -62,6 +64,8 @@ struct SWKernelArgs {
     const double *relax;       // [4][nE][np]
     const double *nrm;         // [nE][4][3][n1] own normals (nx, ny, |n|)
     const double *fnrm;        // [nE][4][3][n1] edge-primary normals seen from this side
+    const double *fnM;         // [nE][4][3] per-side reference values (mixed)
+    const float *fnD;          // [nE][4][3][n1] FP32 deviations from them (mixed)
     const int2 *side;          // [nE][4]
     const double *trIn;        // [nE][4][3][n1]
     double *trOut;
@@ -104,17 +108,15 @@ __device__ __forceinline__ void element_trace(const double *qs, int s, int k, co
 }
 
 // Source, ConservativeShallowWaterClass.f90:1662-1694
+// dragOverH = LinearDrag / h (0 without drag), so no division per call.
 __device__ __forceinline__ void sw_source(const double s[3], double vort, double g, double hx, double hy,
-                                          double drag, double h, bool relax, const double r[4],
-                                          double q[3])
+                                          double dragOverH, bool relax, const double r[4], double q[3])
 {
     q[0] = vort * s[1] + g * s[2] * hx;
     q[1] = -vort * s[0] + g * s[2] * hy;
     q[2] = 0.0;
-    if (drag != 0.0) {
-        q[0] = q[0] - drag * s[0] / h;
-        q[1] = q[1] - drag * s[1] / h;
-    }
+    q[0] = q[0] - dragOverH * s[0];
+    q[1] = q[1] - dragOverH * s[1];
     if (relax) {
         q[0] = q[0] - (s[0] - r[0]) * r[3];
         q[1] = q[1] - (s[1] - r[1]) * r[3];
@@ -357,11 +359,12 @@ __device__ __forceinline__ void dg_divergence_dev32(const NodalLDS<N> &L, const 
 
 // contravariant volume fluxes of one node into LDS (MappedTimeDerivative
 // :1383-1398 and :1434-1449)
+template <bool FAST = false>
 __device__ __forceinline__ void contravariant(const double s[3], const double met[4], double g, double *sC,
                                               double *pC, int np)
 {
     double xF[3], yF[3];
-    sw_fluxes(s, g, xF, yF);
+    sw_fluxes<FAST>(s, g, xF, yF);
     #pragma unroll
     for (int f = 0; f < 3; ++f) {
         sC[f * np] = met[3] * xF[f] - met[1] * yF[f];   //  dydp*f - dxdp*g
@@ -383,7 +386,7 @@ __global__ __launch_bounds__(256, 4) void k_sw_mapped(SWKernelArgs A)
     const int i = nd % n1, j = nd / n1;
     const size_t mstride = (size_t)A.nE * np, base = (size_t)e * np + nd;
     stage_nodal<N, false>(L, A.dS, nullptr, nullptr);
-    double s[3], met[4], invJ = 0, hx = 0, hy = 0, vort = 0, h = 1.0, r[4] = {0, 0, 0, 0};
+    double s[3], met[4], invJ = 0, hx = 0, hy = 0, vort = 0, h = 0.0, r[4] = {0, 0, 0, 0};  // h: drag / depth
     if (act) {
         #pragma unroll
         for (int f = 0; f < 3; ++f) s[f] = A.q[f * mstride + base];
@@ -393,7 +396,7 @@ __global__ __launch_bounds__(256, 4) void k_sw_mapped(SWKernelArgs A)
         hx = A.bathy[mstride + base];
         hy = A.bathy[2 * mstride + base];
         vort = A.plv[base];
-        if (A.drag != 0.0) h = A.bathy[base];
+        if (A.drag != 0.0) h = A.drag / A.bathy[base];
         if (A.hasRelax) {
             #pragma unroll
             for (int k = 0; k < 4; ++k) r[k] = A.relax[k * mstride + base];
@@ -405,7 +408,7 @@ __global__ __launch_bounds__(256, 4) void k_sw_mapped(SWKernelArgs A)
     if (!act) return;
     double tend[3], src[3];
     dg_divergence64<N, false>(L, sC + le * 3 * np, pC + le * 3 * np, bfs + le * 12 * n1, i, j, invJ, tend);
-    sw_source(s, vort, A.g, hx, hy, A.drag, h, A.hasRelax, r, src);
+    sw_source(s, vort, A.g, hx, hy, h, A.hasRelax, r, src);
     #pragma unroll
     for (int f = 0; f < 3; ++f) A.tend[f * mstride + base] = tend[f] + src[f];
 }
@@ -432,34 +435,59 @@ __global__ __launch_bounds__(256) void k_sw_rk_update(double *__restrict__ q, do
 // written there by the neighbours in the previous stage.  sw_process runs
 // the four phases on the loaded values.
 struct SWPre {
-    double s[3], met[4], invJ, hx, hy, vort, h, gold[3], r[4], fn0[3], oth0[3];
+    double s[3], met[4], invJ, hx, hy, vort, dragH, gold[3], r[4], fn0[3], oth0[3];  // dragH: LinearDrag / depth
     int2 info0;
 };
 
+// (nx, ny, |n|) of side node k of side sd of element e, in the edge's
+// primary orientation: FP64, or (mixed) the side's FP64 reference value plus
+// an FP32 deviation (exact for straight sides, where the deviation is 0).
+template <int N, bool MIXED>
+__device__ __forceinline__ void side_normal(const SWKernelArgs &A, size_t e, int sd, int k, double fn[3])
+{
+    constexpr int n1 = N + 1;
+    const size_t es = e * 4 + sd;
+    if constexpr (MIXED) {
+        #pragma unroll
+        for (int c = 0; c < 3; ++c) fn[c] = A.fnM[es * 3 + c] + (double)A.fnD[(es * 3 + c) * n1 + k];
+    } else {
+        const size_t nb = es * 3 * n1 + k;
+        fn[0] = A.fnrm[nb]; fn[1] = A.fnrm[nb + n1]; fn[2] = A.fnrm[nb + 2 * n1];
+    }
+}
+
+// Loads of one node (element e, node nd), and for nd < 4 (N+1) of that
+// element's side node nd (side nd / (N+1), node nd mod (N+1)).
 template <int N, bool MIXED>
 __device__ __forceinline__ void sw_load(const SWKernelArgs &A, int e, int nd, bool act, SWPre &P)
 {
     constexpr int n1 = N + 1, np = n1 * n1;
     const size_t mstride = (size_t)A.nE * np, base = (size_t)e * np + nd;
-    P.invJ = 0; P.hx = 0; P.hy = 0; P.vort = 0; P.h = 1.0;
+    P.invJ = 0; P.hx = 0; P.hy = 0; P.vort = 0; P.dragH = 0.0;
     #pragma unroll
     for (int k = 0; k < 3; ++k) { P.gold[k] = 0; P.fn0[k] = 0; P.oth0[k] = 0; }
     #pragma unroll
     for (int k = 0; k < 4; ++k) P.r[k] = 0;
     P.info0 = make_int2(0, 0);
     if (!act) return;
-    #pragma unroll
-    for (int f = 0; f < 3; ++f) P.s[f] = A.q[f * mstride + base];
-    #pragma unroll
-    for (int k = 0; k < 4; ++k) P.met[k] = A.metric[k * mstride + base];
-    const double Jv = A.metric[4 * mstride + base];
     if (nd < 4 * n1) {  // this thread's side node (all of them for N >= 3)
         const int sd0 = nd / n1, k0 = nd - sd0 * n1;
         P.info0 = A.side[(size_t)e * 4 + sd0];
-        const size_t nb = ((size_t)e * 4 + sd0) * 3 * n1 + k0;
-        P.fn0[0] = A.fnrm[nb]; P.fn0[1] = A.fnrm[nb + n1]; P.fn0[2] = A.fnrm[nb + 2 * n1];
+        side_normal<N, MIXED>(A, e, sd0, k0, P.fn0);
         const double *tr = A.trIn + (((size_t)e * 4 + sd0) * 3) * n1 + k0;
         P.oth0[0] = tr[0]; P.oth0[1] = tr[n1]; P.oth0[2] = tr[2 * n1];
+    }
+    #pragma unroll
+    for (int f = 0; f < 3; ++f) P.s[f] = A.q[f * mstride + base];
+    double Jv;
+    if constexpr (MIXED) {
+        #pragma unroll
+        for (int k = 0; k < 4; ++k) P.met[k] = A.metM[(size_t)e * 5 + k] + (double)A.metD[k * mstride + base];
+        Jv = A.metM[(size_t)e * 5 + 4] + (double)A.metD[4 * mstride + base];
+    } else {
+        #pragma unroll
+        for (int k = 0; k < 4; ++k) P.met[k] = A.metric[k * mstride + base];
+        Jv = A.metric[4 * mstride + base];
     }
     if constexpr (MIXED) {
         P.hx = A.bathy32[base]; P.hy = A.bathy32[mstride + base]; P.vort = A.plv32[base];
@@ -474,12 +502,36 @@ __device__ __forceinline__ void sw_load(const SWKernelArgs &A, int e, int nd, bo
             for (int f = 0; f < 3; ++f) P.gold[f] = A.G64[f * mstride + base];
         }
     }
-    if (A.drag != 0.0) P.h = A.bathy[base];
+    if (A.drag != 0.0) P.dragH = A.drag / A.bathy[base];
     if (A.hasRelax) {
         #pragma unroll
         for (int k = 0; k < 4; ++k) P.r[k] = A.relax[k * mstride + base];
     }
-    P.invJ = 1.0 / Jv;
+    P.invJ = MIXED ? rcp_refined(Jv) : 1.0 / Jv;
+}
+
+// Rusanov flux of one side node in the edge's primary orientation (so the
+// value is the one EdgeFlux stores on both sides), scaled by |n| and signed
+// for this side: one Riemann evaluation whichever way the side faces.
+template <int N, bool MIXED>
+__device__ __forceinline__ void side_flux(const SWKernelArgs &A, const NodalLDS<N> &L, const double *qsT, int sd,
+                                          int k, int2 info, const double fn[3], double oth[3], double out[3])
+{
+    double own[3], lft[3], rgt[3], fl[3];
+    trace3<N>(L, qsT, sd, k, own);
+    const bool bnd = info.x < 0;
+    if (bnd) sw_external(info.x, fn[0], fn[1], own, oth);
+    const bool prim = bnd || side_primary(info.y);
+    #pragma unroll
+    for (int f = 0; f < 3; ++f) {
+        lft[f] = prim ? own[f] : oth[f];
+        rgt[f] = prim ? oth[f] : own[f];
+    }
+    if constexpr (MIXED) sw_riemann_mixed(lft, rgt, fn[0], fn[1], A.g, fl);
+    else sw_riemann(lft, rgt, fn[0], fn[1], A.g, fl);
+    const double sign = prim ? 1.0 : -1.0;
+    #pragma unroll
+    for (int f = 0; f < 3; ++f) out[f] = sign * (fl[f] * fn[2]);
 }
 
 // Every thread of the block calls this (it synchronizes the block).
@@ -496,7 +548,7 @@ __device__ __forceinline__ void sw_process(const SWKernelArgs &A, const NodalLDS
     if (act) {
         #pragma unroll
         for (int f = 0; f < 3; ++f) qsE[f * np + nd] = P.s[f];
-        contravariant(P.s, P.met, A.g, sCE + nd, pCE + nd, np);
+        contravariant<MIXED>(P.s, P.met, A.g, sCE + nd, pCE + nd, np);
     }
     __syncthreads();
 
@@ -515,31 +567,20 @@ __device__ __forceinline__ void sw_process(const SWKernelArgs &A, const NodalLDS
         for (int w = nd; w < 4 * n1; w += np) {
             const int sd = w / n1, k = w - sd * n1;
             int2 info;
-            double fn[3], oth[3], own[3], fl[3];
+            double fn[3], oth[3], fl[3];
             if (w == nd) {
                 info = P.info0;
                 fn[0] = P.fn0[0]; fn[1] = P.fn0[1]; fn[2] = P.fn0[2];
                 oth[0] = P.oth0[0]; oth[1] = P.oth0[1]; oth[2] = P.oth0[2];
             } else {  // N <= 2: more side nodes than element nodes
                 info = A.side[(size_t)e * 4 + sd];
-                const size_t nb = ((size_t)e * 4 + sd) * 3 * n1 + k;
-                fn[0] = A.fnrm[nb]; fn[1] = A.fnrm[nb + n1]; fn[2] = A.fnrm[nb + 2 * n1];
+                side_normal<N, MIXED>(A, e, sd, k, fn);
                 const double *tr = A.trIn + (((size_t)e * 4 + sd) * 3) * n1 + k;
                 oth[0] = tr[0]; oth[1] = tr[n1]; oth[2] = tr[2 * n1];
             }
-            trace3<N>(L, qsE, sd, k, own);
-            double sign = 1.0;
-            if (info.x < 0) {
-                sw_external(info.x, fn[0], fn[1], own, oth);
-                sw_riemann(own, oth, fn[0], fn[1], A.g, fl);
-            } else if (side_primary(info.y)) {
-                sw_riemann(own, oth, fn[0], fn[1], A.g, fl);
-            } else {
-                sw_riemann(oth, own, fn[0], fn[1], A.g, fl);
-                sign = -1.0;
-            }
+            side_flux<N, MIXED>(A, L, qsE, sd, k, info, fn, oth, fl);
             #pragma unroll
-            for (int f = 0; f < 3; ++f) bfE[(sd * 3 + f) * n1 + k] = sign * (fl[f] * fn[2]);
+            for (int f = 0; f < 3; ++f) bfE[(sd * 3 + f) * n1 + k] = fl[f];
         }
     }
     __syncthreads();
@@ -549,7 +590,7 @@ __device__ __forceinline__ void sw_process(const SWKernelArgs &A, const NodalLDS
         double tend[3], src[3];
         if constexpr (MIXED) dg_divergence_dev32<N>(L, sCE, pCE, sDE, pDE, bfE, i, j, P.invJ, tend);
         else dg_divergence64<N, (N <= 5 || N % 2 == 1)>(L, sCE, pCE, bfE, i, j, P.invJ, tend);
-        sw_source(P.s, P.vort, A.g, P.hx, P.hy, A.drag, P.h, A.hasRelax, P.r, src);
+        sw_source(P.s, P.vort, A.g, P.hx, P.hy, P.dragH, A.hasRelax, P.r, src);
         #pragma unroll
         for (int f = 0; f < 3; ++f) {
             // G = a*G + L; the first stage has a = 0 and ForwardStepRK3 zeroes
@@ -697,6 +738,8 @@ struct self_sw2d_s {
     double *q = nullptr, *G64 = nullptr, *tend = nullptr, *metric = nullptr, *bathy = nullptr;
     float *G32 = nullptr, *bathy32 = nullptr, *plv32 = nullptr, *dS32 = nullptr, *dP32 = nullptr;
     double *plv = nullptr, *relax = nullptr, *nrm = nullptr, *fnrm = nullptr;
+    double *metM = nullptr, *fnM = nullptr;  // mixed: reference values of the geometry
+    float *metD = nullptr, *fnD = nullptr;   // mixed: FP32 deviations from them
     double *bsol = nullptr, *bflux = nullptr, *tr[2] = {nullptr, nullptr};
     double *nodal = nullptr;  // dS, dP, qwS, qwP, lag[4]
     int2 *side = nullptr;
@@ -722,7 +765,7 @@ void free_all(self_sw2d_s *h)
 {
     void *ptrs[] = {h->q, h->G64, h->tend, h->metric, h->bathy, h->G32, h->bathy32, h->plv32,
                     h->dS32, h->dP32, h->plv, h->relax, h->nrm, h->fnrm, h->bsol, h->bflux,
-                    h->tr[0], h->tr[1], h->nodal, h->side, h->edges};
+                    h->tr[0], h->tr[1], h->nodal, h->side, h->edges, h->metM, h->fnM, h->metD, h->fnD};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
 }
@@ -747,6 +790,10 @@ SWKernelArgs make_args(self_sw2d_s *h)
     A.G32 = h->G32;
     A.tend = h->tend;
     A.metric = h->metric;
+    A.metM = h->metM;
+    A.metD = h->metD;
+    A.fnM = h->fnM;
+    A.fnD = h->fnD;
     A.bathy = h->bathy;
     A.bathy32 = h->bathy32;
     A.plv = h->plv;
@@ -947,13 +994,17 @@ try {
     if ((rc = dalloc(h, &h->bflux, nE * 12 * n1))) return fail(rc);
     if ((rc = dalloc(h, &h->tr[0], nE * 12 * n1))) return fail(rc);
     if ((rc = dalloc(h, &h->tr[1], nE * 12 * n1))) return fail(rc);
-    if ((rc = dalloc(h, &h->nodal, 2 * n1 * n1 + 2 * n1 + 4 * n1))) return fail(rc);
+    if ((rc = dalloc(h, &h->nodal, nodal_buffer_size(n1)))) return fail(rc);
     if (desc->precision == SELF_MIXED) {
         if ((rc = dalloc(h, &h->G32, 3 * nfield))) return fail(rc);
         if ((rc = dalloc(h, &h->bathy32, 2 * nfield))) return fail(rc);
         if ((rc = dalloc(h, &h->plv32, nfield))) return fail(rc);
         if ((rc = dalloc(h, &h->dS32, n1 * n1))) return fail(rc);
         if ((rc = dalloc(h, &h->dP32, n1 * n1))) return fail(rc);
+        if ((rc = dalloc(h, &h->metM, nE * 5))) return fail(rc);
+        if ((rc = dalloc(h, &h->metD, 5 * nfield))) return fail(rc);
+        if ((rc = dalloc(h, &h->fnM, nE * 12))) return fail(rc);
+        if ((rc = dalloc(h, &h->fnD, nE * 12 * n1))) return fail(rc);
     }
     e = hipMemsetAsync(h->q, 0, 3 * nfield * sizeof(double), h->stream);
     if (e == hipSuccess) e = hipMemsetAsync(h->bathy, 0, 3 * nfield * sizeof(double), h->stream);
@@ -994,7 +1045,7 @@ try {
                  "self_sw2d_set_nodal: null argument");
     SELF_HIP(hipSetDevice(h->device));
     const int n1 = h->n1;
-    std::vector<double> buf(2 * n1 * n1 + 6 * n1);
+    std::vector<double> buf(nodal_buffer_size(n1));
     std::memcpy(buf.data(), dMatS, sizeof(double) * n1 * n1);
     std::memcpy(buf.data() + n1 * n1, dMatP, sizeof(double) * n1 * n1);
     double *p = buf.data() + 2 * n1 * n1;
@@ -1004,6 +1055,10 @@ try {
     std::memcpy(p + 3 * n1, lE, sizeof(double) * n1);
     std::memcpy(p + 4 * n1, lS, sizeof(double) * n1);
     std::memcpy(p + 5 * n1, lN, sizeof(double) * n1);
+    for (int k = 0; k < n1; ++k) {
+        p[6 * n1 + k] = 1.0 / qwS[k];
+        p[7 * n1 + k] = 1.0 / qwP[k];
+    }
     SELF_HIP(hipMemcpy(h->nodal, buf.data(), sizeof(double) * buf.size(), hipMemcpyHostToDevice));
     if (h->dS32) {
         std::vector<float> f(2 * n1 * n1);
@@ -1080,6 +1135,28 @@ try {
     SELF_HIP(hipMemcpy(h->metric, met.data(), sizeof(double) * met.size(), hipMemcpyHostToDevice));
     SELF_HIP(hipMemcpy(h->nrm, nrm.data(), sizeof(double) * nrm.size(), hipMemcpyHostToDevice));
     SELF_HIP(hipMemcpy(h->fnrm, fnrm.data(), sizeof(double) * fnrm.size(), hipMemcpyHostToDevice));
+    if (h->metD) {
+        // mixed form: FP64 reference value per element (metrics, node 0) and
+        // per side (normals, node 0) plus FP32 deviations; an affine element
+        // or a straight side has zero deviations and is reproduced exactly
+        std::vector<double> mM(nE * 5), fM(nE * 12);
+        std::vector<float> mD(5 * nE * np), fD(nE * 12 * n1);
+        for (int k = 0; k < 5; ++k)
+            for (size_t e = 0; e < nE; ++e) {
+                const double *m = met.data() + k * nE * np + e * np;
+                mM[e * 5 + k] = m[0];
+                for (size_t nd = 0; nd < np; ++nd) mD[k * nE * np + e * np + nd] = (float)(m[nd] - m[0]);
+            }
+        for (size_t ec = 0; ec < nE * 12; ++ec) {
+            const double *f = fnrm.data() + ec * n1;
+            fM[ec] = f[0];
+            for (int k = 0; k < n1; ++k) fD[ec * n1 + k] = (float)(f[k] - f[0]);
+        }
+        SELF_HIP(hipMemcpy(h->metM, mM.data(), sizeof(double) * mM.size(), hipMemcpyHostToDevice));
+        SELF_HIP(hipMemcpy(h->metD, mD.data(), sizeof(float) * mD.size(), hipMemcpyHostToDevice));
+        SELF_HIP(hipMemcpy(h->fnM, fM.data(), sizeof(double) * fM.size(), hipMemcpyHostToDevice));
+        SELF_HIP(hipMemcpy(h->fnD, fD.data(), sizeof(float) * fD.size(), hipMemcpyHostToDevice));
+    }
     SELF_HIP(hipMemcpy(h->side, side.data(), sizeof(int2) * side.size(), hipMemcpyHostToDevice));
     SELF_HIP(hipMemcpy(h->edges, edges, sizeof(int) * 6 * nEd, hipMemcpyHostToDevice));
     h->geomSet = true;
