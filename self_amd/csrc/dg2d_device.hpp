@@ -151,22 +151,38 @@ struct NodalLDS {
 // then 1/qwS, 1/qwP (computed on the host: the same IEEE quotients)
 constexpr int nodal_buffer_size(int n1) { return 2 * n1 * n1 + 8 * n1; }
 
+// lane / lanes: the threads that stage this copy (the whole workgroup by
+// default, or one wavefront staging its own copy).
 template <int N, bool F32>
 __device__ __forceinline__ void stage_nodal(NodalLDS<N> &L, const double *__restrict__ g,
-                                            const float *__restrict__ dS32, const float *__restrict__ dP32)
+                                            const float *__restrict__ dS32, const float *__restrict__ dP32,
+                                            int lane = -1, int lanes = 0)
 {
     constexpr int n1 = N + 1, nd = 2 * n1 * n1 + 6 * n1;
+    if (lane < 0) { lane = threadIdx.x; lanes = blockDim.x; }
     double *dst = L.dS;  // members up to lN are contiguous doubles: dS, dP, iwS, iwP, lags
-    for (int k = threadIdx.x; k < nd; k += blockDim.x) {
+    for (int k = lane; k < nd; k += lanes) {
         const bool w = (k >= 2 * n1 * n1) && (k < 2 * n1 * n1 + 2 * n1);
         dst[k] = g[w ? k + 6 * n1 : k];
     }
     if constexpr (F32) {
-        for (int k = threadIdx.x; k < n1 * n1; k += blockDim.x) {
+        for (int k = lane; k < n1 * n1; k += lanes) {
             L.dS32[k] = dS32[k];
             L.dP32[k] = dP32[k];
         }
     }
+}
+
+// Orders a wavefront's LDS writes before its following LDS reads of other
+// lanes' data: the compiler keeps the order (fences, wave barrier) and the
+// wavefront waits for its outstanding LDS operations (s_waitcnt lgkmcnt(0),
+// gfx9 encoding with vmcnt and expcnt left at their maxima).
+__device__ __forceinline__ void wave_sync()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_s_waitcnt(0xc07f);
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
 }  // namespace self

@@ -434,9 +434,16 @@ __global__ __launch_bounds__(256) void k_sw_rk_update(double *__restrict__ q, do
 // load depends on another: neighbour traces sit in the element's own slots,
 // written there by the neighbours in the previous stage.  sw_process runs
 // the four phases on the loaded values.
+// Per-thread values of one node (and of its side node): sw_load only issues
+// the loads (raw values, no arithmetic, so no wait sits between them and
+// every load of the phase is in flight at once); sw_finish derives the
+// working values once the loads have all been issued.
 struct SWPre {
-    double s[3], met[4], invJ, hx, hy, vort, dragH, gold[3], r[4], fn0[3], oth0[3];  // dragH: LinearDrag / depth
+    double s[3], met[4], invJ, hx, hy, vort, dragH, gold[3], r[4], fn0[3], oth0[3];
     int2 info0;
+    // raw loads
+    double Jv, hRaw, metM[5], fnM[3];
+    float metD[5], fnD[3], hx32, hy32, vort32, g32[3];
 };
 
 // (nx, ny, |n|) of side node k of side sd of element e, in the edge's
@@ -463,51 +470,76 @@ __device__ __forceinline__ void sw_load(const SWKernelArgs &A, int e, int nd, bo
 {
     constexpr int n1 = N + 1, np = n1 * n1;
     const size_t mstride = (size_t)A.nE * np, base = (size_t)e * np + nd;
-    P.invJ = 0; P.hx = 0; P.hy = 0; P.vort = 0; P.dragH = 0.0;
+    P.Jv = 1.0; P.hRaw = 1.0;
     #pragma unroll
-    for (int k = 0; k < 3; ++k) { P.gold[k] = 0; P.fn0[k] = 0; P.oth0[k] = 0; }
+    for (int k = 0; k < 3; ++k) {
+        P.s[k] = 0; P.gold[k] = 0; P.fn0[k] = 0; P.oth0[k] = 0; P.g32[k] = 0; P.fnD[k] = 0; P.fnM[k] = 0;
+    }
     #pragma unroll
-    for (int k = 0; k < 4; ++k) P.r[k] = 0;
+    for (int k = 0; k < 4; ++k) { P.r[k] = 0; P.met[k] = 0; }
+    #pragma unroll
+    for (int k = 0; k < 5; ++k) { P.metD[k] = 0; P.metM[k] = 0; }
+    P.hx = 0; P.hy = 0; P.vort = 0; P.hx32 = 0; P.hy32 = 0; P.vort32 = 0;
     P.info0 = make_int2(0, 0);
     if (!act) return;
     if (nd < 4 * n1) {  // this thread's side node (all of them for N >= 3)
         const int sd0 = nd / n1, k0 = nd - sd0 * n1;
-        P.info0 = A.side[(size_t)e * 4 + sd0];
-        side_normal<N, MIXED>(A, e, sd0, k0, P.fn0);
-        const double *tr = A.trIn + (((size_t)e * 4 + sd0) * 3) * n1 + k0;
+        const size_t es = (size_t)e * 4 + sd0;
+        P.info0 = A.side[es];
+        if constexpr (MIXED) {
+            #pragma unroll
+            for (int c = 0; c < 3; ++c) { P.fnM[c] = A.fnM[es * 3 + c]; P.fnD[c] = A.fnD[(es * 3 + c) * n1 + k0]; }
+        } else {
+            const size_t nb = es * 3 * n1 + k0;
+            P.fn0[0] = A.fnrm[nb]; P.fn0[1] = A.fnrm[nb + n1]; P.fn0[2] = A.fnrm[nb + 2 * n1];
+        }
+        const double *tr = A.trIn + es * 3 * n1 + k0;
         P.oth0[0] = tr[0]; P.oth0[1] = tr[n1]; P.oth0[2] = tr[2 * n1];
     }
     #pragma unroll
     for (int f = 0; f < 3; ++f) P.s[f] = A.q[f * mstride + base];
-    double Jv;
     if constexpr (MIXED) {
         #pragma unroll
-        for (int k = 0; k < 4; ++k) P.met[k] = A.metM[(size_t)e * 5 + k] + (double)A.metD[k * mstride + base];
-        Jv = A.metM[(size_t)e * 5 + 4] + (double)A.metD[4 * mstride + base];
+        for (int k = 0; k < 5; ++k) { P.metM[k] = A.metM[(size_t)e * 5 + k]; P.metD[k] = A.metD[k * mstride + base]; }
+        P.hx32 = A.bathy32[base]; P.hy32 = A.bathy32[mstride + base]; P.vort32 = A.plv32[base];
+        if (A.a != 0.0) {
+            #pragma unroll
+            for (int f = 0; f < 3; ++f) P.g32[f] = A.G32[f * mstride + base];
+        }
     } else {
         #pragma unroll
         for (int k = 0; k < 4; ++k) P.met[k] = A.metric[k * mstride + base];
-        Jv = A.metric[4 * mstride + base];
-    }
-    if constexpr (MIXED) {
-        P.hx = A.bathy32[base]; P.hy = A.bathy32[mstride + base]; P.vort = A.plv32[base];
-        if (A.a != 0.0) {
-            #pragma unroll
-            for (int f = 0; f < 3; ++f) P.gold[f] = A.G32[f * mstride + base];
-        }
-    } else {
+        P.Jv = A.metric[4 * mstride + base];
         P.hx = A.bathy[mstride + base]; P.hy = A.bathy[2 * mstride + base]; P.vort = A.plv[base];
         if (A.a != 0.0) {
             #pragma unroll
             for (int f = 0; f < 3; ++f) P.gold[f] = A.G64[f * mstride + base];
         }
     }
-    if (A.drag != 0.0) P.dragH = A.drag / A.bathy[base];
+    if (A.drag != 0.0) P.hRaw = A.bathy[base];
     if (A.hasRelax) {
         #pragma unroll
         for (int k = 0; k < 4; ++k) P.r[k] = A.relax[k * mstride + base];
     }
-    P.invJ = MIXED ? rcp_refined(Jv) : 1.0 / Jv;
+}
+
+// Working values from the raw loads (after every load has been issued).
+template <bool MIXED>
+__device__ __forceinline__ void sw_finish(const SWKernelArgs &A, bool act, SWPre &P)
+{
+    if constexpr (MIXED) {
+        #pragma unroll
+        for (int k = 0; k < 4; ++k) P.met[k] = P.metM[k] + (double)P.metD[k];
+        P.Jv = P.metM[4] + (double)P.metD[4];
+        #pragma unroll
+        for (int c = 0; c < 3; ++c) P.fn0[c] = P.fnM[c] + (double)P.fnD[c];
+        P.hx = P.hx32; P.hy = P.hy32; P.vort = P.vort32;
+        #pragma unroll
+        for (int f = 0; f < 3; ++f) P.gold[f] = P.g32[f];
+    }
+    P.invJ = act ? (MIXED ? rcp_refined(P.Jv) : 1.0 / P.Jv) : 0.0;
+    P.dragH = 0.0;
+    if (A.drag != 0.0) P.dragH = A.drag / P.hRaw;  // LinearDrag / depth
 }
 
 // Rusanov flux of one side node in the edge's primary orientation (so the
@@ -534,12 +566,23 @@ __device__ __forceinline__ void side_flux(const SWKernelArgs &A, const NodalLDS<
     for (int f = 0; f < 3; ++f) out[f] = sign * (fl[f] * fn[2]);
 }
 
-// Every thread of the block calls this (it synchronizes the block).
-template <int N, bool MIXED>
+// Synchronization between the phases of one element: the whole workgroup,
+// or (WL) only the element's own wavefront, when every element lies inside
+// one wavefront (64 % (N+1)^2 == 0) and the nodal constants are per wave.
+template <bool WL>
+__device__ __forceinline__ void phase_sync()
+{
+    if constexpr (WL) wave_sync();
+    else __syncthreads();
+}
+
+// Every thread of the block (WL: of the wavefront) calls this.
+template <int N, bool MIXED, bool WL>
 __device__ __forceinline__ void sw_process(const SWKernelArgs &A, const NodalLDS<N> &L, double *qsE, double *sCE,
                                            double *pCE, float *sDE, float *pDE, double *bfE, int e, int nd, bool act,
-                                           const SWPre &P)
+                                           SWPre &P)
 {
+    sw_finish<MIXED>(A, act, P);
     constexpr int n1 = N + 1, np = n1 * n1;
     const int i = nd % n1, j = nd / n1;
     const size_t mstride = (size_t)A.nE * np, base = (size_t)e * np + nd;
@@ -550,7 +593,7 @@ __device__ __forceinline__ void sw_process(const SWKernelArgs &A, const NodalLDS
         for (int f = 0; f < 3; ++f) qsE[f * np + nd] = P.s[f];
         contravariant<MIXED>(P.s, P.met, A.g, sCE + nd, pCE + nd, np);
     }
-    __syncthreads();
+    phase_sync<WL>();
 
     // ---- phase B: FP32 deviations of the contravariant fluxes (mixed), and
     // the Rusanov flux of each side node in the edge's primary orientation
@@ -583,7 +626,7 @@ __device__ __forceinline__ void sw_process(const SWKernelArgs &A, const NodalLDS
             for (int f = 0; f < 3; ++f) bfE[(sd * 3 + f) * n1 + k] = fl[f];
         }
     }
-    __syncthreads();
+    phase_sync<WL>();
 
     // ---- phase C: weak divergence + lift + source + RK3 update
     if (act) {
@@ -607,7 +650,7 @@ __device__ __forceinline__ void sw_process(const SWKernelArgs &A, const NodalLDS
             qsE[f * np + nd] = qn;  // phase B finished reading qs before the barrier
         }
     }
-    __syncthreads();
+    phase_sync<WL>();
 
     // ---- phase D: traces of the updated state, pushed into the
     // neighbours' slots for the next stage
@@ -634,7 +677,14 @@ __global__ __launch_bounds__(256, 4 / G) void k_sw_stage_fused(SWKernelArgs A)
 {
     using S = Shape<N>;
     constexpr int np = S::np, epb = S::epb, n1 = S::n1;
-    __shared__ NodalLDS<N> L;
+    // WL: each element lies inside one wavefront, so the phases synchronize
+    // per wavefront and the wavefronts of a workgroup never wait for each
+    // other (one that has its data computes while the others still load).
+    // Measured at N = 7: mixed 10.45 -> 10.32 us per stage, FP64 11.29 ->
+    // 11.63 us, so the FP64 form keeps workgroup barriers.
+    constexpr bool WL = MIXED && (G == 1) && (64 % np == 0);
+    __shared__ NodalLDS<N> Lw[WL ? 4 : 1];
+    NodalLDS<N> &L = Lw[WL ? threadIdx.x / 64 : 0];
     __shared__ double qs[epb * 3 * np], sC[epb * 3 * np], pC[epb * 3 * np], bfs[epb * 12 * n1];
     __shared__ float sD[MIXED ? epb * 3 * np : 1], pD[MIXED ? epb * 3 * np : 1];
     const int t = threadIdx.x, le = t / np, nd = t - le * np;
@@ -648,14 +698,19 @@ __global__ __launch_bounds__(256, 4 / G) void k_sw_stage_fused(SWKernelArgs A)
         act[g] = le < epb && e[g] < A.nE;
         sw_load<N, MIXED>(A, e[g], nd, act[g], P[g]);
     }
-    stage_nodal<N, MIXED>(L, A.dS, A.dS32, A.dP32);
+    if constexpr (WL) {
+        stage_nodal<N, MIXED>(L, A.dS, A.dS32, A.dP32, threadIdx.x % 64, 64);
+        wave_sync();
+    } else {
+        stage_nodal<N, MIXED>(L, A.dS, A.dS32, A.dP32);
+    }
     const int ls = le < epb ? le : 0;
     double *qsE = qs + ls * 3 * np, *sCE = sC + ls * 3 * np, *pCE = pC + ls * 3 * np, *bfE = bfs + ls * 12 * n1;
     float *sDE = MIXED ? sD + ls * 3 * np : sD, *pDE = MIXED ? pD + ls * 3 * np : pD;
     #pragma unroll
     for (int g = 0; g < G; ++g) {
         if (g > 0) __syncthreads();  // the previous group's phase D read qs
-        sw_process<N, MIXED>(A, L, qsE, sCE, pCE, sDE, pDE, bfE, e[g], nd, act[g], P[g]);
+        sw_process<N, MIXED, WL>(A, L, qsE, sCE, pCE, sDE, pDE, bfE, e[g], nd, act[g], P[g]);
     }
 }
 
