@@ -45,11 +45,15 @@ struct CgLDS {
 // Element operator on gathered values -> Lbuf (local copies).
 // RESID: Dirichlet nodes take the prescribed values uD (Residual of the
 // solution); otherwise they are zero (search directions are masked there).
-template <int N, typename T, bool RESID>
+// DIR: the gathered value is the CG search direction formed on the fly,
+// v = x (iteration 1) or x + beta vOld, and every copy also stores it to
+// vNew[g] (the copies of a DOF store the same bits).
+template <int N, typename T, bool RESID, bool DIR = false>
 __global__ __launch_bounds__(256) void k_cg_local(const T *__restrict__ x, const int *__restrict__ gid,
                                                   const double *__restrict__ uD, const T *__restrict__ G3,
                                                   const T *__restrict__ Dg, T *__restrict__ Lbuf, int nE,
-                                                  const SolverState *st)
+                                                  const SolverState *st, const T *__restrict__ vOld = nullptr,
+                                                  T *__restrict__ vNew = nullptr)
 {
     using S = Shape<N>;
     constexpr int n1 = S::n1, np = S::np, epb = S::epb;
@@ -66,8 +70,17 @@ __global__ __launch_bounds__(256) void k_cg_local(const T *__restrict__ x, const
     if (act) {
         const int g = gid[lid];
         T u;
-        if (g >= 0) u = x[g];
-        else u = RESID ? (T)uD[lid] : (T)0;
+        if (g >= 0) {
+            if constexpr (DIR) {
+                const T wv = x[g];
+                u = (st->iter == 1) ? wv : wv + (T)st->beta * vOld[g];
+                vNew[g] = u;
+            } else {
+                u = x[g];
+            }
+        } else {
+            u = RESID ? (T)uD[lid] : (T)0;
+        }
         us[le * np + nd] = u;
         g11 = G3[lid];
         g12 = G3[stride + lid];
@@ -221,6 +234,36 @@ struct CgsemOp final : DevOp {
         if (fin != kFinNone && st)
             hipLaunchKernelGGL((k_cg_assemble<float, 0, true>), dim3(1), dim3(kRedThreads), 0, s, Lbuf32, cs, ci,
                                (const double *)nullptr, nG, y, dotWith, st, fin, slot);
+        SELF_HIP(hipGetLastError());
+        return SELF_OK;
+    }
+
+    bool fuses_direction() const override { return true; }
+
+    int matvec_dir(const double *w, const double *vOld, double *vNew, double *y, SolverState *st, int fin,
+                   const RedSlot &slot, hipStream_t s) override
+    {
+        CG_DISPATCH_N(N, hipLaunchKernelGGL((k_cg_local<NN, double, false, true>), dim3(cg_grid<NN>(nE)), dim3(256),
+                                            0, s, w, gid, uD, G3, D, Lbuf, (int)nE, (const SolverState *)st, vOld,
+                                            vNew));
+        hipLaunchKernelGGL((k_cg_assemble<double, 0, false>), dim3(kRedBlocks), dim3(kRedThreads), 0, s, Lbuf, cs,
+                           ci, (const double *)nullptr, nG, y, (const double *)vNew, st, fin, slot);
+        hipLaunchKernelGGL((k_cg_assemble<double, 0, true>), dim3(1), dim3(kRedThreads), 0, s, Lbuf, cs, ci,
+                           (const double *)nullptr, nG, y, (const double *)vNew, st, fin, slot);
+        SELF_HIP(hipGetLastError());
+        return SELF_OK;
+    }
+
+    int matvec_dir32(const float *w, const float *vOld, float *vNew, float *y, SolverState *st, int fin,
+                     const RedSlot &slot, hipStream_t s) override
+    {
+        CG_DISPATCH_N(N, hipLaunchKernelGGL((k_cg_local<NN, float, false, true>), dim3(cg_grid<NN>(nE)), dim3(256),
+                                            0, s, w, gid, uD, G3f, Df, Lbuf32, (int)nE, (const SolverState *)st,
+                                            vOld, vNew));
+        hipLaunchKernelGGL((k_cg_assemble<float, 0, false>), dim3(kRedBlocks), dim3(kRedThreads), 0, s, Lbuf32, cs,
+                           ci, (const double *)nullptr, nG, y, (const float *)vNew, st, fin, slot);
+        hipLaunchKernelGGL((k_cg_assemble<float, 0, true>), dim3(1), dim3(kRedThreads), 0, s, Lbuf32, cs, ci,
+                           (const double *)nullptr, nG, y, (const float *)vNew, st, fin, slot);
         SELF_HIP(hipGetLastError());
         return SELF_OK;
     }

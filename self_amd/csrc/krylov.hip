@@ -8,8 +8,11 @@
 // Every vector and scalar lives on the device.  Dot products are fused into
 // the vector kernels that produce their operands and reduced
 // deterministically (reduce.hpp); the scalar recurrences (alpha, beta,
-// Givens rotations, the stop tests) run in the last workgroup of the
-// reduction or in a one-thread kernel.  A `done` flag in the device state
+// Givens rotations, the stop tests) run in the one-workgroup final launch of
+// a reduction or in a one-thread kernel.  A CG iteration ends with one sweep
+// that also starts the next (preconditioner and numerator), and operators
+// that gather their input fold the direction update into the gather, so an
+// iteration is five launches.  A `done` flag in the device state
 // turns the kernels of iterations after convergence into no-ops, so the
 // host launches iterations in chunks and reads the flag between chunks.
 
@@ -88,14 +91,20 @@ __global__ __launch_bounds__(256) void k_update_v(const T *__restrict__ w, T *__
         v[g] = first ? w[g] : w[g] + beta * v[g];
 }
 
-// u += alpha v; r -= alpha z; |r|^2 -> rNorm, stop test
-template <typename T, bool FIN>
-__global__ __launch_bounds__(kRedThreads) void k_update_ur(T *__restrict__ u, const T *__restrict__ v,
-                                                           T *__restrict__ r, const T *__restrict__ z, size_t n,
-                                                           SolverState *st, RedSlot slot)
+// The end of one CG iteration and the start of the next in one sweep:
+// u += alpha v; r -= alpha z; |r|^2 -> rNorm and the stop test (row 0);
+// w = P r and num = r.w, beta = num/den for the next iteration (row 1).
+// Without a preconditioner w is r itself and row 1 repeats row 0.  Each
+// thread visits the same indices in the same order as the separate
+// precondition/dot and update kernels did, so the sums are the same.
+template <typename T, bool FIN, bool PRE>
+__global__ __launch_bounds__(kRedThreads) void k_update_urw(T *__restrict__ u, const T *__restrict__ v,
+                                                            T *__restrict__ r, const T *__restrict__ z,
+                                                            const T *__restrict__ dinv, T *__restrict__ w, size_t n,
+                                                            SolverState *st, RedSlot slot)
 {
     if (st_done(st)) return;
-    double acc[1] = {0.0};
+    double acc[2] = {0.0, 0.0};
     if constexpr (!FIN) {
         const T alpha = (T)st->alpha;
         for (size_t g = (size_t)blockIdx.x * blockDim.x + threadIdx.x; g < n; g += (size_t)gridDim.x * blockDim.x) {
@@ -103,10 +112,16 @@ __global__ __launch_bounds__(kRedThreads) void k_update_ur(T *__restrict__ u, co
             const T rv = r[g] - alpha * z[g];
             r[g] = rv;
             acc[0] += (double)rv * (double)rv;
+            if constexpr (PRE) {
+                const T wv = dinv[g] * rv;
+                w[g] = wv;
+                acc[1] += (double)rv * (double)wv;
+            }
         }
+        if constexpr (!PRE) acc[1] = acc[0];
     }
-    const int f[1] = {kFinRnorm};
-    reduce_rows<1, FIN>(slot, acc, st, f);
+    const int f[2] = {kFinRnorm, kFinNumNext};
+    reduce_rows<2, FIN>(slot, acc, st, f);
 }
 
 // ---------------------------------------------------------------- GMRES
@@ -341,10 +356,11 @@ int KrylovWork::ensure(size_t nn, int mm, int rl, bool f32)
     if ((rc = A((void **)&r, vb)) || (rc = A((void **)&v, vb)) || (rc = A((void **)&z, vb)) ||
         (rc = A((void **)&w, vb)) || (rc = A((void **)&u, vb)))
         return rc;
+    if ((rc = A((void **)&vB, vb))) return rc;
     if (f32) {
         const size_t fb = sizeof(float) * n;
         if ((rc = A((void **)&r32, fb)) || (rc = A((void **)&v32, fb)) || (rc = A((void **)&z32, fb)) ||
-            (rc = A((void **)&w32, fb)) || (rc = A((void **)&u32, fb)))
+            (rc = A((void **)&w32, fb)) || (rc = A((void **)&u32, fb)) || (rc = A((void **)&vB32, fb)))
             return rc;
     }
     if (m > 0) {
@@ -363,11 +379,11 @@ int KrylovWork::ensure(size_t nn, int mm, int rl, bool f32)
 
 void KrylovWork::release()
 {
-    void *p[] = {r, v, z, w, u, r32, v32, z32, w32, u32, V, Z, small, st, resi, slot.partials};
+    void *p[] = {r, v, z, w, u, vB, r32, v32, z32, w32, u32, vB32, V, Z, small, st, resi, slot.partials};
     for (void *q : p)
         if (q) (void)hipFree(q);
-    r = v = z = w = u = nullptr;
-    r32 = v32 = z32 = w32 = u32 = nullptr;
+    r = v = z = w = u = vB = nullptr;
+    r32 = v32 = z32 = w32 = u32 = vB32 = nullptr;
     V = Z = small = nullptr;
     st = nullptr;
     resi = nullptr;
@@ -413,6 +429,62 @@ int read_state(KrylovWork &wk, SolverState &h, hipStream_t s)
 
 constexpr int kChunk = 32;  // iterations launched between flag reads
 
+inline int op_matvec(DevOp &op, const double *x, double *y, SolverState *st, const RedSlot &slot, hipStream_t s)
+{
+    return op.matvec(x, y, x, st, kFinAlpha, slot, s);
+}
+inline int op_matvec(DevOp &op, const float *x, float *y, SolverState *st, const RedSlot &slot, hipStream_t s)
+{
+    return op.matvec32(x, y, x, st, kFinAlpha, slot, s);
+}
+inline int op_matvec_dir(DevOp &op, const double *w, const double *vOld, double *vNew, double *y, SolverState *st,
+                         const RedSlot &slot, hipStream_t s)
+{
+    return op.matvec_dir(w, vOld, vNew, y, st, kFinAlpha, slot, s);
+}
+inline int op_matvec_dir(DevOp &op, const float *w, const float *vOld, float *vNew, float *y, SolverState *st,
+                         const RedSlot &slot, hipStream_t s)
+{
+    return op.matvec_dir32(w, vOld, vNew, y, st, kFinAlpha, slot, s);
+}
+
+// CG iterations it0..it1 (the loop body of the reference's CG/PCG), three
+// reductions' worth of launches each:
+//   v = w + beta v, z = A v, alpha = num / v.z   (direction update folded
+//                                                 into the operator's gather
+//                                                 when it offers that)
+//   u += alpha v, r -= alpha z, rNorm, stop test, w = P r, num, beta
+// With the folded update v alternates between vA (odd iterations) and vB.
+template <typename T>
+int cg_chunk(DevOp &op, T *u, T *r, T *w, T *vA, T *vB, T *z, const T *dinv, size_t n, SolverState *st,
+             const RedSlot &slot, int it0, int it1, hipStream_t s)
+{
+    const bool fused = op.fuses_direction();
+    const T *wsrc = dinv ? w : r;
+    const int gv = vgrid(n);
+    for (int it = it0; it <= it1; ++it) {
+        T *v = vA;
+        int rc;
+        if (fused) {
+            const T *vOld = (it & 1) ? vB : vA;
+            v = (it & 1) ? vA : vB;
+            rc = op_matvec_dir(op, wsrc, vOld, v, z, st, slot, s);
+        } else {
+            hipLaunchKernelGGL(k_update_v<T>, dim3(gv), dim3(256), 0, s, wsrc, vA, n, st);
+            rc = op_matvec(op, vA, z, st, slot, s);
+        }
+        if (rc) return rc;
+        if (dinv)
+            SELF_RED_LAUNCH((k_update_urw<T, false, true>), (k_update_urw<T, true, true>), s, u, (const T *)v, r,
+                               (const T *)z, dinv, w, n, st, slot);
+        else
+            SELF_RED_LAUNCH((k_update_urw<T, false, false>), (k_update_urw<T, true, false>), s, u, (const T *)v, r,
+                               (const T *)z, dinv, w, n, st, slot);
+    }
+    SELF_HIP(hipGetLastError());
+    return SELF_OK;
+}
+
 // CG / PCG of IterativeSolvers (stopRule sqrt-norm) and the CGSEM PCG (norm rule)
 int run_cg(DevOp &op, KrylovWork &wk, bool precond, int stopRule, double tol, int maxIters, double *sol,
            KrylovResult &res, hipStream_t s)
@@ -424,17 +496,14 @@ int run_cg(DevOp &op, KrylovWork &wk, bool precond, int stopRule, double tol, in
     if ((rc = op.residual(sol, wk.r, wk.st, fin0, wk.slot, s))) return rc;
     SELF_HIP(hipMemcpyAsync(wk.u, sol, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
     const double *dinv = precond ? op.inv_diag() : nullptr;
-    const int gv = vgrid(n);
+    // iteration 1's w = P r and num = r.w; later iterations get theirs from
+    // the update sweep of the iteration before
+    SELF_RED_LAUNCH((k_precond_dot<double, false>), (k_precond_dot<double, true>), s, wk.r, dinv, wk.w, n, wk.st,
+                       (int)kFinNum, wk.slot);
     for (int it0 = 1; it0 <= maxIters; it0 += kChunk) {
-        for (int it = it0; it < it0 + kChunk && it <= maxIters; ++it) {
-            SELF_RED_LAUNCH((k_precond_dot<double, false>), (k_precond_dot<double, true>), s, wk.r, dinv, wk.w, n,
-                               wk.st, (int)kFinNum, wk.slot);
-            hipLaunchKernelGGL(k_update_v<double>, dim3(gv), dim3(256), 0, s, wk.w, wk.v, n, wk.st);
-            if ((rc = op.matvec(wk.v, wk.z, wk.v, wk.st, kFinAlpha, wk.slot, s))) return rc;
-            SELF_RED_LAUNCH((k_update_ur<double, false>), (k_update_ur<double, true>), s, wk.u, wk.v, wk.r,
-                               wk.z, n, wk.st, wk.slot);
-        }
-        SELF_HIP(hipGetLastError());
+        if ((rc = cg_chunk<double>(op, wk.u, wk.r, wk.w, wk.v, wk.vB, wk.z, dinv, n, wk.st, wk.slot, it0,
+                                   std::min(it0 + kChunk - 1, maxIters), s)))
+            return rc;
         SolverState h{};
         if ((rc = read_state(wk, h, s))) return rc;
         if (h.done) break;
@@ -582,16 +651,12 @@ int krylov_solve_ir(DevOp &op, KrylovWork &wk, int precond, double tol, double i
         // r0 of the inner problem is |r/|r|| = 1
         SELF_RED_LAUNCH((k_dot<float, false>), (k_dot<float, true>), s, wk.r32, wk.r32, n, wk.st,
                            (int)kFinR0Plain, wk.slot);
+        SELF_RED_LAUNCH((k_precond_dot<float, false>), (k_precond_dot<float, true>), s, wk.r32, dinv, wk.w32, n,
+                           wk.st, (int)kFinNum, wk.slot);
         for (int it0 = 1; it0 <= maxInner; it0 += kChunk) {
-            for (int it = it0; it < it0 + kChunk && it <= maxInner; ++it) {
-                SELF_RED_LAUNCH((k_precond_dot<float, false>), (k_precond_dot<float, true>), s, wk.r32, dinv,
-                                   wk.w32, n, wk.st, (int)kFinNum, wk.slot);
-                hipLaunchKernelGGL(k_update_v<float>, dim3(gv), dim3(256), 0, s, wk.w32, wk.v32, n, wk.st);
-                if ((rc = op.matvec32(wk.v32, wk.z32, wk.v32, wk.st, kFinAlpha, wk.slot, s))) return rc;
-                SELF_RED_LAUNCH((k_update_ur<float, false>), (k_update_ur<float, true>), s, wk.u32, wk.v32,
-                                   wk.r32, wk.z32, n, wk.st, wk.slot);
-            }
-            SELF_HIP(hipGetLastError());
+            if ((rc = cg_chunk<float>(op, wk.u32, wk.r32, wk.w32, wk.v32, wk.vB32, wk.z32, dinv, n, wk.st, wk.slot,
+                                      it0, std::min(it0 + kChunk - 1, maxInner), s)))
+                return rc;
             if ((rc = read_state(wk, h, s))) return rc;
             if (h.done) break;
         }

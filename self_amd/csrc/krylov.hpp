@@ -46,6 +46,7 @@ enum Finalize {
     kFinR0Plain = 5, // r0 = sqrt(S); resi[0] = r0 (IterativeSolvers form: no early return)
     kFinNumCG = 6,   // num = S (r.r); beta = num/den unless iter == 1
     kFinBnorm = 7,   // bnorm = sqrt(S)
+    kFinNumNext = 8, // after kFinRnorm in the same epilogue: num = S, beta = num/den unless done
 };
 
 // Scalar recurrences run by the last workgroup of a fused reduction.
@@ -87,6 +88,12 @@ __device__ inline void finalize_scalar(SolverState *st, int kind, double S)
     case kFinBnorm:
         st->bnorm = sqrt(S);
         break;
+    case kFinNumNext:
+        if (!st->done) {
+            st->num = S;
+            st->beta = S / st->den;
+        }
+        break;
     default:
         break;
     }
@@ -105,6 +112,16 @@ struct DevOp {
     virtual bool has_f32() const { return false; }
     virtual int matvec32(const float *x, float *y, const float *dotWith, SolverState *st, int fin,
                          const RedSlot &slot, hipStream_t s) { return SELF_ERR_STATE; }
+    // Search-direction update folded into the operator's input gather:
+    // vNew = w (iteration 1) or w + beta vOld, y = A vNew, dot(vNew, y)
+    // finalized with fin.  vNew and vOld are distinct buffers (other
+    // elements still gather vOld while vNew is written).  Operators without
+    // it leave fuses_direction() false and the solver updates v in place.
+    virtual bool fuses_direction() const { return false; }
+    virtual int matvec_dir(const double *w, const double *vOld, double *vNew, double *y, SolverState *st, int fin,
+                           const RedSlot &slot, hipStream_t s) { return SELF_ERR_STATE; }
+    virtual int matvec_dir32(const float *w, const float *vOld, float *vNew, float *y, SolverState *st, int fin,
+                             const RedSlot &slot, hipStream_t s) { return SELF_ERR_STATE; }
     // r = b - A x (the model's Residual); |r|^2 fused and finalized with fin.
     virtual int residual(const double *x, double *r, SolverState *st, int fin, const RedSlot &slot,
                          hipStream_t s) = 0;
@@ -119,6 +136,8 @@ struct KrylovWork {
     int m = 0;
     double *r = nullptr, *v = nullptr, *z = nullptr, *w = nullptr, *u = nullptr;
     float *r32 = nullptr, *v32 = nullptr, *z32 = nullptr, *w32 = nullptr, *u32 = nullptr;
+    double *vB = nullptr;  // second search-direction buffer (fused direction update)
+    float *vB32 = nullptr;
     double *V = nullptr, *Z = nullptr;  // GMRES bases (n x (m+1))
     double *small = nullptr;            // GMRES small matrices
     SolverState *st = nullptr;          // device
