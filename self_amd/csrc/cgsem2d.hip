@@ -45,21 +45,43 @@ struct CgLDS {
 // Element operator on gathered values -> Lbuf (local copies).
 // RESID: Dirichlet nodes take the prescribed values uD (Residual of the
 // solution); otherwise they are zero (search directions are masked there).
-// DIR: the gathered value is the CG search direction formed on the fly,
-// v = x (iteration 1) or x + beta vOld, and every copy also stores it to
-// vNew[g] (the copies of a DOF store the same bits).
+// DIR: the folded CG direction step of iteration `it`.  When `pending`,
+// the end of iteration it-1 comes first: every workgroup sums the update
+// sweep's partial rows 1 and 2 itself (same order, same bits), workgroup 0
+// records it in the state, and all exit if the loop stops; otherwise beta
+// is in the state.  The gathered value is the search direction formed on
+// the fly, v = x (iteration 1) or x + beta vOld, and every copy also stores
+// it to vNew[g] (the copies of a DOF store the same bits).
 template <int N, typename T, bool RESID, bool DIR = false>
 __global__ __launch_bounds__(256) void k_cg_local(const T *__restrict__ x, const int *__restrict__ gid,
                                                   const double *__restrict__ uD, const T *__restrict__ G3,
                                                   const T *__restrict__ Dg, T *__restrict__ Lbuf, int nE,
-                                                  const SolverState *st, const T *__restrict__ vOld = nullptr,
-                                                  T *__restrict__ vNew = nullptr)
+                                                  SolverState *st, const T *__restrict__ vOld = nullptr,
+                                                  T *__restrict__ vNew = nullptr, RedSlot slot = RedSlot{},
+                                                  int it = 0, int pending = 0)
 {
     using S = Shape<N>;
     constexpr int n1 = S::n1, np = S::np, epb = S::epb;
     if (st && (st->done || (!RESID && st->innerExit))) return;
     __shared__ CgLDS<N, T> L;
     __shared__ T us[epb * np], f1[epb * np], f2[epb * np];
+    T beta = 0;
+    if constexpr (DIR) {
+        if (pending) {
+            __shared__ double sFold[2];
+            if (threadIdx.x < 128) {  // wavefront q sums row 1 + q
+                const int q = threadIdx.x >> 6;
+                const double Sq = wave_fold_sum(slot.partials + (1 + q) * kRedBlocks);
+                if ((threadIdx.x & 63) == 0) sFold[q] = Sq;
+            }
+            __syncthreads();
+            if (blockIdx.x == 0 && threadIdx.x == 0) cg_end_apply(st, it - 1, sFold[0], sFold[1]);
+            if (cg_stop(st, sqrt(sFold[0]))) return;  // uniform over the workgroup
+            beta = (T)(sFold[1] / st->numP[(it - 1) & 1]);
+        } else {
+            beta = (T)st->beta;
+        }
+    }
     for (int k = threadIdx.x; k < n1 * n1; k += blockDim.x) L.D[k] = Dg[k];
     const int t = threadIdx.x, le = t / np, nd = t - le * np;
     const int e = blockIdx.x * epb + le;
@@ -73,7 +95,7 @@ __global__ __launch_bounds__(256) void k_cg_local(const T *__restrict__ x, const
         if (g >= 0) {
             if constexpr (DIR) {
                 const T wv = x[g];
-                u = (st->iter == 1) ? wv : wv + (T)st->beta * vOld[g];
+                u = (it == 1) ? wv : wv + beta * vOld[g];
                 vNew[g] = u;
             } else {
                 u = x[g];
@@ -117,8 +139,8 @@ __global__ __launch_bounds__(256) void k_cg_local(const T *__restrict__ x, const
 // y[g] = sum over the copies of g (GlobalSum order); optional fused dot.
 // MODE 0: matvec.  MODE 1: residual  y = (b[c0] - sum L) + sum_{c>=1} b[c],
 // with |y|^2 reduced.
-template <typename T, int MODE, bool FIN>
-__global__ __launch_bounds__(kRedThreads) void k_cg_assemble(const T *__restrict__ Lbuf, const int *__restrict__ cs,
+template <typename T, int MODE, bool FIN, int TPB = kRedThreads>
+__global__ __launch_bounds__(TPB) void k_cg_assemble(const T *__restrict__ Lbuf, const int *__restrict__ cs,
                                                              const int *__restrict__ ci, const double *__restrict__ bl,
                                                              size_t nG, T *__restrict__ y, const T *__restrict__ dotWith,
                                                              SolverState *st, int fin, RedSlot slot)
@@ -148,7 +170,8 @@ __global__ __launch_bounds__(kRedThreads) void k_cg_assemble(const T *__restrict
         y[g] = out;
     }
     if (fin == kFinNone || !st) return;
-    store_partial(slot, 0, acc, lds4);
+    if constexpr (TPB == kRedThreads) store_partial(slot, 0, acc, lds4);
+    else store_partial1<TPB>(slot, 0, acc);
 }
 
 #define CG_DISPATCH_N(N_, CALL)                          \
@@ -240,30 +263,26 @@ struct CgsemOp final : DevOp {
 
     bool fuses_direction() const override { return true; }
 
-    int matvec_dir(const double *w, const double *vOld, double *vNew, double *y, SolverState *st, int fin,
-                   const RedSlot &slot, hipStream_t s) override
+    int cg_direction(const double *w, const double *vOld, double *vNew, double *y, SolverState *st,
+                     const RedSlot &slot, int it, bool pending, hipStream_t s) override
     {
         CG_DISPATCH_N(N, hipLaunchKernelGGL((k_cg_local<NN, double, false, true>), dim3(cg_grid<NN>(nE)), dim3(256),
-                                            0, s, w, gid, uD, G3, D, Lbuf, (int)nE, (const SolverState *)st, vOld,
-                                            vNew));
-        hipLaunchKernelGGL((k_cg_assemble<double, 0, false>), dim3(kRedBlocks), dim3(kRedThreads), 0, s, Lbuf, cs,
-                           ci, (const double *)nullptr, nG, y, (const double *)vNew, st, fin, slot);
-        hipLaunchKernelGGL((k_cg_assemble<double, 0, true>), dim3(1), dim3(kRedThreads), 0, s, Lbuf, cs, ci,
-                           (const double *)nullptr, nG, y, (const double *)vNew, st, fin, slot);
+                                            0, s, w, gid, uD, G3, D, Lbuf, (int)nE, st, vOld, vNew, slot, it,
+                                            (int)pending));
+        hipLaunchKernelGGL((k_cg_assemble<double, 0, false, kFoldThreads>), dim3(kFoldParts), dim3(kFoldThreads), 0, s, Lbuf, cs,
+                           ci, (const double *)nullptr, nG, y, (const double *)vNew, st, (int)kFinAlpha, slot);
         SELF_HIP(hipGetLastError());
         return SELF_OK;
     }
 
-    int matvec_dir32(const float *w, const float *vOld, float *vNew, float *y, SolverState *st, int fin,
-                     const RedSlot &slot, hipStream_t s) override
+    int cg_direction32(const float *w, const float *vOld, float *vNew, float *y, SolverState *st,
+                       const RedSlot &slot, int it, bool pending, hipStream_t s) override
     {
         CG_DISPATCH_N(N, hipLaunchKernelGGL((k_cg_local<NN, float, false, true>), dim3(cg_grid<NN>(nE)), dim3(256),
-                                            0, s, w, gid, uD, G3f, Df, Lbuf32, (int)nE, (const SolverState *)st,
-                                            vOld, vNew));
-        hipLaunchKernelGGL((k_cg_assemble<float, 0, false>), dim3(kRedBlocks), dim3(kRedThreads), 0, s, Lbuf32, cs,
-                           ci, (const double *)nullptr, nG, y, (const float *)vNew, st, fin, slot);
-        hipLaunchKernelGGL((k_cg_assemble<float, 0, true>), dim3(1), dim3(kRedThreads), 0, s, Lbuf32, cs, ci,
-                           (const double *)nullptr, nG, y, (const float *)vNew, st, fin, slot);
+                                            0, s, w, gid, uD, G3f, Df, Lbuf32, (int)nE, st, vOld, vNew, slot, it,
+                                            (int)pending));
+        hipLaunchKernelGGL((k_cg_assemble<float, 0, false, kFoldThreads>), dim3(kFoldParts), dim3(kFoldThreads), 0, s, Lbuf32, cs,
+                           ci, (const double *)nullptr, nG, y, (const float *)vNew, st, (int)kFinAlpha, slot);
         SELF_HIP(hipGetLastError());
         return SELF_OK;
     }

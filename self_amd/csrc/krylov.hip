@@ -10,9 +10,10 @@
 // deterministically (reduce.hpp); the scalar recurrences (alpha, beta,
 // Givens rotations, the stop tests) run in the one-workgroup final launch of
 // a reduction or in a one-thread kernel.  A CG iteration ends with one sweep
-// that also starts the next (preconditioner and numerator), and operators
-// that gather their input fold the direction update into the gather, so an
-// iteration is five launches.  A `done` flag in the device state
+// that also starts the next (preconditioner and numerator); for operators
+// that gather their input the direction update is folded into the gather
+// and each final sum into the kernel that consumes it, so an iteration is
+// two launches of the operator and one sweep.  A `done` flag in the device state
 // turns the kernels of iterations after convergence into no-ops, so the
 // host launches iterations in chunks and reads the flag between chunks.
 
@@ -122,6 +123,56 @@ __global__ __launch_bounds__(kRedThreads) void k_update_urw(T *__restrict__ u, c
     }
     const int f[2] = {kFinRnorm, kFinNumNext};
     reduce_rows<2, FIN>(slot, acc, st, f);
+}
+
+// Folded form of k_update_urw (launched with kFoldParts workgroups): alpha
+// = num / v.z from row 0's partials (each workgroup sums them itself), the
+// same sweep, and r.r, r.w left as partials in rows 1 and 2 for the next
+// direction step or k_cg_fold_end.
+template <typename T, bool PRE>
+__global__ __launch_bounds__(kFoldThreads) void k_update_urw_fold(T *__restrict__ u, const T *__restrict__ v,
+                                                                 T *__restrict__ r, const T *__restrict__ z,
+                                                                 const T *__restrict__ dinv, T *__restrict__ w,
+                                                                 size_t n, SolverState *st, RedSlot slot, int it)
+{
+    if (st_done(st)) return;
+    __shared__ double sAlpha;
+    if (threadIdx.x < 64) {
+        const double vz = wave_fold_sum(slot.partials);
+        if (threadIdx.x == 0) {
+            const double a = st->numP[it & 1] / vz;
+            sAlpha = a;
+            if (blockIdx.x == 0) {
+                st->vz = vz;
+                st->alpha = a;
+            }
+        }
+    }
+    __syncthreads();
+    const T alpha = (T)sAlpha;
+    double acc0 = 0.0, acc1 = 0.0;
+    for (size_t g = (size_t)blockIdx.x * blockDim.x + threadIdx.x; g < n; g += (size_t)gridDim.x * blockDim.x) {
+        u[g] = u[g] + alpha * v[g];
+        const T rv = r[g] - alpha * z[g];
+        r[g] = rv;
+        acc0 += (double)rv * (double)rv;
+        if constexpr (PRE) {
+            const T wv = dinv[g] * rv;
+            w[g] = wv;
+            acc1 += (double)rv * (double)wv;
+        }
+    }
+    if constexpr (!PRE) acc1 = acc0;
+    store_partials2<kFoldThreads>(slot, 1, acc0, 2, acc1);
+}
+
+// end of the last iteration of a chunk (one wavefront)
+__global__ __launch_bounds__(64) void k_cg_fold_end(SolverState *st, RedSlot slot, int it)
+{
+    if (st_done(st)) return;
+    const double S0 = wave_fold_sum(slot.partials + kRedBlocks);
+    const double S1 = wave_fold_sum(slot.partials + 2 * kRedBlocks);
+    if (threadIdx.x == 0) cg_end_apply(st, it, S0, S1);
 }
 
 // ---------------------------------------------------------------- GMRES
@@ -437,48 +488,58 @@ inline int op_matvec(DevOp &op, const float *x, float *y, SolverState *st, const
 {
     return op.matvec32(x, y, x, st, kFinAlpha, slot, s);
 }
-inline int op_matvec_dir(DevOp &op, const double *w, const double *vOld, double *vNew, double *y, SolverState *st,
-                         const RedSlot &slot, hipStream_t s)
+inline int op_cg_direction(DevOp &op, const double *w, const double *vOld, double *vNew, double *y,
+                           SolverState *st, const RedSlot &slot, int it, bool pending, hipStream_t s)
 {
-    return op.matvec_dir(w, vOld, vNew, y, st, kFinAlpha, slot, s);
+    return op.cg_direction(w, vOld, vNew, y, st, slot, it, pending, s);
 }
-inline int op_matvec_dir(DevOp &op, const float *w, const float *vOld, float *vNew, float *y, SolverState *st,
-                         const RedSlot &slot, hipStream_t s)
+inline int op_cg_direction(DevOp &op, const float *w, const float *vOld, float *vNew, float *y, SolverState *st,
+                           const RedSlot &slot, int it, bool pending, hipStream_t s)
 {
-    return op.matvec_dir32(w, vOld, vNew, y, st, kFinAlpha, slot, s);
+    return op.cg_direction32(w, vOld, vNew, y, st, slot, it, pending, s);
 }
 
-// CG iterations it0..it1 (the loop body of the reference's CG/PCG), three
-// reductions' worth of launches each:
-//   v = w + beta v, z = A v, alpha = num / v.z   (direction update folded
-//                                                 into the operator's gather
-//                                                 when it offers that)
+// CG iterations it0..it1 (the loop body of the reference's CG/PCG):
+//   v = w + beta v, z = A v, alpha = num / v.z
 //   u += alpha v, r -= alpha z, rNorm, stop test, w = P r, num, beta
-// With the folded update v alternates between vA (odd iterations) and vB.
+// Operators that gather their input take the folded form: two launches per
+// iteration (direction step, update sweep), the bookkeeping of each
+// iteration's end done by the next direction step, and by k_cg_fold_end
+// after the chunk's last; v alternates between vA (odd iterations) and vB.
+// Other operators: in-place direction update, operator, update sweep, each
+// reduction with its final launch.
 template <typename T>
 int cg_chunk(DevOp &op, T *u, T *r, T *w, T *vA, T *vB, T *z, const T *dinv, size_t n, SolverState *st,
              const RedSlot &slot, int it0, int it1, hipStream_t s)
 {
-    const bool fused = op.fuses_direction();
     const T *wsrc = dinv ? w : r;
+    if (op.fuses_direction()) {
+        for (int it = it0; it <= it1; ++it) {
+            const T *vOld = (it & 1) ? vB : vA;
+            T *v = (it & 1) ? vA : vB;
+            int rc = op_cg_direction(op, wsrc, vOld, v, z, st, slot, it, it > it0, s);
+            if (rc) return rc;
+            if (dinv)
+                hipLaunchKernelGGL((k_update_urw_fold<T, true>), dim3(kFoldParts), dim3(kFoldThreads), 0, s, u,
+                                   (const T *)v, r, (const T *)z, dinv, w, n, st, slot, it);
+            else
+                hipLaunchKernelGGL((k_update_urw_fold<T, false>), dim3(kFoldParts), dim3(kFoldThreads), 0, s, u,
+                                   (const T *)v, r, (const T *)z, dinv, w, n, st, slot, it);
+        }
+        hipLaunchKernelGGL(k_cg_fold_end, dim3(1), dim3(64), 0, s, st, slot, it1);
+        SELF_HIP(hipGetLastError());
+        return SELF_OK;
+    }
     const int gv = vgrid(n);
     for (int it = it0; it <= it1; ++it) {
-        T *v = vA;
-        int rc;
-        if (fused) {
-            const T *vOld = (it & 1) ? vB : vA;
-            v = (it & 1) ? vA : vB;
-            rc = op_matvec_dir(op, wsrc, vOld, v, z, st, slot, s);
-        } else {
-            hipLaunchKernelGGL(k_update_v<T>, dim3(gv), dim3(256), 0, s, wsrc, vA, n, st);
-            rc = op_matvec(op, vA, z, st, slot, s);
-        }
+        hipLaunchKernelGGL(k_update_v<T>, dim3(gv), dim3(256), 0, s, wsrc, vA, n, st);
+        int rc = op_matvec(op, vA, z, st, slot, s);
         if (rc) return rc;
         if (dinv)
-            SELF_RED_LAUNCH((k_update_urw<T, false, true>), (k_update_urw<T, true, true>), s, u, (const T *)v, r,
+            SELF_RED_LAUNCH((k_update_urw<T, false, true>), (k_update_urw<T, true, true>), s, u, (const T *)vA, r,
                                (const T *)z, dinv, w, n, st, slot);
         else
-            SELF_RED_LAUNCH((k_update_urw<T, false, false>), (k_update_urw<T, true, false>), s, u, (const T *)v, r,
+            SELF_RED_LAUNCH((k_update_urw<T, false, false>), (k_update_urw<T, true, false>), s, u, (const T *)vA, r,
                                (const T *)z, dinv, w, n, st, slot);
     }
     SELF_HIP(hipGetLastError());

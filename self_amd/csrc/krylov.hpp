@@ -23,6 +23,7 @@ namespace self {
 // Scalars of one solve, in device memory.
 struct SolverState {
     double num, den, alpha, beta, rNorm, r0, tol, vz, bnorm;
+    double numP[2];  // CG numerator of iteration k in numP[k & 1]
     int iter, done, maxIters, stopRule, ioerr, l, nr, innerExit, m;
     double *resi;
     int resiLen;
@@ -67,6 +68,7 @@ __device__ inline void finalize_scalar(SolverState *st, int kind, double S)
     case kFinNum:
     case kFinNumCG:
         st->num = S;
+        st->numP[st->iter & 1] = S;
         st->beta = (st->iter == 1) ? 0.0 : S / st->den;
         break;
     case kFinAlpha:
@@ -100,6 +102,53 @@ __device__ inline void finalize_scalar(SolverState *st, int kind, double S)
 }
 
 
+// ---------------------------------------------------- folded CG reductions
+// In the folded CG iteration no reduction has a final launch: the kernel
+// that needs a scalar sums the producer's kFoldParts partials itself, every
+// workgroup in the same fixed order (so every workgroup has the same bits),
+// and workgroup 0 records the scalar bookkeeping in the state.  Partial rows
+// of RedSlot: 0 = v.z, 1 = r.r, 2 = r.w.
+constexpr int kFoldParts = 512;    // workgroups of the producing kernels
+constexpr int kFoldThreads = 1024; // their size
+static_assert(kFoldParts % 64 == 0 && kFoldParts <= kRedBlocks, "fold partials: whole wavefront loads, one row");
+
+// fixed-order sum of one partial row by one wavefront, valid in every lane
+__device__ __forceinline__ double wave_fold_sum(const double *row)
+{
+    const int lane = threadIdx.x & 63;
+    double v = 0.0;
+    #pragma unroll
+    for (int b = 0; b < kFoldParts / 64; ++b) v += row[lane + 64 * b];
+    v = wave_sum(v);
+    return __shfl(v, 0, 64);
+}
+
+__device__ __forceinline__ bool cg_stop(const SolverState *st, double rn)
+{
+    return (st->stopRule == kStopSqrtNormRel) ? (sqrt(rn) / st->r0 < st->tol) : (rn / st->r0 < st->tol);
+}
+
+// End of CG iteration it from S0 = r.r and S1 = r.w: rNorm, resi(it),
+// den = num and the stop test, then num and beta of iteration it + 1 (the
+// kFinRnorm and kFinNumNext epilogues).  One thread.
+__device__ inline void cg_end_apply(SolverState *st, int it, double S0, double S1)
+{
+    const double rn = sqrt(S0), den = st->numP[it & 1];
+    st->rNorm = rn;
+    if (it < st->resiLen) st->resi[it] = rn;
+    st->den = den;
+    if (cg_stop(st, rn)) {
+        st->done = 1;
+    } else if (it >= st->maxIters) {
+        st->done = 2;
+    } else {
+        st->iter = it + 1;
+        st->num = S1;
+        st->numP[(it + 1) & 1] = S1;
+        st->beta = S1 / den;
+    }
+}
+
 // Operator plug-in (the deferred procedures of IterativeSolver).
 struct DevOp {
     size_t n = 0;                   // vector length the solver sees
@@ -112,16 +161,20 @@ struct DevOp {
     virtual bool has_f32() const { return false; }
     virtual int matvec32(const float *x, float *y, const float *dotWith, SolverState *st, int fin,
                          const RedSlot &slot, hipStream_t s) { return SELF_ERR_STATE; }
-    // Search-direction update folded into the operator's input gather:
-    // vNew = w (iteration 1) or w + beta vOld, y = A vNew, dot(vNew, y)
-    // finalized with fin.  vNew and vOld are distinct buffers (other
-    // elements still gather vOld while vNew is written).  Operators without
-    // it leave fuses_direction() false and the solver updates v in place.
+    // The folded CG direction step of iteration it, for operators that
+    // gather their input: when `pending`, first the end of iteration it-1
+    // from the update sweep's partial rows 1 and 2 (cg_end_apply in
+    // workgroup 0; every workgroup exits on the stop test), otherwise beta
+    // from the state; then vNew = w (it == 1) or w + beta vOld formed in the
+    // gather, y = A vNew, and v.y left as kFoldParts partials in row 0.
+    // vNew and vOld are distinct buffers (other elements still gather vOld
+    // while vNew is written).  Operators without it leave
+    // fuses_direction() false and the solver updates v in place.
     virtual bool fuses_direction() const { return false; }
-    virtual int matvec_dir(const double *w, const double *vOld, double *vNew, double *y, SolverState *st, int fin,
-                           const RedSlot &slot, hipStream_t s) { return SELF_ERR_STATE; }
-    virtual int matvec_dir32(const float *w, const float *vOld, float *vNew, float *y, SolverState *st, int fin,
-                             const RedSlot &slot, hipStream_t s) { return SELF_ERR_STATE; }
+    virtual int cg_direction(const double *w, const double *vOld, double *vNew, double *y, SolverState *st,
+                             const RedSlot &slot, int it, bool pending, hipStream_t s) { return SELF_ERR_STATE; }
+    virtual int cg_direction32(const float *w, const float *vOld, float *vNew, float *y, SolverState *st,
+                               const RedSlot &slot, int it, bool pending, hipStream_t s) { return SELF_ERR_STATE; }
     // r = b - A x (the model's Residual); |r|^2 fused and finalized with fin.
     virtual int residual(const double *x, double *r, SolverState *st, int fin, const RedSlot &slot,
                          hipStream_t s) = 0;

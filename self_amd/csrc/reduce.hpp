@@ -51,6 +51,50 @@ __device__ __forceinline__ void store_partial(const RedSlot &slot, int k, double
     if (threadIdx.x == 0) slot.partials[(size_t)k * kRedBlocks + blockIdx.x] = p;
 }
 
+// partial kernel with TPB-thread workgroups: rows k0 and k1 of this
+// workgroup (wavefront sums, then wavefronts in index order)
+template <int TPB>
+__device__ __forceinline__ void store_partials2(const RedSlot &slot, int k0, double a0, int k1, double a1)
+{
+    constexpr int W = TPB / 64;
+    __shared__ double l0[W], l1[W];
+    a0 = wave_sum(a0);
+    a1 = wave_sum(a1);
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    if (lane == 0) {
+        l0[wid] = a0;
+        l1[wid] = a1;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double p0 = 0.0, p1 = 0.0;
+        #pragma unroll
+        for (int q = 0; q < W; ++q) {
+            p0 += l0[q];
+            p1 += l1[q];
+        }
+        slot.partials[(size_t)k0 * kRedBlocks + blockIdx.x] = p0;
+        slot.partials[(size_t)k1 * kRedBlocks + blockIdx.x] = p1;
+    }
+}
+
+template <int TPB>
+__device__ __forceinline__ void store_partial1(const RedSlot &slot, int k, double a)
+{
+    constexpr int W = TPB / 64;
+    __shared__ double l0[W];
+    a = wave_sum(a);
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    if (lane == 0) l0[wid] = a;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double p = 0.0;
+        #pragma unroll
+        for (int q = 0; q < W; ++q) p += l0[q];
+        slot.partials[(size_t)k * kRedBlocks + blockIdx.x] = p;
+    }
+}
+
 // final kernel (one workgroup): fixed-order sum of row k, valid in thread 0
 __device__ __forceinline__ double final_sum(const RedSlot &slot, int k, double *lds4)
 {
