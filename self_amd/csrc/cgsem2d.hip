@@ -42,46 +42,53 @@ struct CgLDS {
     T D[n1 * n1];
 };
 
+// F1' = G11 du/ds + G12 du/dp and F2' = G12 du/ds + G22 du/dp at node
+// (i, j) of one element's gathered values ue
+template <int N, typename T>
+__device__ __forceinline__ void cg_flux(const T *D, const T *ue, int i, int j, T g11, T g12, T g22, T *f1, T *f2)
+{
+    constexpr int n1 = N + 1;
+    // du/ds = D u along s (row j), du/dp = D u along p (column i)
+    T duds = D[i] * ue[n1 * j];
+    T dudp = D[j] * ue[i];
+    #pragma unroll
+    for (int m = 1; m < n1; ++m) {
+        duds = fma(D[i + n1 * m], ue[m + n1 * j], duds);
+        dudp = fma(D[j + n1 * m], ue[i + n1 * m], dudp);
+    }
+    f1[i + n1 * j] = g11 * duds + g12 * dudp;
+    f2[i + n1 * j] = g12 * duds + g22 * dudp;
+}
+
+// Lu(i, j) = -( sum_m D(m,i) F1'(m,j) + sum_m D(m,j) F2'(i,m) )
+template <int N, typename T>
+__device__ __forceinline__ T cg_divergence(const T *D, const T *a, const T *b, int i, int j)
+{
+    constexpr int n1 = N + 1;
+    T s1 = D[n1 * i] * a[n1 * j];  // D(0,i) F1'(0,j)
+    T s2 = D[n1 * j] * b[i];       // D(0,j) F2'(i,0)
+    #pragma unroll
+    for (int m = 1; m < n1; ++m) {
+        s1 = fma(D[m + n1 * i], a[m + n1 * j], s1);
+        s2 = fma(D[m + n1 * j], b[i + n1 * m], s2);
+    }
+    return -(s1 + s2);
+}
+
 // Element operator on gathered values -> Lbuf (local copies).
 // RESID: Dirichlet nodes take the prescribed values uD (Residual of the
 // solution); otherwise they are zero (search directions are masked there).
-// DIR: the folded CG direction step of iteration `it`.  When `pending`,
-// the end of iteration it-1 comes first: every workgroup sums the update
-// sweep's partial rows 1 and 2 itself (same order, same bits), workgroup 0
-// records it in the state, and all exit if the loop stops; otherwise beta
-// is in the state.  The gathered value is the search direction formed on
-// the fly, v = x (iteration 1) or x + beta vOld, and every copy also stores
-// it to vNew[g] (the copies of a DOF store the same bits).
-template <int N, typename T, bool RESID, bool DIR = false>
+template <int N, typename T, bool RESID>
 __global__ __launch_bounds__(256) void k_cg_local(const T *__restrict__ x, const int *__restrict__ gid,
                                                   const double *__restrict__ uD, const T *__restrict__ G3,
                                                   const T *__restrict__ Dg, T *__restrict__ Lbuf, int nE,
-                                                  SolverState *st, const T *__restrict__ vOld = nullptr,
-                                                  T *__restrict__ vNew = nullptr, RedSlot slot = RedSlot{},
-                                                  int it = 0, int pending = 0)
+                                                  const SolverState *st)
 {
     using S = Shape<N>;
     constexpr int n1 = S::n1, np = S::np, epb = S::epb;
     if (st && (st->done || (!RESID && st->innerExit))) return;
     __shared__ CgLDS<N, T> L;
     __shared__ T us[epb * np], f1[epb * np], f2[epb * np];
-    T beta = 0;
-    if constexpr (DIR) {
-        if (pending) {
-            __shared__ double sFold[2];
-            if (threadIdx.x < 128) {  // wavefront q sums row 1 + q
-                const int q = threadIdx.x >> 6;
-                const double Sq = wave_fold_sum(slot.partials + (1 + q) * kRedBlocks);
-                if ((threadIdx.x & 63) == 0) sFold[q] = Sq;
-            }
-            __syncthreads();
-            if (blockIdx.x == 0 && threadIdx.x == 0) cg_end_apply(st, it - 1, sFold[0], sFold[1]);
-            if (cg_stop(st, sqrt(sFold[0]))) return;  // uniform over the workgroup
-            beta = (T)(sFold[1] / st->numP[(it - 1) & 1]);
-        } else {
-            beta = (T)st->beta;
-        }
-    }
     for (int k = threadIdx.x; k < n1 * n1; k += blockDim.x) L.D[k] = Dg[k];
     const int t = threadIdx.x, le = t / np, nd = t - le * np;
     const int e = blockIdx.x * epb + le;
@@ -92,48 +99,135 @@ __global__ __launch_bounds__(256) void k_cg_local(const T *__restrict__ x, const
     if (act) {
         const int g = gid[lid];
         T u;
-        if (g >= 0) {
-            if constexpr (DIR) {
-                const T wv = x[g];
-                u = (it == 1) ? wv : wv + beta * vOld[g];
-                vNew[g] = u;
-            } else {
-                u = x[g];
-            }
-        } else {
-            u = RESID ? (T)uD[lid] : (T)0;
-        }
+        if (g >= 0) u = x[g];
+        else u = RESID ? (T)uD[lid] : (T)0;
         us[le * np + nd] = u;
         g11 = G3[lid];
         g12 = G3[stride + lid];
         g22 = G3[2 * stride + lid];
     }
     __syncthreads();
-    if (act) {
-        const T *ue = us + le * np;
-        // du/ds = D u along s (row j), du/dp = D u along p (column i)
-        T duds = L.D[i] * ue[n1 * j];
-        T dudp = L.D[j] * ue[i];
-        #pragma unroll
-        for (int m = 1; m < n1; ++m) {
-            duds = fma(L.D[i + n1 * m], ue[m + n1 * j], duds);
-            dudp = fma(L.D[j + n1 * m], ue[i + n1 * m], dudp);
-        }
-        f1[le * np + nd] = g11 * duds + g12 * dudp;
-        f2[le * np + nd] = g12 * duds + g22 * dudp;
-    }
+    if (act) cg_flux<N>(L.D, us + le * np, i, j, g11, g12, g22, f1 + le * np, f2 + le * np);
     __syncthreads();
-    if (act) {
-        const T *a = f1 + le * np, *b = f2 + le * np;
-        T s1 = L.D[n1 * i] * a[n1 * j];  // D(0,i) F1'(0,j)
-        T s2 = L.D[n1 * j] * b[i];       // D(0,j) F2'(i,0)
-        #pragma unroll
-        for (int m = 1; m < n1; ++m) {
-            s1 = fma(L.D[m + n1 * i], a[m + n1 * j], s1);
-            s2 = fma(L.D[m + n1 * j], b[i + n1 * m], s2);
+    if (act) Lbuf[lid] = cg_divergence<N>(L.D, f1 + le * np, f2 + le * np, i, j);
+}
+
+// The folded CG direction step of iteration `it` (grid-stride over groups
+// of epb elements, at most kRedBlocks workgroups).  When `pending`, the end
+// of iteration it-1 comes first: every workgroup sums the update sweep's
+// partial rows 1 and 2 itself (same order, same bits), workgroup 0 records
+// it in the state, and all exit if the loop stops; otherwise beta is in
+// the state.  The gathered value is the search direction formed on the
+// fly, v = x (iteration 1) or x + beta vOld; every copy also stores it to
+// vNew[g] (the copies of a DOF store the same bits).  After the element
+// operator, v.Av is accumulated as the sum over elements of v_e . A_e v_e
+// (Dirichlet copies hold v = 0), one partial per workgroup in row 0: the
+// same number as the dot product of v with the assembled A v, so the
+// assembly can be fused with the vector update that follows.
+template <int N, typename T>
+__global__ __launch_bounds__(256) void k_cg_direction(const T *__restrict__ x, const T *__restrict__ vOld,
+                                                      T *__restrict__ vNew, const int *__restrict__ gid,
+                                                      const T *__restrict__ G3, const T *__restrict__ Dg,
+                                                      T *__restrict__ Lbuf, int nE, SolverState *st, RedSlot slot,
+                                                      int it, int pending)
+{
+    using S = Shape<N>;
+    constexpr int n1 = S::n1, np = S::np, epb = S::epb;
+    __shared__ CgLDS<N, T> L;
+    __shared__ T us[epb * np], f1[epb * np], f2[epb * np];
+    __shared__ double lds4[4];
+    T beta;
+    if (pending) {
+        // workgroup 0 may set `done` during this kernel, so each workgroup
+        // reads it once (thread 0) and decides as a whole
+        __shared__ double sFold[2];
+        __shared__ int sDone;
+        if (threadIdx.x < 128) {  // wavefront q sums row 1 + q
+            const int q = threadIdx.x >> 6;
+            const double Sq = wave_fold_sum(slot.partials + (1 + q) * kRedBlocks);
+            if ((threadIdx.x & 63) == 0) sFold[q] = Sq;
         }
-        Lbuf[lid] = -(s1 + s2);
+        if (threadIdx.x == 0) sDone = st->done;
+        __syncthreads();
+        if (sDone) return;
+        if (blockIdx.x == 0 && threadIdx.x == 0) cg_end_apply(st, it - 1, sFold[0], sFold[1]);
+        if (cg_stop(st, sqrt(sFold[0]))) return;  // uniform over the workgroup
+        beta = (T)(sFold[1] / st->numP[(it - 1) & 1]);
+    } else {
+        if (st->done) return;  // nothing writes it during this kernel
+        beta = (T)st->beta;
     }
+    for (int k = threadIdx.x; k < n1 * n1; k += blockDim.x) L.D[k] = Dg[k];
+    const int t = threadIdx.x, le = t / np, nd = t - le * np;
+    const int i = nd % n1, j = nd / n1;
+    const size_t stride = (size_t)nE * np;
+    const int nGroups = (nE + epb - 1) / epb;
+    double acc = 0.0;
+    for (int grp = blockIdx.x; grp < nGroups; grp += gridDim.x) {
+        const int e = grp * epb + le;
+        const bool act = le < epb && e < nE;
+        const size_t lid = (size_t)e * np + nd;
+        T u = 0, g11 = 0, g12 = 0, g22 = 0;
+        if (act) {
+            const int g = gid[lid];
+            if (g >= 0) {
+                const T wv = x[g];
+                u = (it == 1) ? wv : wv + beta * vOld[g];
+                vNew[g] = u;
+            }
+            us[le * np + nd] = u;
+            g11 = G3[lid];
+            g12 = G3[stride + lid];
+            g22 = G3[2 * stride + lid];
+        }
+        __syncthreads();
+        if (act) cg_flux<N>(L.D, us + le * np, i, j, g11, g12, g22, f1 + le * np, f2 + le * np);
+        __syncthreads();
+        if (act) {
+            const T Lv = cg_divergence<N>(L.D, f1 + le * np, f2 + le * np, i, j);
+            Lbuf[lid] = Lv;
+            acc += (double)u * (double)Lv;
+        }
+    }
+    store_partial(slot, 0, acc, lds4);
+}
+
+// The rest of the folded CG iteration in one sweep over the global DOFs:
+// alpha = num / v.Av (each workgroup sums the direction step's nPart
+// partials itself), z = A v assembled from the copies in GlobalSum order and
+// used at once: u += alpha v, r -= alpha z, w = P r; r.r and r.w partials in
+// rows 1 and 2.  z is never stored.
+template <typename T, bool PRE>
+__global__ __launch_bounds__(kFoldThreads) void k_cg_assemble_update(
+    const T *__restrict__ Lbuf, const int *__restrict__ cs, const int *__restrict__ ci, size_t nG, T *__restrict__ u,
+    const T *__restrict__ v, T *__restrict__ r, const T *__restrict__ dinv, T *__restrict__ w, SolverState *st,
+    RedSlot slot, int it, int nPart)
+{
+    if (st->done) return;
+    const double vz = block_fold_sum<kFoldThreads>(slot.partials, nPart);
+    const double a = st->numP[it & 1] / vz;
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        st->vz = vz;
+        st->alpha = a;
+    }
+    const T alpha = (T)a;
+    double acc0 = 0.0, acc1 = 0.0;
+    for (size_t g = (size_t)blockIdx.x * blockDim.x + threadIdx.x; g < nG; g += (size_t)gridDim.x * blockDim.x) {
+        const int c0 = cs[g], c1 = cs[g + 1];
+        T z = Lbuf[ci[c0]];
+        for (int c = c0 + 1; c < c1; ++c) z = z + Lbuf[ci[c]];
+        u[g] = u[g] + alpha * v[g];
+        const T rv = r[g] - alpha * z;
+        r[g] = rv;
+        acc0 += (double)rv * (double)rv;
+        if constexpr (PRE) {
+            const T wv = dinv[g] * rv;
+            w[g] = wv;
+            acc1 += (double)rv * (double)wv;
+        }
+    }
+    if constexpr (!PRE) acc1 = acc0;
+    store_partials2<kFoldThreads>(slot, 1, acc0, 2, acc1);
 }
 
 // y[g] = sum over the copies of g (GlobalSum order); optional fused dot.
@@ -263,28 +357,35 @@ struct CgsemOp final : DevOp {
 
     bool fuses_direction() const override { return true; }
 
-    int cg_direction(const double *w, const double *vOld, double *vNew, double *y, SolverState *st,
-                     const RedSlot &slot, int it, bool pending, hipStream_t s) override
+    template <typename T>
+    int cg_iter(const T *dinvT, T *u, T *r, T *w, const T *vOld, T *vNew, const T *G3T, const T *DT, T *LbufT,
+                SolverState *st, const RedSlot &slot, int it, bool pending, hipStream_t s)
     {
-        CG_DISPATCH_N(N, hipLaunchKernelGGL((k_cg_local<NN, double, false, true>), dim3(cg_grid<NN>(nE)), dim3(256),
-                                            0, s, w, gid, uD, G3, D, Lbuf, (int)nE, st, vOld, vNew, slot, it,
-                                            (int)pending));
-        hipLaunchKernelGGL((k_cg_assemble<double, 0, false, kFoldThreads>), dim3(kFoldParts), dim3(kFoldThreads), 0, s, Lbuf, cs,
-                           ci, (const double *)nullptr, nG, y, (const double *)vNew, st, (int)kFinAlpha, slot);
+        const T *src = dinvT ? w : r;
+        int nPart = 0;
+        CG_DISPATCH_N(N, nPart = std::min(cg_grid<NN>(nE), kRedBlocks);
+                      hipLaunchKernelGGL((k_cg_direction<NN, T>), dim3(nPart), dim3(256), 0, s, src, vOld, vNew, gid,
+                                         G3T, DT, LbufT, (int)nE, st, slot, it, (int)pending));
+        if (dinvT)
+            hipLaunchKernelGGL((k_cg_assemble_update<T, true>), dim3(kFoldParts), dim3(kFoldThreads), 0, s,
+                               (const T *)LbufT, cs, ci, nG, u, (const T *)vNew, r, dinvT, w, st, slot, it, nPart);
+        else
+            hipLaunchKernelGGL((k_cg_assemble_update<T, false>), dim3(kFoldParts), dim3(kFoldThreads), 0, s,
+                               (const T *)LbufT, cs, ci, nG, u, (const T *)vNew, r, dinvT, w, st, slot, it, nPart);
         SELF_HIP(hipGetLastError());
         return SELF_OK;
     }
 
-    int cg_direction32(const float *w, const float *vOld, float *vNew, float *y, SolverState *st,
-                       const RedSlot &slot, int it, bool pending, hipStream_t s) override
+    int cg_iteration(const double *dinvA, double *u, double *r, double *w, const double *vOld, double *vNew,
+                     SolverState *st, const RedSlot &slot, int it, bool pending, hipStream_t s) override
     {
-        CG_DISPATCH_N(N, hipLaunchKernelGGL((k_cg_local<NN, float, false, true>), dim3(cg_grid<NN>(nE)), dim3(256),
-                                            0, s, w, gid, uD, G3f, Df, Lbuf32, (int)nE, st, vOld, vNew, slot, it,
-                                            (int)pending));
-        hipLaunchKernelGGL((k_cg_assemble<float, 0, false, kFoldThreads>), dim3(kFoldParts), dim3(kFoldThreads), 0, s, Lbuf32, cs,
-                           ci, (const double *)nullptr, nG, y, (const float *)vNew, st, (int)kFinAlpha, slot);
-        SELF_HIP(hipGetLastError());
-        return SELF_OK;
+        return cg_iter<double>(dinvA, u, r, w, vOld, vNew, G3, D, Lbuf, st, slot, it, pending, s);
+    }
+
+    int cg_iteration32(const float *dinvA, float *u, float *r, float *w, const float *vOld, float *vNew,
+                       SolverState *st, const RedSlot &slot, int it, bool pending, hipStream_t s) override
+    {
+        return cg_iter<float>(dinvA, u, r, w, vOld, vNew, G3f, Df, Lbuf32, st, slot, it, pending, s);
     }
 
     int residual(const double *x, double *r, SolverState *st, int fin, const RedSlot &slot, hipStream_t s) override

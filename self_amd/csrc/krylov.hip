@@ -125,47 +125,6 @@ __global__ __launch_bounds__(kRedThreads) void k_update_urw(T *__restrict__ u, c
     reduce_rows<2, FIN>(slot, acc, st, f);
 }
 
-// Folded form of k_update_urw (launched with kFoldParts workgroups): alpha
-// = num / v.z from row 0's partials (each workgroup sums them itself), the
-// same sweep, and r.r, r.w left as partials in rows 1 and 2 for the next
-// direction step or k_cg_fold_end.
-template <typename T, bool PRE>
-__global__ __launch_bounds__(kFoldThreads) void k_update_urw_fold(T *__restrict__ u, const T *__restrict__ v,
-                                                                 T *__restrict__ r, const T *__restrict__ z,
-                                                                 const T *__restrict__ dinv, T *__restrict__ w,
-                                                                 size_t n, SolverState *st, RedSlot slot, int it)
-{
-    if (st_done(st)) return;
-    __shared__ double sAlpha;
-    if (threadIdx.x < 64) {
-        const double vz = wave_fold_sum(slot.partials);
-        if (threadIdx.x == 0) {
-            const double a = st->numP[it & 1] / vz;
-            sAlpha = a;
-            if (blockIdx.x == 0) {
-                st->vz = vz;
-                st->alpha = a;
-            }
-        }
-    }
-    __syncthreads();
-    const T alpha = (T)sAlpha;
-    double acc0 = 0.0, acc1 = 0.0;
-    for (size_t g = (size_t)blockIdx.x * blockDim.x + threadIdx.x; g < n; g += (size_t)gridDim.x * blockDim.x) {
-        u[g] = u[g] + alpha * v[g];
-        const T rv = r[g] - alpha * z[g];
-        r[g] = rv;
-        acc0 += (double)rv * (double)rv;
-        if constexpr (PRE) {
-            const T wv = dinv[g] * rv;
-            w[g] = wv;
-            acc1 += (double)rv * (double)wv;
-        }
-    }
-    if constexpr (!PRE) acc1 = acc0;
-    store_partials2<kFoldThreads>(slot, 1, acc0, 2, acc1);
-}
-
 // end of the last iteration of a chunk (one wavefront)
 __global__ __launch_bounds__(64) void k_cg_fold_end(SolverState *st, RedSlot slot, int it)
 {
@@ -488,48 +447,42 @@ inline int op_matvec(DevOp &op, const float *x, float *y, SolverState *st, const
 {
     return op.matvec32(x, y, x, st, kFinAlpha, slot, s);
 }
-inline int op_cg_direction(DevOp &op, const double *w, const double *vOld, double *vNew, double *y,
-                           SolverState *st, const RedSlot &slot, int it, bool pending, hipStream_t s)
+inline int op_cg_iteration(DevOp &op, const double *dinv, double *u, double *r, double *w, const double *vOld,
+                           double *vNew, SolverState *st, const RedSlot &slot, int it, bool pending, hipStream_t s)
 {
-    return op.cg_direction(w, vOld, vNew, y, st, slot, it, pending, s);
+    return op.cg_iteration(dinv, u, r, w, vOld, vNew, st, slot, it, pending, s);
 }
-inline int op_cg_direction(DevOp &op, const float *w, const float *vOld, float *vNew, float *y, SolverState *st,
-                           const RedSlot &slot, int it, bool pending, hipStream_t s)
+inline int op_cg_iteration(DevOp &op, const float *dinv, float *u, float *r, float *w, const float *vOld,
+                           float *vNew, SolverState *st, const RedSlot &slot, int it, bool pending, hipStream_t s)
 {
-    return op.cg_direction32(w, vOld, vNew, y, st, slot, it, pending, s);
+    return op.cg_iteration32(dinv, u, r, w, vOld, vNew, st, slot, it, pending, s);
 }
 
 // CG iterations it0..it1 (the loop body of the reference's CG/PCG):
 //   v = w + beta v, z = A v, alpha = num / v.z
 //   u += alpha v, r -= alpha z, rNorm, stop test, w = P r, num, beta
-// Operators that gather their input take the folded form: two launches per
-// iteration (direction step, update sweep), the bookkeeping of each
-// iteration's end done by the next direction step, and by k_cg_fold_end
-// after the chunk's last; v alternates between vA (odd iterations) and vB.
-// Other operators: in-place direction update, operator, update sweep, each
-// reduction with its final launch.
+// Operators that gather their input run each iteration folded
+// (DevOp::cg_iteration, no final launches): the bookkeeping of an
+// iteration's end is done by the next iteration's first kernel, and by
+// k_cg_fold_end after the chunk's last; v alternates between vA (odd
+// iterations) and vB.  Other operators: in-place direction update,
+// operator, update sweep, each reduction with its final launch.
 template <typename T>
 int cg_chunk(DevOp &op, T *u, T *r, T *w, T *vA, T *vB, T *z, const T *dinv, size_t n, SolverState *st,
              const RedSlot &slot, int it0, int it1, hipStream_t s)
 {
-    const T *wsrc = dinv ? w : r;
     if (op.fuses_direction()) {
         for (int it = it0; it <= it1; ++it) {
             const T *vOld = (it & 1) ? vB : vA;
             T *v = (it & 1) ? vA : vB;
-            int rc = op_cg_direction(op, wsrc, vOld, v, z, st, slot, it, it > it0, s);
+            int rc = op_cg_iteration(op, dinv, u, r, w, vOld, v, st, slot, it, it > it0, s);
             if (rc) return rc;
-            if (dinv)
-                hipLaunchKernelGGL((k_update_urw_fold<T, true>), dim3(kFoldParts), dim3(kFoldThreads), 0, s, u,
-                                   (const T *)v, r, (const T *)z, dinv, w, n, st, slot, it);
-            else
-                hipLaunchKernelGGL((k_update_urw_fold<T, false>), dim3(kFoldParts), dim3(kFoldThreads), 0, s, u,
-                                   (const T *)v, r, (const T *)z, dinv, w, n, st, slot, it);
         }
         hipLaunchKernelGGL(k_cg_fold_end, dim3(1), dim3(64), 0, s, st, slot, it1);
         SELF_HIP(hipGetLastError());
         return SELF_OK;
     }
+    const T *wsrc = dinv ? w : r;
     const int gv = vgrid(n);
     for (int it = it0; it <= it1; ++it) {
         hipLaunchKernelGGL(k_update_v<T>, dim3(gv), dim3(256), 0, s, wsrc, vA, n, st);

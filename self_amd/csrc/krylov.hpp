@@ -104,10 +104,11 @@ __device__ inline void finalize_scalar(SolverState *st, int kind, double S)
 
 // ---------------------------------------------------- folded CG reductions
 // In the folded CG iteration no reduction has a final launch: the kernel
-// that needs a scalar sums the producer's kFoldParts partials itself, every
-// workgroup in the same fixed order (so every workgroup has the same bits),
-// and workgroup 0 records the scalar bookkeeping in the state.  Partial rows
-// of RedSlot: 0 = v.z, 1 = r.r, 2 = r.w.
+// that needs a scalar sums the producer's partials itself, every workgroup
+// in the same fixed order (so every workgroup has the same bits), and
+// workgroup 0 records the scalar bookkeeping in the state.  Partial rows of
+// RedSlot: 0 = v.Av (one per workgroup of the operator kernel), 1 = r.r,
+// 2 = r.w (kFoldParts each).
 constexpr int kFoldParts = 512;    // workgroups of the producing kernels
 constexpr int kFoldThreads = 1024; // their size
 static_assert(kFoldParts % 64 == 0 && kFoldParts <= kRedBlocks, "fold partials: whole wavefront loads, one row");
@@ -161,20 +162,29 @@ struct DevOp {
     virtual bool has_f32() const { return false; }
     virtual int matvec32(const float *x, float *y, const float *dotWith, SolverState *st, int fin,
                          const RedSlot &slot, hipStream_t s) { return SELF_ERR_STATE; }
-    // The folded CG direction step of iteration it, for operators that
-    // gather their input: when `pending`, first the end of iteration it-1
-    // from the update sweep's partial rows 1 and 2 (cg_end_apply in
-    // workgroup 0; every workgroup exits on the stop test), otherwise beta
-    // from the state; then vNew = w (it == 1) or w + beta vOld formed in the
-    // gather, y = A vNew, and v.y left as kFoldParts partials in row 0.
+    // One folded CG iteration `it` for operators that gather their input
+    // (fuses_direction() true), no final launches:
+    //  - when `pending`, the end of iteration it-1 from partial rows 1 and
+    //    2 (cg_end_apply in workgroup 0; every workgroup exits on the stop
+    //    test), otherwise beta from the state;
+    //  - vNew = w (it == 1) or w + beta vOld formed in the input gather
+    //    (w is r without a preconditioner), the operator, v.Av;
+    //  - alpha = num / v.Av, u += alpha vNew, r -= alpha A vNew, w = P r,
+    //    r.r and r.w as kFoldParts partials in rows 1 and 2.
     // vNew and vOld are distinct buffers (other elements still gather vOld
     // while vNew is written).  Operators without it leave
     // fuses_direction() false and the solver updates v in place.
     virtual bool fuses_direction() const { return false; }
-    virtual int cg_direction(const double *w, const double *vOld, double *vNew, double *y, SolverState *st,
-                             const RedSlot &slot, int it, bool pending, hipStream_t s) { return SELF_ERR_STATE; }
-    virtual int cg_direction32(const float *w, const float *vOld, float *vNew, float *y, SolverState *st,
-                               const RedSlot &slot, int it, bool pending, hipStream_t s) { return SELF_ERR_STATE; }
+    virtual int cg_iteration(const double *dinv, double *u, double *r, double *w, const double *vOld, double *vNew,
+                             SolverState *st, const RedSlot &slot, int it, bool pending, hipStream_t s)
+    {
+        return SELF_ERR_STATE;
+    }
+    virtual int cg_iteration32(const float *dinv, float *u, float *r, float *w, const float *vOld, float *vNew,
+                               SolverState *st, const RedSlot &slot, int it, bool pending, hipStream_t s)
+    {
+        return SELF_ERR_STATE;
+    }
     // r = b - A x (the model's Residual); |r|^2 fused and finalized with fin.
     virtual int residual(const double *x, double *r, SolverState *st, int fin, const RedSlot &slot,
                          hipStream_t s) = 0;

@@ -95,6 +95,25 @@ __device__ __forceinline__ void store_partial1(const RedSlot &slot, int k, doubl
     }
 }
 
+// fixed-order sum of the first `count` entries of a partial row by a whole
+// TPB-thread workgroup, valid in every thread (every workgroup that calls
+// it gets the same bits)
+template <int TPB>
+__device__ __forceinline__ double block_fold_sum(const double *row, int count)
+{
+    constexpr int W = TPB / 64;
+    __shared__ double lw[W];
+    double v = 0.0;
+    for (int b = threadIdx.x; b < count; b += TPB) v += row[b];
+    v = wave_sum(v);
+    if ((threadIdx.x & 63) == 0) lw[threadIdx.x >> 6] = v;
+    __syncthreads();
+    double sum = 0.0;
+    #pragma unroll
+    for (int q = 0; q < W; ++q) sum += lw[q];
+    return sum;
+}
+
 // final kernel (one workgroup): fixed-order sum of row k, valid in thread 0
 __device__ __forceinline__ double final_sum(const RedSlot &slot, int k, double *lds4)
 {
