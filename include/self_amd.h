@@ -423,6 +423,15 @@ int self_cgsem2d_solve(self_cgsem2d_t h, int method, int precond, double tol, in
 int self_cgsem2d_solve_mixed(self_cgsem2d_t h, int precond, double tol, double innerTol, int maxInner,
                              int maxOuter, int *ioerr, int *outerIters, int *innerIters, double *relResidual,
                              double *elapsed_ms);
+/* Mixed precision with reliable updates: one FP32 PCG sequence that is
+ * never restarted; whenever its residual has dropped by delta (0 < delta
+ * < 1) the FP32 iterate is added to the FP64 solution and the residual is
+ * recomputed in FP64, until |b - A x| <= tol |b - A x0|.  maxIters bounds
+ * the FP32 iterations, maxUpdates the refinements; ioerr 0 converged, -1
+ * not.  (No reference counterpart: the reference solves in FP64 only.) */
+int self_cgsem2d_solve_reliable(self_cgsem2d_t h, int precond, double tol, double delta, int maxIters,
+                                int maxUpdates, int *ioerr, int *updates, int *iterations, double *relResidual,
+                                double *elapsed_ms);
 
 /* --------------------------------------------------------------- serving
  * Request loop (no reference counterpart: the reference's drivers run one

@@ -833,4 +833,34 @@ try {
 }
 SELF_CATCH_ALL
 
+int self_cgsem2d_solve_reliable(self_cgsem2d_t h, int precond, double tol, double delta, int maxIters,
+                                int maxUpdates, int *ioerr, int *updates, int *iterations, double *relResidual,
+                                double *elapsed_ms)
+try {
+    SELF_REQUIRE(h && ioerr && updates && iterations, SELF_ERR_ARG, "null argument");
+    SELF_REQUIRE(maxIters >= 1 && maxUpdates >= 1 && tol > 0.0, SELF_ERR_ARG,
+                 "need maxIters >= 1, maxUpdates >= 1 and tol > 0");
+    SELF_HIP(hipSetDevice(h->device));
+    CgsemOp &op = h->op;
+    KrylovResult res;
+    int its = 0, ups = 0;
+    SELF_HIP(hipEventRecord(h->ev0, op.stream));
+    int rc = krylov_solve_reliable(op, h->wk, precond, tol, delta, maxIters, maxUpdates, op.xg, res, its, ups,
+                                   op.stream);
+    if (rc) return rc;
+    SELF_HIP(hipEventRecord(h->ev1, op.stream));
+    SELF_HIP(hipEventSynchronize(h->ev1));
+    if (elapsed_ms) {
+        float ms = 0;
+        SELF_HIP(hipEventElapsedTime(&ms, h->ev0, h->ev1));
+        *elapsed_ms = ms;
+    }
+    *ioerr = res.ioerr;
+    *updates = ups;
+    *iterations = its;
+    if (relResidual) *relResidual = res.r0 > 0 ? res.rNorm / res.r0 : 0.0;
+    return SELF_OK;
+}
+SELF_CATCH_ALL
+
 }  // extern "C"

@@ -343,6 +343,28 @@ __global__ __launch_bounds__(256) void k_fill(T *__restrict__ a, size_t n, T v)
     for (size_t g = (size_t)blockIdx.x * blockDim.x + threadIdx.x; g < n; g += (size_t)gridDim.x * blockDim.x) a[g] = v;
 }
 
+// reliable updates (krylov_solve_reliable): let the FP64 residual kernels run
+__global__ void k_reliable_begin(SolverState *st)
+{
+    if (threadIdx.x == 0) st->done = 0;
+}
+
+// after an update: stop (4 converged, 5 out of updates or iterations) or
+// continue the FP32 sequence at the next iteration, its stop test relative
+// to the new residual
+__global__ void k_reliable_next(SolverState *st, double r0, double scale, double tol, double delta, int maxUpdates)
+{
+    if (threadIdx.x != 0) return;
+    const double rn = st->bnorm;
+    st->l += 1;
+    st->rc = rn;
+    if (rn <= tol * r0) { st->done = 4; return; }
+    if (st->l >= maxUpdates || st->iter >= st->maxIters) { st->done = 5; return; }
+    st->iter += 1;
+    st->r0 = rn / scale;
+    st->tol = fmax(delta, tol * r0 / rn);
+}
+
 inline int vgrid(size_t n) { return (int)std::min<size_t>(4096, (n + 255) / 256 > 0 ? (n + 255) / 256 : 1); }
 
 }  // namespace
@@ -681,6 +703,101 @@ int krylov_solve_ir(DevOp &op, KrylovWork &wk, int precond, double tol, double i
     }
     res.ioerr = -1;
     res.iterations = maxOuter;
+    return SELF_OK;
+}
+
+
+// Mixed precision with reliable updates (the solver of Clark et al.,
+// "Solving lattice QCD systems of equations using mixed precision solvers on
+// GPUs", 2010, applied to PCG): one FP32 PCG sequence on A e = r / |r0| that
+// is never restarted.  Whenever its recursive residual has dropped by
+// `delta` since the last update (or reaches the overall target), the FP32
+// iterate is added to the FP64 solution, the true residual is recomputed in
+// FP64 and replaces the FP32 one, and the iteration continues with the same
+// search direction (beta from the new residual).  Every vector update and
+// operator application runs in FP32; solution, residual and convergence
+// test are FP64.  The update and the decision to continue run on the
+// device (k_reliable_next), so an update costs the host no extra wait.
+// maxIters bounds the FP32 iterations, maxUpdates the refinements.
+// State: done 1 = FP32 segment reached its target, 2 = maxIters,
+// 4 = converged after an update, 5 = maxUpdates reached; l counts updates,
+// rc holds the last true residual norm.
+int krylov_solve_reliable(DevOp &op, KrylovWork &wk, int precond, double tol, double delta, int maxIters,
+                          int maxUpdates, double *x, KrylovResult &res, int &iters, int &updates, hipStream_t s)
+{
+    if (!op.has_f32()) { set_error("operator has no FP32 form"); return SELF_ERR_STATE; }
+    if (!(delta > 0.0 && delta < 1.0)) { set_error("reliable-update factor %g outside (0, 1)", delta); return SELF_ERR_ARG; }
+    const size_t n = op.n;
+    int rc = wk.ensure(n, 0, maxIters + 1, true);
+    if (rc) return rc;
+    const int gv = vgrid(n);
+    const float *dinv = precond ? op.inv_diag32() : nullptr;
+    iters = 0;
+    updates = 0;
+    // FP64 residual of the initial guess; it fixes the FP32 scaling
+    if ((rc = init_state(wk, tol, 1, kStopNormRel, 0, s))) return rc;
+    if ((rc = op.residual(x, wk.r, wk.st, kFinBnorm, wk.slot, s))) return rc;
+    SolverState h{};
+    if ((rc = read_state(wk, h, s))) return rc;
+    const double r0 = h.bnorm, scale = r0;
+    res.r0 = r0;
+    res.rNorm = r0;
+    res.ioerr = 0;
+    res.iterations = 0;
+    if (r0 == 0.0) return SELF_OK;
+    hipLaunchKernelGGL((k_convert<double, float>), dim3(gv), dim3(256), 0, s, wk.r, wk.r32, n, 1.0 / scale);
+    hipLaunchKernelGGL(k_fill<float>, dim3(gv), dim3(256), 0, s, wk.u32, n, 0.0f);
+    // FP32 state: iteration 1, stop test relative to |r32| = 1
+    if ((rc = init_state(wk, std::max(delta, tol), maxIters, kStopNormRel, 0, s))) return rc;
+    SELF_RED_LAUNCH((k_dot<float, false>), (k_dot<float, true>), s, wk.r32, wk.r32, n, wk.st, (int)kFinR0Plain,
+                       wk.slot);
+    SELF_RED_LAUNCH((k_precond_dot<float, false>), (k_precond_dot<float, true>), s, wk.r32, dinv, wk.w32, n, wk.st,
+                       (int)kFinNum, wk.slot);
+    // chunks sized from the last segment's length, so that the host sees a
+    // segment's end soon after it happens (iterations launched after it
+    // are no-ops)
+    int it0 = 1, seg0 = 1, chunk = kChunk;
+    for (;;) {
+        // FP32 iterations until the state reports a stop
+        for (;;) {
+            const int it1 = std::min(it0 + chunk - 1, maxIters);
+            if ((rc = cg_chunk<float>(op, wk.u32, wk.r32, wk.w32, wk.v32, wk.vB32, wk.z32, dinv, n, wk.st, wk.slot,
+                                      it0, it1, s)))
+                return rc;
+            if ((rc = read_state(wk, h, s))) return rc;
+            if (h.done) break;
+            it0 = it1 + 1;
+        }
+        if (h.done == 4 || h.done == 5) break;
+        chunk = std::min(kChunk, std::max(4, h.iter - seg0 + 3));
+        seg0 = h.iter + 1;
+        // done 1 or 2: reliable update, x += scale e and r = b - A x in FP64
+        hipLaunchKernelGGL(k_reliable_begin, dim3(1), dim3(64), 0, s, wk.st);
+        hipLaunchKernelGGL(k_add_scaled32, dim3(gv), dim3(256), 0, s, x, wk.u32, n, scale);
+        if ((rc = op.residual(x, wk.r, wk.st, kFinBnorm, wk.slot, s))) return rc;
+        hipLaunchKernelGGL(k_reliable_next, dim3(1), dim3(64), 0, s, wk.st, r0, scale, tol, delta, maxUpdates);
+        if (h.done == 2) {  // FP32 iterations exhausted: this update is the last
+            if ((rc = read_state(wk, h, s))) return rc;
+            break;
+        }
+        // continue from the true residual with the same direction (no-ops
+        // once k_reliable_next has stopped the solve)
+        hipLaunchKernelGGL((k_convert<double, float>), dim3(gv), dim3(256), 0, s, wk.r, wk.r32, n, 1.0 / scale);
+        hipLaunchKernelGGL(k_fill<float>, dim3(gv), dim3(256), 0, s, wk.u32, n, 0.0f);
+        SELF_RED_LAUNCH((k_precond_dot<float, false>), (k_precond_dot<float, true>), s, wk.r32, dinv, wk.w32, n,
+                           wk.st, (int)kFinNum, wk.slot);
+        it0 = h.iter + 1;
+        if (it0 > maxIters) {  // k_reliable_next stops the solve (done 5)
+            if ((rc = read_state(wk, h, s))) return rc;
+            break;
+        }
+    }
+    SELF_HIP(hipGetLastError());
+    iters = h.iter;
+    updates = h.l;
+    res.iterations = h.l;
+    res.rNorm = h.l > 0 ? h.rc : r0;
+    res.ioerr = (h.done == 4 || res.rNorm <= tol * r0) ? 0 : -1;
     return SELF_OK;
 }
 

@@ -230,4 +230,11 @@ int krylov_solve(DevOp &op, KrylovWork &wk, int method, int precond, double tol,
 int krylov_solve_ir(DevOp &op, KrylovWork &wk, int precond, double tol, double innerTol, int maxInner,
                     int maxOuter, double *sol, KrylovResult &res, int &innerTotal, hipStream_t s);
 
+// Mixed precision with reliable updates: one FP32 PCG sequence, never
+// restarted; the FP32 iterate is folded into the FP64 solution and the
+// residual recomputed in FP64 whenever the FP32 residual has dropped by
+// delta.  res.iterations counts the updates, iters the FP32 iterations.
+int krylov_solve_reliable(DevOp &op, KrylovWork &wk, int precond, double tol, double delta, int maxIters,
+                          int maxUpdates, double *sol, KrylovResult &res, int &iters, int &updates, hipStream_t s);
+
 }  // namespace self

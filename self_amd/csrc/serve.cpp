@@ -177,11 +177,11 @@ int run_cgsem(self_server_s *s, Job &j)
     if (!rc) rc = self_cgsem2d_set_solution(h, zero.data());
     int ioerr = 0, iters = 0;
     if (!rc) {
-        if (r.precision == SELF_MIXED) {
-            int outer = 0;
+        if (r.precision == SELF_MIXED) {  // FP32 PCG with FP64 reliable updates
+            int updates = 0;
             double rel = 0.0;
-            rc = self_cgsem2d_solve_mixed(h, r.precond, r.tol, 1e-5, r.maxIters, 30, &ioerr, &outer, &iters, &rel,
-                                          nullptr);
+            rc = self_cgsem2d_solve_reliable(h, r.precond, r.tol, 0.2, r.maxIters, 400, &ioerr, &updates, &iters,
+                                             &rel, nullptr);
         } else {
             rc = self_cgsem2d_solve(h, SELF_KRYLOV_CGSEM_PCG, r.precond, r.tol, r.maxIters, 30, 0, &ioerr, &iters,
                                     nullptr, 0, nullptr);

@@ -187,6 +187,10 @@ def test_cgsem_solvers_agree_and_converge_spectrally(gpu_lib):
         io3, outer, inner, rel, _ = g.SolveMixed(tolerance=1e-10)
         assert io3 == 0 and rel <= 1e-10
         assert np.abs(g.GetSolution() - u1).max() < 1e-8
+        g.SetSolution(np.zeros_like(ue))
+        io4, updates, its, rel4, _ = g.SolveReliable(tolerance=1e-10)
+        assert io4 == 0 and rel4 <= 1e-10 and updates >= 1
+        assert np.abs(g.GetSolution() - u1).max() < 1e-8
         g.Trash()
     assert errs[0] > 1e3 * errs[2] and errs[2] < 1e-9, errs
 
@@ -204,3 +208,42 @@ def test_cgsem_solve_is_deterministic(gpu_lib):
     assert np.array_equal(r1, g.resi) and np.array_equal(s1, g.GetSolution())
     g.Trash()
     g2.Trash()
+
+
+def test_cgsem_reliable_updates_reach_the_fp64_solution(gpu_lib):
+    """FP32 PCG with FP64 reliable updates: the true FP64 residual (recomputed
+    through the model's Residual, not the solver's own bookkeeping) meets the
+    tolerance, the solution is the FP64 Jacobi-PCG solution (the device
+    solver pinned against the oracle above), the FP32 sequence needs about as
+    many iterations as FP64 PCG (it is never restarted), and a repeat solve
+    is bitwise the same.  Inhomogeneous Dirichlet values."""
+    from self_amd import _lib as L
+    _, g, shape = _cgsem_pair(N=5, nX=8, nY=6, dirichlet=0.2)
+    io_o, it_o, _ = g.Solve("cgsem_pcg", precond="jacobi", tolerance=1e-12, maxIters=4000)
+    assert io_o == -2
+    ref = g.GetSolution()
+    g.SetSolution(np.zeros(shape))
+    r0 = np.linalg.norm(g.Residual())
+    io, updates, its, rel, _ = g.SolveReliable(tolerance=1e-12, delta=0.2)
+    assert io == 0 and updates >= 5
+    true = np.linalg.norm(g.Residual()) / r0
+    assert true <= 1e-12 and abs(true - rel) <= 1e-3 * rel
+    sol = g.GetSolution()
+    assert rel_l2(sol.ravel(), ref.ravel()) < 1e-9
+    assert its <= 1.3 * it_o, (its, it_o)
+    g.SetSolution(np.zeros(shape))
+    io2, updates2, its2, rel2, _ = g.SolveReliable(tolerance=1e-12, delta=0.2)
+    assert (io2, updates2, its2, rel2) == (io, updates, its, rel)
+    assert np.array_equal(g.GetSolution(), sol)
+    # out of FP32 iterations: not converged, the update made at the end still counts
+    g.SetSolution(np.zeros(shape))
+    io3, updates3, its3, rel3, _ = g.SolveReliable(tolerance=1e-12, delta=0.2, maxIters=40)
+    assert io3 == -1 and its3 == 40 and rel3 > 1e-12
+    assert abs(np.linalg.norm(g.Residual()) / r0 - rel3) <= 1e-3 * rel3
+    # out of updates
+    g.SetSolution(np.zeros(shape))
+    io4, updates4, _, rel4, _ = g.SolveReliable(tolerance=1e-12, delta=0.2, maxUpdates=2)
+    assert io4 == -1 and updates4 == 2 and rel4 > 1e-12
+    with pytest.raises(L.SelfError):
+        g.SolveReliable(delta=1.0)
+    g.Trash()
