@@ -298,18 +298,18 @@ struct CgsemOp final : DevOp {
     std::vector<int> gidH, csH, ciH;
     std::vector<double> wH, DH;
     std::vector<double> met[5];  // dxds, dxdp, dyds, dydp, J (local)
-    std::vector<double> kH;      // flux coefficient (local)
     std::vector<int> dirH;       // 1 at Dirichlet local nodes
     // device
     int *gid = nullptr, *cs = nullptr, *ci = nullptr;
     double *G3 = nullptr, *D = nullptr, *bl = nullptr, *uD = nullptr, *dinv = nullptr, *Lbuf = nullptr;
+    double *metD = nullptr, *kD = nullptr, *wD = nullptr;  // metric terms (5 x local), flux coefficient, weights
     float *G3f = nullptr, *Df = nullptr, *dinvf = nullptr, *Lbuf32 = nullptr;
     double *xg = nullptr;  // global solution vector
     size_t bytes = 0;
 
     ~CgsemOp() override
     {
-        void *p[] = {gid, cs, ci, G3, D, bl, uD, dinv, Lbuf, G3f, Df, dinvf, Lbuf32, xg};
+        void *p[] = {gid, cs, ci, G3, D, bl, uD, dinv, Lbuf, G3f, Df, dinvf, Lbuf32, xg, metD, kD, wD};
         for (void *q : p)
             if (q) (void)hipFree(q);
         if (stream) (void)hipStreamDestroy(stream);
@@ -423,8 +423,10 @@ int build_maps(CgsemOp &op, int nEdges, int nNodes, const int *edges, const int 
 {
     const int N = op.N, n1 = op.n1, np = op.np;
     const size_t nE = op.nE;
-    std::vector<std::vector<int>> copies;  // per global DOF: local ids in summation order
-    std::vector<int> owner(nE * np, -1);   // local id of the owner copy, or -1 (Dirichlet)
+    // per DOF (creation order): its local copies in summation order, CSR
+    std::vector<int> cpStart{0}, cpList;
+    cpStart.reserve(nE * np + 1);
+    cpList.reserve(nE * np);
     op.dirH.assign(nE * np, 0);
     // node -> element list (increasing element order) and Dirichlet nodes
     std::vector<std::vector<std::pair<int, int>>> nodeList(nNodes);
@@ -451,12 +453,10 @@ int build_maps(CgsemOp &op, int nEdges, int nNodes, const int *edges, const int 
             dirNode[elemNodes[4 * e1 + kEdgeMapC[1][s1 - 1] - 1] - 1] = 1;
         }
     }
-    // interior nodes
-    std::vector<int> dofOfLocal(nE * np, -1);
-    std::vector<int> ownerOfDof;
-    auto new_dof = [&](std::vector<int> &&cp) {
-        const int d = (int)copies.size();
-        copies.push_back(std::move(cp));
+    auto new_dof = [&](std::initializer_list<int> cp) {
+        const int d = (int)cpStart.size() - 1;
+        for (int l : cp) cpList.push_back(l);
+        cpStart.push_back((int)cpList.size());
         return d;
     };
     // assign DOF identities (not yet numbered): edges
@@ -499,31 +499,37 @@ int build_maps(CgsemOp &op, int nEdges, int nNodes, const int *edges, const int 
                 op.dirH[pe.first * np + kCornerI[pe.second] * N + n1 * (kCornerJ[pe.second] * N)] = 1;
             continue;
         }
-        std::vector<int> cp;
-        for (auto &pe : nodeList[nd]) cp.push_back(pe.first * np + kCornerI[pe.second] * N + n1 * (kCornerJ[pe.second] * N));
-        const int d = new_dof(std::move(cp));
-        for (int l : copies[d]) dofKey[l] = d;
+        const int d = (int)cpStart.size() - 1;
+        for (auto &pe : nodeList[nd]) {
+            const int l = pe.first * np + kCornerI[pe.second] * N + n1 * (kCornerJ[pe.second] * N);
+            cpList.push_back(l);
+            dofKey[l] = d;
+        }
+        cpStart.push_back((int)cpList.size());
     }
     for (size_t l = 0; l < nE * (size_t)np; ++l)
         SELF_REQUIRE(dofKey[l] >= 0 || op.dirH[l], SELF_ERR_ARG,
                      "local node %zu of element %zu is on no edge (mesh tables incomplete)", l % np, l / np + 1);
     // number DOFs in the order their owner copies appear, element by element
-    std::vector<int> number(copies.size(), -1);
+    const size_t nDof = cpStart.size() - 1;
+    std::vector<int> number(nDof, -1);
     int next = 0;
     for (size_t l = 0; l < nE * (size_t)np; ++l) {
         const int d = dofKey[l];
-        if (d >= 0 && copies[d][0] == (int)l) number[d] = next++;
+        if (d >= 0 && cpList[cpStart[d]] == (int)l) number[d] = next++;
     }
     op.nG = (size_t)next;
     op.gidH.assign(nE * np, -1);
     for (size_t l = 0; l < nE * (size_t)np; ++l)
         if (dofKey[l] >= 0) op.gidH[l] = number[dofKey[l]];
-    std::vector<int> order(copies.size());
-    for (size_t d = 0; d < copies.size(); ++d) order[number[d]] = (int)d;
+    std::vector<int> order(nDof);
+    for (size_t d = 0; d < nDof; ++d) order[number[d]] = (int)d;
     op.csH.assign(op.nG + 1, 0);
     op.ciH.clear();
+    op.ciH.reserve(cpList.size());
     for (size_t g = 0; g < op.nG; ++g) {
-        for (int l : copies[order[g]]) op.ciH.push_back(l);
+        const int d = order[g];
+        for (int c = cpStart[d]; c < cpStart[d + 1]; ++c) op.ciH.push_back(cpList[c]);
         op.csH[g + 1] = (int)op.ciH.size();
     }
     return SELF_OK;
@@ -544,53 +550,87 @@ struct self_cgsem2d_s {
 
 namespace {
 
-int upload_geometry(self_cgsem2d_s *h)
+// Geometric factors, element diagonal and Jacobi inverse, on the device,
+// with the rounding of the plain expressions (no contraction):
+//   c = k ws wp / J, G11 = c (dydp^2 + dxdp^2), G12 = -c (dyds dydp + dxds dxdp),
+//   G22 = c (dxds^2 + dyds^2);
+//   A(l,l) = -( sum_m G11(m,j) D(m,i)^2 + 2 G12(i,j) D(i,i) D(j,j) + sum_m G22(i,m) D(m,j)^2 ),
+//   summed over the copies of each global DOF in GlobalSum order.
+__global__ __launch_bounds__(256) void k_cg_geometry(const double *__restrict__ met, const double *__restrict__ k,
+                                                     const double *__restrict__ w, int n1, size_t nl,
+                                                     double *__restrict__ G3, float *__restrict__ G3f)
 {
-    CgsemOp &op = h->op;
-    const int n1 = op.n1, np = op.np;
-    const size_t nl = op.nLocal;
-    std::vector<double> G(3 * nl);
-    std::vector<float> Gf(3 * nl);
-    std::vector<double> diagLocal(nl);
-    for (size_t e = 0; e < op.nE; ++e)
-        for (int j = 0; j < n1; ++j)
-            for (int i = 0; i < n1; ++i) {
-                const size_t l = e * np + i + n1 * j;
-                const double dxds = op.met[0][l], dxdp = op.met[1][l], dyds = op.met[2][l], dydp = op.met[3][l];
-                const double J = op.met[4][l], k = op.kH[l];
-                const double c = k * op.wH[i] * op.wH[j] / J;
-                G[l] = c * (dydp * dydp + dxdp * dxdp);
-                G[nl + l] = -c * (dyds * dydp + dxds * dxdp);
-                G[2 * nl + l] = c * (dxds * dxds + dyds * dyds);
-            }
-    for (size_t q = 0; q < 3 * nl; ++q) Gf[q] = (float)G[q];
-    // element diagonal of the operator:
-    //  A(l,l) = -( sum_m G11(m,j) D(m,i)^2 + 2 G12(i,j) D(i,i) D(j,j) + sum_m G22(i,m) D(m,j)^2 )
-    const double *Dm = op.DH.data();
-    for (size_t e = 0; e < op.nE; ++e)
-        for (int j = 0; j < n1; ++j)
-            for (int i = 0; i < n1; ++i) {
-                double a = 0.0, b = 0.0;
-                for (int m = 0; m < n1; ++m) {
-                    const double dmi = Dm[m + n1 * i], dmj = Dm[m + n1 * j];
-                    a += G[e * np + m + n1 * j] * dmi * dmi;
-                    b += G[2 * nl + e * np + i + n1 * m] * dmj * dmj;
-                }
-                const size_t l = e * np + i + n1 * j;
-                diagLocal[l] = -(a + 2.0 * G[nl + l] * Dm[i + n1 * i] * Dm[j + n1 * j] + b);
-            }
-    std::vector<double> dinv(op.nG);
-    std::vector<float> dinvf(op.nG);
-    for (size_t g = 0; g < op.nG; ++g) {
-        double sum = diagLocal[op.ciH[op.csH[g]]];
-        for (int c = op.csH[g] + 1; c < op.csH[g + 1]; ++c) sum += diagLocal[op.ciH[c]];
-        dinv[g] = 1.0 / sum;
-        dinvf[g] = (float)dinv[g];
+#pragma clang fp contract(off)
+    const int np = n1 * n1;
+    for (size_t l = (size_t)blockIdx.x * blockDim.x + threadIdx.x; l < nl; l += (size_t)gridDim.x * blockDim.x) {
+        const int nd = (int)(l % np), i = nd % n1, j = nd / n1;
+        const double dxds = met[l], dxdp = met[nl + l], dyds = met[2 * nl + l], dydp = met[3 * nl + l];
+        const double J = met[4 * nl + l];
+        const double c = k[l] * w[i] * w[j] / J;
+        const double g11 = c * (dydp * dydp + dxdp * dxdp);
+        const double g12 = -c * (dyds * dydp + dxds * dxdp);
+        const double g22 = c * (dxds * dxds + dyds * dyds);
+        G3[l] = g11;
+        G3[nl + l] = g12;
+        G3[2 * nl + l] = g22;
+        G3f[l] = (float)g11;
+        G3f[nl + l] = (float)g12;
+        G3f[2 * nl + l] = (float)g22;
     }
-    SELF_HIP(hipMemcpy(op.G3, G.data(), sizeof(double) * G.size(), hipMemcpyHostToDevice));
-    SELF_HIP(hipMemcpy(op.G3f, Gf.data(), sizeof(float) * Gf.size(), hipMemcpyHostToDevice));
-    SELF_HIP(hipMemcpy(op.dinv, dinv.data(), sizeof(double) * op.nG, hipMemcpyHostToDevice));
-    SELF_HIP(hipMemcpy(op.dinvf, dinvf.data(), sizeof(float) * op.nG, hipMemcpyHostToDevice));
+}
+
+__global__ __launch_bounds__(256) void k_cg_diagonal(const double *__restrict__ G3, const double *__restrict__ D,
+                                                     int n1, size_t nl, double *__restrict__ diag)
+{
+#pragma clang fp contract(off)
+    const int np = n1 * n1;
+    for (size_t l = (size_t)blockIdx.x * blockDim.x + threadIdx.x; l < nl; l += (size_t)gridDim.x * blockDim.x) {
+        const size_t e0 = l - l % np;
+        const int nd = (int)(l % np), i = nd % n1, j = nd / n1;
+        double a = 0.0, b = 0.0;
+        for (int m = 0; m < n1; ++m) {
+            const double dmi = D[m + n1 * i], dmj = D[m + n1 * j];
+            a += G3[e0 + m + n1 * j] * dmi * dmi;
+            b += G3[2 * nl + e0 + i + n1 * m] * dmj * dmj;
+        }
+        diag[l] = -(a + 2.0 * G3[nl + l] * D[i + n1 * i] * D[j + n1 * j] + b);
+    }
+}
+
+__global__ __launch_bounds__(256) void k_cg_jacobi_inverse(const double *__restrict__ diag, const int *__restrict__ cs,
+                                                           const int *__restrict__ ci, size_t nG,
+                                                           double *__restrict__ dinv, float *__restrict__ dinvf)
+{
+#pragma clang fp contract(off)
+    for (size_t g = (size_t)blockIdx.x * blockDim.x + threadIdx.x; g < nG; g += (size_t)gridDim.x * blockDim.x) {
+        double sum = diag[ci[cs[g]]];
+        for (int c = cs[g] + 1; c < cs[g + 1]; ++c) sum += diag[ci[c]];
+        const double v = 1.0 / sum;
+        dinv[g] = v;
+        dinvf[g] = (float)v;
+    }
+}
+
+__global__ __launch_bounds__(256) void k_fill_f64(double *__restrict__ a, size_t n, double v)
+{
+    for (size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x; q < n; q += (size_t)gridDim.x * blockDim.x) a[q] = v;
+}
+
+// G3 and the Jacobi inverse from the device copies of the metric terms and
+// flux coefficient (the element diagonal goes through Lbuf, free here)
+int device_geometry(CgsemOp &op)
+{
+    const size_t nl = op.nLocal;
+    const int grid = (int)std::min<size_t>(8192, (nl + 255) / 256);
+    hipLaunchKernelGGL(k_cg_geometry, dim3(grid), dim3(256), 0, op.stream, (const double *)op.metD,
+                       (const double *)op.kD, (const double *)op.wD, op.n1, nl, op.G3, op.G3f);
+    hipLaunchKernelGGL(k_cg_diagonal, dim3(grid), dim3(256), 0, op.stream, (const double *)op.G3,
+                       (const double *)op.D, op.n1, nl, op.Lbuf);
+    const int gridG = (int)std::min<size_t>(8192, (op.nG + 255) / 256);
+    hipLaunchKernelGGL(k_cg_jacobi_inverse, dim3(gridG), dim3(256), 0, op.stream, (const double *)op.Lbuf,
+                       (const int *)op.cs, (const int *)op.ci, op.nG, op.dinv, op.dinvf);
+    SELF_HIP(hipGetLastError());
+    SELF_HIP(hipStreamSynchronize(op.stream));
     return SELF_OK;
 }
 
@@ -628,13 +668,13 @@ try {
     if (rc) return fail(rc);
     const double *src[5] = {dxds, dxdp, dyds, dydp, J};
     for (int k = 0; k < 5; ++k) op.met[k].assign(src[k], src[k] + op.nLocal);
-    op.kH.assign(op.nLocal, 1.0);  // Build_CGsemElliptic_2D: fluxCoeff = ONE
     if ((rc = build_maps(op, nEdges, nNodes, edges, elemNodes))) return fail(rc);
     if ((rc = op.dev(&op.gid, op.nLocal)) || (rc = op.dev(&op.cs, op.nG + 1)) || (rc = op.dev(&op.ci, op.ciH.size())) ||
         (rc = op.dev(&op.G3, 3 * op.nLocal)) || (rc = op.dev(&op.G3f, 3 * op.nLocal)) ||
         (rc = op.dev(&op.D, op.np)) || (rc = op.dev(&op.Df, op.np)) || (rc = op.dev(&op.bl, op.nLocal)) ||
         (rc = op.dev(&op.uD, op.nLocal)) || (rc = op.dev(&op.dinv, op.nG)) || (rc = op.dev(&op.dinvf, op.nG)) ||
-        (rc = op.dev(&op.Lbuf, op.nLocal)) || (rc = op.dev(&op.Lbuf32, op.nLocal)) || (rc = op.dev(&op.xg, op.nG)))
+        (rc = op.dev(&op.Lbuf, op.nLocal)) || (rc = op.dev(&op.Lbuf32, op.nLocal)) || (rc = op.dev(&op.xg, op.nG)) ||
+        (rc = op.dev(&op.metD, 5 * op.nLocal)) || (rc = op.dev(&op.kD, op.nLocal)) || (rc = op.dev(&op.wD, op.n1)))
         return fail(rc);
     op.n = op.nG;
     std::vector<float> Df(op.np);
@@ -647,10 +687,16 @@ try {
     if (he == hipSuccess) he = hipMemset(op.bl, 0, sizeof(double) * op.nLocal);
     if (he == hipSuccess) he = hipMemset(op.uD, 0, sizeof(double) * op.nLocal);
     if (he == hipSuccess) he = hipMemset(op.xg, 0, sizeof(double) * op.nG);
+    for (int k = 0; k < 5 && he == hipSuccess; ++k)
+        he = hipMemcpy(op.metD + k * op.nLocal, src[k], sizeof(double) * op.nLocal, hipMemcpyHostToDevice);
+    if (he == hipSuccess) he = hipMemcpy(op.wD, op.wH.data(), sizeof(double) * op.n1, hipMemcpyHostToDevice);
     if (he == hipSuccess) he = hipEventCreate(&h->ev0);
     if (he == hipSuccess) he = hipEventCreate(&h->ev1);
     if (he != hipSuccess) return fail(hip_fail(he, "cgsem2d upload", __FILE__, __LINE__));
-    if ((rc = upload_geometry(h))) return fail(rc);
+    // Build_CGsemElliptic_2D: fluxCoeff = ONE
+    hipLaunchKernelGGL(k_fill_f64, dim3((int)std::min<size_t>(8192, (op.nLocal + 255) / 256)), dim3(256), 0,
+                       op.stream, op.kD, op.nLocal, 1.0);
+    if ((rc = device_geometry(op))) return fail(rc);
     *out = h;
     return SELF_OK;
 }
@@ -679,8 +725,8 @@ int self_cgsem2d_set_flux_coefficient(self_cgsem2d_t h, const double *k)
 try {
     SELF_REQUIRE(h && k, SELF_ERR_ARG, "null argument");
     SELF_HIP(hipSetDevice(h->device));
-    h->op.kH.assign(k, k + h->op.nLocal);
-    return upload_geometry(h);
+    SELF_HIP(hipMemcpy(h->op.kD, k, sizeof(double) * h->op.nLocal, hipMemcpyHostToDevice));
+    return device_geometry(h->op);
 }
 SELF_CATCH_ALL
 
