@@ -5,11 +5,14 @@
     python -m self_amd.profile_run adv2d --steps 10
     python -m self_amd.profile_run adv3d --steps 5
     python -m self_amd.profile_run cgsem --nx 128 --precond 1
+    python -m self_amd.profile_run cgsem --precision mixed --steps 2000
 
 `copy` runs the STREAM-style copy kernel on 64 MiB (21 launches) and 1 GiB
 (11 launches): the calibration for hardware byte counters (FETCH_SIZE /
 WRITE_SIZE of a copy of known size).  Every other mode builds the named
-workload with its seeded inputs and runs the HIP path only.
+workload with its seeded inputs and runs the HIP path only.  For cgsem,
+--precision fp64 runs the reference's PCG loop and mixed the FP32 PCG with
+FP64 reliable updates, each for at most --steps iterations.
 """
 from __future__ import annotations
 
@@ -57,8 +60,12 @@ def main(argv=None):
     elif a.mode == "cgsem":
         from .workloads import cgsem_helmholtz
         m, _ = cgsem_helmholtz(nX=a.nx or 128, N=a.N)
-        r = m.Solve("cgsem_pcg", precond=("none", "jacobi")[a.precond], tolerance=1e-10, maxIters=a.steps)
-        print(f"cgsem: {r}", flush=True)
+        pc = ("none", "jacobi")[a.precond]
+        if a.precision == "mixed":
+            r = m.SolveReliable(tolerance=1e-10, delta=0.2, maxIters=a.steps, precond=pc)
+        else:
+            r = m.Solve("cgsem_pcg", precond=pc, tolerance=1e-10, maxIters=a.steps)
+        print(f"cgsem {a.precision}: {r}", flush=True)
     return 0
 
 
