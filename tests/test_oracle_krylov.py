@@ -122,3 +122,37 @@ def test_planted_mistake_is_caught_by_dense_check():
         v = rn + (rn @ rn) / (r @ r) * v
         r = rn
     assert np.linalg.norm(x - np.linalg.solve(A, b)) > 1e-3
+
+
+def test_oracle_cgsem_first_solve_quirk():
+    """CGsemElliptic_2D Solve (:1155-1156) takes its first residual from the
+    stored solution; on a fresh model its Dirichlet nodes are still zero, so
+    the first solve is the homogeneous problem with the Dirichlet values
+    pasted afterwards (:1205), and only a second solve includes them.  With
+    zero Dirichlet data both solves agree."""
+    import math
+    N, nX, nY = 4, 4, 3
+    def model(d):
+        o = po.CGSEM2D(N, nX, nY, 1.0, 1.5, 0.0, tolerance=1e-12, maxIters=2000)
+        x, y = o.mesh.x, o.mesh.y
+        o.fluxCoeff[:] = 1.0 + 0.5 * np.sin(2 * math.pi * x) * np.sin(2 * math.pi * y)
+        o.source[:] = np.sin(math.pi * x) * np.sin(2 * math.pi * y / 1.5) + 0.3 * x * y
+        o.s.dirichletValue = d
+        return o
+    o0 = model(0.0)
+    o0.solve(1)
+    o = model(0.2)
+    o.solve(1)
+    first = o.solution.copy()
+    o.solve(1)
+    second = o.solution.copy()
+    inner = np.abs(o0.solution) > 0  # interior nodes of the homogeneous solve
+    # first solve: interior equals the homogeneous solution, boundary holds 0.2
+    assert np.allclose(first[inner], o0.solution[inner], rtol=0, atol=1e-12)
+    assert np.isclose(first.max(), 0.2)
+    # second solve: the boundary data enter the interior (u ~ 0.2 + the homogeneous part)
+    assert np.abs(second[inner] - first[inner]).min() > 0.1
+    o2 = model(0.0)
+    o2.solve(1)
+    o2.solve(1)  # starts from the converged solution: the same to solver tolerance
+    assert np.abs(o2.solution - o0.solution).max() <= 1e-9 * np.abs(o0.solution).max()
