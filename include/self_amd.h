@@ -231,6 +231,15 @@ int self_sw2d_synchronize(self_sw2d_t h);
  * waves), 2 (each workgroup prefetches its second group while computing the
  * first), 0 = default (1).  No reference counterpart. */
 int self_sw2d_set_groups(self_sw2d_t h, int groups);
+/* Ensembles (no reference counterpart: the reference runs one state per
+ * model object).  `members` states on the same mesh, nodal storage,
+ * bathymetry, vorticity, relaxation and time step advance together, one
+ * fused launch per RK stage for all of them (fused form only).  Setting
+ * the count reallocates the states (zeroed); self_sw2d_set_field /
+ * get_field (SOLUTION) address member 0. */
+int self_sw2d_set_members(self_sw2d_t h, int members);
+int self_sw2d_set_member_solution(self_sw2d_t h, int member, const double *data);
+int self_sw2d_get_member_solution(self_sw2d_t h, int member, double *data);
 /* Autotuner: times nSteps RK3 steps with each setting on a copy of the
  * current state (the state is restored), keeps the faster one, returns it
  * and its time per step. */

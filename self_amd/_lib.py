@@ -180,6 +180,9 @@ _SIGS = {
     "self_coarse_to_fine_3d": ([C.c_int, C.c_size_t, dp, C.c_int, dp, dp], C.c_int),
     "self_coarse_to_fine_3d_dev": ([C.c_int, C.c_size_t, dp, C.c_int, dp, dp, C.c_void_p], C.c_int),
     "self_sw2d_set_groups": ([C.c_void_p, C.c_int], C.c_int),
+    "self_sw2d_set_members": ([C.c_void_p, C.c_int], C.c_int),
+    "self_sw2d_set_member_solution": ([C.c_void_p, C.c_int, dp], C.c_int),
+    "self_sw2d_get_member_solution": ([C.c_void_p, C.c_int, dp], C.c_int),
     "self_sw2d_autotune": ([C.c_void_p, C.c_int, ip, dp], C.c_int),
     "self_serve_create": ([C.c_int, C.c_double, C.POINTER(C.c_void_p)], C.c_int),
     "self_serve_submit": ([C.c_void_p, C.POINTER(Request), C.POINTER(C.c_longlong)], C.c_int),

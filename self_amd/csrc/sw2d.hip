@@ -42,6 +42,7 @@ constexpr double kRkG[3] = {1.0 / 3.0, 15.0 / 16.0, 8.0 / 15.0};
 
 struct SWKernelArgs {
     int nE;
+    int members;               // ensemble members (fused form): q, G and traces per member
     double g, drag, a, gdt;
     int hasRelax;
     // nodal storage
@@ -166,6 +167,8 @@ __global__ __launch_bounds__(256) void k_sw_traces_push(const double *__restrict
     const int t = threadIdx.x, le = t / np, nd = t - le * np;
     const int e = blockIdx.x * epb + le;
     const bool act = le < epb && e < nE;
+    q += (size_t)blockIdx.y * 3 * nE * np;  // ensemble member
+    out += (size_t)blockIdx.y * nE * 12 * n1;
     double lW[n1], lE[n1], lS[n1], lN[n1];
     load_lag<N>(lag, lW, lE, lS, lN);
     if (act)
@@ -672,11 +675,27 @@ __device__ __forceinline__ void sw_process(const SWKernelArgs &A, const NodalLDS
 // second group's loads before the first group computes, so its memory
 // traffic overlaps the first group's arithmetic (the stage is otherwise
 // one round of waves that all load, then all compute).
+//
+// Ensembles: the grid holds `members` workgroups per element group, element
+// group major, so the members of one group run side by side on one XCD and
+// share its geometry in L2; each member has its own q, G and traces.
 template <int N, bool MIXED, int G>
-__global__ __launch_bounds__(256, 4 / G) void k_sw_stage_fused(SWKernelArgs A)
+__global__ __launch_bounds__(256, 4 / G) void k_sw_stage_fused(SWKernelArgs A0)
 {
     using S = Shape<N>;
     constexpr int np = S::np, epb = S::epb, n1 = S::n1;
+    int wg = xcd_block(blockIdx.x, gridDim.x);
+    SWKernelArgs A = A0;
+    if (A0.members > 1) {  // uniform: a single state skips the member arithmetic
+        const int mbr = wg % A0.members;
+        wg /= A0.members;
+        const size_t qOff = (size_t)mbr * 3 * A0.nE * np, tOff = (size_t)mbr * A0.nE * 12 * n1;
+        A.q += qOff;
+        if (MIXED) A.G32 += qOff;
+        else A.G64 += qOff;
+        A.trIn += tOff;
+        A.trOut += tOff;
+    }
     // WL: each element lies inside one wavefront, so the phases synchronize
     // per wavefront and the wavefronts of a workgroup never wait for each
     // other (one that has its data computes while the others still load).
@@ -688,7 +707,6 @@ __global__ __launch_bounds__(256, 4 / G) void k_sw_stage_fused(SWKernelArgs A)
     __shared__ double qs[epb * 3 * np], sC[epb * 3 * np], pC[epb * 3 * np], bfs[epb * 12 * n1];
     __shared__ float sD[MIXED ? epb * 3 * np : 1], pD[MIXED ? epb * 3 * np : 1];
     const int t = threadIdx.x, le = t / np, nd = t - le * np;
-    const int wg = xcd_block(blockIdx.x, gridDim.x);
     SWPre P[G];
     int e[G];
     bool act[G];
@@ -802,6 +820,7 @@ struct self_sw2d_s {
     int trCur = 0;
     bool trValid = false, nodalSet = false, geomSet = false, hasRelax = false;
     int groups = 0;  // element groups per workgroup of the fused kernel (0: heuristic)
+    int members = 1; // ensemble members (fused form)
     size_t bytes = 0;
 };
 
@@ -830,6 +849,7 @@ SWKernelArgs make_args(self_sw2d_s *h)
     SWKernelArgs A{};
     const int n1 = h->n1;
     A.nE = (int)h->nE;
+    A.members = h->members;
     A.g = h->d.g;
     A.drag = h->d.linearDrag;
     A.hasRelax = h->hasRelax ? 1 : 0;
@@ -913,8 +933,8 @@ int launch_traces(self_sw2d_s *h, const double *src, double *dst, int nf)
 int launch_traces_push(self_sw2d_s *h, double *dst)
 {
     const double *lag = h->nodal + 2 * h->n1 * h->n1 + 2 * h->n1;
-    SELF_DISPATCH_N(h->d.N, hipLaunchKernelGGL(k_sw_traces_push<NN>, dim3(grid_for<NN>(h->nE)), dim3(256), 0,
-                                               h->stream, h->q, lag, h->side, dst, (int)h->nE));
+    SELF_DISPATCH_N(h->d.N, hipLaunchKernelGGL(k_sw_traces_push<NN>, dim3(grid_for<NN>(h->nE), h->members),
+                                               dim3(256), 0, h->stream, h->q, lag, h->side, dst, (int)h->nE));
     SELF_HIP(hipGetLastError());
     return SELF_OK;
 }
@@ -959,6 +979,20 @@ int step_plain(self_sw2d_s *h)
     return SELF_OK;
 }
 
+// one fused stage for every ensemble member: grid_fused x members workgroups
+int launch_stage_fused(self_sw2d_s *h, const SWKernelArgs &A)
+{
+    const bool two = fused_groups(h->groups, h->nE, 0) == 2;
+    const unsigned M = (unsigned)h->members;
+    if (h->d.precision == SELF_MIXED) {
+        SELF_DISPATCH_N(h->d.N, if (two) { hipLaunchKernelGGL((k_sw_stage_fused<NN, true, 2>), dim3(grid_fused<NN>(h->nE, 2) * M), dim3(256), 0, h->stream, A); } else { hipLaunchKernelGGL((k_sw_stage_fused<NN, true, 1>), dim3(grid_fused<NN>(h->nE, 1) * M), dim3(256), 0, h->stream, A); });
+    } else {
+        SELF_DISPATCH_N(h->d.N, if (two) { hipLaunchKernelGGL((k_sw_stage_fused<NN, false, 2>), dim3(grid_fused<NN>(h->nE, 2) * M), dim3(256), 0, h->stream, A); } else { hipLaunchKernelGGL((k_sw_stage_fused<NN, false, 1>), dim3(grid_fused<NN>(h->nE, 1) * M), dim3(256), 0, h->stream, A); });
+    }
+    SELF_HIP(hipGetLastError());
+    return SELF_OK;
+}
+
 int step_fused(self_sw2d_s *h)
 {
     if (!h->trValid) {
@@ -972,12 +1006,7 @@ int step_fused(self_sw2d_s *h)
         A.gdt = kRkG[m] * h->d.dt;
         A.trIn = h->tr[h->trCur];
         A.trOut = h->tr[1 - h->trCur];
-        if (h->d.precision == SELF_MIXED) {
-            SELF_DISPATCH_N(h->d.N, if (fused_groups(h->groups, h->nE, Shape<NN>::epb) == 2) { hipLaunchKernelGGL((k_sw_stage_fused<NN, true, 2>), dim3(grid_fused<NN>(h->nE, 2)), dim3(256), 0, h->stream, A); } else { hipLaunchKernelGGL((k_sw_stage_fused<NN, true, 1>), dim3(grid_fused<NN>(h->nE, 1)), dim3(256), 0, h->stream, A); });
-        } else {
-            SELF_DISPATCH_N(h->d.N, if (fused_groups(h->groups, h->nE, Shape<NN>::epb) == 2) { hipLaunchKernelGGL((k_sw_stage_fused<NN, false, 2>), dim3(grid_fused<NN>(h->nE, 2)), dim3(256), 0, h->stream, A); } else { hipLaunchKernelGGL((k_sw_stage_fused<NN, false, 1>), dim3(grid_fused<NN>(h->nE, 1)), dim3(256), 0, h->stream, A); });
-        }
-        SELF_HIP(hipGetLastError());
+        if (int rc = launch_stage_fused(h, A)) return rc;
         h->trCur = 1 - h->trCur;
     }
     return SELF_OK;
@@ -1416,12 +1445,7 @@ try {
                 A.trIn = h->tr[h->trCur];
                 A.trOut = h->tr[1 - h->trCur];
                 SELF_HIP(mark(0));
-                if (h->d.precision == SELF_MIXED) {
-                    SELF_DISPATCH_N(h->d.N, if (fused_groups(h->groups, h->nE, Shape<NN>::epb) == 2) { hipLaunchKernelGGL((k_sw_stage_fused<NN, true, 2>), dim3(grid_fused<NN>(h->nE, 2)), dim3(256), 0, h->stream, A); } else { hipLaunchKernelGGL((k_sw_stage_fused<NN, true, 1>), dim3(grid_fused<NN>(h->nE, 1)), dim3(256), 0, h->stream, A); });
-                } else {
-                    SELF_DISPATCH_N(h->d.N, if (fused_groups(h->groups, h->nE, Shape<NN>::epb) == 2) { hipLaunchKernelGGL((k_sw_stage_fused<NN, false, 2>), dim3(grid_fused<NN>(h->nE, 2)), dim3(256), 0, h->stream, A); } else { hipLaunchKernelGGL((k_sw_stage_fused<NN, false, 1>), dim3(grid_fused<NN>(h->nE, 1)), dim3(256), 0, h->stream, A); });
-                }
-                SELF_HIP(hipGetLastError());
+                if ((rc = launch_stage_fused(h, A))) break;
                 SELF_HIP(mark(1));
                 ++li;
                 h->trCur = 1 - h->trCur;
@@ -1466,6 +1490,69 @@ int self_sw2d_calculate_bathymetry_gradient(self_sw2d_t h)
     return SELF_OK;
 }
 
+int self_sw2d_set_members(self_sw2d_t h, int members)
+try {
+    SELF_REQUIRE(h, SELF_ERR_ARG, "null handle");
+    SELF_REQUIRE(members >= 1 && members <= 4096, SELF_ERR_ARG, "members %d outside 1..4096", members);
+    SELF_REQUIRE(h->d.form == SELF_FUSED || members == 1, SELF_ERR_ARG, "ensembles run in the fused form only");
+    SELF_HIP(hipSetDevice(h->device));
+    SELF_HIP(hipStreamSynchronize(h->stream));
+    const size_t nf = 3 * h->nE * h->np, nt = h->nE * 12 * h->n1, M = (size_t)members;
+    // reallocate the per-member state, RK register and trace buffers
+    void *old[] = {h->q, h->G64, h->G32, h->tr[0], h->tr[1]};
+    for (void *p : old)
+        if (p) (void)hipFree(p);
+    h->bytes -= h->members * (nf * sizeof(double) + nf * (h->G32 ? sizeof(float) : sizeof(double)) +
+                              2 * nt * sizeof(double));
+    h->q = nullptr; h->G64 = nullptr; h->G32 = nullptr; h->tr[0] = nullptr; h->tr[1] = nullptr;
+    h->members = 1;
+    int rc = dalloc(h, &h->q, M * nf);
+    if (!rc) rc = (h->d.precision == SELF_MIXED) ? dalloc(h, &h->G32, M * nf) : dalloc(h, &h->G64, M * nf);
+    if (!rc) rc = dalloc(h, &h->tr[0], M * nt);
+    if (!rc) rc = dalloc(h, &h->tr[1], M * nt);
+    if (rc) return rc;
+    SELF_HIP(hipMemsetAsync(h->q, 0, M * nf * sizeof(double), h->stream));
+    // boundary slots of the pushed traces are never written
+    SELF_HIP(hipMemsetAsync(h->tr[0], 0, M * nt * sizeof(double), h->stream));
+    SELF_HIP(hipMemsetAsync(h->tr[1], 0, M * nt * sizeof(double), h->stream));
+    SELF_HIP(hipStreamSynchronize(h->stream));
+    h->members = members;
+    h->trValid = false;
+    return SELF_OK;
+}
+SELF_CATCH_ALL
+
+int self_sw2d_set_member_solution(self_sw2d_t h, int member, const double *data)
+try {
+    SELF_REQUIRE(h && data, SELF_ERR_ARG, "null argument");
+    SELF_REQUIRE(member >= 0 && member < h->members, SELF_ERR_ARG, "member %d outside 0..%d", member, h->members - 1);
+    SELF_HIP(hipSetDevice(h->device));
+    SELF_HIP(hipStreamSynchronize(h->stream));
+    const size_t nE = h->nE, np = h->np;
+    std::vector<double> buf(3 * nE * np);
+    to_device_layout(data, buf.data(), nE, 3, (int)np);
+    SELF_HIP(hipMemcpy(h->q + (size_t)member * buf.size(), buf.data(), sizeof(double) * buf.size(),
+                       hipMemcpyHostToDevice));
+    h->trValid = false;
+    return SELF_OK;
+}
+SELF_CATCH_ALL
+
+int self_sw2d_get_member_solution(self_sw2d_t h, int member, double *data)
+try {
+    SELF_REQUIRE(h && data, SELF_ERR_ARG, "null argument");
+    SELF_REQUIRE(member >= 0 && member < h->members, SELF_ERR_ARG, "member %d outside 0..%d", member, h->members - 1);
+    SELF_HIP(hipSetDevice(h->device));
+    SELF_HIP(hipStreamSynchronize(h->stream));
+    const size_t nE = h->nE, np = h->np;
+    std::vector<double> buf(3 * nE * np);
+    SELF_HIP(hipMemcpy(buf.data(), h->q + (size_t)member * buf.size(), sizeof(double) * buf.size(),
+                       hipMemcpyDeviceToHost));
+    from_device_layout(buf.data(), data, nE, 3, (int)np);
+    return SELF_OK;
+}
+SELF_CATCH_ALL
+
 int self_sw2d_set_groups(self_sw2d_t h, int groups)
 {
     SELF_REQUIRE(h && groups >= 0 && groups <= 2, SELF_ERR_ARG, "groups must be 0 (heuristic), 1 or 2");
@@ -1483,7 +1570,8 @@ try {
         return SELF_OK;
     }
     // the trials advance a copy of the state: save and restore everything a step writes
-    const size_t nq = 3 * h->nE * h->np, nt = h->nE * 12 * h->n1;
+    const size_t M = (size_t)h->members;
+    const size_t nq = M * 3 * h->nE * h->np, nt = M * h->nE * 12 * h->n1;
     const size_t gb = h->G32 ? sizeof(float) * nq : sizeof(double) * nq;
     void *G = h->G32 ? (void *)h->G32 : (void *)h->G64;
     double *sq = nullptr, *st0 = nullptr, *st1 = nullptr;

@@ -199,6 +199,24 @@ class ShallowWater:
         """Element groups per workgroup of the fused kernel (0 = heuristic)."""
         L.check(L.lib().self_sw2d_set_groups(self._h, groups))
 
+    def SetMembers(self, members):
+        """Ensemble of `members` states advanced together (fused form): one
+        launch per RK stage for all of them.  States are reset to zero;
+        Set/GetSolution address member 0."""
+        L.check(L.lib().self_sw2d_set_members(self._h, int(members)))
+        self.members = int(members)
+
+    def SetMemberSolution(self, member, sol):
+        a = L.f64(sol)
+        if a.size != self.nElems * 3 * (self.nS + 1) ** 2:
+            raise ValueError("expected (nElems, 3, N+1, N+1)")
+        L.check(L.lib().self_sw2d_set_member_solution(self._h, int(member), L.ptr(a)))
+
+    def GetMemberSolution(self, member):
+        out = np.zeros(self._shape(3))
+        L.check(L.lib().self_sw2d_get_member_solution(self._h, int(member), L.ptr(out)))
+        return out
+
     def Autotune(self, nSteps=50):
         """Time each fused-kernel setting on a copy of the state; keeps the
         faster.  Returns (groups, ms_per_step)."""

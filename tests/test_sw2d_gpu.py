@@ -251,3 +251,47 @@ def test_autotune_keeps_the_state(gpu_lib):
     assert np.array_equal(m.GetSolution(), ref.GetSolution())
     m.Trash()
     ref.Trash()
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+@pytest.mark.parametrize("groups", [1, 2])
+def test_ensemble_members_equal_single_runs(gpu_lib, precision, groups):
+    """Three states on one mesh advanced together (one fused launch per RK
+    stage for all members) end bitwise where three separate runs end: a
+    member's arithmetic does not depend on its place in the grid."""
+    from self_amd import _lib as L
+    from self_amd.workloads import sw2d_geostrophic
+    base, info = sw2d_geostrophic(16, 5, precision=precision, form="fused")
+    H = info["sol"][:, 2]
+    sols = []
+    for k in range(3):
+        s = info["sol"].copy()
+        s[:, 2] = H + 0.25 * k * np.cos(2e-6 * np.pi * base.mesh.x * (k + 1)).reshape(H.shape)
+        sols.append(s)
+    singles = []
+    for s in sols:
+        m, _ = sw2d_geostrophic(16, 5, precision=precision, form="fused")
+        m.SetGroups(groups)
+        m.SetSolution(s)
+        m.ForwardStepRK3(0.0, 4)
+        singles.append(m.GetSolution())
+        m.Trash()
+    base.SetGroups(groups)
+    base.SetMembers(3)
+    for k, s in enumerate(sols):
+        base.SetMemberSolution(k, s)
+    assert np.array_equal(base.GetSolution(), sols[0])
+    base.ForwardStepRK3(0.0, 4)
+    for k in range(3):
+        out = base.GetMemberSolution(k)
+        assert not np.array_equal(out, sols[k])
+        assert np.array_equal(out, singles[k]), k
+    with pytest.raises(L.SelfError):
+        base.GetMemberSolution(3)
+    with pytest.raises(L.SelfError):
+        base.SetMembers(0)
+    base.Trash()
+    plain, _ = sw2d_geostrophic(8, 3, precision="fp64", form="plain")
+    with pytest.raises(L.SelfError):
+        plain.SetMembers(2)
+    plain.Trash()
