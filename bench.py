@@ -121,6 +121,39 @@ def bench_adv3d(steps=20, warm=2, nX=32, nY=32, nZ=16, N=7):
             "filtered_element_stages": filtered}
 
 
+def bench_sw2d_ensemble(members=8, nX=64, N=7, steps=100, warm=10):
+    """Ensemble of `members` states of the 64x64, N=7 geostrophic case (each
+    with its own 0.1 m depth offset) advanced together: one fused launch per
+    RK stage for all members.  Bytes: per member and element the state and
+    RK register read+write (Np 6 (s_q + s_G)); shared by the members the
+    metrics, bathymetry/Coriolis and normals (Np (5 s_m + 4 s_b) +
+    4 (N+1) 3 s_n), read once per launch."""
+    from self_amd.workloads import sw2d_geostrophic
+    out = {"members": members, "workload": f"{members} x SW2D geostrophic {nX}x{nX} N={N}"}
+    E, np_ = nX * nX, (N + 1) ** 2
+    for prec, s_b in (("mixed", 4), ("fp64", 8)):
+        m, info = sw2d_geostrophic(nX, N, precision=prec, form="fused", seed=SEED)
+        m.ForwardStepRK3(0.0, warm)
+        single = m.ForwardStepRK3Timed(0.0, steps) / steps / 3.0
+        m.SetMembers(members)
+        for k in range(members):
+            sol = info["sol"].copy()
+            sol[:, 2] += 0.1 * k
+            m.SetMemberSolution(k, sol)
+        m.ForwardStepRK3(0.0, warm)
+        stage_ms = m.ForwardStepRK3Timed(0.0, steps) / steps / 3.0
+        m.Trash()
+        per_member = np_ * 6 * (8 + 8)
+        shared = np_ * (5 * 8 + 4 * s_b) + 4 * (N + 1) * 3 * s_b
+        bytes_launch = E * (members * per_member + shared)
+        out[prec] = {"stage_ms": stage_ms, "member_stage_us": 1e3 * stage_ms / members,
+                     "dof_stage_per_s": members * 3 * E * np_ / (stage_ms * 1e-3),
+                     "single_member_stage_us": 1e3 * single, "speedup_vs_single": single * members / stage_ms,
+                     "algorithmic_bytes_per_launch": bytes_launch,
+                     "achieved_GBs": round(bytes_launch / (stage_ms * 1e-3) / 1e9, 1)}
+    return out
+
+
 def bench_cgsem(nX=128, N=7, tol=1e-10):
     """CGSEM 2-D variable-coefficient Poisson (128x128, N=7 GLL, Dirichlet):
     Jacobi-PCG in FP64 (the reference's CGSEM PCG loop), FP32-inner PCG with
@@ -286,6 +319,7 @@ def main():
                      "steps_compared": K + W, "gate": 1e-6, "pass": max(acc) <= 1e-6},
     }
     if not args.no_extras:
+        out["sw2d_ensemble"] = bench_sw2d_ensemble()
         out["adv2d"] = bench_adv2d()
         out["adv3d"] = bench_adv3d()
         out["cgsem"] = bench_cgsem()
