@@ -214,7 +214,9 @@ def bench_serving(n=512, workers=8, steps=200):
         srv.close()
         out[prec] = {"capacity_req_s": sat["throughput_req_s"], "arrival_rate_req_s": rate,
                      "throughput_req_s": poi["throughput_req_s"], "p50_ms": poi["p50_ms"], "p99_ms": poi["p99_ms"],
-                     "mean_ms": poi["mean_ms"], "sw_dof_stage_per_s": poi["sw_dof_stage_per_s"], "ok": poi["ok"]}
+                     "mean_ms": poi["mean_ms"], "sw_dof_stage_per_s": poi["sw_dof_stage_per_s"],
+                     "sw_mean_ensemble": poi["sw_mean_ensemble"], "capacity_sw_mean_ensemble": sat["sw_mean_ensemble"],
+                     "ok": poi["ok"]}
     out["stream"] = f"{n} requests, 80% SW (8..64^2, N 3/5/7, {steps} RK3 steps) / 20% CGSEM (16..128^2, N 3/5/7)"
     out["workers"] = workers
     return out

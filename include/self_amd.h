@@ -233,12 +233,16 @@ int self_sw2d_synchronize(self_sw2d_t h);
 int self_sw2d_set_groups(self_sw2d_t h, int groups);
 /* Ensembles (no reference counterpart: the reference runs one state per
  * model object).  `members` states on the same mesh, nodal storage,
- * bathymetry, vorticity, relaxation and time step advance together, one
+ * bathymetry, vorticity and relaxation advance together, one
  * fused launch per RK stage for all of them (fused form only).  Setting
  * the count reallocates the states (zeroed); self_sw2d_set_field /
  * get_field (SOLUTION) address member 0. */
 int self_sw2d_set_members(self_sw2d_t h, int members);
 int self_sw2d_set_member_solution(self_sw2d_t h, int member, const double *data);
+/* Time step of one member (default: the handle's dt; self_sw2d_set_params
+ * resets every member to its dt).  The stage update of member k uses
+ * g_m * dt_k, the same product a separate run forms. */
+int self_sw2d_set_member_time_step(self_sw2d_t h, int member, double dt);
 int self_sw2d_get_member_solution(self_sw2d_t h, int member, double *data);
 /* Autotuner: times nSteps RK3 steps with each setting on a copy of the
  * current state (the state is restored), keeps the faster one, returns it
@@ -473,6 +477,7 @@ typedef struct {
     int ioerr, iterations;/* CGSEM */
     double queue_ms, run_ms, latency_ms;  /* arrival -> start, start -> end, arrival -> end */
     size_t bytes;         /* device memory reserved for the request */
+    int members;          /* SW: requests that ran in the same ensemble launch (1 = alone) */
 } self_request_result;
 
 typedef struct self_server_s *self_server_t;

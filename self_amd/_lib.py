@@ -85,7 +85,8 @@ class Request(C.Structure):
 class Result(C.Structure):
     """self_request_result of include/self_amd.h"""
     _fields_ = [("status", C.c_int), ("ioerr", C.c_int), ("iterations", C.c_int), ("queue_ms", C.c_double),
-                ("run_ms", C.c_double), ("latency_ms", C.c_double), ("bytes", C.c_size_t)]
+                ("run_ms", C.c_double), ("latency_ms", C.c_double), ("bytes", C.c_size_t),
+                ("members", C.c_int)]
 
 
 _SIGS = {
@@ -183,6 +184,7 @@ _SIGS = {
     "self_sw2d_set_members": ([C.c_void_p, C.c_int], C.c_int),
     "self_sw2d_set_member_solution": ([C.c_void_p, C.c_int, dp], C.c_int),
     "self_sw2d_get_member_solution": ([C.c_void_p, C.c_int, dp], C.c_int),
+    "self_sw2d_set_member_time_step": ([C.c_void_p, C.c_int, C.c_double], C.c_int),
     "self_sw2d_autotune": ([C.c_void_p, C.c_int, ip, dp], C.c_int),
     "self_serve_create": ([C.c_int, C.c_double, C.POINTER(C.c_void_p)], C.c_int),
     "self_serve_submit": ([C.c_void_p, C.POINTER(Request), C.POINTER(C.c_longlong)], C.c_int),

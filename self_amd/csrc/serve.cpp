@@ -12,8 +12,13 @@
 //    against free HBM; requests that do not fit wait for memory to free);
 //  * each model has its own HIP stream (default priority), so the kernels
 //    of concurrent requests overlap on the device;
+//  * continuous batching: when a worker takes a shallow-water request, the
+//    queued requests that can share its ensemble (same mesh, degree,
+//    precision, steps, constants and bathymetry) join it, up to 8 members
+//    and as memory allows, and advance in one fused launch per RK stage,
+//    each with its own state and time step;
 //  * results are written to the caller's output array; wait() returns the
-//    request's status, queue time, run time and latency.
+//    request's status, queue time, run time, latency and ensemble size.
 // Every kernel of both models is deterministic (fixed reduction trees, no
 // atomics on values), so a request's result is bitwise the same alone and
 // in any mix.  Meshes and nodal storage are cached per (kind, nX, N).
@@ -22,6 +27,7 @@
 
 #include <chrono>
 #include <condition_variable>
+#include <cstdint>
 #include <cstring>
 #include <deque>
 #include <map>
@@ -54,6 +60,7 @@ struct Job {
     long long id = 0;
     self_request req{};
     size_t bytes = 0;
+    uint64_t inHash[2] = {0, 0};  // SW: two independent hashes of the bathymetry
     Clock::time_point arrive, start, end;
     bool done = false;
     self_request_result res{};
@@ -124,6 +131,34 @@ std::shared_ptr<MeshCache> mesh_for(self_server_s *s, int kind, int nX, int N, d
     return m;
 }
 
+// two independent 64-bit hashes of an array's bytes (word-wise)
+void hash_words(const double *a, size_t n, uint64_t out[2])
+{
+    uint64_t h0 = 0xcbf29ce484222325ull, h1 = 0x9e3779b97f4a7c15ull;
+    for (size_t k = 0; k < n; ++k) {
+        uint64_t w;
+        std::memcpy(&w, a + k, sizeof(w));
+        h0 = (h0 ^ w) * 0x100000001b3ull;
+        h1 = (h1 + w) * 0xff51afd7ed558ccdull;
+        h1 ^= h1 >> 29;
+    }
+    out[0] = h0;
+    out[1] = h1;
+}
+
+// Shallow-water requests that can advance as members of one ensemble: the
+// same mesh, degree, precision, step count, constants and bathymetry (each
+// member keeps its own state and time step)
+bool sw_batchable(const Job &a, const Job &b)
+{
+    const self_request &x = a.req, &y = b.req;
+    return x.kind == SELF_REQ_SW2D && y.kind == SELF_REQ_SW2D && x.nX == y.nX && x.N == y.N &&
+           x.precision == y.precision && x.steps == y.steps && x.L == y.L && x.g == y.g && x.f0 == y.f0 &&
+           a.inHash[0] == b.inHash[0] && a.inHash[1] == b.inHash[1];
+}
+
+constexpr int kMaxMembers = 8;  // ensemble size at which the fused stage stops gaining (C2: 6.4 us/member at 8)
+
 // device bytes a request needs (state, geometry, work vectors), with margin
 size_t estimate_bytes(const self_request &r)
 {
@@ -132,6 +167,38 @@ size_t estimate_bytes(const self_request &r)
         return (size_t)(1.25 * (double)(nodes * 8 * 24 + nE * 4 * n1 * 8 * 18)) + (1u << 20);
     // CGSEM: local and global vectors, geometry, Krylov work vectors
     return (size_t)(1.25 * (double)(nodes * 8 * 22)) + (1u << 20);
+}
+
+// several batchable shallow-water requests as one ensemble: one fused launch
+// per RK stage for all of them, each member with its own state and dt
+int run_sw_batch(self_server_s *s, std::vector<std::shared_ptr<Job>> &jobs)
+{
+    const self_request &r = jobs[0]->req;
+    const int M = (int)jobs.size();
+    auto m = mesh_for(s, SELF_REQ_SW2D, r.nX, r.N, r.L);
+    SELF_REQUIRE(m, SELF_ERR_OTHER, "request %lld: mesh set-up failed", jobs[0]->id);
+    self_sw2d_desc d{r.N, m->nE, m->nEdges, r.precision, SELF_FUSED, r.g, r.dt, 0.0};
+    self_sw2d_t h = nullptr;
+    int rc = self_sw2d_create(&d, &h);
+    if (rc) return rc;
+    const size_t nv = (size_t)m->nE * (r.N + 1) * (r.N + 1);
+    std::vector<double> plv(nv, r.f0);
+    if (!rc) rc = self_sw2d_set_nodal(h, m->dS.data(), m->dP.data(), m->w.data(), m->w.data(), m->lW.data(),
+                                      m->lE.data(), m->lS.data(), m->lN.data());
+    if (!rc) rc = self_sw2d_set_geometry(h, m->edges.data(), m->dxds.data(), m->dxdp.data(), m->dyds.data(),
+                                         m->dydp.data(), m->J.data(), m->nHat.data(), m->nLen.data());
+    if (!rc) rc = self_sw2d_set_field(h, SELF_SW_BATHYMETRY, r.in1);
+    if (!rc) rc = self_sw2d_set_field(h, SELF_SW_PLANETARY_VORTICITY, plv.data());
+    if (!rc) rc = self_sw2d_calculate_bathymetry_gradient(h);
+    if (!rc) rc = self_sw2d_set_members(h, M);
+    for (int k = 0; k < M && !rc; ++k) {
+        rc = self_sw2d_set_member_solution(h, k, jobs[k]->req.in0);
+        if (!rc) rc = self_sw2d_set_member_time_step(h, k, jobs[k]->req.dt);
+    }
+    if (!rc) rc = self_sw2d_forward_step_rk3(h, 0.0, r.steps);
+    for (int k = 0; k < M && !rc; ++k) rc = self_sw2d_get_member_solution(h, k, jobs[k]->req.out);
+    self_sw2d_destroy(h);
+    return rc;
 }
 
 int run_sw(self_server_s *s, Job &j)
@@ -198,7 +265,7 @@ void worker_main(self_server_s *s)
 {
     (void)hipSetDevice(s->device);
     for (;;) {
-        std::shared_ptr<Job> j;
+        std::vector<std::shared_ptr<Job>> batch;
         {
             std::unique_lock<std::mutex> g(s->mu);
             for (;;) {
@@ -208,30 +275,49 @@ void worker_main(self_server_s *s)
                 for (auto it = s->queue.begin(); it != s->queue.end(); ++it) {
                     const size_t b = (*it)->bytes;
                     if (s->reserved + b <= s->budget || (s->running == 0 && s->reserved == 0)) {
-                        j = *it;
+                        batch.push_back(*it);
                         s->queue.erase(it);
                         break;
                     }
                 }
-                if (j) break;
+                if (!batch.empty()) break;
                 s->cvWork.wait(g);
             }
-            s->reserved += j->bytes;
-            s->running++;
+            size_t bytes = batch[0]->bytes;
+            // continuous batching: queued shallow-water requests that can run
+            // as members of the same ensemble join it while memory allows
+            for (auto it = s->queue.begin(); it != s->queue.end() && (int)batch.size() < kMaxMembers;) {
+                if (sw_batchable(*batch[0], **it) && s->reserved + bytes + (*it)->bytes <= s->budget) {
+                    bytes += (*it)->bytes;
+                    batch.push_back(*it);
+                    it = s->queue.erase(it);
+                } else {
+                    ++it;
+                }
+            }
+            s->reserved += bytes;
+            s->running += (int)batch.size();
         }
-        j->start = Clock::now();
-        int rc = j->req.kind == SELF_REQ_SW2D ? run_sw(s, *j) : run_cgsem(s, *j);
-        j->end = Clock::now();
+        const Clock::time_point start = Clock::now();
+        int rc;
+        if (batch.size() > 1) rc = run_sw_batch(s, batch);
+        else rc = batch[0]->req.kind == SELF_REQ_SW2D ? run_sw(s, *batch[0]) : run_cgsem(s, *batch[0]);
+        const Clock::time_point end = Clock::now();
         {
             std::lock_guard<std::mutex> g(s->mu);
-            j->res.status = rc;
-            j->res.queue_ms = ms_between(j->arrive, j->start);
-            j->res.run_ms = ms_between(j->start, j->end);
-            j->res.latency_ms = ms_between(j->arrive, j->end);
-            j->res.bytes = j->bytes;
-            j->done = true;
-            s->reserved -= j->bytes;
-            s->running--;
+            for (auto &j : batch) {
+                j->start = start;
+                j->end = end;
+                j->res.status = rc;
+                j->res.queue_ms = ms_between(j->arrive, j->start);
+                j->res.run_ms = ms_between(j->start, j->end);
+                j->res.latency_ms = ms_between(j->arrive, j->end);
+                j->res.bytes = j->bytes;
+                j->res.members = (int)batch.size();
+                j->done = true;
+                s->reserved -= j->bytes;
+                s->running--;
+            }
         }
         s->cvDone.notify_all();
         s->cvWork.notify_all();
@@ -271,6 +357,8 @@ int self_serve_submit(self_server_t s, const self_request *req, long long *id)
     auto j = std::make_shared<Job>();
     j->req = *req;
     j->bytes = estimate_bytes(*req);
+    if (req->kind == SELF_REQ_SW2D)
+        hash_words(req->in1, 3 * (size_t)req->nX * req->nX * (req->N + 1) * (req->N + 1), j->inHash);
     j->arrive = Clock::now();
     {
         std::lock_guard<std::mutex> g(s->mu);

@@ -44,6 +44,8 @@ struct SWKernelArgs {
     int nE;
     int members;               // ensemble members (fused form): q, G and traces per member
     double g, drag, a, gdt;
+    double gcoef;              // RK stage coefficient g_m: member k steps with gcoef * dtM[k]
+    const double *dtM;         // per-member time steps (members > 1)
     int hasRelax;
     // nodal storage
     const double *dS, *dP;     // (N+1)^2 column-major
@@ -695,6 +697,7 @@ __global__ __launch_bounds__(256, 4 / G) void k_sw_stage_fused(SWKernelArgs A0)
         else A.G64 += qOff;
         A.trIn += tOff;
         A.trOut += tOff;
+        A.gdt = A0.gcoef * A0.dtM[mbr];
     }
     // WL: each element lies inside one wavefront, so the phases synchronize
     // per wavefront and the wavefronts of a workgroup never wait for each
@@ -821,6 +824,8 @@ struct self_sw2d_s {
     bool trValid = false, nodalSet = false, geomSet = false, hasRelax = false;
     int groups = 0;  // element groups per workgroup of the fused kernel (0: heuristic)
     int members = 1; // ensemble members (fused form)
+    double *dtM = nullptr;          // per-member time steps (device, members > 1)
+    std::vector<double> dtH;        // host copy
     size_t bytes = 0;
 };
 
@@ -839,7 +844,8 @@ void free_all(self_sw2d_s *h)
 {
     void *ptrs[] = {h->q, h->G64, h->tend, h->metric, h->bathy, h->G32, h->bathy32, h->plv32,
                     h->dS32, h->dP32, h->plv, h->relax, h->nrm, h->fnrm, h->bsol, h->bflux,
-                    h->tr[0], h->tr[1], h->nodal, h->side, h->edges, h->metM, h->fnM, h->metD, h->fnD};
+                    h->tr[0], h->tr[1], h->nodal, h->side, h->edges, h->metM, h->fnM, h->metD, h->fnD,
+                    h->dtM};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
 }
@@ -850,6 +856,7 @@ SWKernelArgs make_args(self_sw2d_s *h)
     const int n1 = h->n1;
     A.nE = (int)h->nE;
     A.members = h->members;
+    A.dtM = h->dtM;
     A.g = h->d.g;
     A.drag = h->d.linearDrag;
     A.hasRelax = h->hasRelax ? 1 : 0;
@@ -1004,6 +1011,7 @@ int step_fused(self_sw2d_s *h)
     for (int m = 0; m < 3; ++m) {
         A.a = kRkA[m];
         A.gdt = kRkG[m] * h->d.dt;
+        A.gcoef = kRkG[m];
         A.trIn = h->tr[h->trCur];
         A.trOut = h->tr[1 - h->trCur];
         if (int rc = launch_stage_fused(h, A)) return rc;
@@ -1350,6 +1358,12 @@ int self_sw2d_set_params(self_sw2d_t h, double g, double dt, double linearDrag)
     h->d.g = g;
     h->d.dt = dt;
     h->d.linearDrag = linearDrag;
+    if (h->dtM) {  // every member takes the common step again
+        SELF_HIP(hipSetDevice(h->device));
+        SELF_HIP(hipStreamSynchronize(h->stream));
+        h->dtH.assign(h->dtH.size(), dt);
+        SELF_HIP(hipMemcpy(h->dtM, h->dtH.data(), sizeof(double) * h->dtH.size(), hipMemcpyHostToDevice));
+    }
     return SELF_OK;
 }
 
@@ -1442,6 +1456,7 @@ try {
             for (int m = 0; m < 3; ++m) {
                 A.a = kRkA[m];
                 A.gdt = kRkG[m] * h->d.dt;
+                A.gcoef = kRkG[m];
                 A.trIn = h->tr[h->trCur];
                 A.trOut = h->tr[1 - h->trCur];
                 SELF_HIP(mark(0));
@@ -1511,6 +1526,16 @@ try {
     if (!rc) rc = dalloc(h, &h->tr[0], M * nt);
     if (!rc) rc = dalloc(h, &h->tr[1], M * nt);
     if (rc) return rc;
+    if (h->dtM) {
+        (void)hipFree(h->dtM);
+        h->dtM = nullptr;
+        h->bytes -= h->dtH.size() * sizeof(double);
+    }
+    h->dtH.assign(M, h->d.dt);
+    if (M > 1) {
+        if ((rc = dalloc(h, &h->dtM, M))) return rc;
+        SELF_HIP(hipMemcpy(h->dtM, h->dtH.data(), sizeof(double) * M, hipMemcpyHostToDevice));
+    }
     SELF_HIP(hipMemsetAsync(h->q, 0, M * nf * sizeof(double), h->stream));
     // boundary slots of the pushed traces are never written
     SELF_HIP(hipMemsetAsync(h->tr[0], 0, M * nt * sizeof(double), h->stream));
@@ -1534,6 +1559,23 @@ try {
     SELF_HIP(hipMemcpy(h->q + (size_t)member * buf.size(), buf.data(), sizeof(double) * buf.size(),
                        hipMemcpyHostToDevice));
     h->trValid = false;
+    return SELF_OK;
+}
+SELF_CATCH_ALL
+
+int self_sw2d_set_member_time_step(self_sw2d_t h, int member, double dt)
+try {
+    SELF_REQUIRE(h, SELF_ERR_ARG, "null handle");
+    SELF_REQUIRE(member >= 0 && member < h->members, SELF_ERR_ARG, "member %d outside 0..%d", member, h->members - 1);
+    SELF_REQUIRE(dt > 0.0, SELF_ERR_ARG, "time step %g", dt);
+    if (h->members == 1) {  // a single state steps with the common dt
+        h->d.dt = dt;
+        return SELF_OK;
+    }
+    SELF_HIP(hipSetDevice(h->device));
+    SELF_HIP(hipStreamSynchronize(h->stream));
+    h->dtH[member] = dt;
+    SELF_HIP(hipMemcpy(h->dtM + member, &dt, sizeof(double), hipMemcpyHostToDevice));
     return SELF_OK;
 }
 SELF_CATCH_ALL

@@ -212,6 +212,9 @@ class ShallowWater:
             raise ValueError("expected (nElems, 3, N+1, N+1)")
         L.check(L.lib().self_sw2d_set_member_solution(self._h, int(member), L.ptr(a)))
 
+    def SetMemberTimeStep(self, member, dt):
+        L.check(L.lib().self_sw2d_set_member_time_step(self._h, int(member), float(dt)))
+
     def GetMemberSolution(self, member):
         out = np.zeros(self._shape(3))
         L.check(L.lib().self_sw2d_get_member_solution(self._h, int(member), L.ptr(out)))

@@ -77,3 +77,28 @@ def test_bad_requests_are_rejected(gpu_lib):
     with pytest.raises(L.SelfError):
         srv.submit(spec)
     srv.close()
+
+
+def test_batched_shallow_water_requests_equal_alone(gpu_lib):
+    """Queued shallow-water requests on the same mesh, degree, precision and
+    step count join one ensemble (one fused launch per stage for all); each
+    keeps its own state and CFL time step and ends bitwise where it ends
+    alone."""
+    from self_amd import serving as S
+    specs = []
+    for i in range(9):
+        nX, N, prec = ((16, 5, "mixed"), (8, 3, "fp64"), (16, 5, "mixed"))[i % 3]
+        specs.append(S.make_inputs(S.Spec(S.REQ_SW2D, nX, N, 900 + i, prec, steps=12)))
+    dts = {s.params["dt"] for s in specs}
+    assert len(dts) > 2  # the members step with different time steps
+    srv = S.Server(workers=1)
+    mix, summary = S.run_stream(srv, specs)
+    srv.close()
+    assert summary["ok"]
+    assert max(r.members for _, r, _ in mix) > 1
+    alone = S.Server(workers=1)
+    for spec, res, out in mix:
+        r1, o1 = alone.wait(alone.submit(spec))
+        assert r1.status == 0 and r1.members == 1
+        assert np.array_equal(o1, out), (spec.nX, spec.N, spec.precision, res.members)
+    alone.close()
