@@ -250,6 +250,31 @@ int self_sw2d_get_member_solution(self_sw2d_t h, int member, double *data);
 int self_sw2d_autotune(self_sw2d_t h, int nSteps, int *groups, double *ms_per_step);
 /* Bytes of HBM state the handle holds. */
 int self_sw2d_device_bytes(self_sw2d_t h, size_t *bytes);
+/* WritePickup (ConservativeShallowWaterClass.f90:2013-2065) of the state,
+ * bathymetry and vorticity (member 0) to `path`; ReadPickup (:2069-2125)
+ * from it, followed by the bathymetry gradient as Build does (:224-228). */
+int self_sw2d_write_pickup(self_sw2d_t h, const char *path);
+int self_sw2d_read_pickup(self_sw2d_t h, const char *path);
+
+/* --------------------------------------------------------- pickup files
+ * The reference's restart files: unformatted direct access, big-endian,
+ * RECL = 8 (N+1)^2 bytes, one record per (0:N,0:N) element field, elements
+ * in order.  Shallow water (ConservativeShallowWaterClass.f90:2013-2125):
+ * U, V, H, h, f per element; reading leaves the bathymetry gradient zero
+ * (bathy = ZERO, :2088).  Advection (AdvectionClass.f90:2040-2171):
+ * c(1..nTracers), relaxation field(1..nTracers), relaxation
+ * factor(1..nTracers), u, v per element.  Host arrays in the layouts of
+ * the set/get calls above (element slowest).  A file whose size is not
+ * that of nElems elements is rejected. */
+/* '<model>.<iter as 10 digits>.pickup' (WRITE(iterChar,'(I10.10)')), e.g. model "ShallowWater" */
+int self_pickup_filename(const char *model, int iter, char *out, int outLen);
+int self_sw2d_pickup_write(const char *path, int N, int nElems, const double *sol, const double *bathy,
+                           const double *plv);
+int self_sw2d_pickup_read(const char *path, int N, int nElems, double *sol, double *bathy, double *plv);
+int self_adv2d_pickup_write(const char *path, int N, int nElems, int nTracers, const double *c, const double *relax,
+                            const double *rFac, const double *vel);
+int self_adv2d_pickup_read(const char *path, int N, int nElems, int nTracers, double *c, double *relax, double *rFac,
+                           double *vel);
 
 /* ------------------------------------------------------------ Krylov solvers
  * The reference's plug-in point is the abstract IterativeSolver type with
@@ -320,7 +345,10 @@ int self_adv2d_destroy(self_adv2d_t h);
  * slowest in every host array. */
 int self_adv2d_set_field(self_adv2d_t h, int field, const double *data);
 /* field 0: solution, 1: tendency (after self_adv2d_global_time_derivative),
- * 2: dc/dx, 3: dc/dy (CalculateConcentrationGradient :1123-1264) */
+ * 2: dc/dx, 3: dc/dy (CalculateConcentrationGradient :1123-1264),
+ * 4: velocity (0:N,0:N,1:2) (GetVelocity :415-430), 5: relaxation field,
+ * 6: relaxation factor (GetRelaxationField :937-950, GetRelaxationFactor
+ * :1005-1018; zeros if never set). */
 int self_adv2d_get_field(self_adv2d_t h, int field, double *data);
 /* GlobalTimeDerivative (:1316-1378) at time t: tendency of the current state. */
 int self_adv2d_global_time_derivative(self_adv2d_t h, double t);
@@ -328,6 +356,10 @@ int self_adv2d_global_time_derivative(self_adv2d_t h, double t);
  * (nullable) is the device time of the steps. */
 int self_adv2d_forward_step_rk3(self_adv2d_t h, double tn, int nSteps, double *elapsed_ms);
 int self_adv2d_synchronize(self_adv2d_t h);
+/* WritePickup (:2040-2103) / ReadPickup (:2107-2171) of the handle's
+ * fields to / from `path` (see self_adv2d_pickup_write). */
+int self_adv2d_write_pickup(self_adv2d_t h, const char *path);
+int self_adv2d_read_pickup(self_adv2d_t h, const char *path);
 
 /* ------------------------------------------------------ 3-D set-up (CPU)
  * BuildNodalStorage_3D (src/nodal/NodalStorage_3D_Class.f90:107-264) with

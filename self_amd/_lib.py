@@ -182,6 +182,15 @@ _SIGS = {
     "self_coarse_to_fine_3d_dev": ([C.c_int, C.c_size_t, dp, C.c_int, dp, dp, C.c_void_p], C.c_int),
     "self_sw2d_set_groups": ([C.c_void_p, C.c_int], C.c_int),
     "self_sw2d_set_members": ([C.c_void_p, C.c_int], C.c_int),
+    "self_sw2d_write_pickup": ([C.c_void_p, C.c_char_p], C.c_int),
+    "self_sw2d_read_pickup": ([C.c_void_p, C.c_char_p], C.c_int),
+    "self_adv2d_write_pickup": ([C.c_void_p, C.c_char_p], C.c_int),
+    "self_adv2d_read_pickup": ([C.c_void_p, C.c_char_p], C.c_int),
+    "self_pickup_filename": ([C.c_char_p, C.c_int, C.c_char_p, C.c_int], C.c_int),
+    "self_sw2d_pickup_write": ([C.c_char_p, C.c_int, C.c_int, dp, dp, dp], C.c_int),
+    "self_sw2d_pickup_read": ([C.c_char_p, C.c_int, C.c_int, dp, dp, dp], C.c_int),
+    "self_adv2d_pickup_write": ([C.c_char_p, C.c_int, C.c_int, C.c_int, dp, dp, dp, dp], C.c_int),
+    "self_adv2d_pickup_read": ([C.c_char_p, C.c_int, C.c_int, C.c_int, dp, dp, dp, dp], C.c_int),
     "self_sw2d_set_member_solution": ([C.c_void_p, C.c_int, dp], C.c_int),
     "self_sw2d_get_member_solution": ([C.c_void_p, C.c_int, dp], C.c_int),
     "self_sw2d_set_member_time_step": ([C.c_void_p, C.c_int, C.c_double], C.c_int),

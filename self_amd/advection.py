@@ -12,6 +12,7 @@ from __future__ import annotations
 
 import ctypes as C
 from dataclasses import dataclass
+import os
 
 import numpy as np
 
@@ -106,6 +107,33 @@ class Advection:
 
     def SetRelaxationFactor(self, f):
         self._set(3, f, self.nEq)
+
+    def GetVelocity(self):
+        """(u, v), each (nElems, N+1, N+1)."""
+        out = np.zeros(self._shape(2))
+        L.check(L.lib().self_adv2d_get_field(self._h, 4, L.ptr(out)))
+        return out[:, 0].copy(), out[:, 1].copy()
+
+    def GetRelaxationField(self):
+        return self._get(5)
+
+    def GetRelaxationFactor(self):
+        return self._get(6)
+
+    def WritePickup(self, iter, directory="."):
+        """WritePickup (AdvectionClass.f90:2040-2103) to
+        <directory>/Advection.<iter>.pickup."""
+        from .pickup import filename
+        path = os.path.join(directory, filename("Advection", iter))
+        L.check(L.lib().self_adv2d_write_pickup(self._h, path.encode()))
+        return path
+
+    def ReadPickup(self, iter, directory="."):
+        """ReadPickup (:2107-2171): solution, relaxation field and factor, velocity."""
+        from .pickup import filename
+        path = os.path.join(directory, filename("Advection", iter))
+        L.check(L.lib().self_adv2d_read_pickup(self._h, path.encode()))
+        return path
 
     def GetTendency(self):
         return self._get(1)

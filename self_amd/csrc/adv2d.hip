@@ -581,20 +581,29 @@ try {
 }
 SELF_CATCH_ALL
 
-// field: 0 concentration, 1 tendency, 2 dc/dx, 3 dc/dy
+// field: 0 concentration, 1 tendency, 2 dc/dx, 3 dc/dy, 4 velocity
+// (0:N,0:N,1:2), 5 relaxation field, 6 relaxation factor (zero if never set)
 int self_adv2d_get_field(self_adv2d_t h, int field, double *data)
 try {
     SELF_REQUIRE(h && data, SELF_ERR_ARG, "null argument");
     SELF_HIP(hipSetDevice(h->device));
     SELF_HIP(hipStreamSynchronize(h->stream));
-    const size_t nE = h->nE, np = h->np, nq = h->nq;
+    const size_t nE = h->nE, np = h->np;
+    size_t nq = h->nq;
     const double *src = nullptr;
     switch (field) {
     case 0: src = h->c; break;
     case 1: src = h->tend; break;
     case 2: src = h->grad; break;
     case 3: src = h->grad + nq * nE * np; break;
+    case 4: src = h->vel; nq = 2; break;
+    case 5: src = h->relax; break;
+    case 6: src = h->rFac; break;
     default: SELF_REQUIRE(false, SELF_ERR_ARG, "field %d", field);
+    }
+    if (!src) {  // a field never set holds zeros, as the reference's Build leaves it
+        std::memset(data, 0, sizeof(double) * nq * nE * np);
+        return SELF_OK;
     }
     std::vector<double> buf(nq * nE * np);
     SELF_HIP(hipMemcpy(buf.data(), src, sizeof(double) * buf.size(), hipMemcpyDeviceToHost));
@@ -643,6 +652,36 @@ int self_adv2d_forward_step_rk3(self_adv2d_t h, double tn, int nSteps, double *e
     }
     return rc;
 }
+
+// WritePickup_Advection (AdvectionClass.f90:2040-2103)
+int self_adv2d_write_pickup(self_adv2d_t h, const char *path)
+try {
+    SELF_REQUIRE(h && path, SELF_ERR_ARG, "null argument");
+    const size_t nv = h->nE * h->np, q = (size_t)h->nq;
+    std::vector<double> c(q * nv), relax(q * nv), rFac(q * nv), vel(2 * nv);
+    int rc = self_adv2d_get_field(h, 0, c.data());
+    if (!rc) rc = self_adv2d_get_field(h, 5, relax.data());
+    if (!rc) rc = self_adv2d_get_field(h, 6, rFac.data());
+    if (!rc) rc = self_adv2d_get_field(h, 4, vel.data());
+    if (!rc) rc = self_adv2d_pickup_write(path, h->N, (int)h->nE, h->nq, c.data(), relax.data(), rFac.data(), vel.data());
+    return rc;
+}
+SELF_CATCH_ALL
+
+// ReadPickup_Advection (:2107-2171): solution, relaxation field and factor, velocity
+int self_adv2d_read_pickup(self_adv2d_t h, const char *path)
+try {
+    SELF_REQUIRE(h && path, SELF_ERR_ARG, "null argument");
+    const size_t nv = h->nE * h->np, q = (size_t)h->nq;
+    std::vector<double> c(q * nv), relax(q * nv), rFac(q * nv), vel(2 * nv);
+    int rc = self_adv2d_pickup_read(path, h->N, (int)h->nE, h->nq, c.data(), relax.data(), rFac.data(), vel.data());
+    if (!rc) rc = self_adv2d_set_field(h, 0, c.data());
+    if (!rc) rc = self_adv2d_set_field(h, 2, relax.data());
+    if (!rc) rc = self_adv2d_set_field(h, 3, rFac.data());
+    if (!rc) rc = self_adv2d_set_field(h, 1, vel.data());
+    return rc;
+}
+SELF_CATCH_ALL
 
 int self_adv2d_synchronize(self_adv2d_t h)
 {

@@ -1595,6 +1595,35 @@ try {
 }
 SELF_CATCH_ALL
 
+// WritePickup_ShallowWater (ConservativeShallowWaterClass.f90:2013-2065) of member 0
+int self_sw2d_write_pickup(self_sw2d_t h, const char *path)
+try {
+    SELF_REQUIRE(h && path, SELF_ERR_ARG, "null argument");
+    const size_t nv = h->nE * h->np;
+    std::vector<double> sol(3 * nv), bathy(3 * nv), plv(nv);
+    int rc = self_sw2d_get_field(h, SELF_SW_SOLUTION, sol.data());
+    if (!rc) rc = self_sw2d_get_field(h, SELF_SW_BATHYMETRY, bathy.data());
+    if (!rc) rc = self_sw2d_get_field(h, SELF_SW_PLANETARY_VORTICITY, plv.data());
+    if (!rc) rc = self_sw2d_pickup_write(path, h->d.N, (int)h->nE, sol.data(), bathy.data(), plv.data());
+    return rc;
+}
+SELF_CATCH_ALL
+
+// ReadPickup_ShallowWater (:2069-2125) and the gradient Build computes after it (:224-228)
+int self_sw2d_read_pickup(self_sw2d_t h, const char *path)
+try {
+    SELF_REQUIRE(h && path, SELF_ERR_ARG, "null argument");
+    const size_t nv = h->nE * h->np;
+    std::vector<double> sol(3 * nv), bathy(3 * nv), plv(nv);
+    int rc = self_sw2d_pickup_read(path, h->d.N, (int)h->nE, sol.data(), bathy.data(), plv.data());
+    if (!rc) rc = self_sw2d_set_field(h, SELF_SW_SOLUTION, sol.data());
+    if (!rc) rc = self_sw2d_set_field(h, SELF_SW_BATHYMETRY, bathy.data());
+    if (!rc) rc = self_sw2d_set_field(h, SELF_SW_PLANETARY_VORTICITY, plv.data());
+    if (!rc) rc = self_sw2d_calculate_bathymetry_gradient(h);
+    return rc;
+}
+SELF_CATCH_ALL
+
 int self_sw2d_set_groups(self_sw2d_t h, int groups)
 {
     SELF_REQUIRE(h && groups >= 0 && groups <= 2, SELF_ERR_ARG, "groups must be 0 (heuristic), 1 or 2");

@@ -191,6 +191,61 @@ MODULE SELF_AMD_CABI
       TYPE(C_PTR), VALUE :: elapsedMs
     END FUNCTION
 
+    ! ------------------------------------------------------------ pickup files and ensembles
+    ! path / model: NUL-terminated (pass TRIM(name)//C_NULL_CHAR)
+    INTEGER(C_INT) FUNCTION self_pickup_filename(model, iter, out, outLen) BIND(C, name="self_pickup_filename")
+      IMPORT :: C_INT, C_CHAR
+      CHARACTER(KIND=C_CHAR), INTENT(in) :: model(*)
+      INTEGER(C_INT), VALUE :: iter, outLen
+      CHARACTER(KIND=C_CHAR), INTENT(out) :: out(*)
+    END FUNCTION
+    INTEGER(C_INT) FUNCTION self_sw2d_write_pickup(h, path) BIND(C, name="self_sw2d_write_pickup")
+      IMPORT :: C_INT, C_PTR, C_CHAR
+      TYPE(C_PTR), VALUE :: h
+      CHARACTER(KIND=C_CHAR), INTENT(in) :: path(*)
+    END FUNCTION
+    INTEGER(C_INT) FUNCTION self_sw2d_read_pickup(h, path) BIND(C, name="self_sw2d_read_pickup")
+      IMPORT :: C_INT, C_PTR, C_CHAR
+      TYPE(C_PTR), VALUE :: h
+      CHARACTER(KIND=C_CHAR), INTENT(in) :: path(*)
+    END FUNCTION
+    INTEGER(C_INT) FUNCTION self_adv2d_write_pickup(h, path) BIND(C, name="self_adv2d_write_pickup")
+      IMPORT :: C_INT, C_PTR, C_CHAR
+      TYPE(C_PTR), VALUE :: h
+      CHARACTER(KIND=C_CHAR), INTENT(in) :: path(*)
+    END FUNCTION
+    INTEGER(C_INT) FUNCTION self_adv2d_read_pickup(h, path) BIND(C, name="self_adv2d_read_pickup")
+      IMPORT :: C_INT, C_PTR, C_CHAR
+      TYPE(C_PTR), VALUE :: h
+      CHARACTER(KIND=C_CHAR), INTENT(in) :: path(*)
+    END FUNCTION
+    INTEGER(C_INT) FUNCTION self_sw2d_set_members(h, members) BIND(C, name="self_sw2d_set_members")
+      IMPORT :: C_INT, C_PTR
+      TYPE(C_PTR), VALUE :: h
+      INTEGER(C_INT), VALUE :: members
+    END FUNCTION
+    INTEGER(C_INT) FUNCTION self_sw2d_set_member_solution(h, member, dat) &
+                                                         BIND(C, name="self_sw2d_set_member_solution")
+      IMPORT :: C_INT, C_PTR, C_DOUBLE
+      TYPE(C_PTR), VALUE :: h
+      INTEGER(C_INT), VALUE :: member
+      REAL(C_DOUBLE), INTENT(in) :: dat(*)
+    END FUNCTION
+    INTEGER(C_INT) FUNCTION self_sw2d_get_member_solution(h, member, dat) &
+                                                         BIND(C, name="self_sw2d_get_member_solution")
+      IMPORT :: C_INT, C_PTR, C_DOUBLE
+      TYPE(C_PTR), VALUE :: h
+      INTEGER(C_INT), VALUE :: member
+      REAL(C_DOUBLE), INTENT(out) :: dat(*)
+    END FUNCTION
+    INTEGER(C_INT) FUNCTION self_sw2d_set_member_time_step(h, member, dt) &
+                                                          BIND(C, name="self_sw2d_set_member_time_step")
+      IMPORT :: C_INT, C_PTR, C_DOUBLE
+      TYPE(C_PTR), VALUE :: h
+      INTEGER(C_INT), VALUE :: member
+      REAL(C_DOUBLE), VALUE :: dt
+    END FUNCTION
+
     ! ------------------------------------------------------------ Krylov and CGSEM
     INTEGER(C_INT) FUNCTION self_linop_dense_create(n, A, b, op) BIND(C, name="self_linop_dense_create")
       IMPORT :: C_INT, C_PTR, C_DOUBLE
@@ -211,6 +266,65 @@ MODULE SELF_AMD_CABI
       REAL(C_DOUBLE), INTENT(inout) :: sol(*)
       REAL(C_DOUBLE), INTENT(out) :: resi(*)
       INTEGER(C_INT), INTENT(out) :: iterations, ioerr
+    END FUNCTION
+    ! CGsemElliptic_2D (CGsemElliptic_2D_Class.f90): element-local arrays (0:N,0:N,nElems)
+    INTEGER(C_INT) FUNCTION self_cgsem2d_create(N, nElems, nEdges, nNodes, edges, elemNodes, dxds, dxdp, dyds, &
+                                                dydp, J, h) BIND(C, name="self_cgsem2d_create")
+      IMPORT :: C_INT, C_PTR, C_DOUBLE
+      INTEGER(C_INT), VALUE :: N, nElems, nEdges, nNodes
+      INTEGER(C_INT), INTENT(in) :: edges(*), elemNodes(*)
+      REAL(C_DOUBLE), INTENT(in) :: dxds(*), dxdp(*), dyds(*), dydp(*), J(*)
+      TYPE(C_PTR), INTENT(out) :: h
+    END FUNCTION
+    INTEGER(C_INT) FUNCTION self_cgsem2d_destroy(h) BIND(C, name="self_cgsem2d_destroy")
+      IMPORT :: C_INT, C_PTR
+      TYPE(C_PTR), VALUE :: h
+    END FUNCTION
+    INTEGER(C_INT) FUNCTION self_cgsem2d_set_flux_coefficient(h, k) BIND(C, name="self_cgsem2d_set_flux_coefficient")
+      IMPORT :: C_INT, C_PTR, C_DOUBLE
+      TYPE(C_PTR), VALUE :: h
+      REAL(C_DOUBLE), INTENT(in) :: k(*)
+    END FUNCTION
+    INTEGER(C_INT) FUNCTION self_cgsem2d_set_source(h, src) BIND(C, name="self_cgsem2d_set_source")
+      IMPORT :: C_INT, C_PTR, C_DOUBLE
+      TYPE(C_PTR), VALUE :: h
+      REAL(C_DOUBLE), INTENT(in) :: src(*)
+    END FUNCTION
+    INTEGER(C_INT) FUNCTION self_cgsem2d_set_dirichlet(h, vals) BIND(C, name="self_cgsem2d_set_dirichlet")
+      IMPORT :: C_INT, C_PTR, C_DOUBLE
+      TYPE(C_PTR), VALUE :: h
+      REAL(C_DOUBLE), INTENT(in) :: vals(*)
+    END FUNCTION
+    INTEGER(C_INT) FUNCTION self_cgsem2d_set_solution(h, u) BIND(C, name="self_cgsem2d_set_solution")
+      IMPORT :: C_INT, C_PTR, C_DOUBLE
+      TYPE(C_PTR), VALUE :: h
+      REAL(C_DOUBLE), INTENT(in) :: u(*)
+    END FUNCTION
+    INTEGER(C_INT) FUNCTION self_cgsem2d_get_solution(h, u) BIND(C, name="self_cgsem2d_get_solution")
+      IMPORT :: C_INT, C_PTR, C_DOUBLE
+      TYPE(C_PTR), VALUE :: h
+      REAL(C_DOUBLE), INTENT(out) :: u(*)
+    END FUNCTION
+    INTEGER(C_INT) FUNCTION self_cgsem2d_solve(h, method, precond, tol, maxIters, mInner, ortho, ioerr, &
+                                               iterations, resi, resiLen, elapsedMs) BIND(C, name="self_cgsem2d_solve")
+      IMPORT :: C_INT, C_PTR, C_DOUBLE
+      TYPE(C_PTR), VALUE :: h
+      INTEGER(C_INT), VALUE :: method, precond, maxIters, mInner, ortho, resiLen
+      REAL(C_DOUBLE), VALUE :: tol
+      INTEGER(C_INT), INTENT(out) :: ioerr, iterations
+      REAL(C_DOUBLE), INTENT(out) :: resi(*)
+      TYPE(C_PTR), VALUE :: elapsedMs
+    END FUNCTION
+    INTEGER(C_INT) FUNCTION self_cgsem2d_solve_reliable(h, precond, tol, delta, maxIters, maxUpdates, ioerr, &
+                                                        updates, iterations, relResidual, elapsedMs) &
+                                                        BIND(C, name="self_cgsem2d_solve_reliable")
+      IMPORT :: C_INT, C_PTR, C_DOUBLE
+      TYPE(C_PTR), VALUE :: h
+      INTEGER(C_INT), VALUE :: precond, maxIters, maxUpdates
+      REAL(C_DOUBLE), VALUE :: tol, delta
+      INTEGER(C_INT), INTENT(out) :: ioerr, updates, iterations
+      REAL(C_DOUBLE), INTENT(out) :: relResidual
+      TYPE(C_PTR), VALUE :: elapsedMs
     END FUNCTION
   END INTERFACE
 

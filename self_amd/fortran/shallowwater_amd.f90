@@ -7,7 +7,7 @@
 !   GetTendency (:516-529), SetBathymetry (:929-942),
 !   SetPlanetaryVorticity (:1035-1048), CalculateBathymetryGradient
 !   (:1733-1868, here for all elements at once), ForwardStepRK3 (:1173-1218),
-!   GlobalTimeDerivative (:1222-1262).
+!   GlobalTimeDerivative (:1222-1262), WritePickup/ReadPickup (:2013-2125).
 ! The type keeps reference-shaped host arrays as a mirror; the device holds
 ! the state.  Set* write the mirror and mark it newer; the next device call
 ! uploads it once.  Device calls mark the device newer; the next Get*
@@ -49,6 +49,8 @@ MODULE ShallowWaterAMD_Class
     PROCEDURE :: CalculateBathymetryGradient => CalculateBathymetryGradient_ShallowWaterAMD
     PROCEDURE :: GlobalTimeDerivative => GlobalTimeDerivative_ShallowWaterAMD
     PROCEDURE :: ForwardStepRK3 => ForwardStepRK3_ShallowWaterAMD
+    PROCEDURE :: WritePickup => WritePickup_ShallowWaterAMD
+    PROCEDURE :: ReadPickup => ReadPickup_ShallowWaterAMD
     PROCEDURE, PRIVATE :: Upload => Upload_ShallowWaterAMD
   END TYPE ShallowWaterAMD
 
@@ -235,5 +237,35 @@ CONTAINS
     CALL self_check(self_sw2d_forward_step_rk3(myDGSEM % handle, tn, 1_C_INT), 'ForwardStepRK3')
     myDGSEM % deviceSolutionNewer = .TRUE.
   END SUBROUTINE ForwardStepRK3_ShallowWaterAMD
+
+  ! WritePickup (:2013-2065): ShallowWater.<iter, 10 digits>.pickup in the
+  ! working directory, the reference's big-endian direct-access records
+  SUBROUTINE WritePickup_ShallowWaterAMD(myDGSEM, iter)
+    CLASS(ShallowWaterAMD), INTENT(inout) :: myDGSEM
+    INTEGER, INTENT(in)                   :: iter
+    CHARACTER(KIND=C_CHAR, LEN=64) :: fname
+    CALL myDGSEM % Upload()
+    CALL self_check(self_pickup_filename('ShallowWater'//C_NULL_CHAR, INT(iter, C_INT), fname, 64_C_INT), &
+                    'WritePickup')
+    CALL self_check(self_sw2d_write_pickup(myDGSEM % handle, fname), 'WritePickup')
+  END SUBROUTINE WritePickup_ShallowWaterAMD
+
+  ! ReadPickup (:2069-2125) and the bathymetry gradient Build computes after it
+  ! (:224-228); the host mirrors follow the device
+  SUBROUTINE ReadPickup_ShallowWaterAMD(myDGSEM, iter)
+    CLASS(ShallowWaterAMD), INTENT(inout) :: myDGSEM
+    INTEGER, INTENT(in)                   :: iter
+    CHARACTER(KIND=C_CHAR, LEN=64) :: fname
+    CALL self_check(self_pickup_filename('ShallowWater'//C_NULL_CHAR, INT(iter, C_INT), fname, 64_C_INT), &
+                    'ReadPickup')
+    CALL self_check(self_sw2d_read_pickup(myDGSEM % handle, fname), 'ReadPickup')
+    CALL self_check(self_sw2d_get_field(myDGSEM % handle, SELF_SW_PLANETARY_VORTICITY, myDGSEM % vorticity), &
+                    'ReadPickup')
+    myDGSEM % hostSolutionNewer = .FALSE.
+    myDGSEM % hostBathymetryNewer = .FALSE.
+    myDGSEM % hostVorticityNewer = .FALSE.
+    myDGSEM % deviceSolutionNewer = .TRUE.
+    myDGSEM % deviceBathymetryNewer = .TRUE.
+  END SUBROUTINE ReadPickup_ShallowWaterAMD
 
 END MODULE ShallowWaterAMD_Class

@@ -14,6 +14,8 @@ from __future__ import annotations
 import ctypes as C
 from dataclasses import dataclass
 
+import os
+
 import numpy as np
 
 from . import _lib as L
@@ -198,6 +200,22 @@ class ShallowWater:
     def SetGroups(self, groups):
         """Element groups per workgroup of the fused kernel (0 = heuristic)."""
         L.check(L.lib().self_sw2d_set_groups(self._h, groups))
+
+    def WritePickup(self, iter, directory="."):
+        """WritePickup (ConservativeShallowWaterClass.f90:2013-2065): state,
+        bathymetry and vorticity to <directory>/ShallowWater.<iter>.pickup."""
+        from .pickup import filename
+        path = os.path.join(directory, filename("ShallowWater", iter))
+        L.check(L.lib().self_sw2d_write_pickup(self._h, path.encode()))
+        return path
+
+    def ReadPickup(self, iter, directory="."):
+        """ReadPickup (:2069-2125) and the bathymetry gradient Build computes
+        after it (:224-228)."""
+        from .pickup import filename
+        path = os.path.join(directory, filename("ShallowWater", iter))
+        L.check(L.lib().self_sw2d_read_pickup(self._h, path.encode()))
+        return path
 
     def SetMembers(self, members):
         """Ensemble of `members` states advanced together (fused form): one

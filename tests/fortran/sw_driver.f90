@@ -4,13 +4,16 @@
 ! src/highend/shallowwater/ShallowWater_Driver.f90:60-66).
 ! Usage: sw_driver OUTFILE PRECISION(0|1) FORM(0|1) NSTEPS
 ! Writes (stream, native endian): initial solution, bathymetry (after the
-! gradient), final solution, each (0:N,0:N,1:3,1:nElems) FP64.
+! gradient), final solution, each (0:N,0:N,1:3,1:nElems) FP64; then
+! WritePickup(NSTEPS) (ShallowWater.<NSTEPS>.pickup in the working
+! directory), ReadPickup of it into a second model, one more step of both,
+! and both solutions (two more blocks).
 PROGRAM sw_driver
   USE ISO_C_BINDING
   USE SELF_AMD_CABI
   USE ShallowWaterAMD_Class
   IMPLICIT NONE
-  TYPE(ShallowWaterAMD) :: sw
+  TYPE(ShallowWaterAMD) :: sw, sw2
   TYPE(SWParamsAMD)     :: p
   REAL(C_DOUBLE), ALLOCATABLE :: sol(:,:,:), bathy(:,:,:), allSol(:,:,:,:), allBathy(:,:,:,:)
   REAL(C_DOUBLE) :: x, y, tn, kx, ky
@@ -77,7 +80,24 @@ PROGRAM sw_driver
     allSol(:,:,:,iEl) = sol
   ENDDO
   WRITE(u) allSol
+
+  CALL sw % WritePickup(nSteps)
+  CALL sw2 % Build(p)
+  CALL sw2 % ReadPickup(nSteps)
+  CALL sw % ForwardStepRK3(tn)
+  CALL sw2 % ForwardStepRK3(tn)
+  DO iEl = 1, sw % nElems
+    CALL sw % GetSolution(iEl, sol)
+    allSol(:,:,:,iEl) = sol
+  ENDDO
+  WRITE(u) allSol
+  DO iEl = 1, sw2 % nElems
+    CALL sw2 % GetSolution(iEl, sol)
+    allSol(:,:,:,iEl) = sol
+  ENDDO
+  WRITE(u) allSol
   CLOSE(u)
+  CALL sw2 % Trash()
   CALL sw % Trash()
   PRINT '(A,I0,A,I0,A)', 'sw_driver: ', nSteps, ' steps on ', sw % nEdges, ' edges done'
 END PROGRAM sw_driver

@@ -289,3 +289,60 @@ def run_nodal(N, quadrature, form, nNew, f2, f3, timeout=60):
     out["fine3"] = t(np.float64, (m1, m1, m1))
     r.done()
     return out
+
+
+def _execute_files(model, params_text, header, input_arrays, want, timeout=60, place=None):
+    """As _execute, with files placed in the directory beforehand (place:
+    name -> bytes) and the named output files returned (want: names)."""
+    exe = os.path.join(REF_DIR, f"ref_{model}")
+    if not available(model):
+        raise FileNotFoundError(exe)
+    with tempfile.TemporaryDirectory() as d:
+        with open(os.path.join(d, "advection.params"), "w") as f:
+            f.write(params_text)
+        with open(os.path.join(d, "ref_in.bin"), "wb") as f:
+            f.write(header)
+            for a in input_arrays:
+                f.write(np.asfortranarray(a, dtype=np.float64).tobytes(order="F"))
+        for name, data in (place or {}).items():
+            with open(os.path.join(d, name), "wb") as f:
+                f.write(data)
+        p = subprocess.run([exe], cwd=d, capture_output=True, text=True, timeout=timeout)
+        if p.returncode != 0:
+            raise RuntimeError(f"ref_{model} failed ({p.returncode}): {p.stderr[-2000:]}")
+        out = {}
+        for name in want:
+            with open(os.path.join(d, name), "rb") as f:
+                out[name] = f.read()
+        return out
+
+
+def pickup_name(model, it):
+    """'<model>.<it, 10 digits>.pickup' (WRITE(iterChar,'(I10.10)'))."""
+    return f"{model}.{it:010d}.pickup"
+
+
+def run_pickup_write(N, nX, nY, nTracers, it, inputs, timeout=60):
+    """The reference's WritePickup_Advection of the given fields: returns the
+    bytes of Advection.<it>.pickup (inputs as random_inputs)."""
+    params = _params_text(N, nX, nY, nTracers, 1e-3, 1e-4, 1.0, 1.0)
+    header = np.array([1, it], dtype=np.int32).tobytes()
+    arrays = [inputs[k] for k in ("c", "u", "v", "relax", "rFac")]
+    name = pickup_name("Advection", it)
+    return _execute_files("pickup", params, header, arrays, [name], timeout)[name]
+
+
+def run_pickup_read(N, nX, nY, nTracers, it, data, timeout=60):
+    """The reference's ReadPickup_Advection of the given file bytes; returns
+    the fields read back (c, u, v, relax, rFac; element-last shapes)."""
+    params = _params_text(N, nX, nY, nTracers, 1e-3, 1e-4, 1.0, 1.0)
+    header = np.array([2, it], dtype=np.int32).tobytes()
+    raw = _execute_files("pickup", params, header, [], ["ref_out.bin"], timeout,
+                         place={pickup_name("Advection", it): data})["ref_out.bin"]
+    n1, E = N + 1, nX * nY
+    r = _Reader(raw)
+    out = {"c": r.take("<f8", (n1, n1, nTracers, E)), "u": r.take("<f8", (n1, n1, E)),
+           "v": r.take("<f8", (n1, n1, E)), "relax": r.take("<f8", (n1, n1, nTracers, E)),
+           "rFac": r.take("<f8", (n1, n1, nTracers, E))}
+    r.done()
+    return out

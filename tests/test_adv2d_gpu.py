@@ -138,3 +138,31 @@ def test_errors_are_reported(gpu_lib):
     with pytest.raises(ValueError):
         m.SetSolution(np.zeros((9, 2, 4, 4)))
     m.Trash()
+
+
+def test_pickup_write_then_read_resumes_the_run(gpu_lib, tmp_path):
+    """WritePickup of a model with relaxation set and ReadPickup into a fresh
+    one give the same solution, velocity and relaxation fields, and the same
+    states after further steps."""
+    import os
+    from self_amd.workloads import adv2d_blob
+    a, info = adv2d_blob(nX=4, N=5, nTracers=2)
+    rng = np.random.default_rng(3)
+    shape = a.GetSolution().shape
+    a.SetRelaxationField(rng.standard_normal(shape))
+    a.SetRelaxationFactor(rng.uniform(0, 2, shape))
+    a.ForwardStepRK3(0.0, 3)
+    path = a.WritePickup(3, directory=str(tmp_path))
+    assert os.path.basename(path) == "Advection.0000000003.pickup"
+    b, _ = adv2d_blob(nX=4, N=5, nTracers=2)
+    b.ReadPickup(3, directory=str(tmp_path))
+    assert np.array_equal(b.GetSolution(), a.GetSolution())
+    for x, y in zip(b.GetVelocity(), a.GetVelocity()):
+        assert np.array_equal(x, y)
+    assert np.array_equal(b.GetRelaxationField(), a.GetRelaxationField())
+    assert np.array_equal(b.GetRelaxationFactor(), a.GetRelaxationFactor())
+    a.ForwardStepRK3(3e-4, 4)
+    b.ForwardStepRK3(3e-4, 4)
+    assert np.array_equal(b.GetSolution(), a.GetSolution())
+    a.Trash()
+    b.Trash()

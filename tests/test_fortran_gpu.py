@@ -21,13 +21,19 @@ def test_fortran_driver_matches_python_host_path(gpu_lib, tmp_path, precision, f
     out = tmp_path / "sw.bin"
     nSteps = 7
     r = subprocess.run([DRIVER, str(out), str(precision), str(form), str(nSteps)], capture_output=True, text=True,
-                       timeout=120)
+                       timeout=120, cwd=str(tmp_path))
     assert r.returncode == 0, r.stdout + r.stderr
     N, nX, nY = 5, 6, 5
     E, n1 = nX * nY, N + 1
     raw = np.fromfile(out, dtype=np.float64)
-    assert raw.size == 3 * E * 3 * n1 * n1
-    sol0, bathy, final = (a.reshape(E, 3, n1, n1) for a in np.split(raw, 3))
+    assert raw.size == 5 * E * 3 * n1 * n1
+    sol0, bathy, final, after, resumed = (a.reshape(E, 3, n1, n1) for a in np.split(raw, 5))
+    # WritePickup from Fortran: the reference's file, read back by the library
+    from self_amd import pickup as P
+    s, h, f = P.read_sw2d(tmp_path / P.filename("ShallowWater", nSteps), N, E)
+    assert np.array_equal(s, final) and np.array_equal(h[:, 0], bathy[:, 0]) and np.all(f == 0.1)
+    # ReadPickup into a second model resumes the run exactly
+    assert np.array_equal(resumed, after) and not np.array_equal(after, final)
     p = SWParams(polyDeg=N, nXElem=nX, nYElem=nY, xScale=2.0, yScale=3.0, dt=1e-3, g=9.81, f0=0.1, warp=0.03)
     m = ShallowWater(p, precision=("fp64", "mixed")[precision], form=("plain", "fused")[form])
     b = bathy.copy()
