@@ -115,6 +115,26 @@ int self_quadmesh_structured(int nX, int nY, int N, int periodicX, int periodicY
                              int *edges, double *x, double *y, double *dxds, double *dxdp,
                              double *dyds, double *dydp, double *J, double *nHat, double *nLen);
 
+/* ISM-V2 SpecMesh files (ReadSpecMeshFile_QuadMesh, src/geom/
+ * QuadMeshClass.f90:2100-2465): unstructured quads, side curves of degree
+ * gPolyDeg on Chebyshev-Gauss-Lobatto points, side names setting the
+ * boundary flags.  counts: corners, edges (as rebuilt by ConstructEdges,
+ * :1489-1627), elements, curve degree.  read: the edge table (6 ints per
+ * edge as self_quadmesh_structured; e2 = the side's boundary flag on a
+ * named boundary side, NO_NORMAL_FLOW otherwise; start/inc :2421-2431),
+ * elemNodes (4 corners per element, 1-based), the corner node types
+ * (INTERIOR = 1, or the flag of a named side through the corner, :2339-2419),
+ * corner positions, and the element geometry at the interpolation nodes
+ * sNodes by transfinite interpolation of the side curves
+ * (MappedGeometryClass_2D.f90:536-710), in the layouts of
+ * self_quadmesh_structured.  Files without the ISM-V2 version line, with
+ * side flags other than 0/1, or whose edge count disagrees with their
+ * elements are rejected (SELF_ERR_ARG). */
+int self_specmesh_counts(const char *path, int *nNodes, int *nEdges, int *nElems, int *gPolyDeg);
+int self_specmesh_read(const char *path, int N, const double *sNodes, int *edges, int *elemNodes, int *nodeType,
+                       double *nodeX, double *nodeY, double *x, double *y, double *dxds, double *dxdp, double *dyds,
+                       double *dydp, double *J, double *nHat, double *nLen);
+
 /* ------------------------------------------------- nodal library operators
  * Batched over nElems elements stored one after the other, each in the
  * reference's column-major per-element layout.  The plain forms take host

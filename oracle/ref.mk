@@ -4,11 +4,12 @@
 # unmodified from where they lie under the reference tree (never copied into
 # this repository), linked with the checker drivers ref_adv2d.f90,
 # ref_adv3d.f90, ref_cgsem2d.f90 and ref_krylov.f90, the quadrature,
-# nodal-storage and interpolation modules behind ref_nodal.f90, and the 2-D
-# advection model's pickup routines behind ref_pickup.f90.  All outputs
+# nodal-storage and interpolation modules behind ref_nodal.f90, the 2-D
+# advection model's pickup routines behind ref_pickup.f90, and the SpecMesh
+# reader and element geometry behind ref_specmesh.f90.  All outputs
 # (objects, .mod files, binaries) go to oracle/_ref/.
 #
-#   make -f oracle/ref.mk   # -> oracle/_ref/ref_{adv2d,adv3d,cgsem2d,krylov,nodal,pickup}
+#   make -f oracle/ref.mk   # -> oracle/_ref/ref_{adv2d,adv3d,cgsem2d,krylov,nodal,pickup,specmesh}
 #
 # No OpenMP, as in the reference's own build/makefile (OPT=-O3, no -fopenmp):
 # its !$OMP directives stay comments (Advection3DClass.f90 has some that
@@ -41,7 +42,8 @@ ADV3D_SRC := highend/advection3d/Advection3DParamsClass highend/advection3d/Adve
 CGSEM2D_SRC := cgsem/CGsemParams_Class cgsem/CGsemPreconditioner_2D_Class cgsem/CGsemElliptic_2D_Class
 KRYLOV_SRC := iterativesolve/IterativeSolversParams_Class iterativesolve/IterativeSolvers_Class
 
-all: $(OUT)/ref_adv2d $(OUT)/ref_adv3d $(OUT)/ref_cgsem2d $(OUT)/ref_krylov $(OUT)/ref_nodal $(OUT)/ref_pickup
+all: $(OUT)/ref_adv2d $(OUT)/ref_adv3d $(OUT)/ref_cgsem2d $(OUT)/ref_krylov $(OUT)/ref_nodal $(OUT)/ref_pickup \
+     $(OUT)/ref_specmesh
 
 # One recipe compiles the shared chain in order: each module needs the .mod
 # files of the ones before it.
@@ -69,6 +71,7 @@ $(eval $(call model,cgsem2d,$(CGSEM2D_SRC)))
 $(eval $(call model,krylov,$(KRYLOV_SRC)))
 $(eval $(call model,nodal,))
 $(eval $(call model,pickup,$(ADV2D_SRC)))
+$(eval $(call model,specmesh,))
 
 clean:
 	rm -rf $(OUT)

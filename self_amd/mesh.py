@@ -42,6 +42,37 @@ class QuadMesh:
             *[L.ptr(getattr(self, n)) for n in ("x", "y", "dxds", "dxdp", "dyds", "dydp", "J", "nHat", "nLen")]))
 
 
+class SpecMesh:
+    """An ISM-V2 SpecMesh file (QuadMesh%ReadSpecMeshFile, src/geom/
+    QuadMeshClass.f90:2100-2465): unstructured quads with curved boundaries,
+    geometry at the interpolation nodes sNodes by transfinite interpolation
+    of the side curves.  Same arrays as QuadMesh, plus elemNodes
+    (nElems, 4), the corner node types nodeType and positions nodeX, nodeY
+    (nNodes,).  Raises SelfError on a file that is not ISM-V2."""
+
+    def __init__(self, path, N: int, sNodes):
+        import ctypes as C
+        lib = L.lib()
+        nN, nEd, nE, g = C.c_int(), C.c_int(), C.c_int(), C.c_int()
+        p = str(path).encode()
+        L.check(lib.self_specmesh_counts(p, C.byref(nN), C.byref(nEd), C.byref(nE), C.byref(g)))
+        self.N, self.nNodes, self.nEdges, self.nElems, self.gPolyDeg = N, nN.value, nEd.value, nE.value, g.value
+        n1, E = N + 1, self.nElems
+        self.edges = np.zeros((self.nEdges, 6), dtype=np.int32)
+        self.elemNodes = np.zeros((E, 4), dtype=np.int32)
+        self.nodeType = np.zeros(self.nNodes, dtype=np.int32)
+        self.nodeX = np.zeros(self.nNodes)
+        self.nodeY = np.zeros(self.nNodes)
+        for name in ("x", "y", "dxds", "dxdp", "dyds", "dydp", "J"):
+            setattr(self, name, np.zeros((E, n1, n1)))
+        self.nHat = np.zeros((E, 4, n1, 2))
+        self.nLen = np.zeros((E, 4, n1))
+        L.check(lib.self_specmesh_read(
+            p, N, L.ptr(L.f64(sNodes)), L.ptr(self.edges, L.ip), L.ptr(self.elemNodes, L.ip),
+            L.ptr(self.nodeType, L.ip), L.ptr(self.nodeX), L.ptr(self.nodeY),
+            *[L.ptr(getattr(self, n)) for n in ("x", "y", "dxds", "dxdp", "dyds", "dydp", "J", "nHat", "nLen")]))
+
+
 class HexMesh:
     """Structured hex mesh of the unit cube (self_hexmesh_structured).
 

@@ -346,3 +346,35 @@ def run_pickup_read(N, nX, nY, nTracers, it, data, timeout=60):
            "rFac": r.take("<f8", (n1, n1, nTracers, E))}
     r.done()
     return out
+
+
+def run_specmesh(path, N, quadrature=1, timeout=120):
+    """The reference's ReadSpecMeshFile_QuadMesh on a mesh file with the DG
+    nodal storage of degree N (quadrature 1 Gauss, -1 Gauss-Lobatto): edges,
+    element corners, node types and positions, and the element geometry
+    (element-last shapes)."""
+    name = os.path.abspath(str(path)).encode()
+    header = np.array([N, quadrature, len(name)], dtype=np.int32).tobytes() + name
+    exe = os.path.join(REF_DIR, "ref_specmesh")
+    if not available("specmesh"):
+        raise FileNotFoundError(exe)
+    with tempfile.TemporaryDirectory() as d:
+        with open(os.path.join(d, "ref_in.bin"), "wb") as f:
+            f.write(header)
+        p = subprocess.run([exe], cwd=d, capture_output=True, text=True, timeout=timeout)
+        if p.returncode != 0:
+            raise RuntimeError(f"ref_specmesh failed ({p.returncode}): {p.stderr[-2000:]}")
+        with open(os.path.join(d, "ref_out.bin"), "rb") as f:
+            raw = f.read()
+    r = _Reader(raw)
+    nE, nEd, nNd = (int(v) for v in r.take("<i4", (3,)))
+    n1 = N + 1
+    out = {"nElems": nE, "nEdges": nEd, "nNodes": nNd, "edges": r.take("<i4", (6, nEd)),
+           "elemNodes": r.take("<i4", (4, nE)), "nodeType": r.take("<i4", (nNd,)),
+           "nodeX": r.take("<f8", (nNd,)), "nodeY": r.take("<f8", (nNd,))}
+    for k in ("x", "y", "dxds", "dxdp", "dyds", "dydp", "J"):
+        out[k] = r.take("<f8", (n1, n1, nE))
+    out["nHat"] = r.take("<f8", (2, n1, 4, nE))
+    out["nLen"] = r.take("<f8", (n1, 4, nE))
+    r.done()
+    return out
