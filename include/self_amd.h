@@ -131,6 +131,17 @@ int self_quadmesh_structured(int nX, int nY, int N, int periodicX, int periodicY
  * side flags other than 0/1, or whose edge count disagrees with their
  * elements are rejected (SELF_ERR_ARG). */
 int self_specmesh_counts(const char *path, int *nNodes, int *nEdges, int *nElems, int *gPolyDeg);
+/* ScaleTheMesh_QuadMesh (QuadMeshClass.f90:1691-1714) with
+ * ScaleGeometry_MappedGeometry_2D (MappedGeometryClass_2D.f90:921-948), in
+ * place on arrays in the layouts above: corner positions, x, y and the
+ * metric terms scaled per direction, J by xScale*yScale, and the boundary
+ * normals and lengths recomputed from the derivative of each element's
+ * position interpolant on the nodes sNodes (GenerateBoundaryMetrics,
+ * :714-788).  The shallow-water model applies it to every mesh it reads
+ * (ConservativeShallowWaterClass.f90:302-304).  nNodes may be 0 (no corners). */
+int self_quadmesh_scale(int N, const double *sNodes, int nElems, int nNodes, double xScale, double yScale,
+                        double *nodeX, double *nodeY, double *x, double *y, double *dxds, double *dxdp, double *dyds,
+                        double *dydp, double *J, double *nHat, double *nLen);
 int self_specmesh_read(const char *path, int N, const double *sNodes, int *edges, int *elemNodes, int *nodeType,
                        double *nodeX, double *nodeY, double *x, double *y, double *dxds, double *dxdp, double *dyds,
                        double *dydp, double *J, double *nHat, double *nLen);

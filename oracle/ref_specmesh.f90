@@ -7,7 +7,10 @@
 ! recipe oracle/ref.mk) on a mesh file and writes what it built.
 !
 ! Run in a directory holding ref_in.bin (stream): int32 N, quadrature
-! (1 Gauss, -1 Gauss-Lobatto), name length, then the mesh file name.
+! (1 Gauss, -1 Gauss-Lobatto), name length, the mesh file name, then
+! int32 doScale and float64 xScale, yScale: with doScale = 1 the mesh is
+! then scaled by the reference's ScaleTheMesh (QuadMeshClass.f90:1691-1714),
+! as the shallow-water model does after reading it.
 ! It writes ref_out.bin (stream):
 !   int32 nElems, nEdges, nNodes
 !   int32 edges(6,nEdges) = e1, s1, e2, s2, start, increment
@@ -24,9 +27,9 @@ PROGRAM ref_specmesh
  IMPLICIT NONE
  TYPE(NodalStorage_2D) :: ns
  TYPE(QuadMesh) :: mesh
- INTEGER :: N, quad, nameLen, nE, nEd, nNd, iEl, iEdge, iNode, i, j, b, k, uIn, uOut
+ INTEGER :: N, quad, nameLen, doScale, nE, nEd, nNd, iEl, iEdge, iNode, i, j, b, k, uIn, uOut
  INTEGER, ALLOCATABLE :: edges(:,:), en(:,:), nt(:)
- REAL(prec) :: nrm(1:2), length
+ REAL(prec) :: nrm(1:2), length, xs, ys
  REAL(prec), ALLOCATABLE :: g(:,:,:,:), nHat(:,:,:,:), nLen(:,:,:), nx(:), ny(:)
  CHARACTER(LEN=512) :: fname
 
@@ -34,10 +37,12 @@ PROGRAM ref_specmesh
    READ( uIn ) N, quad, nameLen
    fname = ' '
    READ( uIn ) fname(1:nameLen)
+   READ( uIn ) doScale, xs, ys
    CLOSE( uIn )
 
    CALL ns % Build( N, N, quad, DG )
    CALL mesh % ReadSpecMeshFile( ns % interp, TRIM(fname) )
+   IF( doScale == 1 ) CALL mesh % ScaleTheMesh( ns % interp, xs, ys )
    nE  = mesh % nElems
    nEd = mesh % nEdges
    nNd = mesh % nNodes

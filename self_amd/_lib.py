@@ -188,6 +188,7 @@ _SIGS = {
     "self_adv2d_read_pickup": ([C.c_void_p, C.c_char_p], C.c_int),
     "self_pickup_filename": ([C.c_char_p, C.c_int, C.c_char_p, C.c_int], C.c_int),
     "self_specmesh_counts": ([C.c_char_p, ip, ip, ip, ip], C.c_int),
+    "self_quadmesh_scale": ([C.c_int, dp, C.c_int, C.c_int, C.c_double, C.c_double] + [dp] * 11, C.c_int),
     "self_specmesh_read": ([C.c_char_p, C.c_int, dp, ip, ip, ip, dp, dp] + [dp] * 9, C.c_int),
     "self_sw2d_pickup_write": ([C.c_char_p, C.c_int, C.c_int, dp, dp, dp], C.c_int),
     "self_sw2d_pickup_read": ([C.c_char_p, C.c_int, C.c_int, dp, dp, dp], C.c_int),

@@ -337,9 +337,107 @@ int build_edges(const Mesh &m, int N, std::vector<int> &edges, std::vector<int> 
     return SELF_OK;
 }
 
+// EvaluateLagrangePolynomial_1D (Lagrange_1D_Class.f90:448-515)
+void lagrange_values(const Curve &c, double xe, double *l)
+{
+    bool hit = false;
+    for (int j = 0; j <= c.g; ++j) {
+        l[j] = 0.0;
+        if (almost_equal(xe, c.s[j])) {
+            l[j] = 1.0;
+            hit = true;
+        }
+    }
+    if (hit) return;
+    double t1 = 0.0;
+    for (int j = 0; j <= c.g; ++j) {
+        const double t2 = c.w[j] / (xe - c.s[j]);
+        l[j] = t2;
+        t1 = t1 + t2;
+    }
+    for (int j = 0; j <= c.g; ++j) l[j] = l[j] / t1;
+}
+
+// EvaluateDerivative_2D (Lagrange_2D_Class.f90:396-440) of the nodal
+// array f (i fastest) at (s, p): (df/ds, df/dp)
+void gradient_2d(const Curve &c, const double *f, double s, double p, double g[2])
+{
+    const int n1 = c.g + 1;
+    std::vector<double> lS(n1), lP(n1), row(n1);
+    lagrange_values(c, s, lS.data());
+    lagrange_values(c, p, lP.data());
+    double dfds = 0.0, dfdp = 0.0;
+    for (int jP = 0; jP < n1; ++jP) {
+        for (int i = 0; i < n1; ++i) row[i] = f[i + n1 * jP];
+        dfds = dfds + eval_deriv(c, row, s) * lP[jP];
+    }
+    for (int jS = 0; jS < n1; ++jS) {
+        for (int j = 0; j < n1; ++j) row[j] = f[jS + n1 * j];
+        dfdp = dfdp + eval_deriv(c, row, p) * lS[jS];
+    }
+    g[0] = dfds;
+    g[1] = dfdp;
+}
+
 }  // namespace
 
 extern "C" {
+
+int self_quadmesh_scale(int N, const double *sNodes, int nElems, int nNodes, double xScale, double yScale,
+                        double *nodeX, double *nodeY, double *x, double *y, double *dxds, double *dxdp, double *dyds,
+                        double *dydp, double *J, double *nHat, double *nLen)
+try {
+    SELF_REQUIRE(sNodes && x && y && dxds && dxdp && dyds && dydp && J && nHat && nLen && N >= 1 && N <= 64 &&
+                     nElems >= 1 && nNodes >= 0 && (nNodes == 0 || (nodeX && nodeY)),
+                 SELF_ERR_ARG, "self_quadmesh_scale: bad arguments");
+    const int n1 = N + 1;
+    const size_t np = (size_t)n1 * n1;
+    for (int i = 0; i < nNodes; ++i) {
+        nodeX[i] = xScale * nodeX[i];
+        nodeY[i] = yScale * nodeY[i];
+    }
+    const double xy = xScale * yScale;
+    for (size_t k = 0; k < np * nElems; ++k) {
+        x[k] = xScale * x[k];
+        y[k] = yScale * y[k];
+        dxds[k] = xScale * dxds[k];
+        dxdp[k] = xScale * dxdp[k];
+        dyds[k] = yScale * dyds[k];
+        dydp[k] = yScale * dydp[k];
+        J[k] = xy * J[k];
+    }
+    // GenerateBoundaryMetrics: normals from the derivative of the element's
+    // (scaled) position interpolant at the boundary points
+    Curve c;
+    c.g = N;
+    c.s.assign(sNodes, sNodes + n1);
+    c.w.resize(n1);
+    if (int rc = self_barycentric_weights(N, c.s.data(), c.w.data())) return rc;
+    for (int e = 0; e < nElems; ++e) {
+        const double *xe = x + np * e, *ye = y + np * e;
+        for (int side = 0; side < 4; ++side)
+            for (int i = 0; i <= N; ++i) {
+                const double sa = (side == 1) ? 1.0 : (side == 3) ? -1.0 : sNodes[i];
+                const double pa = (side == 0) ? -1.0 : (side == 2) ? 1.0 : sNodes[i];
+                const double sgn = (side == 0 || side == 3) ? -1.0 : 1.0;
+                double gx[2], gy[2];
+                gradient_2d(c, xe, sa, pa, gx);
+                gradient_2d(c, ye, sa, pa, gy);
+                const double Jb = gx[0] * gy[1] - gx[1] * gy[0];
+                const double signJ = std::fabs(Jb) / Jb;
+                double d0, d1;
+                if (side == 0 || side == 2) { d0 = sgn * signJ * (-gy[0]); d1 = sgn * signJ * gx[0]; }
+                else { d0 = sgn * signJ * gy[1]; d1 = sgn * signJ * (-gx[1]); }
+                const double len = std::sqrt(d0 * d0 + d1 * d1);
+                const size_t nb = i + (size_t)n1 * (side + 4 * (size_t)e);
+                nLen[nb] = len;
+                nHat[2 * nb + 0] = d0 / len;
+                nHat[2 * nb + 1] = d1 / len;
+            }
+    }
+    return SELF_OK;
+}
+SELF_CATCH_ALL
 
 int self_specmesh_counts(const char *path, int *nNodes, int *nEdges, int *nElems, int *gPolyDeg)
 try {

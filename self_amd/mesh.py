@@ -71,6 +71,17 @@ class SpecMesh:
             p, N, L.ptr(L.f64(sNodes)), L.ptr(self.edges, L.ip), L.ptr(self.elemNodes, L.ip),
             L.ptr(self.nodeType, L.ip), L.ptr(self.nodeX), L.ptr(self.nodeY),
             *[L.ptr(getattr(self, n)) for n in ("x", "y", "dxds", "dxdp", "dyds", "dydp", "J", "nHat", "nLen")]))
+        self._sNodes = L.f64(sNodes)
+
+    def ScaleTheMesh(self, xScale: float, yScale: float):
+        """QuadMesh%ScaleTheMesh (QuadMeshClass.f90:1691-1714), in place; the
+        shallow-water model applies it after reading a mesh file
+        (ConservativeShallowWaterClass.f90:302-304)."""
+        L.check(L.lib().self_quadmesh_scale(
+            self.N, L.ptr(self._sNodes), self.nElems, self.nNodes, float(xScale), float(yScale),
+            L.ptr(self.nodeX), L.ptr(self.nodeY),
+            *[L.ptr(getattr(self, n)) for n in ("x", "y", "dxds", "dxdp", "dyds", "dydp", "J", "nHat", "nLen")]))
+        return self
 
 
 class HexMesh:

@@ -2,11 +2,14 @@
 
     python -m self_amd.profile_run copy
     python -m self_amd.profile_run sw2d --precision mixed --form fused --steps 10
+    python -m self_amd.profile_run sw2d-dipole --mesh box.mesh --N 10 --form plain --precision fp64
     python -m self_amd.profile_run adv2d --steps 10
     python -m self_amd.profile_run adv3d --steps 5
     python -m self_amd.profile_run cgsem --nx 128 --precond 1
     python -m self_amd.profile_run cgsem --precision mixed --steps 2000
 
+`sw2d-dipole` runs the shallow-water model on an ISM-V2 mesh file (the
+dipole run's box.mesh, scaled by 4e5) with the dipole state.
 `copy` runs the STREAM-style copy kernel on 64 MiB (21 launches) and 1 GiB
 (11 launches): the calibration for hardware byte counters (FETCH_SIZE /
 WRITE_SIZE of a copy of known size).  Every other mode builds the named
@@ -25,7 +28,8 @@ from . import _lib as L
 
 def main(argv=None):
     ap = argparse.ArgumentParser(prog="python -m self_amd.profile_run")
-    ap.add_argument("mode", choices=["copy", "sw2d", "adv2d", "adv3d", "cgsem"])
+    ap.add_argument("mode", choices=["copy", "sw2d", "sw2d-dipole", "adv2d", "adv3d", "cgsem"])
+    ap.add_argument("--mesh", default=None, help="ISM-V2 mesh file (sw2d-dipole)")
     ap.add_argument("--precision", default="mixed", choices=["fp64", "mixed"])
     ap.add_argument("--form", default="fused", choices=["fused", "plain"])
     ap.add_argument("--nx", type=int, default=None)
@@ -45,6 +49,14 @@ def main(argv=None):
         m.ForwardStepRK3(0.0, a.steps)
         m.Synchronize()
         print(f"sw2d {a.precision} {a.form}: {a.steps} RK3 steps", flush=True)
+    elif a.mode == "sw2d-dipole":
+        from .workloads import sw2d_dipole_box
+        if not a.mesh:
+            ap.error("sw2d-dipole needs --mesh")
+        m, _ = sw2d_dipole_box(a.mesh, a.N, precision=a.precision, form=a.form)
+        m.ForwardStepRK3(0.0, a.steps)
+        m.Synchronize()
+        print(f"sw2d-dipole N={a.N} {a.precision} {a.form}: {a.steps} RK3 steps", flush=True)
     elif a.mode == "adv2d":
         from .workloads import adv2d_blob
         m, _ = adv2d_blob(nX=a.nx or 10, N=a.N)

@@ -112,6 +112,53 @@ def sw2d_geostrophic(nX=64, N=7, precision="fp64", form="fused", L=1.0e6, seed=D
     return model, {"sol": sol, "bathy": bathy, "dt": dt, "f0": f0, "g": g, "L": L}
 
 
+def sw2d_dipole_fields(x, y, g=9.81, f0=1.0e-4, vMax=1.75, x0=2.0e5, y0=2.05e5, x1=2.0e5, y1=1.95e5, L=1.0e4,
+                      hMin=200.0, hMax=3000.0, Lshelf=2.5e5):
+    """The dipole run's state (examples/shallowwater/dipole/run/
+    shallowwater.params; localmods/ShallowWater_InitialConditions.f90):
+    shelf bathymetry h = hMin + (hMax - hMin) tanh(x / Lshelf), a free-surface
+    dipole dn (exp(-c0 / 2L^2) - exp(-c1 / 2L^2)) with dn = f0 vMax L / g, and
+    the geostrophic velocity of that surface, here from its analytic gradient
+    (the run takes it from a first GlobalTimeDerivative).  Returns h, U, V, H
+    in the conservative variables."""
+    dn = f0 * vMax * L / g
+    e0 = np.exp(-0.5 * ((x - x0) ** 2 + (y - y0) ** 2) / (L * L))
+    e1 = np.exp(-0.5 * ((x - x1) ** 2 + (y - y1) ** 2) / (L * L))
+    eta = dn * (e0 - e1)
+    detadx = dn * (-(x - x0) * e0 + (x - x1) * e1) / (L * L)
+    detady = dn * (-(y - y0) * e0 + (y - y1) * e1) / (L * L)
+    h = hMin + (hMax - hMin) * np.tanh(x / Lshelf)
+    u = -(g / f0) * detady
+    v = (g / f0) * detadx
+    H = h + eta
+    return h, H * u, H * v, H
+
+
+def sw2d_dipole_box(mesh_path, N=10, precision="fp64", form="fused", dt=0.1, g=9.81, f0=1.0e-4,
+                    scale=4.0e5):
+    """The shallow-water model on the dipole run's unstructured mesh file
+    (448 elements, read with the SpecMesh reader and scaled by 4e5 as the
+    run's parameters ask), degree N, its time step and state
+    (sw2d_dipole_fields).  Returns (model, info)."""
+    from .mesh import SpecMesh
+    from .nodal import NodalStorage2D
+    from .shallow_water import ShallowWater, SWParams
+    from . import _lib as L
+
+    nodal = NodalStorage2D(N, L.GAUSS, L.DG)
+    mesh = SpecMesh(mesh_path, N, nodal.nodes).ScaleTheMesh(scale, scale)
+    p = SWParams(polyDeg=N, dt=dt, g=g, f0=f0)
+    model = ShallowWater(p, precision=precision, form=form, mesh=mesh, nodal=nodal)
+    h, U, V, H = sw2d_dipole_fields(mesh.x, mesh.y, g=g, f0=f0)
+    sol = np.stack([U, V, H], axis=1)
+    bathy = np.stack([h, np.zeros_like(h), np.zeros_like(h)], axis=1)
+    model.SetBathymetry(bathy)
+    model.SetPlanetaryVorticity(np.full_like(h, f0))
+    model.SetSolution(sol)
+    model.CalculateBathymetryGradient()
+    return model, {"sol": sol, "bathy": bathy, "dt": dt, "mesh": mesh}
+
+
 def adv2d_blob_fields(x, y, x0=0.5, y0=0.5, width=0.1, amp=1.0):
     """Gaussian tracer blob amp * exp(-r^2 / (2 width^2)), r measured from
     (x0, y0) (the 2 sigma^2 convention of the commented-out initial condition

@@ -3,9 +3,11 @@ curved quarter annulus (tests/specmesh_gen.py; rotated corner orderings,
 one clockwise element) and what the reference's own ReadSpecMeshFile_QuadMesh
 (src/geom/QuadMeshClass.f90:2100-2465, compiled unmodified by oracle/ref.mk
 into oracle/_ref/ref_specmesh) built from it at N = 3 on Gauss nodes and
-N = 5 on Gauss-Lobatto nodes.
+N = 5 on Gauss-Lobatto nodes, and at N = 4 on Gauss nodes followed by its
+ScaleTheMesh(2, 3) (QuadMeshClass.f90:1691-1714).
 
-The fixture holds the file's text (uint8) and, per case '<N>_<quad>_', the
+The fixture holds the file's text (uint8) and, per case '<N>_<quad>_'
+('<N>_<quad>_s_' when scaled), the
 edge table, corners, node types, corner positions and the element geometry
 in the reference's element-last shapes.
 
@@ -24,7 +26,12 @@ import ref_run as R  # noqa: E402
 import specmesh_gen as G  # noqa: E402
 
 NR, NT, GDEG, SEED, MIRROR = 3, 4, 4, 1, (5,)
-CASES = ((3, 1), (5, -1))
+# (N, quadrature, ScaleTheMesh factors or None)
+CASES = ((3, 1, None), (5, -1, None), (4, 1, (2.0, 3.0)))
+
+
+def tag(N, quad, scale):
+    return f"{N}_{quad}_" + ("" if scale is None else "s_")
 
 
 def main():
@@ -34,10 +41,10 @@ def main():
         path = os.path.join(d, "annulus.mesh")
         with open(path, "w") as f:
             f.write(text)
-        for N, quad in CASES:
-            ref = R.run_specmesh(path, N, quad)
+        for N, quad, scale in CASES:
+            ref = R.run_specmesh(path, N, quad, scale=scale)
             for k, v in ref.items():
-                out[f"{N}_{quad}_{k}"] = np.asarray(v)
+                out[tag(N, quad, scale) + k] = np.asarray(v)
     dst = os.path.join(HERE, "ref_specmesh_annulus.npz")
     np.savez_compressed(dst, **out)
     print(dst, os.path.getsize(dst))

@@ -348,13 +348,16 @@ def run_pickup_read(N, nX, nY, nTracers, it, data, timeout=60):
     return out
 
 
-def run_specmesh(path, N, quadrature=1, timeout=120):
+def run_specmesh(path, N, quadrature=1, scale=None, timeout=120):
     """The reference's ReadSpecMeshFile_QuadMesh on a mesh file with the DG
-    nodal storage of degree N (quadrature 1 Gauss, -1 Gauss-Lobatto): edges,
-    element corners, node types and positions, and the element geometry
-    (element-last shapes)."""
+    nodal storage of degree N (quadrature 1 Gauss, -1 Gauss-Lobatto), then,
+    with scale = (xScale, yScale), its ScaleTheMesh: edges, element corners,
+    node types and positions, and the element geometry (element-last
+    shapes)."""
     name = os.path.abspath(str(path)).encode()
     header = np.array([N, quadrature, len(name)], dtype=np.int32).tobytes() + name
+    header += np.array([0 if scale is None else 1], dtype=np.int32).tobytes()
+    header += np.array(scale or (1.0, 1.0), dtype=np.float64).tobytes()
     exe = os.path.join(REF_DIR, "ref_specmesh")
     if not available("specmesh"):
         raise FileNotFoundError(exe)

@@ -78,13 +78,16 @@ def write(tmp_path, text, name="m.mesh"):
 
 
 # ---------------------------------------------------------------- golden
-@pytest.mark.parametrize("N,quad", [(3, 1), (5, -1)])
-def test_reader_matches_the_reference_fixture(tmp_path, N, quad):
+@pytest.mark.parametrize("N,quad,scale", [(3, 1, None), (5, -1, None), (4, 1, (2.0, 3.0))])
+def test_reader_matches_the_reference_fixture(tmp_path, N, quad, scale):
     g = np.load(GOLDEN)
     path = tmp_path / "annulus.mesh"
     path.write_bytes(g["mesh"].tobytes())
-    ref = {k[len(f"{N}_{quad}_"):]: g[k] for k in g.files if k.startswith(f"{N}_{quad}_")}
+    tag = f"{N}_{quad}_" + ("" if scale is None else "s_")
+    ref = {k[len(tag):]: g[k] for k in g.files if k.startswith(tag) and "_" not in k[len(tag):]}
     m = read(path, N, quad)
+    if scale is not None:
+        m.ScaleTheMesh(*scale)
     assert (m.nElems, m.nEdges, m.nNodes) == (int(ref["nElems"]), int(ref["nEdges"]), int(ref["nNodes"]))
     assert mismatches(m, ref) == []
     # the fixture exercises reversed neighbours and a negative Jacobian
@@ -118,6 +121,18 @@ def test_live_reference_generated(tmp_path, nr, nt, g, seed, mirror, N, quad):
 def test_live_reference_mesh_files(rel):
     path = os.path.join(REF_ROOT, rel)
     assert mismatches(read(path, 4), R.run_specmesh(path, 4, 1)) == []
+
+
+@live
+@pytest.mark.skipif(not os.path.isdir(REF_ROOT), reason="reference mesh files absent")
+@pytest.mark.parametrize("rel,N,quad,scale", [
+    ("examples/shallowwater/dipole/run/box.mesh", 10, 1, (4.0e5, 4.0e5)),  # the dipole run's own
+    ("src/geom/Testing/MeshFiles/CircleFlow.mesh", 5, -1, (1.5, 0.5)),
+])
+def test_live_reference_scale_the_mesh(rel, N, quad, scale):
+    path = os.path.join(REF_ROOT, rel)
+    m = read(path, N, quad).ScaleTheMesh(*scale)
+    assert mismatches(m, R.run_specmesh(path, N, quad, scale=scale)) == []
 
 
 # --------------------------------------------------------------- closed form
@@ -179,6 +194,40 @@ def test_straight_sides_are_the_bilinear_map(tmp_path):
     m = read(path, 5, -1)
     m.x[:] = m.x.transpose(0, 2, 1)
     assert bilinear_error(m, s) > 1e-3
+
+
+def tangent_errors(m):
+    """Largest |nHat . t| over the sides of a straight-sided mesh, t the
+    side's direction from its (scaled) corner positions, and the largest
+    |nLen - |t| / 2| (the side's half length: d/ds of the bilinear map)."""
+    err = 0.0
+    for e in range(m.nElems):
+        c = m.elemNodes[e] - 1
+        for k in range(4):
+            a, b = c[G.EDGE[k][0]], c[G.EDGE[k][1]]
+            t = np.array([m.nodeX[b] - m.nodeX[a], m.nodeY[b] - m.nodeY[a]])
+            err = max(err, np.abs(m.nHat[e, k] @ t).max() / np.hypot(*t),
+                      np.abs(m.nLen[e, k] - 0.5 * np.hypot(*t)).max() / np.hypot(*t))
+    return err
+
+
+def test_scaling_recomputes_normals(tmp_path):
+    path = straight_mesh(tmp_path)
+    s = nodes(4, 1)[0]
+    m = read(path, 4)
+    keep = m.nHat.copy()
+    m.ScaleTheMesh(2.0, 3.0)
+    assert tangent_errors(m) < 1e-13 and bilinear_error(m, s) < 1e-13
+    # planted mistake: normals left as they were before scaling
+    m.nHat[:] = keep
+    assert tangent_errors(m) > 1e-2
+    # uniform scaling: same directions, lengths scaled
+    u = read(path, 4)
+    u0 = read(path, 4)
+    u.ScaleTheMesh(4.0e5, 4.0e5)
+    assert np.allclose(u.nHat, u0.nHat, atol=1e-14)
+    assert np.allclose(u.nLen, 4.0e5 * u0.nLen, rtol=1e-14)
+    assert np.array_equal(u.J, (4.0e5 * 4.0e5) * u0.J)
 
 
 def area_and_arc_errors(m, s, w):
