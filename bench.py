@@ -194,6 +194,18 @@ def bench_sw2d_dipole(N=10, steps=200, warm=10):
                 r["algorithmic_bytes_per_launch"] = b
                 r["achieved_GBs"] = round(b / (stage_ms * 1e-3) / 1e9, 1)
                 sols[name] = m.GetSolution()
+                if prec == "fp64":
+                    # eight members of this case in one launch per stage fill the CUs
+                    # the single 448-element state leaves idle
+                    m.SetMembers(8)
+                    for k in range(8):
+                        sol = info["sol"].copy()
+                        sol[:, 2] += 0.01 * k
+                        m.SetMemberSolution(k, sol)
+                    m.ForwardStepRK3(0.0, warm)
+                    ens_ms = m.ForwardStepRK3Timed(0.0, steps) / steps / 3.0
+                    r["members8_member_stage_us"] = 1e3 * ens_ms / 8
+                    r["members8_speedup_vs_single"] = 8 * stage_ms / ens_ms
             out[name] = r
             m.Trash()
     a, b = sols["mixed_fused"], sols["fp64_fused"]
