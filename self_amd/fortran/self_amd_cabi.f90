@@ -193,6 +193,34 @@ MODULE SELF_AMD_CABI
 
     ! ------------------------------------------------------------ pickup files and ensembles
     ! path / model: NUL-terminated (pass TRIM(name)//C_NULL_CHAR)
+    ! ISM-V2 SpecMesh files and ScaleTheMesh (QuadMeshClass.f90:2100-2465, :1691-1714)
+    INTEGER(C_INT) FUNCTION self_specmesh_counts(path, nNodes, nEdges, nElems, gPolyDeg) &
+                                                 BIND(C, name="self_specmesh_counts")
+      IMPORT :: C_INT, C_CHAR
+      CHARACTER(KIND=C_CHAR), INTENT(in) :: path(*)
+      INTEGER(C_INT), INTENT(out) :: nNodes, nEdges, nElems, gPolyDeg
+    END FUNCTION
+    INTEGER(C_INT) FUNCTION self_specmesh_read(path, N, sNodes, edges, elemNodes, nodeType, nodeX, nodeY, x, y, &
+                                               dxds, dxdp, dyds, dydp, J, nHat, nLen) &
+                                               BIND(C, name="self_specmesh_read")
+      IMPORT :: C_INT, C_DOUBLE, C_CHAR
+      CHARACTER(KIND=C_CHAR), INTENT(in) :: path(*)
+      INTEGER(C_INT), VALUE :: N
+      REAL(C_DOUBLE), INTENT(in) :: sNodes(*)
+      INTEGER(C_INT), INTENT(out) :: edges(*), elemNodes(*), nodeType(*)
+      REAL(C_DOUBLE), INTENT(out) :: nodeX(*), nodeY(*), x(*), y(*), dxds(*), dxdp(*), dyds(*), dydp(*), J(*)
+      REAL(C_DOUBLE), INTENT(out) :: nHat(*), nLen(*)
+    END FUNCTION
+    INTEGER(C_INT) FUNCTION self_quadmesh_scale(N, sNodes, nElems, nNodes, xScale, yScale, nodeX, nodeY, x, y, &
+                                                dxds, dxdp, dyds, dydp, J, nHat, nLen) &
+                                                BIND(C, name="self_quadmesh_scale")
+      IMPORT :: C_INT, C_DOUBLE
+      INTEGER(C_INT), VALUE :: N, nElems, nNodes
+      REAL(C_DOUBLE), VALUE :: xScale, yScale
+      REAL(C_DOUBLE), INTENT(in) :: sNodes(*)
+      REAL(C_DOUBLE), INTENT(inout) :: nodeX(*), nodeY(*), x(*), y(*), dxds(*), dxdp(*), dyds(*), dydp(*), J(*)
+      REAL(C_DOUBLE), INTENT(inout) :: nHat(*), nLen(*)
+    END FUNCTION
     INTEGER(C_INT) FUNCTION self_pickup_filename(model, iter, out, outLen) BIND(C, name="self_pickup_filename")
       IMPORT :: C_INT, C_CHAR
       CHARACTER(KIND=C_CHAR), INTENT(in) :: model(*)

@@ -25,6 +25,9 @@ MODULE ShallowWaterAMD_Class
     REAL(C_DOUBLE) :: linearDrag = 0.0_C_DOUBLE, warp = 0.0_C_DOUBLE
     LOGICAL        :: periodicX = .TRUE., periodicY = .TRUE.
     INTEGER        :: precision = SELF_FP64, form = SELF_FUSED
+    ! an ISM-V2 mesh file instead of the structured nXElem x nYElem mesh
+    ! (SWParams % SpecMeshFile; blank = the structured mesh)
+    CHARACTER(LEN=256) :: SpecMeshFile = ' '
   END TYPE SWParamsAMD
 
   TYPE ShallowWaterAMD
@@ -59,7 +62,10 @@ CONTAINS
   SUBROUTINE Build_ShallowWaterAMD(myDGSEM, params)
     CLASS(ShallowWaterAMD), INTENT(inout) :: myDGSEM
     TYPE(SWParamsAMD), INTENT(in)         :: params
-    INTEGER(C_INT) :: N, nE, nEd, px, py
+    INTEGER(C_INT) :: N, nE, nEd, px, py, nNd, gDeg
+    INTEGER(C_INT), ALLOCATABLE :: elemNodes(:,:), nodeType(:)
+    REAL(C_DOUBLE), ALLOCATABLE :: nodeX(:), nodeY(:)
+    CHARACTER(KIND=C_CHAR, LEN=257) :: path
     REAL(C_DOUBLE), ALLOCATABLE :: dMatS(:,:), dMatP(:,:), lW(:), lE(:), lS(:), lN(:)
     REAL(C_DOUBLE), ALLOCATABLE :: dxds(:,:,:), dxdp(:,:,:), dyds(:,:,:), dydp(:,:,:), J(:,:,:)
     REAL(C_DOUBLE), ALLOCATABLE :: nHat(:,:,:,:), nLen(:,:,:)
@@ -74,18 +80,29 @@ CONTAINS
     ALLOCATE(lW(0:N), lE(0:N), lS(0:N), lN(0:N))
     CALL self_check(self_nodal2d_build(N, SELF_GAUSS, SELF_DG, 0_C_INT, myDGSEM % nodes, myDGSEM % qWeight, &
                                        dMatS, dMatP, lW, lE, lS, lN), 'self_nodal2d_build')
-    px = MERGE(1, 0, params % periodicX)
-    py = MERGE(1, 0, params % periodicY)
-    CALL self_check(self_quadmesh_counts(params % nXElem, params % nYElem, px, py, nE, nEd), 'self_quadmesh_counts')
-    myDGSEM % nElems = nE
-    myDGSEM % nEdges = nEd
-    ALLOCATE(edges(6, nEd), myDGSEM % x(0:N,0:N,nE), myDGSEM % y(0:N,0:N,nE))
-    ALLOCATE(dxds(0:N,0:N,nE), dxdp(0:N,0:N,nE), dyds(0:N,0:N,nE), dydp(0:N,0:N,nE), J(0:N,0:N,nE))
-    ALLOCATE(nHat(2,0:N,4,nE), nLen(0:N,4,nE))
-    CALL self_check(self_quadmesh_structured(params % nXElem, params % nYElem, N, px, py, params % xScale, &
-                                             params % yScale, params % warp, myDGSEM % nodes, edges, &
-                                             myDGSEM % x, myDGSEM % y, dxds, dxdp, dyds, dydp, J, nHat, nLen), &
-                    'self_quadmesh_structured')
+    IF (LEN_TRIM(params % SpecMeshFile) > 0) THEN
+      ! BuildQuadMesh (:289-304): ReadSpecMeshFile, then ScaleTheMesh
+      path = TRIM(params % SpecMeshFile)//C_NULL_CHAR
+      CALL self_check(self_specmesh_counts(path, nNd, nEd, nE, gDeg), 'self_specmesh_counts')
+      CALL alloc_geometry()
+      ALLOCATE(elemNodes(4, nE), nodeType(nNd), nodeX(nNd), nodeY(nNd))
+      CALL self_check(self_specmesh_read(path, N, myDGSEM % nodes, edges, elemNodes, nodeType, nodeX, nodeY, &
+                                         myDGSEM % x, myDGSEM % y, dxds, dxdp, dyds, dydp, J, nHat, nLen), &
+                      'self_specmesh_read')
+      CALL self_check(self_quadmesh_scale(N, myDGSEM % nodes, nE, nNd, params % xScale, params % yScale, nodeX, &
+                                          nodeY, myDGSEM % x, myDGSEM % y, dxds, dxdp, dyds, dydp, J, nHat, nLen), &
+                      'self_quadmesh_scale')
+    ELSE
+      px = MERGE(1, 0, params % periodicX)
+      py = MERGE(1, 0, params % periodicY)
+      CALL self_check(self_quadmesh_counts(params % nXElem, params % nYElem, px, py, nE, nEd), &
+                      'self_quadmesh_counts')
+      CALL alloc_geometry()
+      CALL self_check(self_quadmesh_structured(params % nXElem, params % nYElem, N, px, py, params % xScale, &
+                                               params % yScale, params % warp, myDGSEM % nodes, edges, &
+                                               myDGSEM % x, myDGSEM % y, dxds, dxdp, dyds, dydp, J, nHat, nLen), &
+                      'self_quadmesh_structured')
+    ENDIF
     desc % N = N
     desc % nElems = nE
     desc % nEdges = nEd
@@ -111,6 +128,16 @@ CONTAINS
     myDGSEM % hostSolutionNewer = .TRUE.
     myDGSEM % hostBathymetryNewer = .TRUE.
     myDGSEM % hostVorticityNewer = .TRUE.
+
+  CONTAINS
+
+    SUBROUTINE alloc_geometry()
+      myDGSEM % nElems = nE
+      myDGSEM % nEdges = nEd
+      ALLOCATE(edges(6, nEd), myDGSEM % x(0:N,0:N,nE), myDGSEM % y(0:N,0:N,nE))
+      ALLOCATE(dxds(0:N,0:N,nE), dxdp(0:N,0:N,nE), dyds(0:N,0:N,nE), dydp(0:N,0:N,nE), J(0:N,0:N,nE))
+      ALLOCATE(nHat(2,0:N,4,nE), nLen(0:N,4,nE))
+    END SUBROUTINE alloc_geometry
   END SUBROUTINE Build_ShallowWaterAMD
 
   SUBROUTINE Trash_ShallowWaterAMD(myDGSEM)

@@ -2,7 +2,9 @@
 ! Build, per-element SetSolution/SetBathymetry, CalculateBathymetryGradient,
 ! ForwardStepRK3(tn) in a loop, per-element GetSolution (the pattern of
 ! src/highend/shallowwater/ShallowWater_Driver.f90:60-66).
-! Usage: sw_driver OUTFILE PRECISION(0|1) FORM(0|1) NSTEPS
+! Usage: sw_driver OUTFILE PRECISION(0|1) FORM(0|1) NSTEPS [MESHFILE]
+! With MESHFILE the model reads that ISM-V2 mesh (SpecMeshFile), scaled by
+! the same xScale, yScale, instead of the structured 6 x 5 mesh.
 ! Writes (stream, native endian): initial solution, bathymetry (after the
 ! gradient), final solution, each (0:N,0:N,1:3,1:nElems) FP64; then
 ! WritePickup(NSTEPS) (ShallowWater.<NSTEPS>.pickup in the working
@@ -28,6 +30,7 @@ PROGRAM sw_driver
   READ(arg, *) p % form
   CALL GET_COMMAND_ARGUMENT(4, arg)
   READ(arg, *) nSteps
+  IF (COMMAND_ARGUMENT_COUNT() >= 5) CALL GET_COMMAND_ARGUMENT(5, p % SpecMeshFile)
   p % polyDeg = 5
   p % nXElem = 6
   p % nYElem = 5

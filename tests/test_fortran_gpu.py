@@ -46,3 +46,44 @@ def test_fortran_driver_matches_python_host_path(gpu_lib, tmp_path, precision, f
     assert np.array_equal(m.GetSolution(), final)
     assert np.abs(final - sol0).max() > 1e-6  # the state moved
     m.Trash()
+
+
+@pytest.mark.parametrize("precision", [0, 1])
+def test_fortran_driver_on_a_spec_mesh_file(gpu_lib, tmp_path, precision):
+    """SWParamsAMD % SpecMeshFile: the drop-in type reads an ISM-V2 mesh and
+    scales it (BuildQuadMesh, ConservativeShallowWaterClass.f90:289-304); the
+    result is bitwise the Python path on SpecMesh(...).ScaleTheMesh(2, 3)."""
+    import specmesh_gen as G
+    from self_amd.mesh import SpecMesh
+    from self_amd.nodal import NodalStorage2D
+    from self_amd.shallow_water import ShallowWater, SWParams
+    from self_amd import _lib as L
+    assert os.path.exists(DRIVER), "Fortran driver not built (run __graft_entry__.build())"
+    mesh_file = tmp_path / "annulus.mesh"
+    mesh_file.write_text(G.quarter_annulus(3, 4, 5, seed=4, names=("NO_NORMAL_FLOW",) * 4))
+    out = tmp_path / "sw.bin"
+    nSteps = 5
+    print(f"fortran driver on {mesh_file.name}, precision {precision}", flush=True)
+    r = subprocess.run([DRIVER, str(out), str(precision), "1", str(nSteps), str(mesh_file)], capture_output=True,
+                       text=True, timeout=120, cwd=str(tmp_path))
+    assert r.returncode == 0, r.stdout + r.stderr
+    N = 5
+    nodal = NodalStorage2D(N, L.GAUSS, L.DG)
+    mesh = SpecMesh(mesh_file, N, nodal.nodes).ScaleTheMesh(2.0, 3.0)
+    E, n1 = mesh.nElems, N + 1
+    raw = np.fromfile(out, dtype=np.float64)
+    assert raw.size == 5 * E * 3 * n1 * n1
+    sol0, bathy, final, after, resumed = (a.reshape(E, 3, n1, n1) for a in np.split(raw, 5))
+    assert np.array_equal(resumed, after)
+    p = SWParams(polyDeg=N, xScale=2.0, yScale=3.0, dt=1e-3, g=9.81, f0=0.1)
+    m = ShallowWater(p, precision=("fp64", "mixed")[precision], form="fused", mesh=mesh, nodal=nodal)
+    b = bathy.copy()
+    b[:, 1:] = 0.0
+    m.SetBathymetry(b)
+    m.CalculateBathymetryGradient()
+    assert np.array_equal(m.GetBathymetry(), bathy)
+    m.SetSolution(sol0)
+    m.ForwardStepRK3(0.0, nSteps)
+    assert np.array_equal(m.GetSolution(), final)
+    assert np.abs(final - sol0).max() > 1e-6
+    m.Trash()
