@@ -211,7 +211,13 @@ int parse(const char *path, Mesh &m)
     std::string line;
     auto next = [&](const char *what) -> int {
         SELF_REQUIRE(std::getline(in, line), SELF_ERR_ARG, "SpecMesh %s: file ends before %s", path, what);
+        if (!line.empty() && line.back() == '\r') line.pop_back();  // CRLF files
         return SELF_OK;
+    };
+    // list-directed READ also takes Fortran's D exponent (1.5D+02)
+    auto fortran_reals = [&]() {
+        for (char &ch : line)
+            if (ch == 'D' || ch == 'd') ch = 'E';
     };
     int rc;
     if ((rc = next("the version line"))) return rc;
@@ -229,6 +235,7 @@ int parse(const char *path, Mesh &m)
     m.ny.resize(m.nNodes);
     for (int i = 0; i < m.nNodes; ++i) {
         if ((rc = next("a node"))) return rc;
+        fortran_reals();
         std::istringstream ss(line);
         SELF_REQUIRE(ss >> m.nx[i] >> m.ny[i], SELF_ERR_ARG, "SpecMesh %s: bad node line %d", path, i + 1);
     }
@@ -265,6 +272,7 @@ int parse(const char *path, Mesh &m)
                 Y.resize(m.g + 1);
                 for (int j = 0; j <= m.g; ++j) {
                     if ((rc = next("a boundary curve"))) return rc;
+                    fortran_reals();
                     std::istringstream ss(line);
                     SELF_REQUIRE(ss >> X[j] >> Y[j], SELF_ERR_ARG, "SpecMesh %s: element %d curve point", path,
                                  e + 1);

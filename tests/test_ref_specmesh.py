@@ -319,6 +319,23 @@ def test_edge_table_follows_the_corners(tmp_path):
     assert orientation_errors(m) > 0
 
 
+def test_fortran_exponents_and_crlf_read_the_same(tmp_path):
+    """Node and curve lines written with Fortran's D exponent, and CRLF line
+    ends, read as the plain file (list-directed READ accepts both)."""
+    text = G.quarter_annulus(2, 3, 4, seed=8)
+    lines = text.split("\n")
+    nNodes = int(lines[1].split()[0])
+    for i in range(2, 2 + nNodes):
+        x, y, z = (float(v) for v in lines[i].split())
+        lines[i] = f"{x:.17E} {y:.17E} {z:.1E}".replace("E", "D")
+    a = read(write(tmp_path, text, "a.mesh"), 4)
+    b = read(write(tmp_path, "\r\n".join(lines), "b.mesh"), 4)
+    assert mismatches(b, {"edges": a.edges.T, "elemNodes": a.elemNodes.T, "nodeType": a.nodeType,
+                          "nodeX": a.nodeX, "nodeY": a.nodeY, **{k: getattr(a, k).transpose(2, 1, 0) for k in
+                          ("x", "y", "dxds", "dxdp", "dyds", "dydp", "J", "nLen")},
+                          "nHat": a.nHat.transpose(3, 2, 1, 0)}) == []
+
+
 # --------------------------------------------------------------- bad input
 def test_rejects_files_it_cannot_read(tmp_path):
     text = G.quarter_annulus(2, 2, 2, seed=1)
