@@ -24,14 +24,17 @@ def available(model="adv2d") -> bool:
     return os.access(os.path.join(REF_DIR, f"ref_{model}"), os.X_OK)
 
 
-def _execute(model, params_text, input_arrays, header, timeout, params_file="advection.params"):
-    """Writes the namelist file (params_file) and ref_in.bin into a fresh
-    directory, runs oracle/_ref/ref_<model> there and returns the bytes of
-    ref_out.bin."""
+def _execute(model, params_text, input_arrays, header, timeout, params_file="advection.params", place=None):
+    """Writes the namelist file (params_file), ref_in.bin and the files of
+    `place` ({name: text}) into a fresh directory, runs oracle/_ref/ref_<model>
+    there and returns the bytes of ref_out.bin."""
     exe = os.path.join(REF_DIR, f"ref_{model}")
     if not available(model):
         raise FileNotFoundError(exe)
     with tempfile.TemporaryDirectory() as d:
+        for name, text in (place or {}).items():
+            with open(os.path.join(d, name), "w") as f:
+                f.write(text)
         with open(os.path.join(d, params_file), "w") as f:
             f.write(params_text)
         with open(os.path.join(d, "ref_in.bin"), "wb") as f:
@@ -72,26 +75,32 @@ def random_inputs(N, nX, nY, nTracers, seed):
     return {"c": c, "u": u, "v": v, "relax": rl, "rFac": rf}
 
 
-def _params_text(N, nX, nY, nTracers, kappa, dt, xScale, yScale):
+def _params_text(N, nX, nY, nTracers, kappa, dt, xScale, yScale, spec_mesh=None):
+    mesh = f" SpecMeshFile = '{spec_mesh}',\n" if spec_mesh else ""
     return ("&TracerSetup\n"
             f" nTracers = {nTracers},\n kappa = {kappa!r},\n/\n"
             "&TimeManagement\n"
             f" dt = {dt!r},\n iterInit = 0,\n nTimeSteps = 0,\n dumpFreq = 1,\n/\n"
-            "&SpaceManagement\n"
+            "&SpaceManagement\n" + mesh +
             f" polyDeg = {N},\n nXElem = {nX},\n nYElem = {nY},\n nPlot = 4,\n"
             f" xScale = {xScale!r},\n yScale = {yScale!r},\n/\n")
 
 
-def run(N, nX, nY, nTracers, kappa, dt, t0, nSteps, inputs, xScale=1.0, yScale=1.0, timeout=120):
+def run(N, nX, nY, nTracers, kappa, dt, t0, nSteps, inputs, xScale=1.0, yScale=1.0, timeout=120, mesh_text=None):
     """Returns a dict: the inputs, the reference's nodal arrays (dMatS,
     dMatP (n1, n1), qWeightS, qWeightP, lagWest/East/South/North), its mesh
     (edges (nEdges, 6) int32, x, y, dxds, dxdp, dyds, dydp, J (n1, n1, E),
     nHat (2, n1, 4, E), nLen (n1, 4, E)), the tendency and gradients after
-    GlobalTimeDerivative(t0) and the solution after nSteps RK3 steps."""
+    GlobalTimeDerivative(t0) and the solution after nSteps RK3 steps.
+    With mesh_text the model reads that ISM-V2 file (SpecMeshFile) instead
+    of its default nX x nY mesh; the inputs then have nX * nY = its element
+    count."""
     n1, E = N + 1, nX * nY
-    raw = _execute("adv2d", _params_text(N, nX, nY, nTracers, kappa, dt, xScale, yScale),
+    place = {"m.mesh": mesh_text} if mesh_text else None
+    raw = _execute("adv2d", _params_text(N, nX, nY, nTracers, kappa, dt, xScale, yScale,
+                                         "m.mesh" if mesh_text else None),
                    [inputs[k] for k in ("c", "u", "v", "relax", "rFac")],
-                   np.int32(nSteps).tobytes() + np.float64(t0).tobytes(), timeout)
+                   np.int32(nSteps).tobytes() + np.float64(t0).tobytes(), timeout, place=place)
     r = _Reader(raw)
     take = r.take
     out = dict(inputs)
