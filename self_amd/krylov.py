@@ -110,24 +110,35 @@ class CGsemElliptic2D:
     Arrays are element-local (nElems, N+1, N+1) indexed [e, j(p), i(s)],
     the reference's solution(0:N,0:N,1:nElems)."""
 
-    def __init__(self, params: CGsemParams):
+    def __init__(self, params: CGsemParams, mesh=None):
+        """mesh: a SpecMesh read on the GLL nodes of degree polyDeg (the
+        reference's SpecMeshFile, BuildQuadMesh :169-179); default the
+        structured nXElem x nYElem mesh (LoadDefaultMesh)."""
         self.params = params
         N = params.polyDeg
         self.nS = self.nP = N
         self.nodes, self.qWeight = legendre_quadrature(N, L.GAUSS_LOBATTO)
-        self.mesh = QuadMesh(params.nXElem, params.nYElem, N, self.nodes, False, False, params.xScale,
-                             params.yScale, params.warp)
-        self.nElems = self.mesh.nElems
-        nX, nY = params.nXElem, params.nYElem
-        # corner node ids of LoadDefaultMesh (walled): SW, SE, NE, NW
-        en = np.zeros((self.nElems, 4), dtype=np.int32)
-        for iY in range(1, nY + 1):
-            for iX in range(1, nX + 1):
-                e = iX + (iY - 1) * nX - 1
-                n0 = iX + (iY - 1) * (nX + 1)
-                en[e] = (n0, n0 + 1, n0 + 1 + nX + 1, n0 + nX + 1)
-        self.elemNodes = np.ascontiguousarray(en.ravel())
-        nNodes = (nX + 1) * (nY + 1)
+        if mesh is not None:
+            if mesh.N != N or not np.array_equal(mesh._sNodes, self.nodes):
+                raise L.SelfError(L.SELF_ERR_ARG, "CGsemElliptic2D: the mesh must be read on the GLL nodes of degree N")
+            self.mesh = mesh
+            self.nElems = mesh.nElems
+            self.elemNodes = np.ascontiguousarray(mesh.elemNodes.ravel())
+            nNodes = mesh.nNodes
+        else:
+            self.mesh = QuadMesh(params.nXElem, params.nYElem, N, self.nodes, False, False, params.xScale,
+                                 params.yScale, params.warp)
+            self.nElems = self.mesh.nElems
+            nX, nY = params.nXElem, params.nYElem
+            # corner node ids of LoadDefaultMesh (walled): SW, SE, NE, NW
+            en = np.zeros((self.nElems, 4), dtype=np.int32)
+            for iY in range(1, nY + 1):
+                for iX in range(1, nX + 1):
+                    e = iX + (iY - 1) * nX - 1
+                    n0 = iX + (iY - 1) * (nX + 1)
+                    en[e] = (n0, n0 + 1, n0 + 1 + nX + 1, n0 + nX + 1)
+            self.elemNodes = np.ascontiguousarray(en.ravel())
+            nNodes = (nX + 1) * (nY + 1)
         m = self.mesh
         h = C.c_void_p()
         L.check(L.lib().self_cgsem2d_create(N, m.nElems, m.nEdges, nNodes, L.ptr(m.edges, L.ip),

@@ -211,7 +211,7 @@ def random_inputs_cgsem2d(N, nX, nY, seed):
             "solution": rng.uniform(-0.5, 0.5, shp), "u": rng.uniform(-1, 1, shp), "v": rng.uniform(-1, 1, shp)}
 
 
-def run_cgsem2d(N, nX, nY, maxIters, tolerance, dirValue, inputs, timeout=300):
+def run_cgsem2d(N, nX, nY, maxIters, tolerance, dirValue, inputs, timeout=300, mesh_text=None):
     """The reference's CGsemElliptic_2D with pcIterates = 0 (identity
     preconditioner).  Returns a dict: the inputs; dMat, qWeight; edges
     (nEdges, 6) int32, nodeType (nNodes,) int32; x, y, dxds, dxdp, dyds,
@@ -222,10 +222,11 @@ def run_cgsem2d(N, nX, nY, maxIters, tolerance, dirValue, inputs, timeout=300):
     params = ("&SolverCriteria\n"
               f" MaximumIterates = {maxIters},\n pcIterates = 0,\n tolerance = {tolerance!r},\n"
               " pcTolerance = 1.0e-4,\n pcDegree = 1,\n/\n"
-              "&SpaceManagement\n"
+              "&SpaceManagement\n" + (" SpecMeshFile = 'm.mesh',\n" if mesh_text else "") +
               f" polyDeg = {N},\n nXElem = {nX},\n nYElem = {nY},\n nPlot = 4,\n xScale = 1.0,\n/\n")
     raw = _execute("cgsem2d", params, [inputs[k] for k in ("fluxCoeff", "source", "solution", "u", "v")],
-                   np.float64(dirValue).tobytes(), timeout, params_file="runtime.params")
+                   np.float64(dirValue).tobytes(), timeout, params_file="runtime.params",
+                   place={"m.mesh": mesh_text} if mesh_text else None)
     r = _Reader(raw)
     take = r.take
     out = dict(inputs)
