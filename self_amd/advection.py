@@ -36,6 +36,9 @@ class AdvectionParams:
     periodicX: bool = True
     periodicY: bool = True
     warp: float = 0.0
+    # SpaceManagement % SpecMeshFile: an ISM-V2 mesh file read (and scaled by
+    # xScale, yScale) instead of the structured nXElem x nYElem mesh
+    SpecMeshFile: str = ""
 
 
 class Advection:
@@ -52,6 +55,11 @@ class Advection:
         self.nS = self.nP = N
         self.nEq = params.nTracers
         self.dgStorage = nodal or NodalStorage2D(N, L.GAUSS, L.DG, quirk_dmatp_transposed)
+        if mesh is None and params.SpecMeshFile:
+            # BuildQuadMesh (AdvectionClass.f90:303-330): read, then scale
+            from .mesh import SpecMesh
+            mesh = SpecMesh(params.SpecMeshFile, N, self.dgStorage.nodes).ScaleTheMesh(params.xScale,
+                                                                                       params.yScale)
         self.mesh = mesh or QuadMesh(params.nXElem, params.nYElem, N, self.dgStorage.nodes, params.periodicX,
                                      params.periodicY, params.xScale, params.yScale, params.warp)
         self.nElems = self.mesh.nElems

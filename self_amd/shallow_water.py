@@ -42,6 +42,9 @@ class SWParams:
     periodicX: bool = True
     periodicY: bool = True
     warp: float = 0.0
+    # SpaceManagement % SpecMeshFile: an ISM-V2 mesh file read (and scaled by
+    # xScale, yScale) instead of the structured nXElem x nYElem mesh
+    SpecMeshFile: str = ""
 
 
 _FIELDS = {"solution": (0, 3), "tendency": (1, 3), "bathymetry": (2, 3), "vorticity": (3, 1),
@@ -64,6 +67,11 @@ class ShallowWater:
         self.nS = self.nP = N
         self.nEq = 3
         self.dgStorage = nodal or NodalStorage2D(N, L.GAUSS, L.DG, quirk_dmatp_transposed)
+        if mesh is None and params.SpecMeshFile:
+            # BuildQuadMesh (ConservativeShallowWaterClass.f90:289-304): read, then scale
+            from .mesh import SpecMesh
+            mesh = SpecMesh(params.SpecMeshFile, N, self.dgStorage.nodes).ScaleTheMesh(params.xScale,
+                                                                                       params.yScale)
         self.mesh = mesh or QuadMesh(params.nXElem, params.nYElem, N, self.dgStorage.nodes,
                                      params.periodicX, params.periodicY, params.xScale, params.yScale,
                                      params.warp)

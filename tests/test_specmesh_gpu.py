@@ -179,3 +179,31 @@ def test_dipole_box_N10_mixed_accuracy(gpu_lib, tmp_path):
         assert rel_l2(a[:, f], b[:, f]) < 5e-5, (f, rel_l2(a[:, f], b[:, f]))
     m64.Trash()
     mmx.Trash()
+
+
+def test_spec_mesh_file_parameter(gpu_lib, tmp_path):
+    """SWParams / AdvectionParams SpecMeshFile: the model reads and scales the
+    file itself, as the reference's BuildQuadMesh; same mesh and same steps
+    as passing SpecMesh(...).ScaleTheMesh(...) explicitly."""
+    from self_amd.advection import Advection, AdvectionParams
+    from self_amd.mesh import SpecMesh
+    from self_amd.shallow_water import ShallowWater, SWParams
+    path = tmp_path / "annulus.mesh"
+    path.write_text(G.quarter_annulus(3, 4, 4, seed=6, names=NAMES))
+    sw = ShallowWater(SWParams(polyDeg=5, xScale=2.0, yScale=3.0, dt=1e-4, SpecMeshFile=str(path)))
+    ref = SpecMesh(path, 5, sw.dgStorage.nodes).ScaleTheMesh(2.0, 3.0)
+    for k in ("edges", "x", "y", "dxds", "dxdp", "dyds", "dydp", "J", "nHat", "nLen"):
+        assert np.array_equal(getattr(sw.mesh, k), getattr(ref, k)), k
+    sw2 = ShallowWater(SWParams(polyDeg=5, xScale=2.0, yScale=3.0, dt=1e-4), mesh=ref, nodal=sw.dgStorage)
+    sol = np.zeros((sw.nElems, 3, 6, 6))
+    sol[:, 2] = 10.0 + 0.1 * np.sin(ref.x) * np.cos(ref.y)
+    for m in (sw, sw2):
+        m.SetSolution(sol)
+        m.ForwardStepRK3(0.0, 3)
+    assert np.array_equal(sw.GetSolution(), sw2.GetSolution())
+    sw.Trash()
+    sw2.Trash()
+    adv = Advection(AdvectionParams(polyDeg=4, nTracers=2, xScale=1.5, yScale=1.5, SpecMeshFile=str(path)))
+    ref = SpecMesh(path, 4, adv.dgStorage.nodes).ScaleTheMesh(1.5, 1.5)
+    assert adv.nElems == ref.nElems and np.array_equal(adv.mesh.J, ref.J)
+    adv.Trash()
