@@ -257,6 +257,15 @@ int self_sw2d_kernel_times(self_sw2d_t h, double tn, int nSteps, double *avg_ms,
 /* CalculateBathymetryAtBoundaries + CalculateBathymetryGradient for every
  * element (:1160-1168, :1733-1868): sets bathymetry vars 2 and 3. */
 int self_sw2d_calculate_bathymetry_gradient(self_sw2d_t h);
+/* Which shallow-water class the handle follows: 0 (default) the src one
+ * (src/highend/shallowwater/ConservativeShallowWaterClass.f90, with g for the
+ * undeclared name in its RiemannSolver); 1 the class as the examples ship it
+ * (examples/shallowwater/planewave/localmods/ConservativeShallowWaterClass.f90):
+ * Rusanov wave speed max|(u_n +- sqrt(u_n^2 + 2 g H^3))/(2H)| (:1555-1558),
+ * Source of Coriolis and g H grad(h) only (:1649-1656; linearDrag and the
+ * relaxation field are ignored), bathymetry gradient grad(h^2/2)/h
+ * (:1754-1825).  -5 for any other value. */
+int self_sw2d_set_variant(self_sw2d_t h, int variant);
 int self_sw2d_synchronize(self_sw2d_t h);
 /* Element groups per workgroup of the fused stage kernel: 1 (one round of
  * waves), 2 (each workgroup prefetches its second group while computing the

@@ -160,10 +160,20 @@ typedef struct {
     double *sol, *tend, *bsol, *bflux;
     double *bathy, *bbathy, *relax, *plv;
     double *G;        /* RK3 register, same layout as sol (ForwardStepRK3 G2D) */
+    /* 0: src/highend/shallowwater/ConservativeShallowWaterClass.f90;
+     * 1: the class as the examples ship it
+     *    (examples/shallowwater/planewave/localmods/ConservativeShallowWaterClass.f90):
+     *    wave speed max|(u_n +- sqrt(u_n^2 + 2 g H^3))/(2H)| (:1555-1558),
+     *    Source = g H grad(h) + Coriolis only (:1649-1656; drag and
+     *    relaxation dropped), bathymetry gradient grad(h^2/2)/h (:1754-1825). */
+    int form;
 } orc_sw2d;
 
 void orc_sw2d_riemann(const double *inS, const double *outS, const double *nHat,
                       double g, double *numFlux);                               /* :1555-1604 */
+void orc_sw2d_riemann_form(const double *inS, const double *outS, const double *nHat,
+                           double g, int form, double *numFlux);
+
 void orc_sw2d_external_state(int bcFlag, const double *nHat, const double *inS,
                              double *exS);                                      /* :1698-1729 */
 void orc_sw2d_solution_at_boundaries(orc_sw2d *m);   /* DGSEMSolutionStorageClass_2D.f90:749-802 */

@@ -4,7 +4,10 @@
  *   src/highend/shallowwater/ConservativeShallowWaterClass.f90
  *   src/dgsem/DGSEMSolutionStorageClass_2D.f90:749-802
  *   src/common/ConstantsDictionary.f90:66-68 (low-storage RK3 coefficients)
- * with the one-token fix of quirk Q2 (RiemannSolver's undeclared g -> locG).
+ * with the one-token fix of quirk Q2 (RiemannSolver's undeclared g -> locG),
+ * and, behind m->form = 1, the class as the examples ship it
+ *   examples/shallowwater/planewave/localmods/ConservativeShallowWaterClass.f90
+ * (the form the compiled reference runs: the src copy does not compile).
  * Layouts are documented in oracle.h.  Fortran evaluates a*b*c*d left to
  * right; the C below keeps that association explicitly.
  */
@@ -22,9 +25,10 @@ static const double rk3_g[3] = {1.0 / 3.0, 15.0 / 16.0, 8.0 / 15.0};
 
 #define NQ 3
 
-/* :1555-1604 */
-void orc_sw2d_riemann(const double *inS, const double *outS, const double *nHat,
-                      double locG, double *numFlux)
+/* :1555-1604; form 1: examples/shallowwater/planewave/localmods/
+ * ConservativeShallowWaterClass.f90:1523-1566 (only the wave speed differs) */
+void orc_sw2d_riemann_form(const double *inS, const double *outS, const double *nHat,
+                           double locG, int form, double *numFlux)
 {
     double jump[NQ], aS[NQ];
     for (int k = 0; k < NQ; ++k) jump[k] = outS[k] - inS[k];
@@ -32,16 +36,31 @@ void orc_sw2d_riemann(const double *inS, const double *outS, const double *nHat,
     double uR = outS[0], vR = outS[1], pR = outS[2];
     double uOut = uR * nHat[0] + vR * nHat[1];
     double uIn = uL * nHat[0] + vL * nHat[1];
-    double cR = sqrt(locG * pR), cL = sqrt(locG * pL);
-    double fac = fabs(uOut / pR + cR);
-    double t;
-    t = fabs(uOut / pR - cR); if (t > fac) fac = t;
-    t = fabs(uIn / pL + cL); if (t > fac) fac = t;
-    t = fabs(uIn / pL - cL); if (t > fac) fac = t;
+    double fac, t;
+    if (form == 1) {
+        double sR = sqrt(uOut * uOut + 2.0 * locG * (pR * pR * pR));
+        double sL = sqrt(uIn * uIn + 2.0 * locG * (pL * pL * pL));
+        fac = fabs(0.5 * (uOut + sR) / pR);
+        t = fabs(0.5 * (uOut - sR) / pR); if (t > fac) fac = t;
+        t = fabs(0.5 * (uIn + sL) / pL); if (t > fac) fac = t;
+        t = fabs(0.5 * (uIn - sL) / pL); if (t > fac) fac = t;
+    } else {
+        double cR = sqrt(locG * pR), cL = sqrt(locG * pL);
+        fac = fabs(uOut / pR + cR);
+        t = fabs(uOut / pR - cR); if (t > fac) fac = t;
+        t = fabs(uIn / pL + cL); if (t > fac) fac = t;
+        t = fabs(uIn / pL - cL); if (t > fac) fac = t;
+    }
     aS[0] = (uOut / pR) * uR + 0.5 * locG * pR * pR * nHat[0] + (uIn / pL) * uL + 0.5 * locG * pL * pL * nHat[0];
     aS[1] = (uOut / pR) * vR + 0.5 * locG * pR * pR * nHat[1] + (uIn / pL) * vL + 0.5 * locG * pL * pL * nHat[1];
     aS[2] = uOut + uIn;
     for (int k = 0; k < NQ; ++k) numFlux[k] = 0.5 * (aS[k] - fac * jump[k]);
+}
+
+void orc_sw2d_riemann(const double *inS, const double *outS, const double *nHat,
+                      double locG, double *numFlux)
+{
+    orc_sw2d_riemann_form(inS, outS, nHat, locG, 0, numFlux);
 }
 
 /* :1698-1729 */
@@ -111,7 +130,7 @@ static void edge_flux_one(orc_sw2d *m, int iEdge, double tn)
             int nb = iNode + n1 * (s1 + 4 * e1);
             nh[0] = m->nHat[2 * nb]; nh[1] = m->nHat[2 * nb + 1];
             double len = m->nLen[nb];
-            orc_sw2d_riemann(inS, exS, nh, m->g, flux);
+            orc_sw2d_riemann_form(inS, exS, nh, m->g, m->form, flux);
             for (int f = 0; f < NQ; ++f) {
                 double fl = flux[f] * len;
                 m->bflux[BDX(n1, iNode, f, NQ, s1, e1)] = fl;
@@ -126,7 +145,7 @@ static void edge_flux_one(orc_sw2d *m, int iEdge, double tn)
             double len = m->nLen[nb];
             for (int f = 0; f < NQ; ++f) inS[f] = m->bsol[BDX(n1, iNode, f, NQ, s1, e1)];
             orc_sw2d_external_state(e2, nh, inS, exS);
-            orc_sw2d_riemann(inS, exS, nh, m->g, flux);
+            orc_sw2d_riemann_form(inS, exS, nh, m->g, m->form, flux);
             for (int f = 0; f < NQ; ++f) m->bflux[BDX(n1, iNode, f, NQ, s1, e1)] = flux[f] * len;
         }
     }
@@ -223,9 +242,16 @@ static void mapped_one(orc_sw2d *m, int e, double tn)
             double dhdx = m->bathy[IDX(n1, iS, iP, 1, 3, e)];
             double dhdy = m->bathy[IDX(n1, iS, iP, 2, 3, e)];
             double lDrag = m->linearDrag, g = m->g;
-            double q0 = vort * s[1] + g * s[2] * dhdx - lDrag * s[0] / h - (s[0] - r[0]) * ts;
-            double q1 = -vort * s[0] + g * s[2] * dhdy - lDrag * s[1] / h - (s[1] - r[1]) * ts;
-            double q2 = -(s[2] - r[2]) * ts;
+            double q0, q1, q2;
+            if (m->form == 1) { /* localmods :1649-1655 */
+                q0 = g * s[2] * dhdx + vort * s[1];
+                q1 = g * s[2] * dhdy - vort * s[0];
+                q2 = 0.0;
+            } else {
+                q0 = vort * s[1] + g * s[2] * dhdx - lDrag * s[0] / h - (s[0] - r[0]) * ts;
+                q1 = -vort * s[0] + g * s[2] * dhdy - lDrag * s[1] / h - (s[1] - r[1]) * ts;
+                q2 = -(s[2] - r[2]) * ts;
+            }
             tend[IDX(n1, iS, iP, 0, NQ, e)] += q0;
             tend[IDX(n1, iS, iP, 1, NQ, e)] += q1;
             tend[IDX(n1, iS, iP, 2, NQ, e)] += q2;
@@ -304,6 +330,7 @@ void orc_sw2d_integrate(orc_sw2d *m, double t0, int nSteps, int nThreads)
 void orc_sw2d_bathymetry_gradient(orc_sw2d *m)
 {
     int N = m->N, n1 = N + 1;
+    const int form1 = (m->form == 1);
     double sC[16][2], pC[16][2], der[16], line[16], fL[2], fR[2];
     for (int e = 0; e < m->nElems; ++e)
         boundaries_one(n1, m->lagWest, m->lagEast, m->lagSouth, m->lagNorth, m->bathy, m->bbathy, 3, e);
@@ -313,11 +340,14 @@ void orc_sw2d_bathymetry_gradient(orc_sw2d *m)
             for (int iS = 0; iS <= N; ++iS) {
                 int g0 = iS + n1 * (iP + n1 * e);
                 double h = m->bathy[IDX(n1, iS, iP, 0, 3, e)];
-                double xF[2] = {h, 0.0}, yF[2] = {0.0, h};
+                /* form 1 (localmods :1754-1755): the flux of h^2/2 */
+                double hf = form1 ? (h * h) * 0.5 : h, z = form1 ? 0.0 * 0.5 : 0.0;
+                double xF[2] = {hf, z}, yF[2] = {z, hf};
                 for (int q = 0; q < 2; ++q) sC[iS][q] = m->dydp[g0] * xF[q] - m->dxdp[g0] * yF[q];
             }
             int nbE = iP + n1 * (1 + 4 * e), nbW = iP + n1 * (3 + 4 * e);
             double hE = m->bbathy[BDX(n1, iP, 0, 3, 1, e)], hW = m->bbathy[BDX(n1, iP, 0, 3, 3, e)];
+            if (form1) { hE = (0.5 * hE) * hE; hW = (0.5 * hW) * hW; } /* :1764,1768 */
             for (int q = 0; q < 2; ++q) {
                 fR[q] = (hE * m->nHat[2 * nbE + q]) * m->nLen[nbE];
                 fL[q] = (hW * m->nHat[2 * nbW + q]) * m->nLen[nbW];
@@ -332,11 +362,13 @@ void orc_sw2d_bathymetry_gradient(orc_sw2d *m)
             for (int iP = 0; iP <= N; ++iP) {
                 int g0 = iS + n1 * (iP + n1 * e);
                 double h = m->bathy[IDX(n1, iS, iP, 0, 3, e)];
-                double xF[2] = {h, 0.0}, yF[2] = {0.0, h};
+                double hf = form1 ? (h * h) * 0.5 : h, z = form1 ? 0.0 * 0.5 : 0.0;
+                double xF[2] = {hf, z}, yF[2] = {z, hf};
                 for (int q = 0; q < 2; ++q) pC[iP][q] = -m->dyds[g0] * xF[q] + m->dxds[g0] * yF[q];
             }
             int nbN = iS + n1 * (2 + 4 * e), nbS = iS + n1 * (0 + 4 * e);
             double hN = m->bbathy[BDX(n1, iS, 0, 3, 2, e)], hS = m->bbathy[BDX(n1, iS, 0, 3, 0, e)];
+            if (form1) { hN = (0.5 * hN) * hN; hS = (0.5 * hS) * hS; } /* :1807,1811 */
             for (int q = 0; q < 2; ++q) {
                 fR[q] = (hN * m->nHat[2 * nbN + q]) * m->nLen[nbN];
                 fL[q] = (hS * m->nHat[2 * nbS + q]) * m->nLen[nbS];
@@ -347,7 +379,10 @@ void orc_sw2d_bathymetry_gradient(orc_sw2d *m)
                 for (int iP = 0; iP <= N; ++iP) {
                     double Jv = m->J[iS + n1 * (iP + n1 * e)];
                     int id = iS + n1 * (iP + n1 * q);
-                    tend[id] = (tend[id] + der[iP]) / Jv;
+                    if (form1) /* :1823-1825, divided by J h */
+                        tend[id] = (tend[id] + der[iP]) / (Jv * m->bathy[IDX(n1, iS, iP, 0, 3, e)]);
+                    else
+                        tend[id] = (tend[id] + der[iP]) / Jv;
                 }
             }
         }

@@ -66,6 +66,7 @@ def _declare(L):
     L.orc_quadmesh_build.argtypes = [C.POINTER(QuadMeshDesc), _dp, _ip, _ip] + [_dp] * 9
     L.orc_quadmesh_build.restype = C.c_int
     L.orc_sw2d_riemann.argtypes = [_dp, _dp, _dp, C.c_double, _dp]
+    L.orc_sw2d_riemann_form.argtypes = [_dp, _dp, _dp, C.c_double, C.c_int, _dp]
     L.orc_sw2d_external_state.argtypes = [C.c_int, _dp, _dp, _dp]
     for name in ("orc_sw2d_solution_at_boundaries", "orc_sw2d_bathymetry_gradient"):
         getattr(L, name).argtypes = [C.POINTER(SW2D)]
@@ -112,7 +113,7 @@ class SW2D(C.Structure):
         (n, _dp) for n in ("dMatS", "dMatP", "qWeightS", "qWeightP", "lagWest", "lagEast",
                            "lagSouth", "lagNorth")] + [("edges", _ip)] + [
         (n, _dp) for n in ("dxds", "dxdp", "dyds", "dydp", "J", "nHat", "nLen", "sol", "tend",
-                           "bsol", "bflux", "bathy", "bbathy", "relax", "plv", "G")]
+                           "bsol", "bflux", "bathy", "bbathy", "relax", "plv", "G")] + [("form", C.c_int)]
 
 
 VecOp = C.CFUNCTYPE(None, C.c_void_p, _dp, _dp)
@@ -290,7 +291,7 @@ class ShallowWater2D:
     """
 
     def __init__(self, nodal: NodalStorage2D, mesh: QuadMesh, g=9.81, dt=1.0,
-                 linearDrag=0.0, dMatP=None, with_relax=False):
+                 linearDrag=0.0, dMatP=None, with_relax=False, form=0):
         N, E = nodal.N, mesh.nElems
         n1 = N + 1
         self.N, self.nElems = N, E
@@ -316,7 +317,7 @@ class ShallowWater2D:
                       *[_p(getattr(mesh, n)) for n in ("dxds", "dxdp", "dyds", "dydp", "J", "nHat", "nLen")],
                       _p(self.sol), _p(self.tend), _p(self.bsol), _p(self.bflux), _p(self.bathy),
                       _p(self.bbathy), _p(self.relax) if self.relax is not None else None,
-                      _p(self.plv), _p(self.G))
+                      _p(self.plv), _p(self.G), form)
         self._edges = np.ascontiguousarray(mesh.edges.ravel())
         self.s.edges = _pi(self._edges)
 
@@ -336,12 +337,12 @@ class ShallowWater2D:
         lib().orc_sw2d_bathymetry_gradient(C.byref(self.s))
 
 
-def riemann(inS, outS, nHat, g):
+def riemann(inS, outS, nHat, g, form=0):
     out = np.zeros(3)
     a = np.ascontiguousarray(inS, dtype=np.float64)
     b = np.ascontiguousarray(outS, dtype=np.float64)
     n = np.ascontiguousarray(nHat, dtype=np.float64)
-    lib().orc_sw2d_riemann(_p(a), _p(b), _p(n), g, _p(out))
+    lib().orc_sw2d_riemann_form(_p(a), _p(b), _p(n), g, form, _p(out))
     return out
 
 

@@ -6,10 +6,11 @@
 # ref_adv3d.f90, ref_cgsem2d.f90 and ref_krylov.f90, the quadrature,
 # nodal-storage and interpolation modules behind ref_nodal.f90, the 2-D
 # advection model's pickup routines behind ref_pickup.f90, and the SpecMesh
-# reader and element geometry behind ref_specmesh.f90.  All outputs
+# reader and element geometry behind ref_specmesh.f90, and the conservative
+# shallow-water class of the planewave example behind ref_sw2d.f90.  All outputs
 # (objects, .mod files, binaries) go to oracle/_ref/.
 #
-#   make -f oracle/ref.mk   # -> oracle/_ref/ref_{adv2d,adv3d,cgsem2d,krylov,nodal,pickup,specmesh}
+#   make -f oracle/ref.mk   # -> oracle/_ref/ref_{adv2d,adv3d,cgsem2d,krylov,nodal,pickup,specmesh,sw2d}
 #
 # No OpenMP, as in the reference's own build/makefile (OPT=-O3, no -fopenmp):
 # its !$OMP directives stay comments (Advection3DClass.f90 has some that
@@ -41,9 +42,13 @@ ADV2D_SRC := highend/advection-diffusion2d/AdvectionParamsClass highend/advectio
 ADV3D_SRC := highend/advection3d/Advection3DParamsClass highend/advection3d/Advection3DClass
 CGSEM2D_SRC := cgsem/CGsemParams_Class cgsem/CGsemPreconditioner_2D_Class cgsem/CGsemElliptic_2D_Class
 KRYLOV_SRC := iterativesolve/IterativeSolversParams_Class iterativesolve/IterativeSolvers_Class
+# The conservative shallow-water class as the planewave example ships it (the
+# src/highend/shallowwater copy does not compile); paths relative to $(REF).
+SW2D_SRC := ../examples/shallowwater/planewave/localmods/SWParamsClass \
+            ../examples/shallowwater/planewave/localmods/ConservativeShallowWaterClass
 
 all: $(OUT)/ref_adv2d $(OUT)/ref_adv3d $(OUT)/ref_cgsem2d $(OUT)/ref_krylov $(OUT)/ref_nodal $(OUT)/ref_pickup \
-     $(OUT)/ref_specmesh
+     $(OUT)/ref_specmesh $(OUT)/ref_sw2d
 
 # One recipe compiles the shared chain in order: each module needs the .mod
 # files of the ones before it.
@@ -72,6 +77,7 @@ $(eval $(call model,krylov,$(KRYLOV_SRC)))
 $(eval $(call model,nodal,))
 $(eval $(call model,pickup,$(ADV2D_SRC)))
 $(eval $(call model,specmesh,))
+$(eval $(call model,sw2d,$(SW2D_SRC)))
 
 clean:
 	rm -rf $(OUT)

@@ -34,11 +34,23 @@ __device__ __forceinline__ double line_dot(const double *__restrict__ l, const T
     return acc;
 }
 
+// Wave speed of the shallow-water class as the examples ship it
+// (examples/shallowwater/planewave/localmods/ConservativeShallowWaterClass.f90:1555-1558):
+// max over both sides of |(u_n +- sqrt(u_n^2 + 2 g H^3)) / (2H)|.
+__device__ __forceinline__ double sw_example_speed(double uOut, double pR, double uIn, double pL, double g)
+{
+    const double sR = sqrt(uOut * uOut + 2.0 * g * (pR * pR * pR));
+    const double sL = sqrt(uIn * uIn + 2.0 * g * (pL * pL * pL));
+    return fmax(fmax(fabs(0.5 * (uOut + sR) / pR), fabs(0.5 * (uOut - sR) / pR)),
+                fmax(fabs(0.5 * (uIn + sL) / pL), fabs(0.5 * (uIn - sL) / pL)));
+}
+
 // Rusanov-type flux of the conservative shallow-water equations,
 // RiemannSolver, ConservativeShallowWaterClass.f90:1555-1604 (with g in the
 // wave speed, quirk Q2 fixed).  in/out are (U, V, H) on the two sides.
+// example: the wave speed of the examples' class (sw_example_speed).
 __device__ __forceinline__ void sw_riemann(const double in[3], const double out[3], double nx,
-                                           double ny, double g, double f[3])
+                                           double ny, double g, double f[3], bool example = false)
 {
     const double uL = in[0], vL = in[1], pL = in[2];
     const double uR = out[0], vR = out[1], pR = out[2];
@@ -46,7 +58,8 @@ __device__ __forceinline__ void sw_riemann(const double in[3], const double out[
     const double uIn = uL * nx + vL * ny;
     const double cR = sqrt(g * pR), cL = sqrt(g * pL);
     const double wR = uOut / pR, wL = uIn / pL;
-    const double fac = fmax(fmax(fabs(wR + cR), fabs(wR - cR)), fmax(fabs(wL + cL), fabs(wL - cL)));
+    const double fac = example ? sw_example_speed(uOut, pR, uIn, pL, g)
+                               : fmax(fmax(fabs(wR + cR), fabs(wR - cR)), fmax(fabs(wL + cL), fabs(wL - cL)));
     const double hgR = 0.5 * g * pR * pR, hgL = 0.5 * g * pL * pL;
     const double a0 = wR * uR + hgR * nx + wL * uL + hgL * nx;
     const double a1 = wR * vR + hgR * ny + wL * vL + hgL * ny;
@@ -72,7 +85,7 @@ __device__ __forceinline__ double rcp_refined(double x)
 // (velocities from refined reciprocals), the dissipation coefficient
 // max |u_n +- c| in FP32 (it multiplies the state jump only).
 __device__ __forceinline__ void sw_riemann_mixed(const double in[3], const double out[3], double nx, double ny,
-                                                 double g, double f[3])
+                                                 double g, double f[3], bool example = false)
 {
     const double uL = in[0], vL = in[1], pL = in[2];
     const double uR = out[0], vR = out[1], pR = out[2];
@@ -81,7 +94,9 @@ __device__ __forceinline__ void sw_riemann_mixed(const double in[3], const doubl
     const double wR = uOut * rcp_refined(pR), wL = uIn * rcp_refined(pL);
     const float cR = __builtin_amdgcn_sqrtf((float)(g * pR)), cL = __builtin_amdgcn_sqrtf((float)(g * pL));
     const float wRf = (float)wR, wLf = (float)wL;
-    const double fac = (double)fmaxf(fmaxf(fabsf(wRf + cR), fabsf(wRf - cR)), fmaxf(fabsf(wLf + cL), fabsf(wLf - cL)));
+    const double fac = example ? sw_example_speed(uOut, pR, uIn, pL, g)
+                               : (double)fmaxf(fmaxf(fabsf(wRf + cR), fabsf(wRf - cR)),
+                                               fmaxf(fabsf(wLf + cL), fabsf(wLf - cL)));
     const double hgR = 0.5 * g * pR * pR, hgL = 0.5 * g * pL * pL;
     const double a0 = wR * uR + hgR * nx + wL * uL + hgL * nx;
     const double a1 = wR * vR + hgR * ny + wL * vL + hgL * ny;

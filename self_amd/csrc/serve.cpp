@@ -76,7 +76,8 @@ struct self_server_s {
     std::condition_variable cvWork, cvDone;
     std::deque<std::shared_ptr<Job>> queue;
     std::unordered_map<long long, std::shared_ptr<Job>> jobs;
-    std::map<std::tuple<int, int, int>, std::shared_ptr<MeshCache>> meshes;
+    // host meshes by (kind, nX, N, bit pattern of the SW domain side L; 0 for CGSEM)
+    std::map<std::tuple<int, int, int, uint64_t>, std::shared_ptr<MeshCache>> meshes;
     std::vector<std::thread> workers;
     long long nextId = 1;
     int running = 0;
@@ -89,13 +90,16 @@ namespace {
 // of side L) or CGSEM (Gauss-Lobatto, walled unit square)
 std::shared_ptr<MeshCache> mesh_for(self_server_s *s, int kind, int nX, int N, double L)
 {
+    const bool sw = kind == SELF_REQ_SW2D;
+    uint64_t Lbits = 0;  // the SW mesh is scaled by L; the CGSEM mesh is the unit square
+    if (sw) std::memcpy(&Lbits, &L, sizeof(Lbits));
+    const auto key = std::make_tuple(kind, nX, N, Lbits);
     {
         std::lock_guard<std::mutex> g(s->mu);
-        auto it = s->meshes.find({kind, nX, N});
+        auto it = s->meshes.find(key);
         if (it != s->meshes.end()) return it->second;
     }
     auto m = std::make_shared<MeshCache>();
-    const bool sw = kind == SELF_REQ_SW2D;
     const int n1 = N + 1, np = n1 * n1;
     m->N = N;
     m->nX = nX;
@@ -127,7 +131,7 @@ std::shared_ptr<MeshCache> mesh_for(self_server_s *s, int kind, int nX, int N, d
             }
     }
     std::lock_guard<std::mutex> g(s->mu);
-    s->meshes[{kind, nX, N}] = m;
+    s->meshes[key] = m;
     return m;
 }
 

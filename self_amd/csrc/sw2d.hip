@@ -42,6 +42,7 @@ constexpr double kRkG[3] = {1.0 / 3.0, 15.0 / 16.0, 8.0 / 15.0};
 
 struct SWKernelArgs {
     int nE;
+    int variant;               // 1: the examples' class (wave speed, no drag/relaxation, grad(h^2/2)/h)
     int members;               // ensemble members (fused form): q, G and traces per member
     double g, drag, a, gdt;
     double gcoef;              // RK stage coefficient g_m: member k steps with gcoef * dtM[k]
@@ -194,8 +195,9 @@ template <int N>
 __global__ __launch_bounds__(256) void k_sw_edge_flux(const int *__restrict__ edges, int nEdges,
                                                       const double *__restrict__ bsol,
                                                       const double *__restrict__ nrm, double g,
-                                                      double *__restrict__ bflux)
+                                                      double *__restrict__ bflux, int variant)
 {
+    const bool ex = variant == 1;
     constexpr int n1 = N + 1;
     const int gid = blockIdx.x * blockDim.x + threadIdx.x;
     if (gid >= nEdges * n1) return;
@@ -203,7 +205,7 @@ __global__ __launch_bounds__(256) void k_sw_edge_flux(const int *__restrict__ ed
     const int *ed = edges + 6 * ie;
     const int e1 = ed[0] - 1, s1 = ed[1] - 1, e2 = ed[2], s2 = abs(ed[3]) - 1;
     const int start = ed[4], inc = ed[5];
-    double in[3], ex[3], fl[3];
+    double in[3], ex_[3], fl[3];
     #pragma unroll
     for (int f = 0; f < 3; ++f) in[f] = bsol[(((size_t)e1 * 4 + s1) * 3 + f) * n1 + iNode];
     const size_t nb = ((size_t)e1 * 4 + s1) * 3 * n1 + iNode;
@@ -211,8 +213,8 @@ __global__ __launch_bounds__(256) void k_sw_edge_flux(const int *__restrict__ ed
     if (e2 > 0) {
         const int k = start - inc + iNode * inc;
         #pragma unroll
-        for (int f = 0; f < 3; ++f) ex[f] = bsol[(((size_t)(e2 - 1) * 4 + s2) * 3 + f) * n1 + k];
-        sw_riemann(in, ex, nx, ny, g, fl);
+        for (int f = 0; f < 3; ++f) ex_[f] = bsol[(((size_t)(e2 - 1) * 4 + s2) * 3 + f) * n1 + k];
+        sw_riemann(in, ex_, nx, ny, g, fl, ex);
         #pragma unroll
         for (int f = 0; f < 3; ++f) {
             const double v = fl[f] * len;
@@ -220,8 +222,8 @@ __global__ __launch_bounds__(256) void k_sw_edge_flux(const int *__restrict__ ed
             bflux[(((size_t)(e2 - 1) * 4 + s2) * 3 + f) * n1 + k] = -v;
         }
     } else {
-        sw_external(e2, nx, ny, in, ex);
-        sw_riemann(in, ex, nx, ny, g, fl);
+        sw_external(e2, nx, ny, in, ex_);
+        sw_riemann(in, ex_, nx, ny, g, fl, ex);
         #pragma unroll
         for (int f = 0; f < 3; ++f) bflux[(((size_t)e1 * 4 + s1) * 3 + f) * n1 + iNode] = fl[f] * len;
     }
@@ -564,8 +566,8 @@ __device__ __forceinline__ void side_flux(const SWKernelArgs &A, const NodalLDS<
         lft[f] = prim ? own[f] : oth[f];
         rgt[f] = prim ? oth[f] : own[f];
     }
-    if constexpr (MIXED) sw_riemann_mixed(lft, rgt, fn[0], fn[1], A.g, fl);
-    else sw_riemann(lft, rgt, fn[0], fn[1], A.g, fl);
+    if constexpr (MIXED) sw_riemann_mixed(lft, rgt, fn[0], fn[1], A.g, fl, A.variant == 1);
+    else sw_riemann(lft, rgt, fn[0], fn[1], A.g, fl, A.variant == 1);
     const double sign = prim ? 1.0 : -1.0;
     #pragma unroll
     for (int f = 0; f < 3; ++f) out[f] = sign * (fl[f] * fn[2]);
@@ -749,16 +751,20 @@ __global__ __launch_bounds__(256) void k_sw_bathy_grad(SWKernelArgs A, double *b
     const size_t mstride = (size_t)A.nE * np, base = (size_t)e * np + nd;
     double lW[n1], lE[n1], lS[n1], lN[n1];
     load_lag<N>(A.lag, lW, lE, lS, lN);
+    // variant 1 (the examples' class, :1754-1825): the weak gradient of
+    // h^2/2, divided by J h at the end
+    const bool half2 = A.variant == 1;
     if (act) {
         const double h = bathy[base];
         hs[le * np + nd] = h;
+        const double hf = half2 ? (h * h) * 0.5 : h;
         const double dxds = A.metric[base], dxdp = A.metric[mstride + base];
         const double dyds = A.metric[2 * mstride + base], dydp = A.metric[3 * mstride + base];
-        // xF = (h, 0), yF = (0, h)
-        sC[(le * 2 + 0) * np + nd] = dydp * h - dxdp * 0.0;
-        sC[(le * 2 + 1) * np + nd] = dydp * 0.0 - dxdp * h;
-        pC[(le * 2 + 0) * np + nd] = -dyds * h + dxds * 0.0;
-        pC[(le * 2 + 1) * np + nd] = -dyds * 0.0 + dxds * h;
+        // xF = (hf, 0), yF = (0, hf)
+        sC[(le * 2 + 0) * np + nd] = dydp * hf - dxdp * 0.0;
+        sC[(le * 2 + 1) * np + nd] = dydp * 0.0 - dxdp * hf;
+        pC[(le * 2 + 0) * np + nd] = -dyds * hf + dxds * 0.0;
+        pC[(le * 2 + 1) * np + nd] = -dyds * 0.0 + dxds * hf;
     }
     __syncthreads();
     if (act) {
@@ -772,6 +778,7 @@ __global__ __launch_bounds__(256) void k_sw_bathy_grad(SWKernelArgs A, double *b
             case 2: tr = line_dot<N>(lN, hb + k, n1); break;
             default: tr = line_dot<N>(lW, hb + n1 * k, 1); break;
             }
+            if (half2) tr = (0.5 * tr) * tr;
             const size_t nb = ((size_t)e * 4 + sd) * 3 * n1 + k;
             const double nx = A.nrm[nb], ny = A.nrm[nb + n1], len = A.nrm[nb + 2 * n1];
             bfs[le * 8 * n1 + (sd * 2 + 0) * n1 + k] = (tr * nx) * len;
@@ -781,7 +788,7 @@ __global__ __launch_bounds__(256) void k_sw_bathy_grad(SWKernelArgs A, double *b
     __syncthreads();
     if (!act) return;
     const double *bf = bfs + le * 8 * n1;
-    const double Jv = A.metric[4 * mstride + base];
+    const double Jv = half2 ? A.metric[4 * mstride + base] * hs[le * np + nd] : A.metric[4 * mstride + base];
     for (int qd = 0; qd < 2; ++qd) {
         const double *sl = sC + (le * 2 + qd) * np + n1 * j;
         double t1 = A.dS[i] * sl[0];
@@ -824,6 +831,7 @@ struct self_sw2d_s {
     bool trValid = false, nodalSet = false, geomSet = false, hasRelax = false;
     int groups = 0;  // element groups per workgroup of the fused kernel (0: heuristic)
     int members = 1; // ensemble members (fused form)
+    int variant = 0; // 1: the examples' class (self_sw2d_set_variant)
     double *dtM = nullptr;          // per-member time steps (device, members > 1)
     std::vector<double> dtH;        // host copy
     size_t bytes = 0;
@@ -858,8 +866,10 @@ SWKernelArgs make_args(self_sw2d_s *h)
     A.members = h->members;
     A.dtM = h->dtM;
     A.g = h->d.g;
-    A.drag = h->d.linearDrag;
-    A.hasRelax = h->hasRelax ? 1 : 0;
+    A.variant = h->variant;
+    // the examples' class has no drag and no relaxation in its Source
+    A.drag = h->variant == 1 ? 0.0 : h->d.linearDrag;
+    A.hasRelax = (h->hasRelax && h->variant != 1) ? 1 : 0;
     A.dS = h->nodal;
     A.dP = h->nodal + n1 * n1;
     A.qwS = h->nodal + 2 * n1 * n1;
@@ -951,7 +961,8 @@ int launch_edge_flux(self_sw2d_s *h)
     const int total = h->d.nEdges * h->n1;
     const int grid = (total + 255) / 256;
     SELF_DISPATCH_N(h->d.N, hipLaunchKernelGGL(k_sw_edge_flux<NN>, dim3(grid), dim3(256), 0, h->stream,
-                                               h->edges, h->d.nEdges, h->bsol, h->nrm, h->d.g, h->bflux));
+                                               h->edges, h->d.nEdges, h->bsol, h->nrm, h->d.g, h->bflux,
+                                               h->variant));
     SELF_HIP(hipGetLastError());
     return SELF_OK;
 }
@@ -1623,6 +1634,14 @@ try {
     return rc;
 }
 SELF_CATCH_ALL
+
+int self_sw2d_set_variant(self_sw2d_t h, int variant)
+{
+    SELF_REQUIRE(h, SELF_ERR_ARG, "null handle");
+    SELF_REQUIRE(variant == 0 || variant == 1, SELF_ERR_ARG, "unknown shallow-water variant %d", variant);
+    h->variant = variant;
+    return SELF_OK;
+}
 
 int self_sw2d_set_groups(self_sw2d_t h, int groups)
 {

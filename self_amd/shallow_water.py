@@ -205,6 +205,12 @@ class ShallowWater:
     def CalculateBathymetryGradient(self):
         L.check(L.lib().self_sw2d_calculate_bathymetry_gradient(self._h))
 
+    def SetVariant(self, variant):
+        """0: the src class; 1: the class as the examples ship it
+        (examples/shallowwater/planewave/localmods/ConservativeShallowWaterClass.f90:
+        its wave speed, a Source without drag and relaxation, grad(h^2/2)/h)."""
+        L.check(L.lib().self_sw2d_set_variant(self._h, int(variant)))
+
     def SetGroups(self, groups):
         """Element groups per workgroup of the fused kernel (0 = heuristic)."""
         L.check(L.lib().self_sw2d_set_groups(self._h, groups))

@@ -391,3 +391,63 @@ def run_specmesh(path, N, quadrature=1, scale=None, timeout=120):
     out["nLen"] = r.take("<f8", (n1, 4, nE))
     r.done()
     return out
+
+
+def random_inputs_sw2d(N, nX, nY, seed):
+    """Shallow-water state per node: H ~ U[1, 2], U, V ~ U[-0.5, 0.5] H,
+    bathymetry h ~ U[0.8, 1.2] (smooth in nothing: the comparison is of the
+    discrete operator), Coriolis f ~ U[-1, 1]; arrays element-last."""
+    rng = np.random.default_rng(seed)
+    n1, E = N + 1, nX * nY
+    H = rng.uniform(1.0, 2.0, (n1, n1, E))
+    sol = np.stack([rng.uniform(-0.5, 0.5, (n1, n1, E)) * H, rng.uniform(-0.5, 0.5, (n1, n1, E)) * H, H], axis=2)
+    h = rng.uniform(0.8, 1.2, (n1, n1, E))
+    f = rng.uniform(-1.0, 1.0, (n1, n1, E))
+    return {"sol": sol, "h": h, "f": f}
+
+
+def _params_text_sw2d(N, nX, nY, g, dt, xScale, yScale):
+    return ("&MODEL_FORM\n/\n"
+            "&TIME_MANAGEMENT\n"
+            f" dt = {dt!r},\n iterInit = 0,\n nTimeSteps = 0,\n dumpFreq = 1,\n/\n"
+            "&SPACE_MANAGEMENT\n"
+            f" polyDeg = {N},\n nXElem = {nX},\n nYElem = {nY},\n nPlot = 4,\n"
+            f" xScale = {xScale!r},\n yScale = {yScale!r},\n/\n"
+            "&PHYSICAL_PARAMETERS\n"
+            f" g = {g!r},\n f0 = 0.0,\n betaX = 0.0,\n betaY = 0.0,\n linearDrag = 0.0,\n/\n"
+            "&DIPOLE_PARAMETERS\n/\n")
+
+
+def run_sw2d(N, nX, nY, g, dt, t0, nSteps, inputs, xScale=1.0, yScale=1.0, timeout=120):
+    """The reference's conservative shallow-water class (planewave example,
+    oracle/ref_sw2d.f90).  Returns a dict: the inputs; the reference's nodal
+    arrays and mesh as run(); bathy (n1, n1, 3, E) after
+    CalculateBathymetryGradient; tend (n1, n1, 3, E) and bflux (n1, 3, 4, E)
+    after GlobalTimeDerivative(t0); sol_out (n1, n1, 3, E) after nSteps RK3
+    steps."""
+    n1, E = N + 1, nX * nY
+    raw = _execute("sw2d", _params_text_sw2d(N, nX, nY, g, dt, xScale, yScale),
+                   [inputs[k] for k in ("sol", "h", "f")],
+                   np.int32(nSteps).tobytes() + np.float64(t0).tobytes(), timeout,
+                   params_file="shallowwater.params")
+    r = _Reader(raw)
+    take = r.take
+    out = dict(inputs)
+    nE, nEd = take(np.int32, (2,))
+    assert nE == E
+    out["dMatS"] = take(np.float64, (n1, n1))
+    out["dMatP"] = take(np.float64, (n1, n1))
+    for k in ("qWeightS", "qWeightP", "lagWest", "lagEast", "lagSouth", "lagNorth"):
+        out[k] = take(np.float64, (n1,))
+    out["edges"] = take(np.int32, (6, int(nEd))).T.copy()
+    for k in ("x", "y", "dxds", "dxdp", "dyds", "dydp", "J"):
+        out[k] = take(np.float64, (n1, n1, E))
+    out["nHat"] = take(np.float64, (2, n1, 4, E))
+    out["nLen"] = take(np.float64, (n1, 4, E))
+    out["bathy"] = take(np.float64, (n1, n1, 3, E))
+    out["tend"] = take(np.float64, (n1, n1, 3, E))
+    out["bflux"] = take(np.float64, (n1, 3, 4, E))
+    out["sol_out"] = take(np.float64, (n1, n1, 3, E))
+    r.done()
+    out.update(N=N, nX=nX, nY=nY, g=g, dt=dt, t0=t0, nSteps=nSteps, xScale=xScale, yScale=yScale)
+    return out

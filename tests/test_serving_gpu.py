@@ -56,6 +56,29 @@ def test_served_shallow_water_equals_the_model_api(gpu_lib):
     m.Trash()
 
 
+def test_same_mesh_size_different_domain_side(gpu_lib):
+    """Two shallow-water requests with the same nX and N on domains of
+    different side L: the server's host-mesh cache must not hand the second
+    one the first one's geometry.  Each result equals a direct model run."""
+    from self_amd import serving as S
+    from self_amd.shallow_water import ShallowWater, SWParams
+    specs = [S.make_inputs(S.Spec(S.REQ_SW2D, 8, 3, 41, "fp64", steps=6), Lside=Ls) for Ls in (1.0e6, 2.5e5)]
+    srv = S.Server(workers=1)
+    results = [srv.wait(srv.submit(sp)) for sp in specs]
+    srv.close()
+    for spec, (res, out) in zip(specs, results):
+        p = spec.params
+        m = ShallowWater(SWParams(polyDeg=3, nXElem=8, nYElem=8, xScale=p["L"], yScale=p["L"], dt=p["dt"],
+                                  g=p["g"], f0=p["f0"]))
+        m.SetBathymetry(spec.arrays["in1"])
+        m.SetPlanetaryVorticity(np.full(spec.arrays["in1"].shape[:1] + spec.arrays["in1"].shape[2:], p["f0"]))
+        m.CalculateBathymetryGradient()
+        m.SetSolution(spec.arrays["in0"])
+        m.ForwardStepRK3(0.0, 6)
+        assert res.status == 0 and np.array_equal(m.GetSolution(), out), p["L"]
+        m.Trash()
+
+
 def test_admission_with_a_tiny_memory_budget_runs_requests_one_at_a_time(gpu_lib):
     from self_amd import serving as S
     specs = _specs()[:4]
