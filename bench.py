@@ -4,20 +4,30 @@ Workload (the configuration the headline metric is quoted on): 2-D nonlinear
 shallow water, 64x64 quad elements on a doubly periodic 1e6 m x 1e6 m
 f-plane (f = 1e-4 1/s, g = 9.81 m/s^2), N = 7 Gauss nodes (262144 nodes x 3
 fields), seeded bathymetry (8 Fourier modes, |h - 1000 m| <= 50 m) and a
-geostrophically balanced 1 m Gaussian, RK3 at CFL 0.5.  One "step" is one
+geostrophically balanced 1 m Gaussian, RK3 at dt = 1.22 s (CFL 0.5 on the
+(N+1)^2 scaling; the scheme diverges above about 2.5-3 s).  One "step" is one
 low-storage RK3 step (3 stages) over the whole state; inputs are resident in
 HBM when the timed region starts.
 
-value = DOF-stage/s of the mixed-precision fused path (DOF = node x field,
-one RK3 stage), whole job on one GPU.  The FP64 fused path and the FP64
-plain drop-in (one kernel per reference phase) are timed beside it, and the
-mixed result is compared with the FP64 result after the same steps.
+value = DOF-stage/s of the FP64 fused path (DOF = node x field, one RK3
+stage), the reference's own arithmetic, whole job on one GPU.  The mixed
+precision fused path (FP32 volume contractions, FP64 state) and the FP64
+plain drop-in (one kernel per reference phase) are timed beside it under
+`variants`; the mixed result is compared with the FP64 one after the same
+steps, and again at dt = 2.5 s.
 
 roofline: algorithmic bytes per stage launch of the dominant kernel (the
-fused stage kernel) over its mean launch time measured with HIP events on
-the model's stream.  cpu_baseline: the FP64 oracle (C restatement of the
-reference, OpenMP over elements and edges) on this host, on a bounded
-sample of the same workload.
+FP64 fused stage kernel, 11 520 B per element at N = 7) over its mean launch
+time measured with HIP events on the model's stream; `traffic` is the HBM
+bytes per launch measured by rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE and
+stored under profiles/ (named in `traffic_source`), not re-measured here.
+
+cpu_baseline: the reference's own conservative shallow-water class (Fortran,
+compiled from the reference's sources into oracle/_ref, serial as the
+reference builds it) on a bounded sample of the same workload (its default
+walled mesh of the same size and state); the oracle (C restatement) is timed
+beside it at 1 thread and at all host cores, and on samples of the other
+configurations.
 """
 import argparse
 import json
@@ -49,16 +59,30 @@ def load_traffic(name):
     return None
 
 
-def cpu_baseline(nX, N, target_s=12.0):
-    """The oracle (FP64, OpenMP) on a bounded sample of the workload."""
+def host_info():
+    model = ""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    try:
+        usable = len(os.sched_getaffinity(0))
+    except AttributeError:
+        usable = os.cpu_count() or 1
+    return {"nproc": os.cpu_count(), "usable_cores": usable, "cpu_model": model}
+
+
+def _c2_oracle(nX, N, periodic=True):
     import numpy as np
     from oracle import pyoracle as po
     from self_amd.workloads import sw2d_cfl_dt, sw2d_geostrophic_fields
-
-    threads = max(1, min(16, os.cpu_count() or 1))
     L = 1.0e6
     nodal = po.NodalStorage2D(N)
-    mesh = po.QuadMesh(nX, nX, N, nodal.nodes, True, True, L, L)
+    mesh = po.QuadMesh(nX, nX, N, nodal.nodes, periodic, periodic, L, L)
     n1 = N + 1
     x = mesh.x.reshape(-1, n1, n1)
     y = mesh.y.reshape(-1, n1, n1)
@@ -69,17 +93,114 @@ def cpu_baseline(nX, N, target_s=12.0):
     orc.bathy[:] = np.stack([h, 0 * h, 0 * h], axis=1).ravel()
     orc.plv[:] = 1.0e-4
     orc.bathymetry_gradient()
+    return orc, (h, U, V, H), dt
+
+
+def _time_oracle_sw(nX, N, threads, target_s):
+    orc, _, _ = _c2_oracle(nX, N)
     orc.integrate(0.0, 2, threads)  # warm the threads and caches
     steps, el = 0, 0.0
     t0 = time.perf_counter()
-    while el < target_s and steps < 5000:  # chunks of 20 steps until the sample is long enough
-        orc.integrate(0.0, 20, threads)
-        steps += 20
+    while el < target_s and steps < 5000:  # chunks of 10 steps until the sample is long enough
+        orc.integrate(0.0, 10, threads)
+        steps += 10
         el = time.perf_counter() - t0
-    dof = 3 * nX * nX * n1 * n1
+    dof = 3 * nX * nX * (N + 1) ** 2
     return {"value": dof * 3 * steps / el, "unit": "DOF-stage/s", "cores": threads, "kind": "port",
             "sample": f"oracle FP64 C restatement, {nX}x{nX} N={N} geostrophic case, {steps} RK3 steps, "
-                      f"{threads} OpenMP threads, {el:.2f} s"}
+                      f"{threads} OpenMP thread(s), {el:.2f} s"}
+
+
+def _time_reference_sw(nX, N, steps):
+    """The reference's own class (oracle/_ref/ref_sw2d, serial) on its
+    default walled nX x nX mesh scaled to 1e6 m, with the geostrophic state."""
+    import numpy as np
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import ref_run as R
+    if not R.available("sw2d"):
+        return None
+    orc, (h, U, V, H), dt = _c2_oracle(nX, N, periodic=False)
+    T = lambda a: np.ascontiguousarray(a.transpose(2, 1, 0))  # noqa: E731  (e, j, i) -> (i, j, e)
+    inp = {"sol": np.stack([T(U), T(V), T(H)], axis=2), "h": T(h), "f": np.full_like(T(h), 1.0e-4)}
+    sec = R.time_sw2d(N, nX, nX, 9.81, dt, steps, inp, xScale=1.0e6, yScale=1.0e6)
+    dof = 3 * nX * nX * (N + 1) ** 2
+    return {"value": dof * 3 * steps / sec, "unit": "DOF-stage/s", "cores": 1, "kind": "reference",
+            "sample": f"reference ConservativeShallowWaterClass (planewave example's copy, Fortran, amdflang -O2, "
+                      f"serial as the reference builds it), default walled {nX}x{nX} mesh scaled to 1e6 m, N={N}, "
+                      f"geostrophic state, {steps} ForwardStepRK3 calls, {sec:.2f} s"}
+
+
+def _time_oracle_others():
+    """1-thread oracle samples of the other configurations (the C
+    restatements of the advection, 3-D advection and CGSEM loops are serial)."""
+    import numpy as np
+    from oracle import pyoracle as po
+    from self_amd.workloads import (adv2d_blob_fields, adv3d_gaussians_fields, cgsem_poisson_fields)
+    out = {}
+    # C1: advection-diffusion blob, 10x10 N=7 periodic, 100 RK3 steps
+    N = 7
+    nodal = po.NodalStorage2D(N)
+    mesh = po.QuadMesh(10, 10, N, nodal.nodes, True, True, 1.0, 1.0)
+    orc = po.Advection2D(nodal, mesh, nTracers=1, kappa=1e-3, dt=1e-4)
+    orc.c[:] = adv2d_blob_fields(mesh.x, mesh.y)
+    orc.u[:] = 1.0
+    orc.v[:] = 0.5
+    t0 = time.perf_counter()
+    for k in range(100):
+        orc.forward_step_rk3(k * 1e-4)
+    el = time.perf_counter() - t0
+    out["C1_adv2d"] = {"value": 6400 * 3 * 100 / el, "unit": "DOF-stage/s", "cores": 1, "kind": "port",
+                       "ms_per_step": 1e3 * el / 100, "sample": f"10x10 N=7, 100 RK3 steps, {el:.2f} s"}
+    # C3: 3-D advection with the dense roll-off filter, sampled on 8x8x4 (per element
+    # the work does not depend on the mesh size)
+    nd = po.Nodal3D(N)
+    hm = po.HexMesh(8, 8, 4, N, nd.nodes, warp=0.05)
+    o3 = po.Advection3D(nd, hm, nTracers=1, dt=1e-4, nCutoff=3, useFilter=True)
+    o3.c[:] = adv3d_gaussians_fields(hm.x, hm.y, hm.z)
+    o3.u[:], o3.v[:], o3.w[:] = 1.0, 0.7, 0.3
+    t0 = time.perf_counter()
+    for k in range(2):
+        o3.forward_step_rk3(k * 1e-4)
+    el = time.perf_counter() - t0
+    dof = hm.nElems * 512
+    out["C3_adv3d"] = {"value": dof * 3 * 2 / el, "unit": "DOF-stage/s", "cores": 1, "kind": "port",
+                       "sample": f"8x8x4 N=7 (1/64 of 32x32x16), dense filter on, 2 RK3 steps, {el:.2f} s"}
+    # C4: CGSEM PCG iterations at full size, 128x128 N=7, 30 iterations
+    cg = po.CGSEM2D(N, 128, 128, tolerance=1e-30, maxIters=30)
+    n1 = N + 1
+    x = cg.mesh.x.reshape(-1, n1, n1)
+    y = cg.mesh.y.reshape(-1, n1, n1)
+    k, f = cgsem_poisson_fields(x, y)
+    cg.fluxCoeff[:] = k.ravel()
+    cg.source[:] = -f.ravel()
+    t0 = time.perf_counter()
+    io, it = cg.solve(precond=0)
+    el = time.perf_counter() - t0
+    out["C4_cgsem_cg"] = {"value": it / el, "unit": "Krylov iterations/s", "cores": 1, "kind": "port",
+                          "sample": f"128x128 N=7 GLL, {it} CG iterations of the reference's CGSEM loop, {el:.2f} s"}
+    return out
+
+
+def cpu_baseline(nX, N, target_s=12.0, others=True):
+    """The reference's own Fortran on a bounded sample of the workload (kind
+    "reference"; the oracle when the reference build is absent), with the
+    oracle at 1 thread and at all usable cores beside it."""
+    info = host_info()
+    allc = max(1, min(16, info["usable_cores"]))  # the GPU box gives one GPU 16 host cores
+    port1 = _time_oracle_sw(nX, N, 1, target_s)
+    portN = _time_oracle_sw(nX, N, allc, target_s) if allc > 1 else port1
+    ref = None
+    try:
+        ref = _time_reference_sw(nX, N, 60)
+    except Exception as exc:  # the reference build is optional; say why it is absent
+        info["reference_error"] = repr(exc)[:200]
+    out = dict(ref if ref else port1)
+    out["host"] = info
+    out["port_1_thread"] = port1
+    out["port_all_cores"] = portN
+    if others:
+        out["other_configs"] = _time_oracle_others()
+    return out
 
 
 def bench_adv2d(steps=100, warm=10):
@@ -328,27 +449,44 @@ def main():
 
     acc = [rel(sols["mixed_fused"][:, f], sols["fp64_fused"][:, f]) for f in range(3)]
     acc_plain = [rel(sols["fp64_fused"][:, f], sols["fp64_plain"][:, f]) for f in range(3)]
+    # the same gate near the scheme's stability limit: 500 steps at dt = 2.5 s
+    # (1250 s of model time; the scheme diverges between 2.5 and 3 s)
+    long_sols = {}
+    for prec in ("fp64", "mixed"):
+        m, info = sw2d_geostrophic(nX, N, precision=prec, form="fused", seed=SEED)
+        m.SetTimeStep(2.5)
+        m.ForwardStepRK3(0.0, 500)
+        long_sols[prec] = m.GetSolution()
+        m.Trash()
+    acc_long = [rel(long_sols["mixed"][:, f], long_sols["fp64"][:, f]) for f in range(3)]
 
-    head = results["mixed_fused"]
-    # algorithmic (compulsory) bytes per stage launch, the path's fixed
-    # per-element figure: FP64 state, register and metrics, with FP32
-    # bathymetry/Coriolis and normals for the mixed variant (10 112 B per
-    # element at N = 7; FP64 11 520 B).  The mixed kernel moves fewer bytes
-    # than this (FP32 register, FP32 geometry deviations), which the
-    # achieved rate credits.
-    b_mixed = sw_bytes_per_element(N, 8, 8, 8, 4, 4) * E
+    head = results["fp64_fused"]
+    # algorithmic (compulsory) bytes per stage launch of the FP64 fused kernel:
+    # q and G read + written, 5 metric terms, (h, h_x, h_y, f) per node and
+    # (n_x, n_y, |n|) per side node, all FP64: 11 520 B per element at N = 7
     b_fp64 = sw_bytes_per_element(N, 8, 8, 8, 8, 8) * E
+    # the mixed kernel's own arrays: q FP64 r/w, G FP32 r/w, metric deviations
+    # and h_x, h_y, f FP32 per node, 5 FP64 metric values per element, FP32
+    # normal deviations per side node and 3 FP64 values per side (h is read
+    # only with drag, absent here)
+    np_ = (N + 1) ** 2
+    b_mixed = E * (np_ * (2 * 3 * 8 + 2 * 3 * 4 + 5 * 4 + 3 * 4) + 5 * 8 + 4 * (N + 1) * 3 * 4 + 4 * 3 * 8)
     # mean stage-kernel duration: the timed region is 3K back-to-back launches
     # of the fused stage kernel on the model's stream
     stage_ms = head["ms_per_step"] / 3.0
-    achieved = b_mixed / (stage_ms * 1e-3) / 1e9
-    traffic = load_traffic("sw2d_stage_fused_mixed_64x64_N7_traffic.json")
+    achieved = b_fp64 / (stage_ms * 1e-3) / 1e9
+    tname = "sw2d_stage_fused_fp64_64x64_N7_traffic.json"
+    traffic = load_traffic(tname)
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 4),
                 "traffic": traffic.get("bytes_per_launch") if traffic else None,
-                "kernel": "k_sw_stage_fused<7,mixed>", "algorithmic_bytes_per_launch": b_mixed,
+                "traffic_source": (f"profiles/{tname}: rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE of this kernel "
+                                   "at this size, calibrated on a copy kernel; stored, not measured in this run")
+                if traffic else None,
+                "kernel": "k_sw_stage_fused<7,fp64>", "algorithmic_bytes_per_launch": b_fp64,
                 "mean_launch_ms": stage_ms, "event_pair_ms": head["kernel_ms"][0]}
-    fp64_stage_ms = results["fp64_fused"]["ms_per_step"] / 3.0
+    mixed = results["mixed_fused"]
+    mixed_stage_ms = mixed["ms_per_step"] / 3.0
     out = {
         "metric": METRIC,
         "value": head["dof_stage_per_s"],
@@ -360,26 +498,30 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64 state + f32 contractions",
+        "dtype": "f64",
         "data": "synthetic",
-        "config": {"workload": "SW2D geostrophic f-plane 64x64 elements N=7 periodic, RK3 CFL 0.5",
+        "config": {"workload": "SW2D geostrophic f-plane 64x64 elements N=7 periodic, RK3 dt 1.22 s",
                    "elements": E, "nodes": E * (N + 1) ** 2, "fields": 3, "N": N, "dt_s": head["dt"],
                    "seed": SEED, "parallelism": "single GPU"},
         "roofline": roofline,
         "variants": {
-            "fp64_fused": {"dof_stage_per_s": results["fp64_fused"]["dof_stage_per_s"],
-                           "ms_per_step": results["fp64_fused"]["ms_per_step"],
-                           "stage_kernel_ms": fp64_stage_ms,
-                           "achieved_GBs": round(b_fp64 / (fp64_stage_ms * 1e-3) / 1e9, 1)},
+            "mixed_fused": {"dtype": "f64 state and RK accumulation + f32 volume contractions and tendency register",
+                            "dof_stage_per_s": mixed["dof_stage_per_s"], "ms_per_step": mixed["ms_per_step"],
+                            "stage_kernel_ms": mixed_stage_ms, "bytes_per_launch": b_mixed,
+                            "achieved_GBs": round(b_mixed / (mixed_stage_ms * 1e-3) / 1e9, 1),
+                            "frac": round(b_mixed / (mixed_stage_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+                            "speedup_vs_fp64_fused": mixed["dof_stage_per_s"] / head["dof_stage_per_s"],
+                            "rel_l2_vs_fp64_UVH": acc},
             "fp64_plain": {"dof_stage_per_s": results["fp64_plain"]["dof_stage_per_s"],
                            "ms_per_step": results["fp64_plain"]["ms_per_step"],
                            "kernel_ms_traces_edgeflux_mapped_rk": results["fp64_plain"]["kernel_ms"]},
         },
-        "speedup_mixed_vs_fp64_plain": head["dof_stage_per_s"] / results["fp64_plain"]["dof_stage_per_s"],
-        "speedup_fp64_fused_vs_fp64_plain": results["fp64_fused"]["dof_stage_per_s"]
-        / results["fp64_plain"]["dof_stage_per_s"],
+        "speedup_fp64_fused_vs_fp64_plain": head["dof_stage_per_s"] / results["fp64_plain"]["dof_stage_per_s"],
+        "speedup_mixed_vs_fp64_plain": mixed["dof_stage_per_s"] / results["fp64_plain"]["dof_stage_per_s"],
         "accuracy": {"mixed_vs_fp64_rel_l2_UVH": acc, "fp64_fused_vs_plain_rel_l2_UVH": acc_plain,
-                     "steps_compared": K + W, "gate": 1e-6, "pass": max(acc) <= 1e-6},
+                     "steps_compared": K + W, "dt_s": head["dt"],
+                     "mixed_vs_fp64_rel_l2_UVH_dt2p5_500_steps": acc_long,
+                     "gate": 1e-6, "pass": max(acc + acc_long) <= 1e-6},
     }
     if not args.no_extras:
         out["sw2d_ensemble"] = bench_sw2d_ensemble()
@@ -390,8 +532,7 @@ def main():
     if not args.no_serve:
         out["serving"] = bench_serving(args.serve_requests)
     if not args.no_cpu:
-        cb = cpu_baseline(nX, N)
-        out["cpu_baseline"] = cb
+        out["cpu_baseline"] = cpu_baseline(nX, N, others=not args.no_extras)
     print(json.dumps(out))
 
 

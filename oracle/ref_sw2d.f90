@@ -26,6 +26,10 @@
 !           after GlobalTimeDerivative(t0)
 !   float64 solution(0:N,0:N,3,nElems) after nSteps ForwardStepRK3 calls
 !
+! Timing mode (nSteps < 0 in ref_in.bin): after the same set-up it runs
+! -nSteps ForwardStepRK3 calls and writes only int32 -nSteps and float64 the
+! elapsed wall seconds of those calls (SYSTEM_CLOCK), for the CPU baseline.
+!
 ! Build is called with forceNoPickup (no pickup file is read); the mesh is
 ! the reference's default walled mesh (NO_NORMAL_FLOW sides) scaled by
 ! xScale, yScale.  The bathymetry traces are computed as ReadPickup does
@@ -36,6 +40,7 @@ PROGRAM ref_sw2d
  IMPLICIT NONE
  TYPE(ShallowWater) :: sw
  INTEGER :: N, nE, nEd, iEl, iEdge, i, j, b, iStep, nSteps, uIn, uOut
+ INTEGER(8) :: c0, c1, cRate
  INTEGER, ALLOCATABLE :: edges(:,:)
  REAL(prec) :: t0, dt, nrm(1:2), length
  REAL(prec), ALLOCATABLE :: s(:,:,:,:), h(:,:,:), f(:,:,:), bath(:,:,:,:), o1(:,:,:,:), bf(:,:,:,:)
@@ -75,6 +80,20 @@ PROGRAM ref_sw2d
    DO iEl = 1, nE
       CALL sw % GetBathymetry( iEl, bath(:,:,:,iEl) )
    ENDDO
+
+   IF( nSteps < 0 )THEN
+      CALL SYSTEM_CLOCK( c0, cRate )
+      DO iStep = 0, -nSteps - 1
+         CALL sw % ForwardStepRK3( t0 + REAL(iStep,prec)*dt )
+      ENDDO
+      CALL SYSTEM_CLOCK( c1 )
+      OPEN( NEWUNIT = uOut, FILE = 'ref_out.bin', ACCESS = 'STREAM', FORM = 'UNFORMATTED', STATUS = 'REPLACE' )
+      WRITE( uOut ) -nSteps
+      WRITE( uOut ) REAL(c1 - c0, prec)/REAL(cRate, prec)
+      CLOSE( uOut )
+      CALL sw % Trash( )
+      STOP
+   ENDIF
 
    CALL sw % dGStorage % GetDerivativeMatrix( dMatS, dMatP )
    CALL sw % dGStorage % GetQuadratureWeights( wS, wP )

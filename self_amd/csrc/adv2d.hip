@@ -510,7 +510,7 @@ try {
     std::memcpy(p + 3 * n1, lagEast, sizeof(double) * n1);
     std::memcpy(p + 4 * n1, lagSouth, sizeof(double) * n1);
     std::memcpy(p + 5 * n1, lagNorth, sizeof(double) * n1);
-    for (int k = 0; k < n1; ++k) p[6 * n1 + k] = p[7 * n1 + k] = 1.0 / qWeight[k];
+    nodal_buffer_derived(nod.data(), n1);
     std::vector<int2> sd(nE * 4);
     for (size_t k = 0; k < nE * 4; ++k) sd[k] = make_int2(G.sideX[k], G.sideY[k]);
     e = hipMemcpy(h->nodal, nod.data(), sizeof(double) * nod.size(), hipMemcpyHostToDevice);

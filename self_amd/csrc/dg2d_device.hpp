@@ -153,18 +153,42 @@ __device__ __forceinline__ int xcd_block(int b, int nb)
 // matrices (column-major, row i col m at i + n1*m), reciprocal quadrature
 // weights and the four boundary Lagrange vectors.  Reading them from LDS
 // (mostly broadcast reads) keeps them out of the VGPR budget.
+// rsS, rsP: row sums sum_m D(i, m) of the two matrices (FP64, host), used by
+// the mixed form to carry a line's reference value exactly: D (r 1) = r rs.
 template <int N>
 struct NodalLDS {
     static constexpr int n1 = N + 1;
     double dS[n1 * n1], dP[n1 * n1];
     double iwS[n1], iwP[n1];
     double lW[n1], lE[n1], lS[n1], lN[n1];
+    double rsS[n1], rsP[n1];
     float dS32[n1 * n1], dP32[n1 * n1];
 };
 
 // global nodal buffer (doubles): dS, dP, qwS, qwP, lagW, lagE, lagS, lagN,
-// then 1/qwS, 1/qwP (computed on the host: the same IEEE quotients)
-constexpr int nodal_buffer_size(int n1) { return 2 * n1 * n1 + 8 * n1; }
+// then 1/qwS, 1/qwP (computed on the host: the same IEEE quotients), then
+// the row sums of dS and dP
+constexpr int nodal_buffer_size(int n1) { return 2 * n1 * n1 + 10 * n1; }
+
+// Host side: fills the inverse weights and row sums of a nodal buffer whose
+// first 2 n1^2 + 6 n1 entries (matrices, weights, Lagrange vectors) are set.
+inline void nodal_buffer_derived(double *buf, int n1)
+{
+    double *p = buf + 2 * n1 * n1;
+    for (int k = 0; k < n1; ++k) {
+        p[6 * n1 + k] = 1.0 / p[k];
+        p[7 * n1 + k] = 1.0 / p[n1 + k];
+    }
+    for (int i = 0; i < n1; ++i) {
+        double a = 0.0, b = 0.0;
+        for (int m = 0; m < n1; ++m) {
+            a += buf[i + n1 * m];
+            b += buf[n1 * n1 + i + n1 * m];
+        }
+        p[8 * n1 + i] = a;
+        p[9 * n1 + i] = b;
+    }
+}
 
 // lane / lanes: the threads that stage this copy (the whole workgroup by
 // default, or one wavefront staging its own copy).
@@ -173,12 +197,13 @@ __device__ __forceinline__ void stage_nodal(NodalLDS<N> &L, const double *__rest
                                             const float *__restrict__ dS32, const float *__restrict__ dP32,
                                             int lane = -1, int lanes = 0)
 {
-    constexpr int n1 = N + 1, nd = 2 * n1 * n1 + 6 * n1;
+    constexpr int n1 = N + 1, nd = 2 * n1 * n1 + 8 * n1;
     if (lane < 0) { lane = threadIdx.x; lanes = blockDim.x; }
-    double *dst = L.dS;  // members up to lN are contiguous doubles: dS, dP, iwS, iwP, lags
+    double *dst = L.dS;  // members up to rsP are contiguous doubles: dS, dP, iwS, iwP, lags, row sums
     for (int k = lane; k < nd; k += lanes) {
         const bool w = (k >= 2 * n1 * n1) && (k < 2 * n1 * n1 + 2 * n1);
-        dst[k] = g[w ? k + 6 * n1 : k];
+        const bool r = k >= 2 * n1 * n1 + 6 * n1;
+        dst[k] = g[w ? k + 6 * n1 : (r ? k + 2 * n1 : k)];
     }
     if constexpr (F32) {
         for (int k = lane; k < n1 * n1; k += lanes) {

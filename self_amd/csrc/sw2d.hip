@@ -300,43 +300,11 @@ __device__ __forceinline__ void dg_divergence64(const NodalLDS<N> &L, const doub
     }
 }
 
-// Same with FP32 contractions applied to the deviation of each flux line
-// from its FP64 mean r; the mean's derivative is carried by the FP64 lift
-// through sum_m Dhat(i,m) = -(l_E(i) - l_W(i))/w_i.
-template <int N>
-__device__ __forceinline__ void dg_divergence_mixed(const NodalLDS<N> &L, const double *sC, const double *pC,
-                                                    const double *bf, int i, int j, double invJ, double tend[3])
-{
-    constexpr int n1 = N + 1, np = n1 * n1;
-    constexpr double inv_n1 = 1.0 / n1;
-    const double lEi = L.lE[i], lWi = L.lW[i], lNj = L.lN[j], lSj = L.lS[j];
-    const double iwS = L.iwS[i], iwP = L.iwP[j];
-    #pragma unroll 1
-    for (int f = 0; f < 3; ++f) {
-        const double *sl = sC + f * np + n1 * j;
-        const double *pl = pC + f * np + i;
-        double a = 0.0, b = 0.0;
-        #pragma unroll
-        for (int m = 0; m < n1; ++m) { a += sl[m]; b += pl[n1 * m]; }
-        const double rs = a * inv_n1, rp = b * inv_n1;
-        float t1 = 0.f, t2 = 0.f;
-        #pragma unroll
-        for (int m = 0; m < n1; ++m) {
-            t1 = __builtin_fmaf(L.dS32[i + n1 * m], (float)(sl[m] - rs), t1);
-            t2 = __builtin_fmaf(L.dP32[j + n1 * m], (float)(pl[n1 * m] - rp), t2);
-        }
-        const double ta = (double)t1 + ((bf[(1 * 3 + f) * n1 + j] - rs) * lEi +
-                                        (bf[(3 * 3 + f) * n1 + j] + rs) * lWi) * iwS;
-        const double tb = (double)t2 + ((bf[(2 * 3 + f) * n1 + i] - rp) * lNj +
-                                        (bf[(0 * 3 + f) * n1 + i] + rp) * lSj) * iwP;
-        tend[f] = (-ta - tb) * invJ;
-    }
-}
-
 // Mixed form with the FP32 deviations prepared once per node in LDS:
 // sD(m, j) = sC(m, j) - sC(0, j) along s-lines, pD(i, m) = pC(i, m) - pC(i, 0)
-// along p-lines; the line's reference value re-enters through the FP64 lift
-// (sum_m Dhat(i,m) = -(l_E(i) - l_W(i))/w_i).
+// along p-lines; the line's reference value re-enters in FP64 through the
+// matrix row sums (D (r 1) = r sum_m D(i, m)), which holds for any operator
+// the model is given, the reference's transposed dMatP included.
 template <int N>
 __device__ __forceinline__ void dg_divergence_dev32(const NodalLDS<N> &L, const double *sC, const double *pC,
                                                     const float *sD, const float *pD, const double *bf, int i, int j,
@@ -344,7 +312,7 @@ __device__ __forceinline__ void dg_divergence_dev32(const NodalLDS<N> &L, const 
 {
     constexpr int n1 = N + 1, np = n1 * n1;
     const double lEi = L.lE[i], lWi = L.lW[i], lNj = L.lN[j], lSj = L.lS[j];
-    const double iwS = L.iwS[i], iwP = L.iwP[j];
+    const double iwS = L.iwS[i], iwP = L.iwP[j], rsS = L.rsS[i], rsP = L.rsP[j];
     #pragma unroll
     for (int f = 0; f < 3; ++f) {
         const float *sl = sD + f * np + n1 * j;
@@ -356,10 +324,10 @@ __device__ __forceinline__ void dg_divergence_dev32(const NodalLDS<N> &L, const 
             t2 = __builtin_fmaf(L.dP32[j + n1 * m], pl[n1 * m], t2);
         }
         const double rs = sC[f * np + n1 * j], rp = pC[f * np + i];
-        const double ta = (double)t1 + ((bf[(1 * 3 + f) * n1 + j] - rs) * lEi +
-                                        (bf[(3 * 3 + f) * n1 + j] + rs) * lWi) * iwS;
-        const double tb = (double)t2 + ((bf[(2 * 3 + f) * n1 + i] - rp) * lNj +
-                                        (bf[(0 * 3 + f) * n1 + i] + rp) * lSj) * iwP;
+        const double ta = (double)t1 + rs * rsS + (bf[(1 * 3 + f) * n1 + j] * lEi +
+                                                    bf[(3 * 3 + f) * n1 + j] * lWi) * iwS;
+        const double tb = (double)t2 + rp * rsP + (bf[(2 * 3 + f) * n1 + i] * lNj +
+                                                    bf[(0 * 3 + f) * n1 + i] * lSj) * iwP;
         tend[f] = (-ta - tb) * invJ;
     }
 }
@@ -1158,10 +1126,7 @@ try {
     std::memcpy(p + 3 * n1, lE, sizeof(double) * n1);
     std::memcpy(p + 4 * n1, lS, sizeof(double) * n1);
     std::memcpy(p + 5 * n1, lN, sizeof(double) * n1);
-    for (int k = 0; k < n1; ++k) {
-        p[6 * n1 + k] = 1.0 / qwS[k];
-        p[7 * n1 + k] = 1.0 / qwP[k];
-    }
+    nodal_buffer_derived(buf.data(), n1);
     SELF_HIP(hipMemcpy(h->nodal, buf.data(), sizeof(double) * buf.size(), hipMemcpyHostToDevice));
     if (h->dS32) {
         std::vector<float> f(2 * n1 * n1);

@@ -407,6 +407,7 @@ def random_inputs_sw2d(N, nX, nY, seed):
 
 
 def _params_text_sw2d(N, nX, nY, g, dt, xScale, yScale):
+    g, dt, xScale, yScale = float(g), float(dt), float(xScale), float(yScale)
     return ("&MODEL_FORM\n/\n"
             "&TIME_MANAGEMENT\n"
             f" dt = {dt!r},\n iterInit = 0,\n nTimeSteps = 0,\n dumpFreq = 1,\n/\n"
@@ -451,3 +452,24 @@ def run_sw2d(N, nX, nY, g, dt, t0, nSteps, inputs, xScale=1.0, yScale=1.0, timeo
     r.done()
     out.update(N=N, nX=nX, nY=nY, g=g, dt=dt, t0=t0, nSteps=nSteps, xScale=xScale, yScale=yScale)
     return out
+
+
+def time_sw2d(N, nX, nY, g, dt, nSteps, inputs, xScale=1.0, yScale=1.0, timeout=600):
+    """Wall seconds of nSteps ForwardStepRK3 calls of the reference's
+    shallow-water class (oracle/ref_sw2d timing mode) on its default walled
+    nX x nY mesh, from the given state; the reference is serial."""
+    import resource
+    try:  # the class keeps its RK3 register on the stack
+        resource.setrlimit(resource.RLIMIT_STACK, (resource.RLIM_INFINITY, resource.RLIM_INFINITY))
+    except (ValueError, OSError):
+        pass
+    raw = _execute("sw2d", _params_text_sw2d(N, nX, nY, g, dt, xScale, yScale),
+                   [inputs[k] for k in ("sol", "h", "f")],
+                   np.int32(-nSteps).tobytes() + np.float64(0.0).tobytes(), timeout,
+                   params_file="shallowwater.params")
+    r = _Reader(raw)
+    steps = int(r.take(np.int32, (1,))[0])
+    sec = float(r.take(np.float64, (1,))[0])
+    r.done()
+    assert steps == nSteps
+    return sec

@@ -212,6 +212,24 @@ def test_geostrophic_500_steps_mixed_within_gate(gpu_lib):
     assert abs(vol1 - vol0) / abs(vol0) < 1e-13
 
 
+def test_geostrophic_gate_near_the_stability_limit(gpu_lib):
+    """The C2 gate at dt = 2.5 s, about twice the workload's step and just
+    below where the scheme diverges (between 2.5 and 3 s on the oracle; the
+    survey's 0.5 dx / ((2N+1) lambda) = 5.2 s diverges within 100 steps):
+    500 steps cover 1250 s of model time; mixed vs FP64 <= 1e-6 per field."""
+    from self_amd.workloads import sw2d_geostrophic
+    res = {}
+    for prec in ("fp64", "mixed"):
+        m, info = sw2d_geostrophic(64, 7, precision=prec, form="fused")
+        m.SetTimeStep(2.5)
+        m.ForwardStepRK3(0.0, 500)
+        res[prec] = m.GetSolution()
+        m.Trash()
+    assert np.all(np.isfinite(res["fp64"]))
+    for f in range(3):
+        assert rel_l2(res["mixed"][:, f], res["fp64"][:, f]) < 1e-6, f
+
+
 @pytest.mark.parametrize("precision", ["fp64", "mixed"])
 @pytest.mark.parametrize("case", [CASES[0], CASES[2], CASES[5]], ids=_ids)
 def test_pipelined_groups_match_the_one_group_schedule(gpu_lib, case, precision):
