@@ -404,6 +404,11 @@ def bench_serving(n=512, workers=8, steps=200):
     return out
 
 
+def progress(msg):
+    """One line per finished part on stderr, so a long run shows it is alive."""
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -443,6 +448,7 @@ def main():
         results[name] = {"ms_per_step": ms / K, "wall_ms_per_step": 1e3 * wall / K,
                          "dof_stage_per_s": dof * 3 * K / (ms * 1e-3), "kernel_ms": kt, "dt": info["dt"]}
         m.Trash()
+        progress(f"{name}: {ms / K:.4f} ms per step")
 
     def rel(a, b):
         return float(np.linalg.norm((a - b).ravel()) / np.linalg.norm(b.ravel()))
@@ -523,14 +529,15 @@ def main():
                      "mixed_vs_fp64_rel_l2_UVH_dt2p5_500_steps": acc_long,
                      "gate": 1e-6, "pass": max(acc + acc_long) <= 1e-6},
     }
+    progress("accuracy gates done")
     if not args.no_extras:
-        out["sw2d_ensemble"] = bench_sw2d_ensemble()
-        out["sw2d_dipole_N10"] = bench_sw2d_dipole()
-        out["adv2d"] = bench_adv2d()
-        out["adv3d"] = bench_adv3d()
-        out["cgsem"] = bench_cgsem()
+        for key, fn in (("sw2d_ensemble", bench_sw2d_ensemble), ("sw2d_dipole_N10", bench_sw2d_dipole),
+                        ("adv2d", bench_adv2d), ("adv3d", bench_adv3d), ("cgsem", bench_cgsem)):
+            out[key] = fn()
+            progress(f"{key} done")
     if not args.no_serve:
         out["serving"] = bench_serving(args.serve_requests)
+        progress("serving done")
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(nX, N, others=not args.no_extras)
     print(json.dumps(out))

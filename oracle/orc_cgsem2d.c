@@ -335,3 +335,318 @@ int orc_cgsem2d_connectivity(orc_cgsem2d *c, const int *elemNodes, int nNodes)
     free(cnt); free(fill);
     return 0;
 }
+
+/* ------------------------------------------------------------------------
+ * CGsemPreconditioner_2D (src/cgsem/CGsemPreconditioner_2D_Class.f90) on the
+ * linear mesh of ConstructLinearMeshFromOther_QuadMesh
+ * (src/geom/QuadMeshClass.f90:1863-2096).
+ */
+
+/* Curve_2D of two nodes at s = -1, 1 (Lagrange_1D, N = 1): value and slope */
+static void curve1_eval(const double *cx, const double *cy, const double *s, const double *bw, double a,
+                        double *P)
+{
+    P[0] = orc_lagrange_interpolate(1, s, bw, a, cx);
+    P[1] = orc_lagrange_interpolate(1, s, bw, a, cy);
+}
+
+static void curve1_slope(const double *cx, const double *cy, const double *s, const double *bw, double a,
+                         double *P)
+{
+    P[0] = orc_lagrange_derivative(1, s, bw, a, cx);
+    P[1] = orc_lagrange_derivative(1, s, bw, a, cy);
+}
+
+/* TransfiniteInterpolationMetrics_2D (GeometrySupportRoutines.f90:109-158) for
+ * four two-node curves cx/cy[side][node], sides south, east, north, west */
+static void tfi_metrics1(double cx[4][2], double cy[4][2], const double *s, const double *bw, double a,
+                         double b, double covT[2][2])
+{
+    double w[2], e[2], so[2], no[2], dW[2], dE[2], dS[2], dN[2], c1[2], c2[2], c3[2], c4[2];
+    curve1_eval(cx[3], cy[3], s, bw, b, w);
+    curve1_eval(cx[1], cy[1], s, bw, b, e);
+    curve1_eval(cx[0], cy[0], s, bw, a, so);
+    curve1_eval(cx[2], cy[2], s, bw, a, no);
+    curve1_slope(cx[3], cy[3], s, bw, b, dW);
+    curve1_slope(cx[1], cy[1], s, bw, b, dE);
+    curve1_slope(cx[0], cy[0], s, bw, a, dS);
+    curve1_slope(cx[2], cy[2], s, bw, a, dN);
+    curve1_eval(cx[0], cy[0], s, bw, -1.0, c1);
+    curve1_eval(cx[0], cy[0], s, bw, 1.0, c2);
+    curve1_eval(cx[2], cy[2], s, bw, -1.0, c3);
+    curve1_eval(cx[2], cy[2], s, bw, 1.0, c4);
+    for (int k = 0; k < 2; ++k) {
+        double P1 = 0.5 * (e[k] - w[k]), Ps = 0.5 * (c2[k] - c1[k]), Pn = 0.5 * (c4[k] - c3[k]);
+        double L = dS[k] - Ps, R = dN[k] - Pn;
+        covT[k][0] = P1 + 0.5 * ((1.0 - b) * L + (1.0 + b) * R);
+        double P2 = 0.5 * (no[k] - so[k]), Pw = 0.5 * (c3[k] - c1[k]), Pe = 0.5 * (c4[k] - c2[k]);
+        L = dW[k] - Pw;
+        R = dE[k] - Pe;
+        covT[k][1] = P2 + 0.5 * ((1.0 - a) * L + (1.0 + a) * R);
+    }
+}
+
+static int cmpll(const void *p, const void *q)
+{
+    long long a = *(const long long *)p, b = *(const long long *)q;
+    return (a > b) - (a < b);
+}
+
+int orc_cgsem2d_pc_build(orc_cgsem2d_pc *pc, int N, int nEl, const double *x, const double *y,
+                         const double *fluxCoeff, double tolerance, int maxIters)
+{
+    memset(pc, 0, sizeof(*pc));
+    const int n1 = N + 1, nLin = nEl * N * N, nLinNodes = nEl * n1 * n1;
+    pc->nS = N;
+    pc->nEl = nEl;
+    pc->nLin = nLin;
+    pc->tolerance = tolerance;
+    pc->maxIters = maxIters;
+    /* unique nodes by AlmostEqual positions, first occurrence numbering (:1900-1925) */
+    int *uniq = (int *)malloc(sizeof(int) * nLinNodes);
+    int nUnique = 0;
+    for (int i = 0; i < nLinNodes; ++i) {
+        int inc = 1;
+        for (int j = 0; j < i; ++j)
+            if (orc_almost_equal(x[i], x[j]) && orc_almost_equal(y[i], y[j])) { uniq[i] = uniq[j]; inc = 0; break; }
+        nUnique += inc;
+        if (inc) uniq[i] = nUnique;
+    }
+    pc->nNodes = nUnique;
+    /* node positions: the last copy visited wins (SetNodePosition, :1933-1944) */
+    double *nx = (double *)malloc(sizeof(double) * nUnique), *ny = (double *)malloc(sizeof(double) * nUnique);
+    for (int i = 0; i < nLinNodes; ++i) { nx[uniq[i] - 1] = x[i]; ny[uniq[i] - 1] = y[i]; }
+    pc->pcToOrig = (int *)calloc((size_t)12 * nLin, sizeof(int));
+    pc->origToPC = (int *)calloc((size_t)3 * nLinNodes, sizeof(int));
+    int *elemNodes = (int *)malloc(sizeof(int) * 4 * nLin);
+
+    const size_t nl1 = (size_t)4 * nLin;
+    pc->dxds = (double *)malloc(sizeof(double) * nl1); pc->dxdp = (double *)malloc(sizeof(double) * nl1);
+    pc->dyds = (double *)malloc(sizeof(double) * nl1); pc->dydp = (double *)malloc(sizeof(double) * nl1);
+    pc->J = (double *)malloc(sizeof(double) * nl1); pc->fluxCoeff = (double *)malloc(sizeof(double) * nl1);
+    pc->D1 = (double *)malloc(sizeof(double) * 4);
+    pc->w1 = (double *)malloc(sizeof(double) * 2);
+    double s1[2], bw1[2], wq[2];
+    orc_generate_legendre_quadrature(1, s1, wq, ORC_GAUSS_LOBATTO);
+    double sL[2] = {-1.0, 1.0};  /* the linear mesh's interpolant nodes (:1929-1931) */
+    orc_barycentric_weights(1, sL, bw1);
+    {
+        double bwq[2];
+        orc_barycentric_weights(1, s1, bwq);
+        orc_derivative_matrix(1, s1, bwq, pc->D1);
+        pc->w1[0] = wq[0]; pc->w1[1] = wq[1];
+    }
+    static const int em[2][4] = {{1, 2, 4, 1}, {2, 3, 3, 4}};  /* edgeMap */
+    for (int iEl = 0; iEl < nEl; ++iEl)
+        for (int iP = 1; iP <= N; ++iP)
+            for (int iS = 1; iS <= N; ++iS) {
+                const int eID = (iS - 1) + N * ((iP - 1) + iEl * N);  /* 0-based (:1960) */
+                const int ci[4] = {iS - 1, iS, iS, iS - 1}, cj[4] = {iP - 1, iP - 1, iP, iP};
+                /* linToOther (0,0), (1,0), (1,1), (0,1) and otherToLin, in the reference's order */
+                const int li[4] = {0, 1, 1, 0}, lj[4] = {0, 0, 1, 1};
+                for (int q = 0; q < 4; ++q) {
+                    int *t = pc->pcToOrig + 3 * (li[q] + 2 * lj[q] + 4 * eID);
+                    t[0] = ci[q]; t[1] = cj[q]; t[2] = iEl;
+                    int *o = pc->origToPC + 3 * (ci[q] + n1 * (cj[q] + n1 * iEl));
+                    o[0] = li[q]; o[1] = lj[q]; o[2] = eID;
+                    elemNodes[4 * eID + q] = uniq[ci[q] + n1 * (cj[q] + n1 * iEl)];
+                    pc->fluxCoeff[li[q] + 2 * lj[q] + 4 * eID] = fluxCoeff[ci[q] + n1 * (cj[q] + n1 * iEl)];
+                }
+                /* straight boundary curves between corner nodes (:1986-1997) */
+                double cx[4][2], cy[4][2];
+                for (int side = 0; side < 4; ++side) {
+                    const int a = elemNodes[4 * eID + em[0][side] - 1] - 1, b = elemNodes[4 * eID + em[1][side] - 1] - 1;
+                    const double x1 = nx[a], y1 = ny[a], x2 = nx[b], y2 = ny[b];
+                    for (int nd = 0; nd < 2; ++nd) {
+                        cx[side][nd] = x1 + (x2 - x1) * 0.5 * (sL[nd] + 1.0);
+                        cy[side][nd] = y1 + (y2 - y1) * 0.5 * (sL[nd] + 1.0);
+                    }
+                }
+                /* GenerateMetrics_MappedGeometry_2D at the corner nodes */
+                for (int i = 0; i < 2; ++i)
+                    for (int j = 0; j < 2; ++j) {
+                        double covT[2][2];
+                        tfi_metrics1(cx, cy, sL, bw1, sL[i], sL[j], covT);
+                        const int g = i + 2 * j + 4 * eID;
+                        pc->dxds[g] = covT[0][0]; pc->dxdp[g] = covT[0][1];
+                        pc->dyds[g] = covT[1][0]; pc->dydp[g] = covT[1][1];
+                        pc->J[g] = covT[0][0] * covT[1][1] - covT[0][1] * covT[1][0];
+                    }
+            }
+    /* edges: element sides shared by two linear elements are interior; the
+     * others are boundary and their nodes Dirichlet (:2047-2063 and :112-118) */
+    int *cnt = (int *)calloc(nUnique, sizeof(int));
+    pc->nodeType = (int *)malloc(sizeof(int) * nUnique);
+    for (int n = 0; n < nUnique; ++n) pc->nodeType[n] = 1;
+    {
+        /* count side occurrences by (min, max) node pair with a hash of sorted pairs */
+        long long *keys = (long long *)malloc(sizeof(long long) * 4 * nLin);
+        for (int e = 0; e < nLin; ++e)
+            for (int side = 0; side < 4; ++side) {
+                long long a = elemNodes[4 * e + em[0][side] - 1], b = elemNodes[4 * e + em[1][side] - 1];
+                if (a > b) { long long t = a; a = b; b = t; }
+                keys[4 * e + side] = a * (long long)(nUnique + 1) + b;
+            }
+        /* sort copy to find unique sides */
+        long long *sorted = (long long *)malloc(sizeof(long long) * 4 * nLin);
+        memcpy(sorted, keys, sizeof(long long) * 4 * nLin);
+        qsort(sorted, 4 * nLin, sizeof(long long), cmpll);
+        for (int q = 0; q < 4 * nLin; ++q) {
+            int m = 0;
+            long long key = sorted[q];
+            while (q + m < 4 * nLin && sorted[q + m] == key) ++m;
+            if (m == 1) {
+                int a = (int)(key / (nUnique + 1)), b = (int)(key % (nUnique + 1));
+                pc->nodeType[a - 1] = ORC_DIRICHLET;
+                pc->nodeType[b - 1] = ORC_DIRICHLET;
+            }
+            q += m - 1;
+        }
+        free(keys); free(sorted);
+    }
+    /* node -> element lists in AddToList order (GetNodeToElementConnectivity) */
+    for (int e = 0; e < nLin; ++e)
+        for (int k = 0; k < 4; ++k) cnt[elemNodes[4 * e + k] - 1]++;
+    pc->nodeElemStart = (int *)malloc(sizeof(int) * (nUnique + 1));
+    pc->nodeElem = (int *)malloc(sizeof(int) * 4 * nLin);
+    pc->nodeLocal = (int *)malloc(sizeof(int) * 4 * nLin);
+    pc->nodeElemStart[0] = 0;
+    for (int n = 0; n < nUnique; ++n) pc->nodeElemStart[n + 1] = pc->nodeElemStart[n] + cnt[n];
+    memset(cnt, 0, sizeof(int) * nUnique);
+    for (int e = 0; e < nLin; ++e)
+        for (int k = 0; k < 4; ++k) {
+            int n = elemNodes[4 * e + k] - 1, pos = pc->nodeElemStart[n] + cnt[n]++;
+            pc->nodeElem[pos] = e;
+            pc->nodeLocal[pos] = k;
+        }
+    pc->edges1 = (int *)malloc(sizeof(int) * 6);  /* no edge loops at N = 1 */
+    pc->solution1 = (double *)calloc(nl1, sizeof(double));
+    pc->source1 = (double *)calloc(nl1, sizeof(double));
+    orc_cgsem2d *l = &pc->lin;
+    memset(l, 0, sizeof(*l));
+    l->N = 1; l->nElems = nLin; l->nEdges = 0; l->nNodes = nUnique;
+    l->edges = pc->edges1; l->nodeElemStart = pc->nodeElemStart; l->nodeElem = pc->nodeElem;
+    l->nodeLocal = pc->nodeLocal; l->nodeType = pc->nodeType; l->D = pc->D1; l->w = pc->w1;
+    l->dxds = pc->dxds; l->dxdp = pc->dxdp; l->dyds = pc->dyds; l->dydp = pc->dydp; l->J = pc->J;
+    l->fluxCoeff = pc->fluxCoeff; l->solution = pc->solution1; l->source = pc->source1;
+    free(uniq); free(nx); free(ny); free(elemNodes); free(cnt);
+    return 0;
+}
+
+void orc_cgsem2d_pc_free(orc_cgsem2d_pc *pc)
+{
+    void *ptrs[] = {pc->pcToOrig, pc->origToPC, pc->D1, pc->w1, pc->dxds, pc->dxdp, pc->dyds, pc->dydp, pc->J,
+                    pc->fluxCoeff, pc->edges1, pc->nodeElemStart, pc->nodeElem, pc->nodeLocal, pc->nodeType,
+                    pc->solution1, pc->source1};
+    for (size_t k = 0; k < sizeof(ptrs) / sizeof(ptrs[0]); ++k) free(ptrs[k]);
+    memset(pc, 0, sizeof(*pc));
+}
+
+/* DotProduct_CGsemPreconditioner_2D (:600-660): masked copies, row sums */
+static double pc_dot(orc_cgsem2d_pc *pc, const double *u, const double *v, double *uu, double *vv)
+{
+    size_t nl = (size_t)4 * pc->nLin;
+    memcpy(uu, u, sizeof(double) * nl);
+    memcpy(vv, v, sizeof(double) * nl);
+    orc_cgsem2d_mask(&pc->lin, uu);
+    orc_cgsem2d_mask(&pc->lin, vv);
+    double s = 0.0;
+    for (int e = 0; e < pc->nLin; ++e)
+        for (int iP = 0; iP <= 1; ++iP) {
+            double d = 0.0;
+            for (int iS = 0; iS <= 1; ++iS) d += uu[iS + 2 * (iP + 2 * e)] * vv[iS + 2 * (iP + 2 * e)];
+            s = s + d;
+        }
+    return s;
+}
+
+/* Solve_CGsemPreconditioner_2D (:664-746) */
+int orc_cgsem2d_pc_solve(orc_cgsem2d_pc *pc, double *x, const double *b, int *iterations)
+{
+    const int nS = pc->nS, n1 = nS + 1;
+    const size_t nl = (size_t)4 * pc->nLin;
+    double *u = (double *)calloc(nl, sizeof(double)), *r = (double *)malloc(sizeof(double) * nl);
+    double *v = (double *)calloc(nl, sizeof(double)), *z = (double *)malloc(sizeof(double) * nl);
+    double *w = (double *)malloc(sizeof(double) * nl), *t1 = (double *)malloc(sizeof(double) * nl);
+    double *t2 = (double *)malloc(sizeof(double) * nl);
+    int ioerr = -2, iter, nIt = pc->maxIters;
+    double TOL = pc->tolerance, rNorm = 0.0, a, c, num, den = 0.0, r0;
+    for (size_t q = 0; q < nl; ++q) {  /* MapFromOrigToPC */
+        const int *t = pc->pcToOrig + 3 * q;
+        r[q] = b[t[0] + n1 * (t[1] + n1 * t[2])];
+    }
+    r0 = sqrt(pc_dot(pc, r, r, t1, t2));
+    *iterations = 0;
+    for (iter = 1; iter <= nIt; ++iter) {
+        memcpy(w, r, sizeof(double) * nl);
+        num = pc_dot(pc, r, w, t1, t2);
+        if (iter == 1) memcpy(v, w, sizeof(double) * nl);
+        else { c = num / den; for (size_t q = 0; q < nl; ++q) v[q] = w[q] + c * v[q]; }
+        orc_cgsem2d_matrix_action(&pc->lin, v, z);
+        a = num / pc_dot(pc, v, z, t1, t2);
+        for (size_t q = 0; q < nl; ++q) u[q] = u[q] + a * v[q];
+        for (size_t q = 0; q < nl; ++q) r[q] = r[q] - a * z[q];
+        den = num;
+        rNorm = sqrt(pc_dot(pc, r, r, t1, t2));
+        *iterations = iter;
+        if (rNorm / r0 < TOL) break;
+    }
+    if (rNorm / r0 > TOL) ioerr = -1;
+    if (nIt == 0) memcpy(u, r, sizeof(double) * nl);
+    orc_cgsem2d_unmask(&pc->lin, u);
+    for (int e = 0; e < pc->nEl; ++e)  /* MapFromPCToOrig */
+        for (int iP = 0; iP <= nS; ++iP)
+            for (int iS = 0; iS <= nS; ++iS) {
+                const int *o = pc->origToPC + 3 * (iS + n1 * (iP + n1 * e));
+                x[iS + n1 * (iP + n1 * e)] = u[o[0] + 2 * (o[1] + 2 * o[2])];
+            }
+    free(u); free(r); free(v); free(z); free(w); free(t1); free(t2);
+    return ioerr;
+}
+
+/* Solve_PCConjugateGradient (:1114-1207) with the preconditioner's Solve (:1169) */
+int orc_cgsem2d_solve_pcg_fem(orc_cgsem2d *c, orc_cgsem2d_pc *pc, int *iterations, int *pcIterations)
+{
+    int N = c->N, n1 = N + 1;
+    size_t nl = (size_t)n1 * n1 * c->nElems;
+    double *u = (double *)malloc(sizeof(double) * nl), *r = (double *)malloc(sizeof(double) * nl);
+    double *v = (double *)malloc(sizeof(double) * nl), *z = (double *)malloc(sizeof(double) * nl);
+    double *w = (double *)malloc(sizeof(double) * nl);
+    int ioerr = -2, iter, pcIt = 0;
+    double a, b, num, den = 0.0, r0, rNorm;
+    memcpy(u, c->solution, sizeof(double) * nl);
+    orc_cgsem2d_residual(c, r);
+    r0 = sqrt(orc_cgsem2d_dot(c, r, r));
+    for (int k = 0; k < c->resiLen; ++k) c->resi[k] = 0.0;
+    c->resi[0] = r0;
+    rNorm = r0;
+    *iterations = 0;
+    *pcIterations = 0;
+    if (r0 < c->tolerance) goto done;
+    for (iter = 1; iter <= c->maxIters; ++iter) {
+        orc_cgsem2d_unmask(c, r);
+        orc_cgsem2d_pc_solve(pc, w, r, &pcIt);
+        *pcIterations += pcIt;
+        num = orc_cgsem2d_dot(c, r, w);
+        if (iter == 1) memcpy(v, w, sizeof(double) * nl);
+        else { b = num / den; for (size_t k = 0; k < nl; ++k) v[k] = w[k] + b * v[k]; }
+        orc_cgsem2d_matrix_action(c, v, z);
+        a = num / orc_cgsem2d_dot(c, v, z);
+        for (size_t k = 0; k < nl; ++k) u[k] = u[k] + a * v[k];
+        for (size_t k = 0; k < nl; ++k) r[k] = r[k] - a * z[k];
+        den = num;
+        rNorm = sqrt(orc_cgsem2d_dot(c, r, r));
+        if (iter < c->resiLen) c->resi[iter] = rNorm;
+        *iterations = iter;
+        if (rNorm / r0 < c->tolerance) break;
+    }
+    if (rNorm / r0 > c->tolerance) ioerr = -1;
+    orc_cgsem2d_unmask(c, u);
+    memcpy(c->solution, u, sizeof(double) * nl);
+    orc_cgsem2d_set_dirichlet(c);
+done:
+    free(u); free(r); free(v); free(z); free(w);
+    return ioerr;
+}
+
+size_t orc_cgsem2d_pc_size(void) { return sizeof(orc_cgsem2d_pc); }

@@ -438,6 +438,11 @@ def _declare_extra_cgsem2d(L):
     L.orc_cgsem2d_dot.restype = C.c_double
     L.orc_cgsem2d_solve_pcg.argtypes = [P, C.c_int, _ip]
     L.orc_cgsem2d_solve_pcg.restype = C.c_int
+    L.orc_cgsem2d_pc_size.restype = C.c_size_t
+    L.orc_cgsem2d_pc_build.argtypes = [C.c_void_p, C.c_int, C.c_int, _dp, _dp, _dp, C.c_double, C.c_int]
+    L.orc_cgsem2d_pc_free.argtypes = [C.c_void_p]
+    L.orc_cgsem2d_pc_solve.argtypes = [C.c_void_p, _dp, _dp, _ip]
+    L.orc_cgsem2d_solve_pcg_fem.argtypes = [P, C.c_void_p, _ip, _ip]
 
 
 class CGSEM2D:
@@ -514,6 +519,38 @@ class CGSEM2D:
         it = C.c_int()
         ioerr = lib().orc_cgsem2d_solve_pcg(C.byref(self.s), precond, C.byref(it))
         return ioerr, it.value
+
+    def build_preconditioner(self, x, y, fluxCoeff=None, pcTolerance=1e-4, pcIterates=50):
+        """CGsemPreconditioner_2D on the linear mesh through the node
+        positions x, y (element-local layout); fluxCoeff defaults to ONE, the
+        value the reference maps at Build."""
+        self._pc = C.create_string_buffer(lib().orc_cgsem2d_pc_size())
+        self._pcx = np.ascontiguousarray(x, dtype=np.float64).ravel().copy()
+        self._pcy = np.ascontiguousarray(y, dtype=np.float64).ravel().copy()
+        k = np.ones_like(self._pcx) if fluxCoeff is None else np.ascontiguousarray(fluxCoeff, dtype=np.float64).ravel()
+        self._pck = k.copy()
+        lib().orc_cgsem2d_pc_build(self._pc, self.N, self.nElems, _p(self._pcx), _p(self._pcy), _p(self._pck),
+                                   pcTolerance, pcIterates)
+
+    def precondition(self, b):
+        x = np.zeros_like(self.solution)
+        bb = np.ascontiguousarray(b, dtype=np.float64).ravel().copy()
+        it = C.c_int()
+        io = lib().orc_cgsem2d_pc_solve(self._pc, _p(x), _p(bb), C.byref(it))
+        return x, io, it.value
+
+    def solve_fem(self):
+        """Solve_PCConjugateGradient with the FEM preconditioner (pcIterates > 0)."""
+        it, pit = C.c_int(), C.c_int()
+        ioerr = lib().orc_cgsem2d_solve_pcg_fem(C.byref(self.s), self._pc, C.byref(it), C.byref(pit))
+        return ioerr, it.value, pit.value
+
+    def __del__(self):
+        if getattr(self, "_pc", None) is not None:
+            try:
+                lib().orc_cgsem2d_pc_free(self._pc)
+            except Exception:
+                pass
 
 
 # --------------------------------------------------------------------------- advection-diffusion 2-D

@@ -211,8 +211,10 @@ def random_inputs_cgsem2d(N, nX, nY, seed):
             "solution": rng.uniform(-0.5, 0.5, shp), "u": rng.uniform(-1, 1, shp), "v": rng.uniform(-1, 1, shp)}
 
 
-def run_cgsem2d(N, nX, nY, maxIters, tolerance, dirValue, inputs, timeout=300, mesh_text=None):
-    """The reference's CGsemElliptic_2D with pcIterates = 0 (identity
+def run_cgsem2d(N, nX, nY, maxIters, tolerance, dirValue, inputs, timeout=300, mesh_text=None, pcIterates=0,
+                pcTolerance=1.0e-4):
+    """The reference's CGsemElliptic_2D, by default with pcIterates = 0
+    (identity preconditioner; pcIterates > 0 runs its inner-CG FEM
     preconditioner).  Returns a dict: the inputs; dMat, qWeight; edges
     (nEdges, 6) int32, nodeType (nNodes,) int32; x, y, dxds, dxdp, dyds,
     dydp, J (n1, n1, E); mask_v, unmask_mask_v, gsum_v; Au, u_masked (after
@@ -220,8 +222,8 @@ def run_cgsem2d(N, nX, nY, maxIters, tolerance, dirValue, inputs, timeout=300, m
     (maxIters + 1,), sol_after_solve."""
     n1, E = N + 1, nX * nY
     params = ("&SolverCriteria\n"
-              f" MaximumIterates = {maxIters},\n pcIterates = 0,\n tolerance = {tolerance!r},\n"
-              " pcTolerance = 1.0e-4,\n pcDegree = 1,\n/\n"
+              f" MaximumIterates = {maxIters},\n pcIterates = {int(pcIterates)},\n tolerance = {float(tolerance)!r},\n"
+              f" pcTolerance = {float(pcTolerance)!r},\n pcDegree = 1,\n/\n"
               "&SpaceManagement\n" + (" SpecMeshFile = 'm.mesh',\n" if mesh_text else "") +
               f" polyDeg = {N},\n nXElem = {nX},\n nYElem = {nY},\n nPlot = 4,\n xScale = 1.0,\n/\n")
     raw = _execute("cgsem2d", params, [inputs[k] for k in ("fluxCoeff", "source", "solution", "u", "v")],
@@ -247,7 +249,8 @@ def run_cgsem2d(N, nX, nY, maxIters, tolerance, dirValue, inputs, timeout=300, m
     out["resi"] = take(np.float64, (maxIters + 1,))
     out["sol_after_solve"] = take(np.float64, shp)
     r.done()
-    out.update(N=N, nX=nX, nY=nY, maxIters=maxIters, tolerance=tolerance, dirValue=dirValue)
+    out.update(N=N, nX=nX, nY=nY, maxIters=maxIters, tolerance=tolerance, dirValue=dirValue, pcIterates=pcIterates,
+               pcTolerance=pcTolerance)
     return out
 
 

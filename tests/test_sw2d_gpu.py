@@ -337,3 +337,41 @@ def test_pickup_write_then_read_resumes_the_run(gpu_lib, tmp_path):
         assert np.array_equal(b.GetSolution(), a.GetSolution())
         a.Trash()
         b.Trash()
+
+
+@pytest.mark.parametrize("precision,form", [("fp64", "plain"), ("fp64", "fused")])
+def test_geostrophic_jet_converges_spectrally(gpu_lib, precision, form):
+    """Spectral convergence against an analytic solution: the straight
+    geostrophic jet eta(xi), xi = 2 pi (x/Lx + y/Ly), u = (g/f) z x grad(eta)
+    over a flat bottom is an exact steady solution of the nonlinear
+    conservative equations (u . grad vanishes on every field).  After 20 RK3
+    steps (0.2 inertial periods) the HIP path's departure from it must fall
+    spectrally with N at a fixed 4x4 periodic mesh."""
+    import math
+    from self_amd.shallow_water import ShallowWater, SWParams
+    f0, H0, A, Lx, Ly, g = 1.0, 10.0, 0.1, 2.0, 3.0, 9.81
+    errs = []
+    for N in (3, 5, 7, 9):
+        p = SWParams(polyDeg=N, nXElem=4, nYElem=4, xScale=Lx, yScale=Ly, g=g, f0=f0)
+        m = ShallowWater(p, precision=precision, form=form)
+        x, y = m.mesh.x, m.mesh.y
+        kx, ky = 2 * math.pi / Lx, 2 * math.pi / Ly
+        eta = A * np.sin(kx * x + ky * y)
+        deta = A * np.cos(kx * x + ky * y)
+        H = H0 + eta
+        sol = np.stack([(g / f0) * (-ky * deta) * H, (g / f0) * (kx * deta) * H, H], axis=1)
+        b = np.zeros_like(sol)
+        b[:, 0] = H0
+        m.SetBathymetry(b)
+        m.SetPlanetaryVorticity(np.full_like(x, f0))
+        m.CalculateBathymetryGradient()
+        m.SetSolution(sol)
+        dt = 0.2 * 2 * math.pi / f0 / 20
+        m.SetTimeStep(min(dt, 0.2 * (Lx / 4) / ((N + 1) ** 2 * (np.abs(sol[:, 0] / H).max() + math.sqrt(g * 11)))))
+        steps = int(round(0.2 * 2 * math.pi / f0 / m.params.dt))
+        m.ForwardStepRK3(0.0, steps)
+        out = m.GetSolution()
+        errs.append(max(rel_l2(out[:, k], sol[:, k]) for k in range(2)))
+        m.Trash()
+    assert errs[0] > 10 * errs[1] > 100 * errs[2], errs
+    assert errs[-1] < 1e-5, errs
