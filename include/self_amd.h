@@ -186,6 +186,16 @@ int self_coarse_to_fine_2d_dev(int nS, int nP, size_t nElems, const double *Ts, 
                                int nNewP, const double *fOld, double *fNew, void *stream);
 /* CoarseToFine_3D (src/interp/Lagrange_3D_Class.f90:545-604): (0:N)^3 ->
  * (0:nNew)^3 with T (0:nNew,0:N) in each direction. */
+/* No counterpart in the reference: self_coarse_to_fine_2d for callers that
+ * cannot allocate device memory (code under HIP graph capture, output loops
+ * of a serving process).  Same arguments and results (host arrays, bitwise
+ * the same values); the device staging lives in a 16 MiB array that the
+ * library holds on every device it runs on until the process ends, so it
+ * calls no hipMalloc/hipFree.  Rejects (-5) calls whose matrices, input and
+ * output need more than 2^21 doubles together.  Calls on one device share
+ * the array and are serialized inside the process. */
+int self_coarse_to_fine_2d_noalloc(int nS, int nP, size_t nElems, const double *Ts, const double *Tp, int nNewS,
+                                   int nNewP, const double *fOld, double *fNew);
 int self_coarse_to_fine_3d(int N, size_t nElems, const double *T, int nNew, const double *fOld, double *fNew);
 int self_coarse_to_fine_3d_dev(int N, size_t nElems, const double *T, int nNew, const double *fOld, double *fNew,
                                void *stream);

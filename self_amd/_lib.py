@@ -176,6 +176,7 @@ _SIGS = {
     "self_nodal3d_boundary_interp": ([C.c_int, C.c_size_t] + [dp] * 4, C.c_int),
     "self_nodal3d_boundary_interp_dev": ([C.c_int, C.c_size_t] + [dp] * 4 + [C.c_void_p], C.c_int),
     "self_coarse_to_fine_2d": ([C.c_int, C.c_int, C.c_size_t, dp, dp, C.c_int, C.c_int, dp, dp], C.c_int),
+    "self_coarse_to_fine_2d_noalloc": ([C.c_int, C.c_int, C.c_size_t, dp, dp, C.c_int, C.c_int, dp, dp], C.c_int),
     "self_coarse_to_fine_2d_dev": ([C.c_int, C.c_int, C.c_size_t, dp, dp, C.c_int, C.c_int, dp, dp, C.c_void_p],
                                    C.c_int),
     "self_coarse_to_fine_3d": ([C.c_int, C.c_size_t, dp, C.c_int, dp, dp], C.c_int),

@@ -16,6 +16,10 @@ namespace self {
 void set_error(const char *fmt, ...);
 const char *last_error();
 
+// Resolves the device address of the no-allocation entry points' scratch
+// array on `device` (current); noalloc.hip.
+int noalloc_prepare(int device);
+
 inline int hip_fail(hipError_t e, const char *what, const char *file, int line)
 {
     set_error("%s failed: %s (%s:%d)", what, hipGetErrorString(e), file, line);

@@ -10,7 +10,7 @@ int self_init(int device)
     SELF_HIP(hipGetDeviceCount(&n));
     SELF_REQUIRE(device >= 0 && device < n, SELF_ERR_ARG, "self_init: device %d of %d", device, n);
     SELF_HIP(hipSetDevice(device));
-    return SELF_OK;
+    return self::noalloc_prepare(device);
 }
 
 int self_device_count(int *n)

@@ -95,3 +95,16 @@ def test_errors_are_reported_not_raised_by_the_library():
     assert rc == -5 and b"periodic" in lib.self_last_error()
     with pytest.raises(L.SelfError):
         L.check(lib.self_nodal2d_build(3, 1, 1234, 0, *[L.ptr(np.zeros(16)) for _ in range(8)]))
+
+
+def test_noalloc_coarse_to_fine_rejects_oversized_calls_before_touching_the_device():
+    """self_coarse_to_fine_2d_noalloc checks its 2^21-double limit first, so
+    an oversized call returns -5 with a message and no HIP call (CPU)."""
+    from self_amd import _lib as L
+    Ts = np.zeros(11 * 8)
+    f = np.zeros(64)
+    out = np.zeros(121)
+    nE = (1 << 21) // (64 + 121) + 1
+    rc = L.lib().self_coarse_to_fine_2d_noalloc(7, 7, nE, L.ptr(Ts), L.ptr(Ts), 10, 10, L.ptr(f), L.ptr(out))
+    assert rc == -5 and b"scratch" in L.lib().self_last_error()
+    assert L.lib().self_coarse_to_fine_2d_noalloc(7, 7, 1, None, None, 10, 10, None, None) == -5
