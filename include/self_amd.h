@@ -507,14 +507,39 @@ int self_cgsem2d_get_solution(self_cgsem2d_t h, double *u);
 int self_cgsem2d_matrix_action(self_cgsem2d_t h, const double *u, double *Au);
 /* Residual (:980-1068) of the current solution, masked local layout */
 int self_cgsem2d_residual(self_cgsem2d_t h, double *r);
+/* CGsemPreconditioner_2D % Build (src/cgsem/CGsemPreconditioner_2D_Class.f90
+ * :88-126, called by Build_CGsemElliptic_2D :146-152): the operator at N = 1
+ * on the linear mesh through the Gauss-Lobatto nodes x, y (element-local
+ * layout as the metric terms), inverted by pcIterates iterations of CG to
+ * the relative tolerance pcTolerance (Solve :664-746).  coefficient 0: k = 1,
+ * which is what the reference maps at Build (fluxCoeff is still ONE there);
+ * 1: the flux coefficient last given to self_cgsem2d_set_flux_coefficient.
+ * Selected by precond = 2 in the solves below. */
+int self_cgsem2d_set_preconditioner(self_cgsem2d_t h, const double *x, const double *y, int pcIterates,
+                                    double pcTolerance, int coefficient);
+/* CGsemPreconditioner_2D % Solve (:664-746): x = M^-1 b for a host array in
+ * the element-local layout (owner copies read; every copy written, 0 on
+ * Dirichlet nodes); iterations (nullable) = inner CG iterations. */
+int self_cgsem2d_apply_preconditioner(self_cgsem2d_t h, const double *b, double *x, int *iterations);
 /* Solve (:1114-1207 for SELF_KRYLOV_CGSEM_PCG; the generic methods above
  * applied to the same operator otherwise).  precond 0 = identity (the
- * reference's pcIterates = 0), 1 = Jacobi (assembled diagonal).
+ * reference's pcIterates = 0), 1 = Jacobi (assembled diagonal), 2 = the
+ * preconditioner set above (CGSEM PCG and PCGMRES, the flexible form
+ * Solve_PCGMRES of IterativeSolvers_Class.f90:656-822).
  * elapsed_ms (nullable): device time of the solve. */
 int self_cgsem2d_solve(self_cgsem2d_t h, int method, int precond, double tol, int maxIters, int mInner,
                        int ortho, int *ioerr, int *iterations, double *resi, int resiLen, double *elapsed_ms);
 /* Mixed precision: FP32 PCG inner solves to innerTol, FP64 residual and
  * iterative refinement until |b - A x| <= tol |b - A x0|. */
+/* Restarted FGMRES(mInner) (IterativeSolvers_Class.f90:656-822 control
+ * flow) whose operator and preconditioner applications run in FP32 while
+ * the basis, Gram-Schmidt (ortho 0 modified, 1 classical twice), Givens
+ * rotations, the update and the true residual of every restart stay FP64,
+ * so each cycle refines the FP64 solution; up to maxCycles cycles to
+ * |r|/|r0| <= tol.  precond as above.  No reference counterpart beyond the
+ * loop it follows. */
+int self_cgsem2d_solve_gmres_mixed(self_cgsem2d_t h, int precond, double tol, int maxCycles, int mInner, int ortho,
+                                   int *ioerr, int *iterations, double *relResidual, double *elapsed_ms);
 int self_cgsem2d_solve_mixed(self_cgsem2d_t h, int precond, double tol, double innerTol, int maxInner,
                              int maxOuter, int *ioerr, int *outerIters, int *innerIters, double *relResidual,
                              double *elapsed_ms);

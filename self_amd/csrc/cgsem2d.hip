@@ -27,10 +27,12 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <memory>
 #include <vector>
 
 #include "common.hpp"
 #include "dg2d_device.hpp"
+#include "fempc.hpp"
 #include "krylov.hpp"
 
 namespace self {
@@ -306,6 +308,8 @@ struct CgsemOp final : DevOp {
     float *G3f = nullptr, *Df = nullptr, *dinvf = nullptr, *Lbuf32 = nullptr;
     double *xg = nullptr;  // global solution vector
     size_t bytes = 0;
+    std::vector<double> kH;         // flux coefficient last set (host, local layout; empty = ONE)
+    std::unique_ptr<FemPC> pc;      // CGsemPreconditioner_2D (self_cgsem2d_set_preconditioner)
 
     ~CgsemOp() override
     {
@@ -324,6 +328,15 @@ struct CgsemOp final : DevOp {
     }
 
     bool has_f32() const override { return true; }
+    bool has_pc() const override { return pc != nullptr; }
+    int precondition(const double *r, double *z, const SolverState *st, hipStream_t s) override
+    {
+        return pc ? pc->apply(r, z, st, s) : SELF_ERR_STATE;
+    }
+    int precondition32(const float *r, float *z, const SolverState *st, hipStream_t s) override
+    {
+        return pc ? pc->apply32(r, z, st, s) : SELF_ERR_STATE;
+    }
     const double *inv_diag() const override { return dinv; }
     const float *inv_diag32() const override { return dinvf; }
 
@@ -726,7 +739,79 @@ try {
     SELF_REQUIRE(h && k, SELF_ERR_ARG, "null argument");
     SELF_HIP(hipSetDevice(h->device));
     SELF_HIP(hipMemcpy(h->op.kD, k, sizeof(double) * h->op.nLocal, hipMemcpyHostToDevice));
+    h->op.kH.assign(k, k + h->op.nLocal);
     return device_geometry(h->op);
+}
+SELF_CATCH_ALL
+
+int self_cgsem2d_set_preconditioner(self_cgsem2d_t h, const double *x, const double *y, int pcIterates,
+                                    double pcTolerance, int coefficient)
+try {
+    SELF_REQUIRE(h && x && y, SELF_ERR_ARG, "null argument");
+    SELF_REQUIRE(pcIterates >= 0 && pcTolerance > 0.0, SELF_ERR_ARG, "pcIterates %d, pcTolerance %g", pcIterates,
+                 pcTolerance);
+    SELF_REQUIRE(coefficient == 0 || coefficient == 1, SELF_ERR_ARG, "coefficient selector %d", coefficient);
+    SELF_HIP(hipSetDevice(h->device));
+    CgsemOp &op = h->op;
+    SELF_HIP(hipStreamSynchronize(op.stream));
+    auto pc = std::make_unique<FemPC>();
+    const double *k = (coefficient == 1 && !op.kH.empty()) ? op.kH.data() : nullptr;
+    int rc = pc->build(op.N, op.nE, x, y, k, op.gidH.data(), op.nG, pcIterates, pcTolerance);
+    if (rc) return rc;
+    op.bytes += pc->bytes;
+    op.pc = std::move(pc);
+    return SELF_OK;
+}
+SELF_CATCH_ALL
+
+// CGsemPreconditioner_2D % Solve (:664-746) on a host array in the
+// element-local layout: owner copies in, every copy out (UnMask +
+// MapFromPCToOrig), Dirichlet nodes 0.  Returns the inner iterations.
+int self_cgsem2d_apply_preconditioner(self_cgsem2d_t h, const double *b, double *x, int *iterations)
+try {
+    SELF_REQUIRE(h && b && x, SELF_ERR_ARG, "null argument");
+    CgsemOp &op = h->op;
+    SELF_REQUIRE(op.pc, SELF_ERR_STATE, "no preconditioner: call self_cgsem2d_set_preconditioner first");
+    SELF_HIP(hipSetDevice(h->device));
+    int rc = h->wk.ensure(op.nG, 0, 1, false);
+    if (rc) return rc;
+    std::vector<double> bg(op.nG), xg(op.nG);
+    for (size_t g = 0; g < op.nG; ++g) bg[g] = b[op.ciH[op.csH[g]]];
+    SELF_HIP(hipMemcpyAsync(h->wk.r, bg.data(), sizeof(double) * op.nG, hipMemcpyHostToDevice, op.stream));
+    if ((rc = op.pc->apply(h->wk.r, h->wk.w, nullptr, op.stream))) return rc;
+    SELF_HIP(hipMemcpyAsync(xg.data(), h->wk.w, sizeof(double) * op.nG, hipMemcpyDeviceToHost, op.stream));
+    int its = 0;
+    if ((rc = op.pc->last_iterations(op.stream, its))) return rc;
+    for (size_t l = 0; l < op.nLocal; ++l) x[l] = op.gidH[l] >= 0 ? xg[op.gidH[l]] : 0.0;
+    if (iterations) *iterations = its;
+    return SELF_OK;
+}
+SELF_CATCH_ALL
+
+int self_cgsem2d_solve_gmres_mixed(self_cgsem2d_t h, int precond, double tol, int maxCycles, int mInner, int ortho,
+                                   int *ioerr, int *iterations, double *relResidual, double *elapsed_ms)
+try {
+    SELF_REQUIRE(h && ioerr && iterations, SELF_ERR_ARG, "null argument");
+    SELF_REQUIRE(maxCycles >= 1 && tol > 0 && mInner >= 1 && mInner <= 200, SELF_ERR_ARG,
+                 "maxCycles %d, tol %g, m %d", maxCycles, tol, mInner);
+    SELF_REQUIRE(precond >= 0 && precond <= 2, SELF_ERR_ARG, "precond %d", precond);
+    SELF_HIP(hipSetDevice(h->device));
+    CgsemOp &op = h->op;
+    KrylovResult res;
+    SELF_HIP(hipEventRecord(h->ev0, op.stream));
+    int rc = krylov_solve_gmres_mixed(op, h->wk, precond, tol, maxCycles, mInner, ortho, op.xg, res, op.stream);
+    if (rc) return rc;
+    SELF_HIP(hipEventRecord(h->ev1, op.stream));
+    SELF_HIP(hipEventSynchronize(h->ev1));
+    if (elapsed_ms) {
+        float ms = 0;
+        SELF_HIP(hipEventElapsedTime(&ms, h->ev0, h->ev1));
+        *elapsed_ms = ms;
+    }
+    *ioerr = res.ioerr;
+    *iterations = res.iterations;
+    if (relResidual) *relResidual = res.r0 > 0 ? res.rNorm / res.r0 : 0.0;
+    return SELF_OK;
 }
 SELF_CATCH_ALL
 

@@ -314,6 +314,17 @@ __global__ void k_gmres_cycle_end(SolverState *st, int j, int nIt)
     if (j >= nIt) st->done = 2;
 }
 
+// Mixed form: the cycle's stop test on the FP64 true residual |b - A x|
+// (bnorm, just computed), not on the Givens estimate, which describes the
+// FP32 operator the cycle applied.
+__global__ void k_gmres_cycle_end_true(SolverState *st, int j, int nIt)
+{
+    if (threadIdx.x != 0 || st->done) return;
+    st->rc = st->bnorm;
+    if (st->rc / st->r0 <= st->tol) { st->ioerr = 0; st->done = 1; return; }
+    if (j >= nIt) st->done = 2;
+}
+
 template <typename T>
 __global__ __launch_bounds__(256) void k_jacobi(const T *__restrict__ v, const T *__restrict__ dinv, T *__restrict__ z,
                                                 size_t n, const SolverState *st)
@@ -559,19 +570,67 @@ int run_cg(DevOp &op, KrylovWork &wk, bool precond, int stopRule, double tol, in
     return SELF_OK;
 }
 
-int run_gmres(DevOp &op, KrylovWork &wk, bool precond, double tol, int nIt, int m, int ortho, double *sol,
+// CGSEM PCG (CGsemElliptic_2D_Class.f90:1114-1207) with the operator's own
+// preconditioner (DevOp::precondition, e.g. CGsemPreconditioner_2D % Solve
+// at :1169): w = M^-1 r, num = r.w, v = w + (num/den) v, z = A v with v.z,
+// u += a v, r -= a z, rNorm and the stop test.  Unfused: the
+// preconditioner is a solve of its own between the sweeps.
+int run_cg_pc(DevOp &op, KrylovWork &wk, double tol, int maxIters, double *sol, KrylovResult &res, hipStream_t s)
+{
+    const size_t n = op.n;
+    int rc = init_state(wk, tol, maxIters, kStopNormRel, 0, s);
+    if (rc) return rc;
+    if ((rc = op.residual(sol, wk.r, wk.st, kFinR0, wk.slot, s))) return rc;
+    SELF_HIP(hipMemcpyAsync(wk.u, sol, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
+    const int gv = vgrid(n);
+    constexpr int kChunkPc = 8;  // outer iterations between host reads of the state
+    for (int it0 = 1; it0 <= maxIters; it0 += kChunkPc) {
+        const int it1 = std::min(it0 + kChunkPc - 1, maxIters);
+        for (int it = it0; it <= it1; ++it) {
+            if ((rc = op.precondition(wk.r, wk.w, wk.st, s))) return rc;
+            SELF_RED_LAUNCH((k_dot<double, false>), (k_dot<double, true>), s, (const double *)wk.r,
+                               (const double *)wk.w, n, wk.st, (int)kFinNum, wk.slot);
+            hipLaunchKernelGGL(k_update_v<double>, dim3(gv), dim3(256), 0, s, (const double *)wk.w, wk.v, n,
+                               (const SolverState *)wk.st);
+            if ((rc = op.matvec(wk.v, wk.z, wk.v, wk.st, kFinAlpha, wk.slot, s))) return rc;
+            SELF_RED_LAUNCH((k_update_urw<double, false, false>), (k_update_urw<double, true, false>), s, wk.u,
+                               (const double *)wk.v, wk.r, (const double *)wk.z, (const double *)nullptr, wk.w, n,
+                               wk.st, wk.slot);
+        }
+        SELF_HIP(hipGetLastError());
+        SolverState h{};
+        if ((rc = read_state(wk, h, s))) return rc;
+        if (h.done) break;
+    }
+    SolverState h{};
+    if ((rc = read_state(wk, h, s))) return rc;
+    SELF_HIP(hipMemcpyAsync(sol, wk.u, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
+    res.r0 = h.r0;
+    res.rNorm = h.rNorm;
+    res.iterations = (h.done == 3) ? 0 : h.iter;
+    res.ioerr = -2;
+    if (h.done != 3 && h.rNorm / h.r0 > tol) res.ioerr = -1;  // :1196-1200
+    return SELF_OK;
+}
+
+// pc: kPcNone, kPcJacobi or kPcOperator (the operator's preconditioner);
+// mixed: operator and preconditioner applications in FP32 (the vectors are
+// converted on the way in and out), everything else FP64.
+int run_gmres(DevOp &op, KrylovWork &wk, int pc, bool mixed, double tol, int nIt, int m, int ortho, double *sol,
               KrylovResult &res, hipStream_t s)
 {
     const size_t n = op.n;
     int rc = init_state(wk, tol, nIt, kStopNormRel, m, s);
     if (rc) return rc;
     if ((rc = op.residual(sol, wk.r, wk.st, kFinR0Plain, wk.slot, s))) return rc;
-    const double *dinv = precond ? op.inv_diag() : nullptr;
+    const bool precond = pc != kPcNone;
+    const double *dinv = (pc == kPcJacobi) ? op.inv_diag() : nullptr;
     const int gv = vgrid(n);
     double *hTmp = wk.small + (size_t)(m + 1) * m + (size_t)m * m + 2 * m + 2 * (m + 1);  // scratch for CGS2
     for (int j = 1; j <= nIt; ++j) {
-        SELF_RED_LAUNCH((k_dot<double, false>), (k_dot<double, true>), s, wk.r, wk.r, n, wk.st,
-                           (int)kFinBnorm, wk.slot);
+        if (!mixed || j == 1)  // mixed: |r| of later cycles comes with their stop test
+            SELF_RED_LAUNCH((k_dot<double, false>), (k_dot<double, true>), s, wk.r, wk.r, n, wk.st,
+                               (int)kFinBnorm, wk.slot);
         hipLaunchKernelGGL(k_gmres_cycle_start, dim3(1), dim3(64), 0, s, wk.st);
         hipLaunchKernelGGL(k_scale<double>, dim3(gv), dim3(256), 0, s, wk.r, wk.V, n, wk.st, 0, 0);
         for (int i = 1; i <= m; ++i) {
@@ -580,12 +639,29 @@ int run_gmres(DevOp &op, KrylovWork &wk, bool precond, double tol, int nIt, int 
             const double *xin = vi;
             if (precond) {
                 double *zi = wk.Z + (size_t)(i - 1) * n;
-                hipLaunchKernelGGL(k_jacobi<double>, dim3(gv), dim3(256), 0, s, vi, dinv, zi, n, wk.st);
+                if (pc == kPcOperator && mixed) {
+                    hipLaunchKernelGGL((k_convert<double, float>), dim3(gv), dim3(256), 0, s, (const double *)vi,
+                                       wk.r32, n, 1.0);
+                    if ((rc = op.precondition32(wk.r32, wk.z32, wk.st, s))) return rc;
+                    hipLaunchKernelGGL((k_convert<float, double>), dim3(gv), dim3(256), 0, s, (const float *)wk.z32,
+                                       zi, n, 1.0);
+                } else if (pc == kPcOperator) {
+                    if ((rc = op.precondition(vi, zi, wk.st, s))) return rc;
+                } else {
+                    hipLaunchKernelGGL(k_jacobi<double>, dim3(gv), dim3(256), 0, s, vi, dinv, zi, n, wk.st);
+                }
                 xin = zi;
             }
             // w = A x ; the matvec kernels honour done but not innerExit, so
             // guard: they run and their result is ignored after an exit.
-            if ((rc = op.matvec(xin, wk.w, nullptr, wk.st, kFinNone, wk.slot, s))) return rc;
+            if (mixed) {
+                hipLaunchKernelGGL((k_convert<double, float>), dim3(gv), dim3(256), 0, s, xin, wk.v32, n, 1.0);
+                if ((rc = op.matvec32(wk.v32, wk.w32, nullptr, wk.st, kFinNone, wk.slot, s))) return rc;
+                hipLaunchKernelGGL((k_convert<float, double>), dim3(gv), dim3(256), 0, s, (const float *)wk.w32,
+                                   wk.w, n, 1.0);
+            } else {
+                if ((rc = op.matvec(xin, wk.w, nullptr, wk.st, kFinNone, wk.slot, s))) return rc;
+            }
             if (ortho == 0) {  // modified Gram-Schmidt, as the reference
                 for (int k = 1; k <= i; ++k) {
                     SELF_RED_LAUNCH((k_gmres_h<false>), (k_gmres_h<true>), s,
@@ -613,12 +689,19 @@ int run_gmres(DevOp &op, KrylovWork &wk, bool precond, double tol, int nIt, int 
         hipLaunchKernelGGL(k_gmres_backsub, dim3(1), dim3(64), 0, s, wk.st);
         hipLaunchKernelGGL(k_gmres_update, dim3(gv), dim3(256), 0, s, sol, (const double *)(precond ? wk.Z : wk.V), n,
                            wk.st);
-        hipLaunchKernelGGL(k_gmres_cycle_end, dim3(1), dim3(64), 0, s, wk.st, j, nIt);
+        if (mixed) {  // FP64 true residual, its norm and the stop test on it: one refinement step
+            if ((rc = op.residual(sol, wk.r, wk.st, kFinNone, wk.slot, s))) return rc;
+            SELF_RED_LAUNCH((k_dot<double, false>), (k_dot<double, true>), s, wk.r, wk.r, n, wk.st,
+                               (int)kFinBnorm, wk.slot);
+            hipLaunchKernelGGL(k_gmres_cycle_end_true, dim3(1), dim3(64), 0, s, wk.st, j, nIt);
+        } else {
+            hipLaunchKernelGGL(k_gmres_cycle_end, dim3(1), dim3(64), 0, s, wk.st, j, nIt);
+        }
         SELF_HIP(hipGetLastError());
         SolverState h{};
         if ((rc = read_state(wk, h, s))) return rc;
         if (h.done) break;
-        if ((rc = op.residual(sol, wk.r, wk.st, kFinNone, wk.slot, s))) return rc;
+        if (!mixed && (rc = op.residual(sol, wk.r, wk.st, kFinNone, wk.slot, s))) return rc;
     }
     SolverState h{};
     if ((rc = read_state(wk, h, s))) return rc;
@@ -641,9 +724,20 @@ int krylov_solve(DevOp &op, KrylovWork &wk, int method, int precond, double tol,
     switch (method) {
     case kCG: rc = run_cg(op, wk, false, kStopSqrtNormRel, tol, maxIters, sol, res, s); break;
     case kPCG: rc = run_cg(op, wk, true, kStopSqrtNormRel, tol, maxIters, sol, res, s); break;
-    case kCGSEM_PCG: rc = run_cg(op, wk, precond != 0, kStopNormRel, tol, maxIters, sol, res, s); break;
-    case kGMRES: rc = run_gmres(op, wk, false, tol, maxIters, m, ortho, sol, res, s); break;
-    case kPCGMRES: rc = run_gmres(op, wk, true, tol, maxIters, m, ortho, sol, res, s); break;
+    case kCGSEM_PCG:
+        if (precond == kPcOperator) {
+            if (!op.has_pc()) { set_error("no preconditioner set on this operator"); return SELF_ERR_STATE; }
+            rc = run_cg_pc(op, wk, tol, maxIters, sol, res, s);
+        } else {
+            rc = run_cg(op, wk, precond != 0, kStopNormRel, tol, maxIters, sol, res, s);
+        }
+        break;
+    case kGMRES: rc = run_gmres(op, wk, kPcNone, false, tol, maxIters, m, ortho, sol, res, s); break;
+    case kPCGMRES:
+        if (precond == kPcOperator && !op.has_pc()) { set_error("no preconditioner set on this operator"); return SELF_ERR_STATE; }
+        rc = run_gmres(op, wk, precond == kPcOperator ? kPcOperator : kPcJacobi, false, tol, maxIters, m, ortho, sol,
+                       res, s);
+        break;
     default: set_error("unknown Krylov method %d", method); return SELF_ERR_ARG;
     }
     if (rc) return rc;
@@ -653,6 +747,16 @@ int krylov_solve(DevOp &op, KrylovWork &wk, int method, int precond, double tol,
         SELF_HIP(hipStreamSynchronize(s));
     }
     return SELF_OK;
+}
+
+int krylov_solve_gmres_mixed(DevOp &op, KrylovWork &wk, int precond, double tol, int maxCycles, int m, int ortho,
+                             double *sol, KrylovResult &res, hipStream_t s)
+{
+    if (!op.has_f32()) { set_error("operator has no FP32 form"); return SELF_ERR_STATE; }
+    if (precond == kPcOperator && !op.has_pc()) { set_error("no preconditioner set on this operator"); return SELF_ERR_STATE; }
+    int rc = wk.ensure(op.n, m, maxCycles * m + 1, true);
+    if (rc) return rc;
+    return run_gmres(op, wk, precond, true, tol, maxCycles, m, ortho, sol, res, s);
 }
 
 // FP32 PCG inner solve of A e = rhs (e0 = 0) to innerTol relative, then

@@ -191,6 +191,19 @@ struct DevOp {
     // inverse diagonal for Jacobi preconditioning (nullptr: identity)
     virtual const double *inv_diag() const { return nullptr; }
     virtual const float *inv_diag32() const { return nullptr; }
+    // A general preconditioner z = M^-1 r (the reference's PreConditionInvert,
+    // IterativeSolvers_Class.f90:168-186, or CGsemPreconditioner_2D % Solve),
+    // in FP64 or FP32 arithmetic; st is the calling solver's state (work
+    // after its done or innerExit flag is skipped).
+    virtual bool has_pc() const { return false; }
+    virtual int precondition(const double *r, double *z, const SolverState *st, hipStream_t s)
+    {
+        return SELF_ERR_STATE;
+    }
+    virtual int precondition32(const float *r, float *z, const SolverState *st, hipStream_t s)
+    {
+        return SELF_ERR_STATE;
+    }
 };
 
 // Device workspace of a solver instance.
@@ -214,6 +227,9 @@ struct KrylovWork {
 
 enum KrylovMethod { kCG = 0, kPCG = 1, kGMRES = 2, kPCGMRES = 3, kCGSEM_PCG = 4 };
 
+// preconditioner selectors of the solvers
+enum KrylovPrecond { kPcNone = 0, kPcJacobi = 1, kPcOperator = 2 };
+
 struct KrylovResult {
     int ioerr = -2;
     int iterations = 0;
@@ -224,6 +240,13 @@ struct KrylovResult {
 // guess in, solution out); resi (host, resiLen) receives the history.
 int krylov_solve(DevOp &op, KrylovWork &wk, int method, int precond, double tol, int maxIters, int m,
                  int ortho, double *sol, double *resiHost, int resiLen, KrylovResult &res, hipStream_t s);
+
+// Restarted FGMRES(m) whose operator and preconditioner applications run
+// in FP32 while the Krylov basis, Gram-Schmidt, Givens rotations, the
+// solution update and the true residual at every restart stay FP64: each
+// cycle is one step of FP64 iterative refinement with an FP32 inner solve.
+int krylov_solve_gmres_mixed(DevOp &op, KrylovWork &wk, int precond, double tol, int maxCycles, int m, int ortho,
+                             double *sol, KrylovResult &res, hipStream_t s);
 
 // Mixed precision: FP32 PCG inner solves, FP64 residual and iterative
 // refinement to |b - A x| / |b - A x0| <= tol.

@@ -104,6 +104,9 @@ class CGsemParams:
     warp: float = 0.0
 
 
+_PRECOND = {"none": 0, "jacobi": 1, "fem": 2}
+
+
 class CGsemElliptic2D:
     """div(k grad u) = s with Dirichlet boundaries, GLL nodes (CG form).
 
@@ -194,10 +197,38 @@ class CGsemElliptic2D:
         rl = mi * (mInner if m in (GMRES, PCGMRES) else 1) + 1
         self.resi = np.zeros(rl)
         io, it, ms = C.c_int(), C.c_int(), C.c_double()
-        pc = {"none": 0, "jacobi": 1}[precond]
+        pc = _PRECOND[precond]
         L.check(L.lib().self_cgsem2d_solve(self._h, m, pc, tol, mi, mInner, ortho, C.byref(io), C.byref(it),
                                            L.ptr(self.resi), rl, C.byref(ms)))
         return io.value, it.value, ms.value
+
+    def SetPreconditioner(self, pcIterates=50, pcTolerance=1.0e-4, coefficient="one"):
+        """CGsemPreconditioner_2D % Build (CGsemPreconditioner_2D_Class.f90:88-126):
+        the operator at N = 1 on the linear mesh through the Gauss-Lobatto
+        nodes, inverted by pcIterates CG iterations to pcTolerance.
+        coefficient "one" is what the reference maps at Build (fluxCoeff =
+        ONE there); "flux" uses the coefficient set by SetFluxCoefficient.
+        Select it with precond="fem" in Solve and SolveGMRESMixed."""
+        x = L.f64(self.mesh.x)
+        y = L.f64(self.mesh.y)
+        L.check(L.lib().self_cgsem2d_set_preconditioner(self._h, L.ptr(x), L.ptr(y), int(pcIterates),
+                                                        float(pcTolerance), {"one": 0, "flux": 1}[coefficient]))
+
+    def ApplyPreconditioner(self, b):
+        """CGsemPreconditioner_2D % Solve: returns (x, inner iterations)."""
+        out = self._new()
+        it = C.c_int()
+        L.check(L.lib().self_cgsem2d_apply_preconditioner(self._h, L.ptr(self._arr(b)), L.ptr(out), C.byref(it)))
+        return out, it.value
+
+    def SolveGMRESMixed(self, tolerance=1e-10, maxCycles=50, mInner=30, precond="fem", ortho=MGS):
+        """Restarted FGMRES(mInner) with FP32 operator and preconditioner
+        applications and FP64 basis, orthogonalisation and restart
+        residuals: (ioerr, iterations, relResidual, ms)."""
+        io, it, rel, ms = C.c_int(), C.c_int(), C.c_double(), C.c_double()
+        L.check(L.lib().self_cgsem2d_solve_gmres_mixed(self._h, _PRECOND[precond], tolerance, maxCycles, mInner,
+                                                       ortho, C.byref(io), C.byref(it), C.byref(rel), C.byref(ms)))
+        return io.value, it.value, rel.value, ms.value
 
     def SolveMixed(self, tolerance=1e-10, innerTolerance=1e-5, maxInner=20000, maxOuter=20, precond="jacobi"):
         """FP32 inner PCG + FP64 iterative refinement: (ioerr, outer, inner, relResidual, ms)."""
