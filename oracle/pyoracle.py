@@ -79,7 +79,7 @@ def _declare(L):
         getattr(L, name).restype = C.c_int
     L.orc_elliptic1d_solve.argtypes = [C.c_int, C.c_int, C.c_int, C.c_double, _dp, _dp, _dp, _ip]
     L.orc_elliptic1d_solve.restype = C.c_int
-    for extra in ("_declare_extra_adv2d", "_declare_extra_adv3d", "_declare_extra_cgsem2d"):
+    for extra in ("_declare_extra_adv2d", "_declare_extra_adv3d", "_declare_extra_cgsem2d", "_declare_extra_hpe"):
         fn = globals().get(extra)
         if fn is not None:
             fn(L)
@@ -743,3 +743,62 @@ class Advection3D:
 
     def forward_step_rk3(self, tn=0.0):
         lib().orc_adv3d_forward_step_rk3(C.byref(self.s), tn)
+
+
+class HPEStruct(C.Structure):
+    _fields_ = [(n, C.c_int) for n in ("N", "M", "nHElems", "nLayers", "nElems", "nFaces", "diagnoseP")] + [
+        ("dt", C.c_double)] + [(n, _dp) for n in ("nodes", "bWeights", "dMat", "qWeight", "lagLeft", "lagRight",
+                                                   "intNodes", "intWeights")] + [("faces", _ip)] + [
+        (n, _dp) for n in ("cov", "J", "nHat", "nLen", "bpos", "sol", "bg", "bsol", "bflux", "tend", "G", "divU",
+                           "bottomW")]
+
+
+def _declare_extra_hpe(L):
+    P = C.POINTER(HPEStruct)
+    for n in ("traces", "divu", "diagnose_p", "diagnose_w", "edge_flux", "mapped", "global_time_derivative",
+              "forward_step_rk3", "forward_step_ab2"):
+        getattr(L, "orc_hpe_" + n).argtypes = [P]
+
+
+class HydrostaticPrimitive:
+    """The hydrostatic primitive equations (hydrostaticprimitives/
+    HydrostaticPrimitivesClass.f90) on the oracle, on a stacked structured hex
+    mesh of nX x nY columns and nLayers layers (orc_hpe.h).
+
+    sol, bg: (nElems, 5, N+1, N+1, N+1) i fastest, variables u v b w p;
+    tend, G: (nElems, 3, ...); divU (nElems, ...); bottomW (nHElems, N+1, N+1)."""
+
+    def __init__(self, N, nX, nY, nLayers, periodic=(False, False), scale=(1.0, 1.0, 1.0), warp=0.0,
+                 warpSign=(1, 1, 1), dt=1e-3, diagnoseP=True):
+        nodal = Nodal3D(N)
+        mesh = HexMesh(nX, nY, nLayers, N, nodal.nodes, periodic=(periodic[0], periodic[1], False), scale=scale,
+                       warp=warp, warpSign=warpSign)
+        n1, E = N + 1, mesh.nElems
+        self.N, self.nElems, self.nHElems, self.nLayers = N, E, nX * nY, nLayers
+        self.M = (N + 1) // 2 + 1
+        self.intNodes, self.intWeights = quadrature(self.M, GAUSS_LOBATTO)
+        self.bWeights = barycentric_weights(nodal.nodes)
+        self.dm = np.ascontiguousarray(nodal.dMat.ravel(order="F"))
+        self.faces = np.ascontiguousarray(mesh.faces.ravel().astype(np.int32))
+        self.sol = np.zeros(5 * n1 ** 3 * E)
+        self.bg = np.zeros(5 * n1 ** 3 * E)
+        self.bsol = np.zeros(5 * n1 * n1 * 6 * E)
+        self.bflux = np.zeros(3 * n1 * n1 * 6 * E)
+        self.tend = np.zeros(3 * n1 ** 3 * E)
+        self.G = np.zeros(3 * n1 ** 3 * E)
+        self.divU = np.zeros(n1 ** 3 * E)
+        self.bottomW = np.zeros(n1 * n1 * self.nHElems)
+        self.mesh, self.nodal = mesh, nodal
+        self.s = HPEStruct(N, self.M, self.nHElems, nLayers, E, mesh.nFaces, int(diagnoseP), dt, _p(nodal.nodes),
+                           _p(self.bWeights), _p(self.dm), _p(nodal.qWeight), _p(nodal.lagLeft),
+                           _p(nodal.lagRight), _p(self.intNodes), _p(self.intWeights), _pi(self.faces),
+                           _p(mesh.cov), _p(mesh.J), _p(mesh.nHat), _p(mesh.nLen), _p(mesh.bpos), _p(self.sol),
+                           _p(self.bg), _p(self.bsol), _p(self.bflux), _p(self.tend), _p(self.G), _p(self.divU),
+                           _p(self.bottomW))
+
+    def view(self, a, nv):
+        n1 = self.N + 1
+        return a.reshape(self.nElems, nv, n1, n1, n1)
+
+    def call(self, name):
+        getattr(lib(), "orc_hpe_" + name)(C.byref(self.s))
