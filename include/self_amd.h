@@ -480,6 +480,50 @@ int self_adv3d_forward_step_rk3(self_adv3d_t h, double tn, int nSteps, double *e
 int self_adv3d_filter_count(self_adv3d_t h, int reset, unsigned *count);
 int self_adv3d_synchronize(self_adv3d_t h);
 
+/* --------------------------------- hydrostatic primitive equations (DGSEM)
+ * Replaces the HydrostaticPrimitive type of src/highend/hydrostaticprimitives/
+ * HydrostaticPrimitivesClass.f90 (:80-210): u, v, b prognostic, w and p
+ * diagnosed (variables 1..5 as :205-209), on a stacked hex mesh whose
+ * element e = hE + nHElems * layer (StackedHexMeshClass.f90:337); bottom
+ * boundary faces are SEA_FLOOR, other boundary faces NO_NORMAL_FLOW
+ * (StackedHexMeshClass.f90:617-621).  The reference class does not compile;
+ * the time derivative follows GlobalTimeDerivative (:1428-1506) with divU
+ * from CalculateDivU (HydrostaticPrimitivesClass_implicit.f90:2520-2691)
+ * and DiagnoseP inside it when diagnoseP is set.  FP64. */
+typedef struct self_hpe_s *self_hpe_t;
+
+/* Build (:219-317) on a mesh of self_hexmesh_structured's form with nZ =
+ * nLayers (no periodicity in z); nodal arrays from self_nodal3d_build (Gauss,
+ * DG); N 1..7.  The column quadrature degree is (N+1)/2 + 1 (:264). */
+int self_hpe_create(int N, int nHElems, int nLayers, int nFaces, const int *faces, const double *cov, const double *J,
+                    const double *nHat, const double *nLen, const double *bpos, const double *nodes,
+                    const double *qWeight, const double *dMat, const double *lagLeft, const double *lagRight,
+                    int diagnoseP, double dt, self_hpe_t *out);
+/* Trash (:321-355) */
+int self_hpe_destroy(self_hpe_t h);
+/* field 0: SetSolution (:420-433) (0:N,0:N,0:N,1:5) per element;
+ * 1: SetBackgroundState (:601-617), same shape (its buoyancy and vertical
+ * velocity enter the fluxes); 2: SetPreviousTendency (:1090-1107)
+ * (0:N,0:N,0:N,1:3), the AB2 history. */
+int self_hpe_set_field(self_hpe_t h, int field, const double *data);
+/* field 0: solution (5 variables), 1: GetTendency (:1111-1128) (3),
+ * 2: divergence of the lateral velocity of the last time derivative (1),
+ * 3: face values of b, w, p of the last time derivative, [e][side][var][node]. */
+int self_hpe_get_field(self_hpe_t h, int field, double *data);
+/* GlobalTimeDerivative (:1428-1506): diagnoses p (when enabled) and w into
+ * the solution and computes the tendency. */
+int self_hpe_global_time_derivative(self_hpe_t h);
+/* method 0: ForwardStepRK3 (HydrostaticPrimitivesClass_implicit.f90:
+ * 1422-1471) without the pressure correction; 1: ForwardStepAB2
+ * (:1378-1424) without the barotropic step.  nSteps steps; elapsed_ms
+ * nullable. */
+int self_hpe_forward_step(self_hpe_t h, int method, int nSteps, double *elapsed_ms);
+/* Average duration in ms of the column kernel (avg_ms[0]) and the stage
+ * kernel (avg_ms[1]) over nSteps RK3 steps (measurement support, no
+ * reference counterpart). */
+int self_hpe_kernel_times(self_hpe_t h, int nSteps, double *avg_ms);
+int self_hpe_synchronize(self_hpe_t h);
+
 /* ---------------------------------------- CG spectral-element elliptic 2-D
  * Replaces CGsemElliptic_2D (src/cgsem/CGsemElliptic_2D_Class.f90:57-93):
  * div(k grad u) = s on GLL nodes of degree N, Dirichlet on every boundary

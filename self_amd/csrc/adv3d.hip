@@ -34,6 +34,7 @@
 
 #include "common.hpp"
 #include "dg2d_device.hpp"
+#include "dg3d_device.hpp"
 
 namespace self {
 void cofactor3(const double *m, int dir, double out[3]);
@@ -77,47 +78,11 @@ __device__ __forceinline__ void stage_nodal3(Nodal3<N> &L, const double *__restr
     }
 }
 
-// side descriptor (int2): x = neighbour element (>= 0) or boundary flag (< 0);
-// y: bits 0-2 neighbour side, 3 primary, 4 swap, 5 i reversed, 6 j reversed,
-//    bits 8.. boundary slot (index into the boundary position array)
-__device__ __forceinline__ int s3_nbr_side(int y) { return y & 7; }
-__device__ __forceinline__ bool s3_primary(int y) { return (y >> 3) & 1; }
-__device__ __forceinline__ int s3_slot(int y) { return y >> 8; }
-
-// node of the neighbour's face matching own face node (a, b): forward map
-// (i,j) -> (k,l) of the reference's face loop when this side is primary,
-// its inverse otherwise
-template <int N>
-__device__ __forceinline__ void s3_map(int y, int a, int b, int &na, int &nb)
-{
-    const int sw = (y >> 4) & 1, fi = (y >> 5) & 1, fj = (y >> 6) & 1;
-    if (s3_primary(y)) {
-        const int ii = fi ? N - a : a, jj = fj ? N - b : b;
-        na = sw ? (fj ? N - b : b) : ii;
-        nb = sw ? (fi ? N - a : a) : jj;
-    } else if (!sw) {
-        na = fi ? N - a : a;
-        nb = fj ? N - b : b;
-    } else {
-        na = fi ? N - b : b;
-        nb = fj ? N - a : a;
-    }
-}
-
 // trace of one element's nodal field q (LDS, i fastest) at side s, face node (a, b)
 template <int N>
 __device__ __forceinline__ double trace3(const Nodal3<N> &L, const double *q, int s, int a, int b)
 {
-    constexpr int n1 = N + 1;
-    const double *l = (s == 1 || s == 2 || s == 5) ? L.lR : L.lL;
-    int off, stride;
-    if (s == 1 || s == 3) { off = n1 * (a + n1 * b); stride = 1; }          // east / west: (p, q)
-    else if (s == 0 || s == 2) { off = a + n1 * n1 * b; stride = n1; }      // south / north: (s, q)
-    else { off = a + n1 * b; stride = n1 * n1; }                            // bottom / top: (s, p)
-    double acc = l[0] * q[off];
-    #pragma unroll
-    for (int m = 1; m < n1; ++m) acc = __builtin_fma(l[m], q[off + m * stride], acc);
-    return acc;
+    return face_trace3<N>(L.lL, L.lR, q, s, a, b);
 }
 
 struct Adv3Args {
