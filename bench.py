@@ -242,6 +242,49 @@ def bench_adv3d(steps=20, warm=2, nX=32, nY=32, nZ=16, N=7):
             "filtered_element_stages": filtered}
 
 
+def hpe_bytes_per_element_stage(N):
+    """Algorithmic bytes of one stage per element (FP64): the column kernel
+    reads u, v, b, six cofactor terms and J at every node, the column
+    height, and the four normal values on six faces, and writes p, w and
+    the face values of b, w, p; the stage kernel reads p, w, u, v, b, the
+    background buoyancy, nine cofactor terms and J, its own and the
+    neighbour's face values, the face data (normal, |n|, background) and
+    the RK register, and writes u, v, b and the register."""
+    n1 = N + 1
+    np_, n2 = n1 ** 3, n1 * n1
+    col = 12 * np_ + 43 * n2
+    stage = 25 * np_ + 72 * n2
+    return 8 * col, 8 * stage
+
+
+def bench_hpe(steps=10, warm=2, nX=32, nY=32, nL=16, N=7):
+    """Hydrostatic primitive equations, 32x32 columns x 16 layers at N=7
+    (16384 elements, as the 3-D advection workload), walled box of depth
+    0.1 with a buoyancy anomaly at rest, FP64, RK3.  Two kernels per stage;
+    each kernel's time comes from events around its launches."""
+    import numpy as np
+    from self_amd.workloads import hpe_buoyancy_anomaly
+    m, info = hpe_buoyancy_anomaly(nX, nY, nL, N)
+    m.ForwardStepRK3(warm)
+    ms = m.ForwardStepRK3Timed(steps)
+    col_ms, st_ms = m.KernelTimes(3)
+    ok = bool(np.all(np.isfinite(m.GetSolution())))
+    E = m.nElems
+    dof = E * (N + 1) ** 3
+    bcol, bst = hpe_bytes_per_element_stage(N)
+    m.Trash()
+    stage_ms = ms / steps / 3.0
+    return {"workload": f"hydrostatic primitive equations {nX}x{nY} columns x {nL} layers N={N}, walled, "
+                        f"buoyancy anomaly at rest, dt={info['dt']}", "steps": steps,
+            "ms_per_step": ms / steps, "stage_ms": stage_ms, "dof_stage_per_s": dof * 3 * steps / (ms * 1e-3),
+            "kernel_ms": {"k_hpe_columns": col_ms, "k_hpe_stage": st_ms},
+            "algorithmic_bytes_per_stage": {"k_hpe_columns": bcol * E, "k_hpe_stage": bst * E},
+            "achieved_GBs": {"k_hpe_columns": round(bcol * E / (col_ms * 1e-3) / 1e9, 1),
+                             "k_hpe_stage": round(bst * E / (st_ms * 1e-3) / 1e9, 1),
+                             "stage": round((bcol + bst) * E / (stage_ms * 1e-3) / 1e9, 1)},
+            "finite": ok}
+
+
 def bench_sw2d_ensemble(members=8, nX=64, N=7, steps=100, warm=10):
     """Ensemble of `members` states of the 64x64, N=7 geostrophic case (each
     with its own 0.1 m depth offset) advanced together: one fused launch per
@@ -340,8 +383,9 @@ def bench_cgsem(nX=128, N=7, tol=1e-10):
     """CGSEM 2-D variable-coefficient Poisson (128x128, N=7 GLL, Dirichlet):
     Jacobi-PCG in FP64 (the reference's CGSEM PCG loop), FP32-inner PCG with
     FP64 iterative refinement (restarted inner solves, and one FP32 sequence
-    with FP64 reliable updates), and restarted GMRES(30) with Jacobi
-    (flexible) preconditioning in FP64."""
+    with FP64 reliable updates), and the reference's FEM preconditioner
+    (inner CG on the linear sub-mesh through the Gauss-Lobatto nodes) in the
+    PCG loop and as the right preconditioner of FGMRES(30), FP64 and mixed."""
     import numpy as np
     from self_amd.workloads import cgsem_helmholtz
     m, info = cgsem_helmholtz(nX, N, tolerance=tol)
@@ -368,12 +412,25 @@ def bench_cgsem(nX=128, N=7, tol=1e-10):
                                              "true_rel_residual": float(np.linalg.norm(m.Residual())) / r0,
                                              "speedup_vs_pcg_fp64": out["pcg_fp64"]["ms"] / ms}
     m.SetSolution(zero)
-    # restarted GMRES(30) stagnates on this SPD problem (about 1e-3 after
-    # 400 cycles), so it is timed over a fixed 20 cycles for iterations/s
-    io, it, ms = m.Solve("pcgmres", precond="jacobi", tolerance=tol, maxIters=20, mInner=30)
-    out["fgmres30_fp64"] = {"ioerr": io, "iterations": it, "cycles": 20, "ms": ms,
-                            "iters_per_s": it / (ms * 1e-3) if ms else None,
-                            "true_rel_residual": float(np.linalg.norm(m.Residual())) / r0}
+    # the reference's own preconditioner (inner CG on the linear sub-mesh)
+    # in its PCG loop, and as the right preconditioner of FGMRES(30) in FP64
+    # and in the mixed form (FP32 operator and preconditioner, FP64 basis
+    # and restart residuals)
+    t0 = time.perf_counter()
+    m.SetPreconditioner(pcIterates=100, pcTolerance=1e-4, coefficient="flux")
+    out["fem_preconditioner_build_s"] = time.perf_counter() - t0
+    io, it, ms = m.Solve("cgsem_pcg", precond="fem", tolerance=tol, maxIters=2000)
+    out["pcg_fem_fp64"] = {"ioerr": io, "iterations": it, "ms": ms, "pcIterates": 100, "pcTolerance": 1e-4,
+                           "true_rel_residual": float(np.linalg.norm(m.Residual())) / r0}
+    m.SetSolution(zero)
+    io, it, ms = m.Solve("pcgmres", precond="fem", tolerance=tol, maxIters=200, mInner=30)
+    out["fgmres30_fem_fp64"] = {"ioerr": io, "iterations": it, "ms": ms, "pcIterates": 100,
+                                "true_rel_residual": float(np.linalg.norm(m.Residual())) / r0}
+    m.SetSolution(zero)
+    io, it, relg, ms = m.SolveGMRESMixed(tolerance=tol, maxCycles=200, mInner=30, precond="fem")
+    out["fgmres30_fem_mixed"] = {"ioerr": io, "iterations": it, "ms": ms, "pcIterates": 100, "rel_residual": relg,
+                                 "true_rel_residual": float(np.linalg.norm(m.Residual())) / r0,
+                                 "speedup_vs_fgmres30_fem_fp64": out["fgmres30_fem_fp64"]["ms"] / ms}
     m.Trash()
     return out
 
@@ -532,7 +589,8 @@ def main():
     progress("accuracy gates done")
     if not args.no_extras:
         for key, fn in (("sw2d_ensemble", bench_sw2d_ensemble), ("sw2d_dipole_N10", bench_sw2d_dipole),
-                        ("adv2d", bench_adv2d), ("adv3d", bench_adv3d), ("cgsem", bench_cgsem)):
+                        ("adv2d", bench_adv2d), ("adv3d", bench_adv3d), ("hpe", bench_hpe),
+                        ("cgsem", bench_cgsem)):
             out[key] = fn()
             progress(f"{key} done")
     if not args.no_serve:

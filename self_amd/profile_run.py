@@ -7,6 +7,7 @@
     python -m self_amd.profile_run adv3d --steps 5
     python -m self_amd.profile_run cgsem --nx 128 --precond 1
     python -m self_amd.profile_run cgsem --precision mixed --steps 2000
+    python -m self_amd.profile_run hpe --steps 5
 
 `sw2d-dipole` runs the shallow-water model on an ISM-V2 mesh file (the
 dipole run's box.mesh, scaled by 4e5) with the dipole state.
@@ -28,7 +29,7 @@ from . import _lib as L
 
 def main(argv=None):
     ap = argparse.ArgumentParser(prog="python -m self_amd.profile_run")
-    ap.add_argument("mode", choices=["copy", "sw2d", "sw2d-dipole", "adv2d", "adv3d", "cgsem"])
+    ap.add_argument("mode", choices=["copy", "sw2d", "sw2d-dipole", "adv2d", "adv3d", "cgsem", "hpe"])
     ap.add_argument("--mesh", default=None, help="ISM-V2 mesh file (sw2d-dipole)")
     ap.add_argument("--precision", default="mixed", choices=["fp64", "mixed"])
     ap.add_argument("--form", default="fused", choices=["fused", "plain"])
@@ -57,6 +58,12 @@ def main(argv=None):
         m.ForwardStepRK3(0.0, a.steps)
         m.Synchronize()
         print(f"sw2d-dipole N={a.N} {a.precision} {a.form}: {a.steps} RK3 steps", flush=True)
+    elif a.mode == "hpe":
+        from .workloads import hpe_buoyancy_anomaly
+        m, _ = hpe_buoyancy_anomaly(a.nx or 32, a.nx or 32, 16, a.N)
+        m.ForwardStepRK3(a.steps)
+        m.Synchronize()
+        print(f"hpe {a.nx or 32}x{a.nx or 32}x16 N={a.N}: {a.steps} RK3 steps", flush=True)
     elif a.mode == "adv2d":
         from .workloads import adv2d_blob
         m, _ = adv2d_blob(nX=a.nx or 10, N=a.N)

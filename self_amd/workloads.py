@@ -264,3 +264,29 @@ def adv3d_gaussians(nX=32, nY=32, nZ=16, N=7, warp=0.05, u=(1.0, 0.7, 0.3), nTra
     model.SetSolution(c)
     model.SetVelocity(np.full_like(m.x, u[0]), np.full_like(m.x, u[1]), np.full_like(m.x, u[2]))
     return model, {"c": c, "dt": dt, "u": u}
+
+
+def hpe_buoyancy_anomaly_fields(x, y, z, H, amp=0.01, width=0.1):
+    """Gaussian buoyancy anomaly of the first vertical mode:
+    b = amp exp(-((x-0.5)^2 + (y-0.5)^2) / (2 width^2)) cos(pi z / H), z in [-H, 0]."""
+    return amp * np.exp(-((x - 0.5) ** 2 + (y - 0.5) ** 2) / (2.0 * width * width)) * np.cos(np.pi * z / H)
+
+
+def hpe_buoyancy_anomaly(nX=32, nY=32, nLayers=16, N=7, H=0.1, B=1.0, dt=2.0e-3, diagnoseP=True):
+    """Hydrostatic primitive equations in the walled unit square of depth H
+    (flat sea floor, NO_NORMAL_FLOW walls and lid): fluid at rest with a
+    Gaussian buoyancy anomaly of the first vertical mode on a constant
+    background buoyancy B.  The fastest wave speed is about sqrt(B H / pi),
+    0.18 for the defaults, so dt = 2e-3 is well inside the RK3 limit at
+    N = 7.  Returns (model, info)."""
+    from .hydrostatic import HydrostaticParams, HydrostaticPrimitive
+    p = HydrostaticParams(polyDeg=N, nXElem=nX, nYElem=nY, nZElem=nLayers, dt=dt, zScale=H, diagnoseP=diagnoseP)
+    model = HydrostaticPrimitive(p)
+    m = model.mesh
+    sol = np.zeros((m.nElems, 5, N + 1, N + 1, N + 1))
+    sol[:, 2] = hpe_buoyancy_anomaly_fields(m.x, m.y, m.z - H, H)
+    bg = np.zeros_like(sol)
+    bg[:, 2] = B
+    model.SetSolution(sol)
+    model.SetBackgroundState(bg)
+    return model, {"sol": sol, "dt": dt, "H": H, "B": B}
