@@ -218,28 +218,44 @@ def bench_adv2d(steps=100, warm=10):
             "dof_stage_per_s": dof * 3 * steps / (ms * 1e-3)}
 
 
-def bench_adv3d(steps=20, warm=2, nX=32, nY=32, nZ=16, N=7):
+def bench_adv3d(steps=20, warm=2, nX=32, nY=32, nZ=16, N=7, acc_steps=200):
     """3-D linear advection (32x32x16 periodic, N=7, seeded warp 0.05,
     u=(1,0.7,0.3), three seeded Gaussians, adaptive roll-off filter on,
-    RK3 at CFL 0.5), FP64.  Bytes per DOF-stage: Np (2 s_c + 2 s_G + 3 s_u +
-    10 s_m) + 6 (N+1)^2 4 s_n per element = 160 B at N=7."""
+    RK3 at CFL 0.5), FP64, and the matrix-core variants of the volume
+    contractions (FP32, split BF16) with their relative L2 against FP64
+    after acc_steps steps.  Bytes per DOF-stage: 2 s_c + 2 s_G + 4 s_m (the
+    static velocity folded into the three cofactor rows, and J) plus
+    6 (N+1)^2 4 s_n per element = 64 B + 12 B at N=7."""
+    import numpy as np
     from self_amd.workloads import adv3d_gaussians
-    m, info = adv3d_gaussians(nX, nY, nZ, N)
-    m.ForwardStepRK3(0.0, warm)
-    m.FilterCount(reset=True)
-    ms = m.ForwardStepRK3Timed(warm * info["dt"], steps)
-    filtered = m.FilterCount(reset=True)
-    E, n1 = m.nElems, N + 1
-    dof = E * n1 ** 3
-    bytes_stage = E * (n1 ** 3 * (2 * 8 + 2 * 8 + 3 * 8 + 10 * 8) + 6 * n1 * n1 * 4 * 8)
-    stage_ms = ms / steps / 3.0
-    m.Trash()
-    return {"workload": f"3D advection {nX}x{nY}x{nZ} N={N} periodic, warp 0.05, filter on (cutoff 3)",
-            "steps": steps, "dt": info["dt"], "ms_per_step": ms / steps, "stage_ms": stage_ms,
-            "dof_stage_per_s": dof * 3 * steps / (ms * 1e-3),
-            "algorithmic_bytes_per_stage": bytes_stage,
-            "achieved_GBs": round(bytes_stage / (stage_ms * 1e-3) / 1e9, 1),
-            "filtered_element_stages": filtered}
+    out = {}
+    finals = {}
+    for prec in ("fp64", "fp32", "bf16x3"):
+        m, info = adv3d_gaussians(nX, nY, nZ, N)
+        m.SetPrecision(prec)
+        m.ForwardStepRK3(0.0, warm)
+        m.FilterCount(reset=True)
+        ms = m.ForwardStepRK3Timed(warm * info["dt"], steps)
+        filtered = m.FilterCount(reset=True)
+        E, n1 = m.nElems, N + 1
+        dof = E * n1 ** 3
+        bytes_stage = E * (n1 ** 3 * (2 * 8 + 2 * 8 + 4 * 8) + 6 * n1 * n1 * 4 * 8)
+        stage_ms = ms / steps / 3.0
+        m.SetSolution(info["c"])
+        m.ForwardStepRK3(0.0, acc_steps)
+        finals[prec] = m.GetSolution()
+        m.Trash()
+        out[prec] = {"ms_per_step": ms / steps, "stage_ms": stage_ms, "dof_stage_per_s": dof * 3 * steps / (ms * 1e-3),
+                     "algorithmic_bytes_per_stage": bytes_stage,
+                     "achieved_GBs": round(bytes_stage / (stage_ms * 1e-3) / 1e9, 1),
+                     "filtered_element_stages": filtered}
+    for prec in ("fp32", "bf16x3"):
+        d = finals[prec] - finals["fp64"]
+        out[prec][f"rel_l2_vs_fp64_after_{acc_steps}_steps"] = float(np.linalg.norm(d) / np.linalg.norm(finals["fp64"]))
+    head = out.pop("fp64")
+    head.update({"workload": f"3D advection {nX}x{nY}x{nZ} N={N} periodic, warp 0.05, filter on (cutoff 3)",
+                 "steps": steps, "dt": info["dt"], "variants": out})
+    return head
 
 
 def hpe_bytes_per_element_stage(N):

@@ -478,6 +478,14 @@ int self_adv3d_forward_step_rk3(self_adv3d_t h, double tn, int nSteps, double *e
 /* Element-tracer pairs the filter replaced since the last reset
  * (measurement support, no reference counterpart). */
 int self_adv3d_filter_count(self_adv3d_t h, int reset, unsigned *count);
+/* Arithmetic of the volume contractions of MappedTimeDerivative
+ * (:1345-1562): 0 FP64 (default, the reference's), 1 FP32 on the matrix
+ * cores, 2 split-BF16 (three bf16 products, FP32 accumulation) on the
+ * matrix cores; the line deviations enter the narrow arithmetic and the
+ * line values are added back in FP64.  State, RK register, geometry, face
+ * fluxes and the filter stay FP64.  No reference counterpart (precision
+ * selection). */
+int self_adv3d_set_precision(self_adv3d_t h, int precision);
 int self_adv3d_synchronize(self_adv3d_t h);
 
 /* --------------------------------- hydrostatic primitive equations (DGSEM)

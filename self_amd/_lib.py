@@ -171,6 +171,7 @@ _SIGS = {
     "self_adv3d_forward_step_rk3": ([C.c_void_p, C.c_double, C.c_int, dp], C.c_int),
     "self_adv3d_filter_count": ([C.c_void_p, C.c_int, C.POINTER(C.c_uint)], C.c_int),
     "self_adv3d_synchronize": ([C.c_void_p], C.c_int),
+    "self_adv3d_set_precision": ([C.c_void_p, C.c_int], C.c_int),
     "self_hpe_create": ([C.c_int] * 4 + [ip] + [dp] * 10 + [C.c_int, C.c_double, C.POINTER(C.c_void_p)], C.c_int),
     "self_hpe_destroy": ([C.c_void_p], C.c_int),
     "self_hpe_set_field": ([C.c_void_p, C.c_int, dp], C.c_int),

@@ -145,5 +145,10 @@ class Advection3D:
         L.check(L.lib().self_adv3d_forward_step_rk3(self._h, tn, nSteps, C.byref(ms)))
         return ms.value
 
+    def SetPrecision(self, precision="fp64"):
+        """Arithmetic of the volume contractions: "fp64" (the reference's),
+        "fp32" (matrix-core FP32) or "bf16x3" (matrix-core split BF16)."""
+        L.check(L.lib().self_adv3d_set_precision(self._h, {"fp64": 0, "fp32": 1, "bf16x3": 2}[precision]))
+
     def Synchronize(self):
         L.check(L.lib().self_adv3d_synchronize(self._h))

@@ -35,6 +35,7 @@
 #include "common.hpp"
 #include "dg2d_device.hpp"
 #include "dg3d_device.hpp"
+#include "mfma.hpp"
 
 namespace self {
 void cofactor3(const double *m, int dir, double out[3]);
@@ -93,6 +94,7 @@ struct Adv3Args {
     const double *nodal;
     double *c, *G, *tend;         // [nq][nE][np]
     const double *vel;            // [3][nE][np]
+    const double *acoef;          // [3][nE][np]: u . cofactor row s, p, q (the static velocity folded in)
     const double *geo;            // [10][nE][np]: 9 cofactors (s row, p row, q row), J
     const double *fdat;           // [nE][6][2][n2]: u.n and |n| of the face's primary side
     const double *bpos;           // [slot][3][n2]
@@ -265,17 +267,35 @@ __global__ __launch_bounds__(Shape3<N>::threads) void k_adv3d_prologue(Adv3Args 
     write_traces<N, NQ>(A, L, cs);
 }
 
-template <int N, int NQ>
+// PREC 0: FP64 contractions; 1: FP32 matrix-core contractions; 2: split-BF16
+// matrix-core contractions (mfma.hpp).  State, register, geometry, face
+// fluxes, lift, Jacobian and the filter are FP64 in every form.
+template <int N, int NQ, int PREC>
 __global__ __launch_bounds__(Shape3<N>::threads) void k_adv3d_stage(Adv3Args A)
 {
     using S = Shape3<N>;
     constexpr int n1 = S::n1, n2 = S::n2, np = S::np, epb = S::epb;
+    constexpr int nT = PREC ? epb * NQ * 3 * np : 1;
     __shared__ Nodal3<N> L;
     __shared__ double cs[epb * NQ * np], Fs[epb * NQ * np], Fp[epb * NQ * np], Fq[epb * NQ * np];
     __shared__ double bf[epb * 6 * NQ * n2];
     __shared__ double red[epb * 2 * n2];
     __shared__ int flag[epb * NQ];
+    __shared__ float Tm[nT];
+    __shared__ float Df[64];
+    __shared__ double rs[n1];
     stage_nodal3<N>(L, A.nodal);
+    if (PREC) {
+        for (int q = threadIdx.x; q < 64; q += blockDim.x) {
+            const int r = q >> 3, m = q & 7;
+            Df[q] = (r < n1 && m < n1) ? (float)A.nodal[r + n1 * m] : 0.0f;
+        }
+        for (int r = threadIdx.x; r < n1; r += blockDim.x) {
+            double acc = 0.0;
+            for (int m = 0; m < n1; ++m) acc += A.nodal[r + n1 * m];
+            rs[r] = acc;
+        }
+    }
     const int t = threadIdx.x, le = t / np, nd = t - le * np;
     const int e = blockIdx.x * epb + le;
     const bool act = le < epb && e < A.nE;
@@ -283,23 +303,30 @@ __global__ __launch_bounds__(Shape3<N>::threads) void k_adv3d_stage(Adv3Args A)
     const size_t ms = (size_t)A.nE * np, base = (size_t)e * np + nd;
     double c[NQ], gold[NQ], J = 1.0;
     if (act) {
-        double g[9];
-        #pragma unroll
-        for (int r = 0; r < 9; ++r) g[r] = A.geo[r * ms + base];
         J = A.geo[9 * ms + base];
-        const double u = A.vel[base], v = A.vel[ms + base], w = A.vel[2 * ms + base];
+        // X/Y/ZFlux (:1624-1672) projected on the cofactor rows, with the
+        // static velocity folded into the rows once
+        const double as = A.acoef[base], ap = A.acoef[ms + base], aq = A.acoef[2 * ms + base];
         for (int f = 0; f < NQ; ++f) {
             c[f] = A.c[f * ms + base];
             gold[f] = (A.mode == 0 && A.a != 0.0) ? A.G[f * ms + base] : 0.0;
-            const double xF = c[f] * u, yF = c[f] * v, zF = c[f] * w;  // X/Y/ZFlux :1624-1672
             const int o = (le * NQ + f) * np + nd;
             cs[o] = c[f];
-            Fs[o] = g[0] * xF + g[1] * yF + g[2] * zF;
-            Fp[o] = g[3] * xF + g[4] * yF + g[5] * zF;
-            Fq[o] = g[6] * xF + g[7] * yF + g[8] * zF;
+            Fs[o] = as * c[f];
+            Fp[o] = ap * c[f];
+            Fq[o] = aq * c[f];
         }
     }
     __syncthreads();
+    if (PREC) {  // matrix-core contractions of the volume fluxes, one 16x16 tile per wavefront
+        const int wave = threadIdx.x >> 6, nw = blockDim.x >> 6, nTiles = epb * NQ * 3 * 2;
+        for (int tile = wave; tile < nTiles; tile += nw) {
+            const int lf = tile / 6, dir = (tile % 6) >> 1, t2 = tile & 1;
+            if (blockIdx.x * epb + lf / NQ >= A.nE || 32 * t2 >= n2) continue;  // wave-uniform
+            const double *X = (dir == 0 ? Fs : dir == 1 ? Fp : Fq) + lf * np;
+            mf_tile<N, PREC>(Df, X, dir, t2, Tm + (lf * 3 + dir) * np);
+        }
+    }
     // face fluxes (EdgeFlux :1237-1341 with RiemannSolver :1595-1620)
     for (int w = t; w < epb * 6 * n2; w += blockDim.x) {
         const int l2 = w / (6 * n2), rem = w - l2 * 6 * n2, sd = rem / n2, node = rem - sd * n2;
@@ -335,11 +362,18 @@ __global__ __launch_bounds__(Shape3<N>::threads) void k_adv3d_stage(Adv3Args A)
         for (int f = 0; f < NQ; ++f) {
             const int o = (le * NQ + f) * np;
             double ts = 0.0, tp = 0.0, tq = 0.0;
-            #pragma unroll
-            for (int m = 0; m < n1; ++m) {
-                ts = __builtin_fma(L.D[i + n1 * m], Fs[o + m + n1 * (j + n1 * k)], ts);
-                tp = __builtin_fma(L.D[j + n1 * m], Fp[o + i + n1 * (m + n1 * k)], tp);
-                tq = __builtin_fma(L.D[k + n1 * m], Fq[o + i + n1 * (j + n1 * m)], tq);
+            if (PREC == 0) {
+                #pragma unroll
+                for (int m = 0; m < n1; ++m) {
+                    ts = __builtin_fma(L.D[i + n1 * m], Fs[o + m + n1 * (j + n1 * k)], ts);
+                    tp = __builtin_fma(L.D[j + n1 * m], Fp[o + i + n1 * (m + n1 * k)], tp);
+                    tq = __builtin_fma(L.D[k + n1 * m], Fq[o + i + n1 * (j + n1 * m)], tq);
+                }
+            } else {  // narrow contraction of the deviations + FP64 line value times the row sum
+                const float *T = Tm + (le * NQ + f) * 3 * np;
+                ts = (double)T[nd] + Fs[o + n1 * (j + n1 * k)] * rs[i];
+                tp = (double)T[np + j + n1 * (i + n1 * k)] + Fp[o + i + n2 * k] * rs[j];
+                tq = (double)T[2 * np + k + n1 * (i + n1 * j)] + Fq[o + i + n1 * j] * rs[k];
             }
             const double bE = bfE[(1 * NQ + f) * n2 + j + n1 * k], bW = bfE[(3 * NQ + f) * n2 + j + n1 * k];
             const double bN = bfE[(2 * NQ + f) * n2 + i + n1 * k], bS = bfE[(0 * NQ + f) * n2 + i + n1 * k];
@@ -397,6 +431,17 @@ __global__ void k_adv3d_face_un(const double *__restrict__ velTr, const double *
     fdat[((size_t)es * 2 + 1) * n2 + node] = len;
 }
 
+// the static velocity folded into the cofactor rows: a_d = (u, v, w) . row d
+__global__ void k_adv3d_fold(const double *__restrict__ vel, const double *__restrict__ geo,
+                             double *__restrict__ acoef, size_t n)
+{
+    const size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= n) return;
+    const double u = vel[q], v = vel[n + q], w = vel[2 * n + q];
+    for (int d = 0; d < 3; ++d)
+        acoef[d * n + q] = geo[(3 * d) * n + q] * u + geo[(3 * d + 1) * n + q] * v + geo[(3 * d + 2) * n + q] * w;
+}
+
 }  // namespace
 }  // namespace self
 
@@ -410,9 +455,10 @@ struct self_adv3d_s {
     double *nodal = nullptr, *c = nullptr, *G = nullptr, *tend = nullptr, *vel = nullptr, *geo = nullptr;
     double *fdat = nullptr, *nrm = nullptr, *bpos = nullptr, *tr[2] = {nullptr, nullptr}, *velTr = nullptr;
     double *relax = nullptr, *rFac = nullptr, *E1 = nullptr, *E2 = nullptr, *lim = nullptr;
+    double *acoef = nullptr;
     int2 *side = nullptr;
     unsigned *nFiltered = nullptr;
-    int cur = 0;
+    int cur = 0, prec = 0;
     bool trValid = false, hasRelax = false;
 };
 
@@ -466,6 +512,7 @@ Adv3Args args3(self_adv3d_s *h)
     A.G = h->G;
     A.tend = h->tend;
     A.vel = h->vel;
+    A.acoef = h->acoef;
     A.geo = h->geo;
     A.fdat = h->fdat;
     A.bpos = h->bpos;
@@ -502,8 +549,19 @@ int stage3(self_adv3d_s *h, double t, int m, int mode, int filterNext)
     A.filterNext = h->useFilter ? filterNext : 0;
     A.trIn = h->tr[h->cur];
     A.trOut = h->tr[1 - h->cur];
-    ADV3_DISPATCH(h->N, h->nq, hipLaunchKernelGGL((k_adv3d_stage<NN, QQ>), dim3(grid3<NN>(h->nE)),
-                                                  dim3(threads3<NN>()), 0, h->stream, A));
+    switch (h->prec) {
+    case 1:
+        ADV3_DISPATCH(h->N, h->nq, hipLaunchKernelGGL((k_adv3d_stage<NN, QQ, 1>), dim3(grid3<NN>(h->nE)),
+                                                      dim3(threads3<NN>()), 0, h->stream, A));
+        break;
+    case 2:
+        ADV3_DISPATCH(h->N, h->nq, hipLaunchKernelGGL((k_adv3d_stage<NN, QQ, 2>), dim3(grid3<NN>(h->nE)),
+                                                      dim3(threads3<NN>()), 0, h->stream, A));
+        break;
+    default:
+        ADV3_DISPATCH(h->N, h->nq, hipLaunchKernelGGL((k_adv3d_stage<NN, QQ, 0>), dim3(grid3<NN>(h->nE)),
+                                                      dim3(threads3<NN>()), 0, h->stream, A));
+    }
     SELF_HIP(hipGetLastError());
     if (mode == 0) h->cur = 1 - h->cur;
     return SELF_OK;
@@ -526,7 +584,7 @@ void d2h_fields(const double *src, double *dst, size_t nE, int nf, int np)
 void free_all(self_adv3d_s *h)
 {
     void *p[] = {h->nodal, h->c, h->G, h->tend, h->vel, h->geo, h->fdat, h->nrm, h->bpos, h->tr[0], h->tr[1],
-                 h->velTr, h->relax, h->rFac, h->E1, h->E2, h->lim, h->side, h->nFiltered};
+                 h->velTr, h->relax, h->rFac, h->E1, h->E2, h->lim, h->acoef, h->side, h->nFiltered};
     for (void *q : p)
         if (q) (void)hipFree(q);
     if (h->stream) (void)hipStreamDestroy(h->stream);
@@ -619,6 +677,7 @@ try {
     M(&h->G, nq * nE * np);
     M(&h->tend, nq * nE * np);
     M(&h->vel, 3 * nE * np);
+    M(&h->acoef, 3 * nE * np);
     M(&h->geo, geo.size());
     M(&h->fdat, nE * 6 * 2 * n2);
     M(&h->nrm, nrm.size());
@@ -638,6 +697,7 @@ try {
     if (e == hipSuccess) e = hipMemcpy(h->side, sd.data(), sizeof(int2) * sd.size(), hipMemcpyHostToDevice);
     if (e == hipSuccess) e = hipMemset(h->c, 0, sizeof(double) * nq * nE * np);
     if (e == hipSuccess) e = hipMemset(h->vel, 0, sizeof(double) * 3 * nE * np);
+    if (e == hipSuccess) e = hipMemset(h->acoef, 0, sizeof(double) * 3 * nE * np);
     if (e == hipSuccess) e = hipMemset(h->fdat, 0, sizeof(double) * nE * 6 * 2 * n2);
     if (e == hipSuccess) e = hipMemset(h->E1, 0, sizeof(double) * nq * nE);
     if (e == hipSuccess) e = hipMemset(h->E2, 0, sizeof(double) * nq * nE);
@@ -695,6 +755,9 @@ try {
             hipLaunchKernelGGL((k_adv3d_face_un<NN>), dim3((nsn + 255) / 256), dim3(256), 0, h->stream, h->velTr,
                                h->nrm, h->side, h->fdat, (int)nE);
         });
+        const size_t nn = nE * np;
+        hipLaunchKernelGGL(k_adv3d_fold, dim3((unsigned)((nn + 255) / 256)), dim3(256), 0, h->stream, h->vel,
+                           h->geo, h->acoef, nn);
         SELF_HIP(hipGetLastError());
         SELF_HIP(hipStreamSynchronize(h->stream));
     }
@@ -782,6 +845,17 @@ try {
     return SELF_OK;
 }
 SELF_CATCH_ALL
+
+// 0: FP64 contractions; 1: FP32 matrix-core contractions; 2: split-BF16
+// matrix-core contractions (the volume weak divergence only)
+int self_adv3d_set_precision(self_adv3d_t h, int precision)
+{
+    SELF_REQUIRE(h, SELF_ERR_ARG, "null handle");
+    SELF_REQUIRE(precision >= 0 && precision <= 2, SELF_ERR_ARG, "precision %d (0 fp64, 1 fp32, 2 bf16x3)",
+                 precision);
+    h->prec = precision;
+    return SELF_OK;
+}
 
 int self_adv3d_synchronize(self_adv3d_t h)
 {

@@ -4,7 +4,7 @@
     python -m self_amd.profile_run sw2d --precision mixed --form fused --steps 10
     python -m self_amd.profile_run sw2d-dipole --mesh box.mesh --N 10 --form plain --precision fp64
     python -m self_amd.profile_run adv2d --steps 10
-    python -m self_amd.profile_run adv3d --steps 5
+    python -m self_amd.profile_run adv3d --steps 5 [--contraction fp32|bf16x3]
     python -m self_amd.profile_run cgsem --nx 128 --precond 1
     python -m self_amd.profile_run cgsem --precision mixed --steps 2000
     python -m self_amd.profile_run hpe --steps 5
@@ -37,6 +37,8 @@ def main(argv=None):
     ap.add_argument("--N", type=int, default=7)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--precond", type=int, default=1)
+    ap.add_argument("--contraction", default="fp64", choices=["fp64", "fp32", "bf16x3"],
+                    help="arithmetic of the 3-D volume contractions (adv3d)")
     a = ap.parse_args(argv)
     L.check(L.lib().self_init(0))
     if a.mode == "copy":
@@ -73,9 +75,10 @@ def main(argv=None):
     elif a.mode == "adv3d":
         from .workloads import adv3d_gaussians
         m, _ = adv3d_gaussians(nX=a.nx or 32, nY=a.nx or 32, nZ=(a.nx or 32) // 2, N=a.N)
+        m.SetPrecision(a.contraction)
         m.ForwardStepRK3(0.0, a.steps)
         m.Synchronize()
-        print(f"adv3d: {a.steps} RK3 steps", flush=True)
+        print(f"adv3d {a.contraction}: {a.steps} RK3 steps", flush=True)
     elif a.mode == "cgsem":
         from .workloads import cgsem_helmholtz
         m, _ = cgsem_helmholtz(nX=a.nx or 128, N=a.N)
