@@ -281,6 +281,10 @@ int self_sw2d_synchronize(self_sw2d_t h);
  * waves), 2 (each workgroup prefetches its second group while computing the
  * first), 0 = default (1).  No reference counterpart. */
 int self_sw2d_set_groups(self_sw2d_t h, int groups);
+/* Fused form: run pairs of RK3 steps as one captured HIP graph (six stage
+ * launches), re-captured whenever a launch it holds would change (default
+ * on); 0 launches every stage from the host.  No reference counterpart. */
+int self_sw2d_set_graph(self_sw2d_t h, int enable);
 /* Ensembles (no reference counterpart: the reference runs one state per
  * model object).  `members` states on the same mesh, nodal storage,
  * bathymetry, vorticity and relaxation advance together, one

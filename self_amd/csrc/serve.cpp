@@ -185,6 +185,9 @@ int run_sw_batch(self_server_s *s, std::vector<std::shared_ptr<Job>> &jobs)
     self_sw2d_t h = nullptr;
     int rc = self_sw2d_create(&d, &h);
     if (rc) return rc;
+    // the workers run their models side by side and allocate while others
+    // run: stages are launched from the host, not captured as graphs
+    rc = self_sw2d_set_graph(h, 0);
     const size_t nv = (size_t)m->nE * (r.N + 1) * (r.N + 1);
     std::vector<double> plv(nv, r.f0);
     if (!rc) rc = self_sw2d_set_nodal(h, m->dS.data(), m->dP.data(), m->w.data(), m->w.data(), m->lW.data(),
@@ -214,6 +217,9 @@ int run_sw(self_server_s *s, Job &j)
     self_sw2d_t h = nullptr;
     int rc = self_sw2d_create(&d, &h);
     if (rc) return rc;
+    // the workers run their models side by side and allocate while others
+    // run: stages are launched from the host, not captured as graphs
+    rc = self_sw2d_set_graph(h, 0);
     const size_t nv = (size_t)m->nE * (r.N + 1) * (r.N + 1);
     std::vector<double> plv(nv, r.f0);
     if (!rc) rc = self_sw2d_set_nodal(h, m->dS.data(), m->dP.data(), m->w.data(), m->w.data(), m->lW.data(),

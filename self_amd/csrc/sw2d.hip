@@ -803,6 +803,12 @@ struct self_sw2d_s {
     double *dtM = nullptr;          // per-member time steps (device, members > 1)
     std::vector<double> dtH;        // host copy
     size_t bytes = 0;
+    // two RK3 steps (six fused stages) captured as one graph, replayed while
+    // the six launches it holds are exactly the ones the model would issue
+    bool useGraph = true;
+    hipGraphExec_t graphExec = nullptr;
+    SWKernelArgs graphArgs[6];
+    int graphDims[4] = {0, 0, 0, 0};
 };
 
 namespace {
@@ -826,9 +832,10 @@ void free_all(self_sw2d_s *h)
         if (p) (void)hipFree(p);
 }
 
-SWKernelArgs make_args(self_sw2d_s *h)
+// fills A member by member (callers that compare argument blocks bytewise
+// zero the object first, so its padding stays zero)
+void fill_args(self_sw2d_s *h, SWKernelArgs &A)
 {
-    SWKernelArgs A{};
     const int n1 = h->n1;
     A.nE = (int)h->nE;
     A.members = h->members;
@@ -863,6 +870,12 @@ SWKernelArgs make_args(self_sw2d_s *h)
     A.fnrm = h->fnrm;
     A.side = h->side;
     A.bflux = h->bflux;
+}
+
+SWKernelArgs make_args(self_sw2d_s *h)
+{
+    SWKernelArgs A{};
+    fill_args(h, A);
     return A;
 }
 
@@ -999,6 +1012,61 @@ int step_fused(self_sw2d_s *h)
     return SELF_OK;
 }
 
+// nPairs x two RK3 steps of the fused form through the captured graph
+int steps_fused_graph(self_sw2d_s *h, int nPairs)
+{
+    if (!h->trValid) {
+        int rc = launch_traces_push(h, h->tr[h->trCur]);
+        if (rc) return rc;
+        h->trValid = true;
+    }
+    SWKernelArgs cand[6];
+    std::memset(static_cast<void *>(cand), 0, sizeof(cand));
+    int cur = h->trCur;
+    for (int k = 0; k < 6; ++k) {
+        const int m = k % 3;
+        fill_args(h, cand[k]);
+        cand[k].a = kRkA[m];
+        cand[k].gdt = kRkG[m] * h->d.dt;
+        cand[k].gcoef = kRkG[m];
+        cand[k].trIn = h->tr[cur];
+        cand[k].trOut = h->tr[1 - cur];
+        cur = 1 - cur;
+    }
+    const int dims[4] = {fused_groups(h->groups, h->nE, 0), h->members, h->d.N, h->d.precision};
+    const bool same = h->graphExec && std::memcmp(cand, h->graphArgs, sizeof(cand)) == 0 &&
+                      std::memcmp(dims, h->graphDims, sizeof(dims)) == 0;
+    if (!same) {
+        if (h->graphExec) {
+            SELF_HIP(hipGraphExecDestroy(h->graphExec));
+            h->graphExec = nullptr;
+        }
+        // a capture the runtime refuses leaves nothing launched: the handle
+        // falls back to host launches for this and every later call
+        hipGraph_t g = nullptr;
+        hipError_t e = hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal);
+        int rc = SELF_OK;
+        if (e == hipSuccess) {
+            for (int k = 0; k < 6 && !rc; ++k) rc = launch_stage_fused(h, cand[k]);
+            e = hipStreamEndCapture(h->stream, &g);
+        }
+        if (e == hipSuccess && !rc) e = hipGraphInstantiate(&h->graphExec, g, nullptr, nullptr, 0);
+        if (g) (void)hipGraphDestroy(g);
+        if (e != hipSuccess || rc) {
+            (void)hipGetLastError();
+            h->graphExec = nullptr;
+            h->useGraph = false;
+            for (int k = 0; k < 2 * nPairs; ++k)
+                if (int r2 = step_fused(h)) return r2;
+            return SELF_OK;
+        }
+        std::memcpy(static_cast<void *>(h->graphArgs), cand, sizeof(cand));
+        std::memcpy(h->graphDims, dims, sizeof(dims));
+    }
+    for (int p = 0; p < nPairs; ++p) SELF_HIP(hipGraphLaunch(h->graphExec, h->stream));
+    return SELF_OK;
+}
+
 int check_ready(self_sw2d_s *h)
 {
     SELF_REQUIRE(h, SELF_ERR_ARG, "null handle");
@@ -1100,6 +1168,7 @@ int self_sw2d_destroy(self_sw2d_t h)
     if (!h) return SELF_OK;
     (void)hipSetDevice(h->device);
     if (h->stream) (void)hipStreamSynchronize(h->stream);
+    if (h->graphExec) (void)hipGraphExecDestroy(h->graphExec);
     free_all(h);
     if (h->ev0) (void)hipEventDestroy(h->ev0);
     if (h->ev1) (void)hipEventDestroy(h->ev1);
@@ -1380,7 +1449,12 @@ int self_sw2d_forward_step_rk3(self_sw2d_t h, double tn, int nSteps)
     int rc = check_ready(h);
     if (rc) return rc;
     SELF_REQUIRE(nSteps >= 0, SELF_ERR_ARG, "negative step count");
-    for (int s = 0; s < nSteps && !rc; ++s) rc = (h->d.form == SELF_FUSED) ? step_fused(h) : step_plain(h);
+    int s = 0;
+    if (h->d.form == SELF_FUSED && h->useGraph && nSteps >= 2) {
+        rc = steps_fused_graph(h, nSteps / 2);
+        s = 2 * (nSteps / 2);
+    }
+    for (; s < nSteps && !rc; ++s) rc = (h->d.form == SELF_FUSED) ? step_fused(h) : step_plain(h);
     return rc;
 }
 
@@ -1605,6 +1679,13 @@ int self_sw2d_set_variant(self_sw2d_t h, int variant)
     SELF_REQUIRE(h, SELF_ERR_ARG, "null handle");
     SELF_REQUIRE(variant == 0 || variant == 1, SELF_ERR_ARG, "unknown shallow-water variant %d", variant);
     h->variant = variant;
+    return SELF_OK;
+}
+
+int self_sw2d_set_graph(self_sw2d_t h, int enable)
+{
+    SELF_REQUIRE(h, SELF_ERR_ARG, "null handle");
+    h->useGraph = enable != 0;
     return SELF_OK;
 }
 

@@ -215,6 +215,11 @@ class ShallowWater:
         """Element groups per workgroup of the fused kernel (0 = heuristic)."""
         L.check(L.lib().self_sw2d_set_groups(self._h, groups))
 
+    def SetGraph(self, enable=True):
+        """Fused form: replay pairs of RK3 steps from one captured HIP graph
+        (default) or launch every stage from the host."""
+        L.check(L.lib().self_sw2d_set_graph(self._h, int(bool(enable))))
+
     def WritePickup(self, iter, directory="."):
         """WritePickup (ConservativeShallowWaterClass.f90:2013-2065): state,
         bathymetry and vorticity to <directory>/ShallowWater.<iter>.pickup."""
