@@ -258,6 +258,41 @@ def bench_adv3d(steps=20, warm=2, nX=32, nY=32, nZ=16, N=7, acc_steps=200):
     return head
 
 
+def bench_sw2d_uneven(nX=16, degrees=(3, 5, 7, 3, 5, 7, 3, 5), steps=50, warm=4):
+    """Eight small shallow-water models (16x16 meshes, N = 3, 5, 7, the
+    request mix of the serving workload; geostrophic state, FP64 fused):
+    stepped one after another with their own launches (24 launches per
+    step), against all eight in one persistent launch per RK stage (3 per
+    step).  Small models cannot fill the GPU one at a time."""
+    from self_amd.shallow_water import forward_step_rk3_multi
+    from self_amd.workloads import sw2d_geostrophic
+    models = [sw2d_geostrophic(nX, N, precision="fp64", form="fused")[0] for N in degrees]
+    dof = sum(m.nElems * (N + 1) ** 2 for m, N in zip(models, degrees))
+    for m in models:
+        m.SetGraph(False)
+        m.ForwardStepRK3(0.0, warm)
+    for m in models:
+        m.Synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        for m in models:
+            m.ForwardStepRK3(0.0, 1)
+    for m in models:
+        m.Synchronize()
+    sep = (time.perf_counter() - t0) * 1e3
+    forward_step_rk3_multi(models, warm)
+    t0 = time.perf_counter()
+    forward_step_rk3_multi(models, steps)
+    one = (time.perf_counter() - t0) * 1e3
+    for m in models:
+        m.Trash()
+    return {"workload": f"{len(degrees)} shallow-water models {nX}x{nX} at N = {', '.join(map(str, degrees))}, "
+                        f"FP64 fused, {steps} steps, wall clock", "dof": dof,
+            "separate_ms_per_step": sep / steps, "one_launch_ms_per_step": one / steps,
+            "separate_dof_stage_per_s": dof * 3 * steps / (sep * 1e-3),
+            "one_launch_dof_stage_per_s": dof * 3 * steps / (one * 1e-3), "speedup": sep / one}
+
+
 def hpe_bytes_per_element_stage(N):
     """Algorithmic bytes of one stage per element (FP64): the column kernel
     reads u, v, b, six cofactor terms and J at every node, the column
@@ -604,7 +639,8 @@ def main():
     }
     progress("accuracy gates done")
     if not args.no_extras:
-        for key, fn in (("sw2d_ensemble", bench_sw2d_ensemble), ("sw2d_dipole_N10", bench_sw2d_dipole),
+        for key, fn in (("sw2d_ensemble", bench_sw2d_ensemble), ("sw2d_uneven_degree", bench_sw2d_uneven),
+                        ("sw2d_dipole_N10", bench_sw2d_dipole),
                         ("adv2d", bench_adv2d), ("adv3d", bench_adv3d), ("hpe", bench_hpe),
                         ("cgsem", bench_cgsem)):
             out[key] = fn()

@@ -285,6 +285,12 @@ int self_sw2d_set_groups(self_sw2d_t h, int groups);
  * launches), re-captured whenever a launch it holds would change (default
  * on); 0 launches every stage from the host.  No reference counterpart. */
 int self_sw2d_set_graph(self_sw2d_t h, int enable);
+/* ForwardStepRK3 nSteps times for nModels fused-form models of any degrees
+ * (1..11, one precision), every RK stage of all of them in one persistent
+ * launch whose workgroups walk the models' element groups; the same
+ * results as stepping each model alone.  Blocking; elapsed_ms (device time
+ * of the launches) nullable.  No reference counterpart (batching). */
+int self_sw2d_forward_step_rk3_multi(self_sw2d_t *models, int nModels, int nSteps, double *elapsed_ms);
 /* Ensembles (no reference counterpart: the reference runs one state per
  * model object).  `members` states on the same mesh, nodal storage,
  * bathymetry, vorticity and relaxation advance together, one

@@ -196,6 +196,7 @@ _SIGS = {
     "self_coarse_to_fine_3d_dev": ([C.c_int, C.c_size_t, dp, C.c_int, dp, dp, C.c_void_p], C.c_int),
     "self_sw2d_set_groups": ([C.c_void_p, C.c_int], C.c_int),
     "self_sw2d_set_graph": ([C.c_void_p, C.c_int], C.c_int),
+    "self_sw2d_forward_step_rk3_multi": ([C.POINTER(C.c_void_p), C.c_int, C.c_int, dp], C.c_int),
     "self_sw2d_set_variant": ([C.c_void_p, C.c_int], C.c_int),
     "self_sw2d_set_members": ([C.c_void_p, C.c_int], C.c_int),
     "self_sw2d_write_pickup": ([C.c_void_p, C.c_char_p], C.c_int),

@@ -27,6 +27,7 @@
 // tendency register, bathymetry gradient and Coriolis parameter.  State,
 // traces, metrics, normals and the RK accumulation stay FP64.
 
+#include <algorithm>
 #include <cstring>
 #include <vector>
 
@@ -651,12 +652,31 @@ __device__ __forceinline__ void sw_process(const SWKernelArgs &A, const NodalLDS
 // Ensembles: the grid holds `members` workgroups per element group, element
 // group major, so the members of one group run side by side on one XCD and
 // share its geometry in L2; each member has its own q, G and traces.
+// LDS of one fused stage workgroup.  WL: each element lies inside one
+// wavefront, so the phases synchronize per wavefront and the wavefronts of
+// a workgroup never wait for each other (one that has its data computes
+// while the others still load).  Measured at N = 7: mixed 10.45 -> 10.32 us
+// per stage, FP64 11.29 -> 11.63 us, so the FP64 form keeps workgroup
+// barriers.
 template <int N, bool MIXED, int G>
-__global__ __launch_bounds__(256, 4 / G) void k_sw_stage_fused(SWKernelArgs A0)
+struct SwStageLds {
+    using S = Shape<N>;
+    static constexpr int np = S::np, epb = S::epb, n1 = S::n1;
+    static constexpr bool WL = MIXED && (G == 1) && (64 % np == 0);
+    NodalLDS<N> Lw[WL ? 4 : 1];
+    double qs[epb * 3 * np], sC[epb * 3 * np], pC[epb * 3 * np], bfs[epb * 12 * n1];
+    float sD[MIXED ? epb * 3 * np : 1], pD[MIXED ? epb * 3 * np : 1];
+};
+
+// One RK stage of element group wg (G groups of epb elements; with
+// ensembles, wg also selects the member) in the LDS block U.
+template <int N, bool MIXED, int G>
+__device__ __forceinline__ void sw_stage_group(const SWKernelArgs &A0, int wg, SwStageLds<N, MIXED, G> &U)
 {
     using S = Shape<N>;
+    using Lds = SwStageLds<N, MIXED, G>;
     constexpr int np = S::np, epb = S::epb, n1 = S::n1;
-    int wg = xcd_block(blockIdx.x, gridDim.x);
+    constexpr bool WL = Lds::WL;
     SWKernelArgs A = A0;
     if (A0.members > 1) {  // uniform: a single state skips the member arithmetic
         const int mbr = wg % A0.members;
@@ -669,16 +689,7 @@ __global__ __launch_bounds__(256, 4 / G) void k_sw_stage_fused(SWKernelArgs A0)
         A.trOut += tOff;
         A.gdt = A0.gcoef * A0.dtM[mbr];
     }
-    // WL: each element lies inside one wavefront, so the phases synchronize
-    // per wavefront and the wavefronts of a workgroup never wait for each
-    // other (one that has its data computes while the others still load).
-    // Measured at N = 7: mixed 10.45 -> 10.32 us per stage, FP64 11.29 ->
-    // 11.63 us, so the FP64 form keeps workgroup barriers.
-    constexpr bool WL = MIXED && (G == 1) && (64 % np == 0);
-    __shared__ NodalLDS<N> Lw[WL ? 4 : 1];
-    NodalLDS<N> &L = Lw[WL ? threadIdx.x / 64 : 0];
-    __shared__ double qs[epb * 3 * np], sC[epb * 3 * np], pC[epb * 3 * np], bfs[epb * 12 * n1];
-    __shared__ float sD[MIXED ? epb * 3 * np : 1], pD[MIXED ? epb * 3 * np : 1];
+    NodalLDS<N> &L = U.Lw[WL ? threadIdx.x / 64 : 0];
     const int t = threadIdx.x, le = t / np, nd = t - le * np;
     SWPre P[G];
     int e[G];
@@ -696,12 +707,92 @@ __global__ __launch_bounds__(256, 4 / G) void k_sw_stage_fused(SWKernelArgs A0)
         stage_nodal<N, MIXED>(L, A.dS, A.dS32, A.dP32);
     }
     const int ls = le < epb ? le : 0;
-    double *qsE = qs + ls * 3 * np, *sCE = sC + ls * 3 * np, *pCE = pC + ls * 3 * np, *bfE = bfs + ls * 12 * n1;
-    float *sDE = MIXED ? sD + ls * 3 * np : sD, *pDE = MIXED ? pD + ls * 3 * np : pD;
+    double *qsE = U.qs + ls * 3 * np, *sCE = U.sC + ls * 3 * np, *pCE = U.pC + ls * 3 * np;
+    double *bfE = U.bfs + ls * 12 * n1;
+    float *sDE = MIXED ? U.sD + ls * 3 * np : U.sD, *pDE = MIXED ? U.pD + ls * 3 * np : U.pD;
     #pragma unroll
     for (int g = 0; g < G; ++g) {
         if (g > 0) __syncthreads();  // the previous group's phase D read qs
         sw_process<N, MIXED, WL>(A, L, qsE, sCE, pCE, sDE, pDE, bfE, e[g], nd, act[g], P[g]);
+    }
+}
+
+// One RK stage for G element groups per workgroup.  G = 2 issues the
+// second group's loads before the first group computes, so its memory
+// traffic overlaps the first group's arithmetic (the stage is otherwise
+// one round of waves that all load, then all compute).
+//
+// Ensembles: the grid holds `members` workgroups per element group, element
+// group major, so the members of one group run side by side on one XCD and
+// share its geometry in L2; each member has its own q, G and traces.
+template <int N, bool MIXED, int G>
+__global__ __launch_bounds__(256, 4 / G) void k_sw_stage_fused(SWKernelArgs A0)
+{
+    __shared__ SwStageLds<N, MIXED, G> U;
+    sw_stage_group<N, MIXED, G>(A0, xcd_block(blockIdx.x, gridDim.x), U);
+}
+
+// Models of uneven degree in one persistent launch: the grid (a few
+// workgroups per CU) walks a list of tiles, each tile one element group (x
+// member) of one model; the model's degree selects the code path and the
+// shared LDS block is reinterpreted for it.  Every model advances one RK
+// stage per launch.
+constexpr int kMultiMax = 8;  // models per launch (the arguments travel as kernel arguments)
+
+struct SWMultiArgs {
+    int nItems, totalTiles;
+    int N[kMultiMax], tileStart[kMultiMax];
+    SWKernelArgs A[kMultiMax];
+};
+
+template <bool MIXED>
+union SwMultiLds {
+    SwStageLds<1, MIXED, 1> n1;
+    SwStageLds<2, MIXED, 1> n2;
+    SwStageLds<3, MIXED, 1> n3;
+    SwStageLds<4, MIXED, 1> n4;
+    SwStageLds<5, MIXED, 1> n5;
+    SwStageLds<6, MIXED, 1> n6;
+    SwStageLds<7, MIXED, 1> n7;
+    SwStageLds<8, MIXED, 1> n8;
+    SwStageLds<9, MIXED, 1> n9;
+    SwStageLds<10, MIXED, 1> n10;
+    SwStageLds<11, MIXED, 1> n11;
+};
+
+// MENU 0: degrees 3, 5 and 7 only (the serving mix); 1: every degree.
+// One kernel has one register budget, so the fewer code paths it holds, the
+// less the paths that need more registers than the others make it spill.
+template <bool MIXED, int MENU>
+__global__ __launch_bounds__(256, 4) void k_sw_stage_multi(SWMultiArgs M)
+{
+    __shared__ SwMultiLds<MIXED> U;
+    for (int tile = blockIdx.x; tile < M.totalTiles; tile += gridDim.x) {
+        int m = 0;
+        while (m + 1 < M.nItems && M.tileStart[m + 1] <= tile) ++m;
+        const int wg = tile - M.tileStart[m];
+        if constexpr (MENU == 0) {
+            switch (M.N[m]) {
+            case 3: sw_stage_group<3, MIXED, 1>(M.A[m], wg, U.n3); break;
+            case 5: sw_stage_group<5, MIXED, 1>(M.A[m], wg, U.n5); break;
+            default: sw_stage_group<7, MIXED, 1>(M.A[m], wg, U.n7); break;
+            }
+        } else {
+            switch (M.N[m]) {
+            case 1: sw_stage_group<1, MIXED, 1>(M.A[m], wg, U.n1); break;
+            case 2: sw_stage_group<2, MIXED, 1>(M.A[m], wg, U.n2); break;
+            case 3: sw_stage_group<3, MIXED, 1>(M.A[m], wg, U.n3); break;
+            case 4: sw_stage_group<4, MIXED, 1>(M.A[m], wg, U.n4); break;
+            case 5: sw_stage_group<5, MIXED, 1>(M.A[m], wg, U.n5); break;
+            case 6: sw_stage_group<6, MIXED, 1>(M.A[m], wg, U.n6); break;
+            case 7: sw_stage_group<7, MIXED, 1>(M.A[m], wg, U.n7); break;
+            case 8: sw_stage_group<8, MIXED, 1>(M.A[m], wg, U.n8); break;
+            case 9: sw_stage_group<9, MIXED, 1>(M.A[m], wg, U.n9); break;
+            case 10: sw_stage_group<10, MIXED, 1>(M.A[m], wg, U.n10); break;
+            default: sw_stage_group<11, MIXED, 1>(M.A[m], wg, U.n11); break;
+            }
+        }
+        __syncthreads();  // the next tile reuses the LDS block
     }
 }
 
@@ -1681,6 +1772,96 @@ int self_sw2d_set_variant(self_sw2d_t h, int variant)
     h->variant = variant;
     return SELF_OK;
 }
+
+// ForwardStepRK3 nSteps times for several fused-form models at once, of
+// any degrees, one persistent launch per RK stage for all of them
+int self_sw2d_forward_step_rk3_multi(self_sw2d_t *models, int nModels, int nSteps, double *elapsed_ms)
+try {
+    SELF_REQUIRE(models && nModels >= 1 && nSteps >= 0, SELF_ERR_ARG, "bad arguments");
+    self_sw2d_s *h0 = models[0];
+    for (int k = 0; k < nModels; ++k) {
+        self_sw2d_s *h = models[k];
+        int rc = check_ready(h);
+        if (rc) return rc;
+        SELF_REQUIRE(h->d.form == SELF_FUSED, SELF_ERR_ARG, "model %d is not in the fused form", k);
+        SELF_REQUIRE(h->d.precision == h0->d.precision, SELF_ERR_ARG, "model %d: precisions differ", k);
+        SELF_REQUIRE(h->device == h0->device, SELF_ERR_ARG, "model %d is on another device", k);
+        SELF_REQUIRE(h->d.N >= 1 && h->d.N <= 11, SELF_ERR_ARG, "model %d: N=%d (1..11)", k, h->d.N);
+        for (int j = 0; j < k; ++j) SELF_REQUIRE(models[j] != h, SELF_ERR_ARG, "model %d is listed twice", k);
+        if (!h->trValid) {
+            rc = launch_traces_push(h, h->tr[h->trCur]);
+            if (rc) return rc;
+            h->trValid = true;
+        }
+        SELF_HIP(hipStreamSynchronize(h->stream));
+    }
+    // six stage sets per batch of up to kMultiMax models (two steps bring
+    // every model back to its trace parity)
+    const int nB = (nModels + kMultiMax - 1) / kMultiMax;
+    std::vector<SWMultiArgs> sets((size_t)6 * nB);
+    std::memset(static_cast<void *>(sets.data()), 0, sizeof(SWMultiArgs) * sets.size());
+    for (int b = 0; b < nB; ++b) {
+        int total = 0;
+        for (int k = b * kMultiMax; k < std::min(nModels, (b + 1) * kMultiMax); ++k) {
+            self_sw2d_s *h = models[k];
+            const int slot = k - b * kMultiMax;
+            int tiles = 0;
+            SELF_DISPATCH_N(h->d.N, tiles = grid_fused<NN>(h->nE, 1) * h->members);
+            int cur = h->trCur;
+            for (int st = 0; st < 6; ++st) {
+                const int m = st % 3;
+                SWMultiArgs &M = sets[(size_t)st * nB + b];
+                M.nItems = slot + 1;
+                M.N[slot] = h->d.N;
+                M.tileStart[slot] = total;
+                fill_args(h, M.A[slot]);
+                M.A[slot].a = kRkA[m];
+                M.A[slot].gdt = kRkG[m] * h->d.dt;
+                M.A[slot].gcoef = kRkG[m];
+                M.A[slot].trIn = h->tr[cur];
+                M.A[slot].trOut = h->tr[1 - cur];
+                cur = 1 - cur;
+            }
+            total += tiles;
+        }
+        for (int st = 0; st < 6; ++st) sets[(size_t)st * nB + b].totalTiles = total;
+    }
+    int cus = 0;
+    SELF_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, h0->device));
+    hipError_t e = elapsed_ms ? hipEventRecord(h0->ev0, h0->stream) : hipSuccess;
+    const bool mixed = h0->d.precision == SELF_MIXED;
+    bool menu357 = true;
+    for (int k = 0; k < nModels; ++k) {
+        const int N = models[k]->d.N;
+        menu357 = menu357 && (N == 3 || N == 5 || N == 7);
+    }
+    for (int s = 0; s < 3 * nSteps && e == hipSuccess; ++s)
+        for (int b = 0; b < nB && e == hipSuccess; ++b) {
+            const SWMultiArgs &M = sets[(size_t)(s % 6) * nB + b];
+            const int grid = std::max(1, std::min(M.totalTiles, 4 * std::max(cus, 1)));
+            if (mixed && menu357)
+                hipLaunchKernelGGL((k_sw_stage_multi<true, 0>), dim3(grid), dim3(256), 0, h0->stream, M);
+            else if (mixed)
+                hipLaunchKernelGGL((k_sw_stage_multi<true, 1>), dim3(grid), dim3(256), 0, h0->stream, M);
+            else if (menu357)
+                hipLaunchKernelGGL((k_sw_stage_multi<false, 0>), dim3(grid), dim3(256), 0, h0->stream, M);
+            else
+                hipLaunchKernelGGL((k_sw_stage_multi<false, 1>), dim3(grid), dim3(256), 0, h0->stream, M);
+            e = hipGetLastError();
+        }
+    if (e == hipSuccess && elapsed_ms) e = hipEventRecord(h0->ev1, h0->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(h0->stream);
+    SELF_HIP(e);
+    if (elapsed_ms) {
+        float ms = 0;
+        SELF_HIP(hipEventElapsedTime(&ms, h0->ev0, h0->ev1));
+        *elapsed_ms = ms;
+    }
+    if ((3 * nSteps) % 2)
+        for (int k = 0; k < nModels; ++k) models[k]->trCur = 1 - models[k]->trCur;
+    return SELF_OK;
+}
+SELF_CATCH_ALL
 
 int self_sw2d_set_graph(self_sw2d_t h, int enable)
 {

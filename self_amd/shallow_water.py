@@ -271,3 +271,14 @@ class ShallowWater:
         b = C.c_size_t()
         L.check(L.lib().self_sw2d_device_bytes(self._h, C.byref(b)))
         return b.value
+
+
+def forward_step_rk3_multi(models, nSteps=1, timed=False):
+    """ForwardStepRK3 nSteps times for several fused-form ShallowWater models
+    of any degrees (one precision), each RK stage of all of them in one
+    persistent launch.  Returns the device milliseconds when timed."""
+    import ctypes as C
+    arr = (C.c_void_p * len(models))(*[m._h for m in models])
+    ms = C.c_double()
+    L.check(L.lib().self_sw2d_forward_step_rk3_multi(arr, len(models), nSteps, C.byref(ms) if timed else None))
+    return ms.value if timed else None
