@@ -304,9 +304,12 @@ int self_sw2d_set_member_solution(self_sw2d_t h, int member, const double *data)
  * g_m * dt_k, the same product a separate run forms. */
 int self_sw2d_set_member_time_step(self_sw2d_t h, int member, double dt);
 int self_sw2d_get_member_solution(self_sw2d_t h, int member, double *data);
-/* Autotuner: times nSteps RK3 steps with each setting on a copy of the
- * current state (the state is restored), keeps the faster one, returns it
- * and its time per step. */
+/* Autotuner: element groups per workgroup (1, 2) x graph replay (off, on).
+ * Takes the process-wide table's entry for this degree, arithmetic and size
+ * class when there is one; otherwise times nSteps RK3 steps with each
+ * setting on a copy of the current state (the state is restored), keeps
+ * the fastest, records it in the table, and returns its groups and time
+ * per step. */
 int self_sw2d_autotune(self_sw2d_t h, int nSteps, int *groups, double *ms_per_step);
 /* Bytes of HBM state the handle holds. */
 int self_sw2d_device_bytes(self_sw2d_t h, size_t *bytes);
@@ -452,6 +455,21 @@ int self_hexmesh_structured(int nX, int nY, int nZ, int N, int periodicX, int pe
                             double xScale, double yScale, double zScale, double warp, const int *warpSign,
                             const double *nodes, int *faces, double *x, double *y, double *z, double *cov,
                             double *J, double *nHat, double *nLen, double *bpos);
+
+/* ---------------------------------------------------------- autotuning
+ * The process-wide table of tuned launch configurations, keyed by model
+ * kind ("sw2d", "sw2d_ensemble"), dimension, degree N, arithmetic (0 FP64,
+ * 1 mixed) and size class floor(log2(elements)); entries hold four
+ * configuration integers and the time per step that chose them.  No
+ * reference counterpart. */
+int self_tune_count(int *n);
+/* entry index: kind (NUL-terminated, kindLen bytes), key4 = dim, N, dtype,
+ * sizeClass; cfg4; ms per step */
+int self_tune_entry(int index, char *kind, int kindLen, int *key4, int *cfg4, double *ms);
+int self_tune_clear(void);
+/* text file, one entry a line: kind dim N dtype sizeClass c0 c1 c2 c3 ms */
+int self_tune_save(const char *path);
+int self_tune_load(const char *path);
 
 /* ---------------------------------------------------- 3-D advection (DGSEM)
  * Replaces the Advection type of src/highend/advection3d/Advection3DClass.f90

@@ -215,6 +215,11 @@ _SIGS = {
     "self_sw2d_get_member_solution": ([C.c_void_p, C.c_int, dp], C.c_int),
     "self_sw2d_set_member_time_step": ([C.c_void_p, C.c_int, C.c_double], C.c_int),
     "self_sw2d_autotune": ([C.c_void_p, C.c_int, ip, dp], C.c_int),
+    "self_tune_count": ([ip], C.c_int),
+    "self_tune_entry": ([C.c_int, C.c_char_p, C.c_int, ip, ip, dp], C.c_int),
+    "self_tune_clear": ([], C.c_int),
+    "self_tune_save": ([C.c_char_p], C.c_int),
+    "self_tune_load": ([C.c_char_p], C.c_int),
     "self_serve_create": ([C.c_int, C.c_double, C.POINTER(C.c_void_p)], C.c_int),
     "self_serve_submit": ([C.c_void_p, C.POINTER(Request), C.POINTER(C.c_longlong)], C.c_int),
     "self_serve_wait": ([C.c_void_p, C.c_longlong, C.POINTER(Result)], C.c_int),

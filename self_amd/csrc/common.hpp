@@ -16,6 +16,12 @@ namespace self {
 void set_error(const char *fmt, ...);
 const char *last_error();
 
+// Autotuning table (tuning.cpp): size class of a mesh, lookup and record of
+// a configuration by (kind, dimension, degree, arithmetic, size class).
+int tune_size_class(size_t nElems);
+bool tune_lookup(const char *kind, int dim, int N, int dtype, int sizeClass, int cfg[4], double *ms);
+void tune_record(const char *kind, int dim, int N, int dtype, int sizeClass, const int cfg[4], double ms);
+
 // Resolves the device address of the no-allocation entry points' scratch
 // array on `device` (current); noalloc.hip.
 int noalloc_prepare(int device);

@@ -1886,6 +1886,19 @@ try {
         *groups = 1;
         return SELF_OK;
     }
+    // the table holds the choice for this degree, arithmetic and size
+    // class once any model of that key has been tuned
+    const char *kind = h->members > 1 ? "sw2d_ensemble" : "sw2d";
+    const int dtype = h->d.precision == SELF_MIXED ? 1 : 0, sc = tune_size_class(h->nE * (size_t)h->members);
+    int cfg[4];
+    double known = 0.0;
+    if (tune_lookup(kind, 2, h->d.N, dtype, sc, cfg, &known)) {
+        h->groups = cfg[0];
+        h->useGraph = cfg[1] != 0;
+        *groups = cfg[0];
+        if (ms_per_step) *ms_per_step = known;
+        return SELF_OK;
+    }
     // the trials advance a copy of the state: save and restore everything a step writes
     const size_t M = (size_t)h->members;
     const size_t nq = M * 3 * h->nE * h->np, nt = M * h->nE * 12 * h->n1;
@@ -1907,14 +1920,18 @@ try {
     if (e == hipSuccess) e = hipMemcpyAsync(st0, h->tr[0], sizeof(double) * nt, hipMemcpyDeviceToDevice, h->stream);
     if (e == hipSuccess) e = hipMemcpyAsync(st1, h->tr[1], sizeof(double) * nt, hipMemcpyDeviceToDevice, h->stream);
     const int cur = h->trCur, saved = h->groups;
+    const bool savedGraph = h->useGraph;
     double best = 1e300;
-    int bestG = 1;
-    for (int g = 1; g <= 2 && e == hipSuccess && !rc; ++g) {
+    int bestG = 1, bestGraph = 1;
+    // candidates: element groups per workgroup x graph replay
+    for (int trial = 0; trial < 4 && e == hipSuccess && !rc; ++trial) {
+        const int g = 1 + (trial >> 1), gr = trial & 1;
         h->groups = g;
+        h->useGraph = gr != 0;
         double ms = 0.0;
-        rc = self_sw2d_forward_step_rk3(h, 0.0, 1);  // warm
+        rc = self_sw2d_forward_step_rk3(h, 0.0, 2);  // warm (captures the graph when on)
         if (!rc) rc = self_sw2d_forward_step_rk3_timed(h, 0.0, nSteps, &ms);
-        if (!rc && ms < best) { best = ms; bestG = g; }
+        if (!rc && ms < best) { best = ms; bestG = g; bestGraph = gr; }
         e = hipMemcpyAsync(h->q, sq, sizeof(double) * nq, hipMemcpyDeviceToDevice, h->stream);
         if (e == hipSuccess) e = hipMemcpyAsync(G, sg, gb, hipMemcpyDeviceToDevice, h->stream);
         if (e == hipSuccess) e = hipMemcpyAsync(h->tr[0], st0, sizeof(double) * nt, hipMemcpyDeviceToDevice, h->stream);
@@ -1924,11 +1941,17 @@ try {
     }
     if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
     (void)hipFree(sq); (void)hipFree(sg); (void)hipFree(st0); (void)hipFree(st1);
-    if (e != hipSuccess) { h->groups = saved; return hip_fail(e, "self_sw2d_autotune", __FILE__, __LINE__); }
-    if (rc) { h->groups = saved; return rc; }
+    if (e != hipSuccess || rc) {
+        h->groups = saved;
+        h->useGraph = savedGraph;
+        return rc ? rc : hip_fail(e, "self_sw2d_autotune", __FILE__, __LINE__);
+    }
     h->groups = bestG;
+    h->useGraph = bestGraph != 0;
     *groups = bestG;
     if (ms_per_step) *ms_per_step = best / nSteps;
+    const int chosen[4] = {bestG, bestGraph, 0, 0};
+    tune_record(kind, 2, h->d.N, dtype, sc, chosen, best / nSteps);
     return SELF_OK;
 }
 SELF_CATCH_ALL

@@ -258,8 +258,10 @@ class ShallowWater:
         return out
 
     def Autotune(self, nSteps=50):
-        """Time each fused-kernel setting on a copy of the state; keeps the
-        faster.  Returns (groups, ms_per_step)."""
+        """Element groups per workgroup x graph replay: the process-wide
+        table's choice for this degree, arithmetic and size class, or the
+        fastest setting timed on a copy of the state (then recorded).
+        Returns (groups, ms_per_step)."""
         g, ms = C.c_int(), C.c_double()
         L.check(L.lib().self_sw2d_autotune(self._h, nSteps, C.byref(g), C.byref(ms)))
         return g.value, ms.value
@@ -282,3 +284,32 @@ def forward_step_rk3_multi(models, nSteps=1, timed=False):
     ms = C.c_double()
     L.check(L.lib().self_sw2d_forward_step_rk3_multi(arr, len(models), nSteps, C.byref(ms) if timed else None))
     return ms.value if timed else None
+
+
+def tuning_table():
+    """The process-wide autotuning table: a list of dicts (kind, dim, N,
+    dtype, sizeClass, config, ms)."""
+    n = C.c_int()
+    L.check(L.lib().self_tune_count(C.byref(n)))
+    out = []
+    for i in range(n.value):
+        kind = C.create_string_buffer(64)
+        key = (C.c_int * 4)()
+        cfg = (C.c_int * 4)()
+        ms = C.c_double()
+        L.check(L.lib().self_tune_entry(i, kind, 64, key, cfg, C.byref(ms)))
+        out.append({"kind": kind.value.decode(), "dim": key[0], "N": key[1], "dtype": key[2], "sizeClass": key[3],
+                    "config": list(cfg), "ms": ms.value})
+    return out
+
+
+def save_tuning_table(path):
+    L.check(L.lib().self_tune_save(os.fsencode(path)))
+
+
+def load_tuning_table(path):
+    L.check(L.lib().self_tune_load(os.fsencode(path)))
+
+
+def clear_tuning_table():
+    L.check(L.lib().self_tune_clear())

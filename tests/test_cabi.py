@@ -108,3 +108,27 @@ def test_noalloc_coarse_to_fine_rejects_oversized_calls_before_touching_the_devi
     rc = L.lib().self_coarse_to_fine_2d_noalloc(7, 7, nE, L.ptr(Ts), L.ptr(Ts), 10, 10, L.ptr(f), L.ptr(out))
     assert rc == -5 and b"scratch" in L.lib().self_last_error()
     assert L.lib().self_coarse_to_fine_2d_noalloc(7, 7, 1, None, None, 10, 10, None, None) == -5
+
+
+def test_tuning_table_file_round_trip_on_the_host(tmp_path):
+    """The autotuning table's text format: load, list, save, reload (no
+    device work)."""
+    from self_amd import shallow_water as SW
+    p = tmp_path / "t.txt"
+    p.write_text("sw2d 2 7 1 12 1 0 0 0 0.0123\nsw2d_ensemble 2 5 0 9 2 1 0 0 0.5\n")
+    SW.clear_tuning_table()
+    SW.load_tuning_table(str(p))
+    t = SW.tuning_table()
+    assert [(e["kind"], e["N"], e["dtype"], e["sizeClass"], e["config"][:2]) for e in t] == [
+        ("sw2d", 7, 1, 12, [1, 0]), ("sw2d_ensemble", 5, 0, 9, [2, 1])]
+    q = tmp_path / "u.txt"
+    SW.save_tuning_table(str(q))
+    SW.clear_tuning_table()
+    SW.load_tuning_table(str(q))
+    assert SW.tuning_table() == t
+    bad = tmp_path / "bad.txt"
+    bad.write_text("sw2d 2 7\n")
+    from self_amd import _lib as L
+    with pytest.raises(L.SelfError):
+        SW.load_tuning_table(str(bad))
+    SW.clear_tuning_table()

@@ -273,6 +273,34 @@ def test_autotune_keeps_the_state(gpu_lib):
     ref.Trash()
 
 
+def test_autotune_table_is_shared_saved_and_loaded(gpu_lib, tmp_path):
+    """A second model of the same degree, arithmetic and size class takes
+    the first one's tuned setting from the table without timing; the table
+    survives a save / clear / load round trip."""
+    from self_amd import shallow_water as SW
+    from self_amd.workloads import sw2d_geostrophic
+    SW.clear_tuning_table()
+    a, _ = sw2d_geostrophic(16, 5, precision="fp64", form="fused")
+    ga, msa = a.Autotune(4)
+    t = SW.tuning_table()
+    assert len(t) == 1 and t[0]["kind"] == "sw2d" and t[0]["N"] == 5 and t[0]["dtype"] == 0
+    assert t[0]["config"][0] == ga and t[0]["ms"] == msa
+    b, _ = sw2d_geostrophic(16, 5, precision="fp64", form="fused")
+    gb, msb = b.Autotune(4)
+    assert (gb, msb) == (ga, msa)  # from the table, not timed again
+    path = str(tmp_path / "tune.txt")
+    SW.save_tuning_table(path)
+    SW.clear_tuning_table()
+    assert SW.tuning_table() == []
+    SW.load_tuning_table(path)
+    assert SW.tuning_table() == t
+    b.ForwardStepRK3(0.0, 3)
+    a.ForwardStepRK3(0.0, 3)
+    assert np.array_equal(a.GetSolution(), b.GetSolution())
+    a.Trash()
+    b.Trash()
+
+
 @pytest.mark.parametrize("precision", ["fp64", "mixed"])
 @pytest.mark.parametrize("groups", [1, 2])
 def test_ensemble_members_equal_single_runs(gpu_lib, precision, groups):
