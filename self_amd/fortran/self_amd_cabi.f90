@@ -61,6 +61,24 @@ MODULE SELF_AMD_CABI
       REAL(C_DOUBLE), INTENT(in) :: x(*), bw(*)
       REAL(C_DOUBLE), INTENT(out) :: D(*)
     END FUNCTION
+    INTEGER(C_INT) FUNCTION self_barycentric_weights(N, x, bw) BIND(C, name="self_barycentric_weights")
+      IMPORT :: C_INT, C_DOUBLE
+      INTEGER(C_INT), VALUE :: N
+      REAL(C_DOUBLE) :: x(*), bw(*)
+    END FUNCTION
+    INTEGER(C_INT) FUNCTION self_interpolation_matrix(N, x, bw, nNew, zNew, T) &
+        BIND(C, name="self_interpolation_matrix")
+      IMPORT :: C_INT, C_DOUBLE
+      INTEGER(C_INT), VALUE :: N, nNew
+      REAL(C_DOUBLE) :: x(*), bw(*), zNew(*), T(*)
+    END FUNCTION
+    INTEGER(C_INT) FUNCTION self_coarse_to_fine_2d(nS, nP, nElems, Ts, Tp, nNewS, nNewP, fOld, fNew) &
+        BIND(C, name="self_coarse_to_fine_2d")
+      IMPORT :: C_INT, C_DOUBLE, C_SIZE_T
+      INTEGER(C_INT), VALUE :: nS, nP, nNewS, nNewP
+      INTEGER(C_SIZE_T), VALUE :: nElems
+      REAL(C_DOUBLE) :: Ts(*), Tp(*), fOld(*), fNew(*)
+    END FUNCTION
     INTEGER(C_INT) FUNCTION self_nodal2d_build(N, quadType, form, quirk, nodes, qWeight, dMatS, dMatP, &
                                                lagWest, lagEast, lagSouth, lagNorth) &
                                                BIND(C, name="self_nodal2d_build")

@@ -35,6 +35,8 @@ MODULE ShallowWaterAMD_Class
     TYPE(SWParamsAMD) :: params
     TYPE(C_PTR)       :: handle = C_NULL_PTR
     REAL(C_DOUBLE), ALLOCATABLE :: nodes(:), qWeight(:), x(:,:,:), y(:,:,:)
+    ! covariant metric terms and Jacobian at the nodes (the mesh's, kept for output)
+    REAL(C_DOUBLE), ALLOCATABLE :: dxds(:,:,:), dxdp(:,:,:), dyds(:,:,:), dydp(:,:,:), J(:,:,:)
     REAL(C_DOUBLE), ALLOCATABLE :: solution(:,:,:,:), tendency(:,:,:,:), bathymetry(:,:,:,:), vorticity(:,:,:)
     LOGICAL :: hostSolutionNewer = .FALSE., deviceSolutionNewer = .FALSE.
     LOGICAL :: hostBathymetryNewer = .FALSE., hostVorticityNewer = .FALSE.
@@ -116,6 +118,11 @@ CONTAINS
                                         lW, lE, lS, lN), 'self_sw2d_set_nodal')
     CALL self_check(self_sw2d_set_geometry(myDGSEM % handle, edges, dxds, dxdp, dyds, dydp, J, nHat, nLen), &
                     'self_sw2d_set_geometry')
+    myDGSEM % dxds = dxds
+    myDGSEM % dxdp = dxdp
+    myDGSEM % dyds = dyds
+    myDGSEM % dydp = dydp
+    myDGSEM % J = J
     ALLOCATE(myDGSEM % solution(0:N,0:N,3,nE), myDGSEM % tendency(0:N,0:N,3,nE))
     ALLOCATE(myDGSEM % bathymetry(0:N,0:N,3,nE), myDGSEM % vorticity(0:N,0:N,nE))
     ! Build_ShallowWater defaults: depth hDefault = 1 with zero gradient,
@@ -149,6 +156,8 @@ CONTAINS
     IF (ALLOCATED(myDGSEM % solution)) DEALLOCATE(myDGSEM % solution, myDGSEM % tendency, &
                                                   myDGSEM % bathymetry, myDGSEM % vorticity, myDGSEM % nodes, &
                                                   myDGSEM % qWeight, myDGSEM % x, myDGSEM % y)
+    IF (ALLOCATED(myDGSEM % J)) DEALLOCATE(myDGSEM % dxds, myDGSEM % dxdp, myDGSEM % dyds, myDGSEM % dydp, &
+                                           myDGSEM % J)
   END SUBROUTINE Trash_ShallowWaterAMD
 
   ! Push host-side changes to the device before a device operation.
