@@ -506,7 +506,8 @@ def bench_serving(n=512, workers=8, steps=200):
                      "throughput_req_s": poi["throughput_req_s"], "p50_ms": poi["p50_ms"], "p99_ms": poi["p99_ms"],
                      "mean_ms": poi["mean_ms"], "sw_dof_stage_per_s": poi["sw_dof_stage_per_s"],
                      "sw_mean_ensemble": poi["sw_mean_ensemble"], "capacity_sw_mean_ensemble": sat["sw_mean_ensemble"],
-                     "ok": poi["ok"]}
+                     "sw_mean_models_per_launch": poi["sw_mean_models_per_launch"],
+                     "capacity_sw_mean_models_per_launch": sat["sw_mean_models_per_launch"], "ok": poi["ok"]}
     out["stream"] = f"{n} requests, 80% SW (8..64^2, N 3/5/7, {steps} RK3 steps) / 20% CGSEM (16..128^2, N 3/5/7)"
     out["workers"] = workers
     return out

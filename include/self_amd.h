@@ -665,6 +665,8 @@ typedef struct {
     double queue_ms, run_ms, latency_ms;  /* arrival -> start, start -> end, arrival -> end */
     size_t bytes;         /* device memory reserved for the request */
     int members;          /* SW: requests that ran in the same ensemble launch (1 = alone) */
+    int models;           /* SW: models (ensembles of any mesh and degree) advanced in the same
+                             persistent launches (1 = its ensemble alone) */
 } self_request_result;
 
 typedef struct self_server_s *self_server_t;

@@ -86,7 +86,7 @@ class Result(C.Structure):
     """self_request_result of include/self_amd.h"""
     _fields_ = [("status", C.c_int), ("ioerr", C.c_int), ("iterations", C.c_int), ("queue_ms", C.c_double),
                 ("run_ms", C.c_double), ("latency_ms", C.c_double), ("bytes", C.c_size_t),
-                ("members", C.c_int)]
+                ("members", C.c_int), ("models", C.c_int)]
 
 
 _SIGS = {

@@ -16,7 +16,12 @@
 //    queued requests that can share its ensemble (same mesh, degree,
 //    precision, steps, constants and bathymetry) join it, up to 8 members
 //    and as memory allows, and advance in one fused launch per RK stage,
-//    each with its own state and time step;
+//    each with its own state and time step; further queued shallow-water
+//    requests of the same precision and step count, on other meshes or
+//    degrees, form up to 7 more such groups, and all groups advance
+//    together in one persistent launch per RK stage
+//    (self_sw2d_forward_step_rk3_multi) -- for small models only (at most
+//    16384 nodes per variable), where the stages are launch-bound;
 //  * results are written to the caller's output array; wait() returns the
 //    request's status, queue time, run time, latency and ensemble size.
 // Every kernel of both models is deterministic (fixed reduction trees, no
@@ -162,6 +167,17 @@ bool sw_batchable(const Job &a, const Job &b)
 }
 
 constexpr int kMaxMembers = 8;  // ensemble size at which the fused stage stops gaining (C2: 6.4 us/member at 8)
+constexpr int kMaxModels = 8;   // groups per persistent launch (the launch's argument block holds 8)
+// models of at most this many nodes per variable join a persistent launch:
+// there the launch-bound stages of small models gain (2.1x for eight 16^2
+// models); larger models run faster in their own launches, side by side
+// on the workers' streams
+constexpr long kMaxGroupedNodes = 16384;
+
+bool sw_groupable(const self_request &r)
+{
+    return r.kind == SELF_REQ_SW2D && (long)r.nX * r.nX * (r.N + 1) * (r.N + 1) <= kMaxGroupedNodes;
+}
 
 // device bytes a request needs (state, geometry, work vectors), with margin
 size_t estimate_bytes(const self_request &r)
@@ -173,9 +189,9 @@ size_t estimate_bytes(const self_request &r)
     return (size_t)(1.25 * (double)(nodes * 8 * 22)) + (1u << 20);
 }
 
-// several batchable shallow-water requests as one ensemble: one fused launch
-// per RK stage for all of them, each member with its own state and dt
-int run_sw_batch(self_server_s *s, std::vector<std::shared_ptr<Job>> &jobs)
+// the model of one group of batchable shallow-water requests (one request:
+// a plain model; several: an ensemble, one member each)
+int sw_group_model(self_server_s *s, std::vector<std::shared_ptr<Job>> &jobs, self_sw2d_t *out)
 {
     const self_request &r = jobs[0]->req;
     const int M = (int)jobs.size();
@@ -197,42 +213,49 @@ int run_sw_batch(self_server_s *s, std::vector<std::shared_ptr<Job>> &jobs)
     if (!rc) rc = self_sw2d_set_field(h, SELF_SW_BATHYMETRY, r.in1);
     if (!rc) rc = self_sw2d_set_field(h, SELF_SW_PLANETARY_VORTICITY, plv.data());
     if (!rc) rc = self_sw2d_calculate_bathymetry_gradient(h);
-    if (!rc) rc = self_sw2d_set_members(h, M);
-    for (int k = 0; k < M && !rc; ++k) {
-        rc = self_sw2d_set_member_solution(h, k, jobs[k]->req.in0);
-        if (!rc) rc = self_sw2d_set_member_time_step(h, k, jobs[k]->req.dt);
+    if (M == 1) {
+        if (!rc) rc = self_sw2d_set_field(h, SELF_SW_SOLUTION, r.in0);
+    } else {
+        if (!rc) rc = self_sw2d_set_members(h, M);
+        for (int k = 0; k < M && !rc; ++k) {
+            rc = self_sw2d_set_member_solution(h, k, jobs[k]->req.in0);
+            if (!rc) rc = self_sw2d_set_member_time_step(h, k, jobs[k]->req.dt);
+        }
     }
-    if (!rc) rc = self_sw2d_forward_step_rk3(h, 0.0, r.steps);
-    for (int k = 0; k < M && !rc; ++k) rc = self_sw2d_get_member_solution(h, k, jobs[k]->req.out);
-    self_sw2d_destroy(h);
+    if (rc) {
+        self_sw2d_destroy(h);
+        return rc;
+    }
+    *out = h;
+    return SELF_OK;
+}
+
+int sw_group_output(self_sw2d_t h, std::vector<std::shared_ptr<Job>> &jobs)
+{
+    int rc = SELF_OK;
+    if (jobs.size() == 1) return self_sw2d_get_field(h, SELF_SW_SOLUTION, jobs[0]->req.out);
+    for (size_t k = 0; k < jobs.size() && !rc; ++k) rc = self_sw2d_get_member_solution(h, (int)k, jobs[k]->req.out);
     return rc;
 }
 
-int run_sw(self_server_s *s, Job &j)
+// groups of shallow-water requests (each an ensemble of batchable requests)
+// advanced together: one group steps alone, several in one persistent
+// launch per RK stage
+int run_sw_groups(self_server_s *s, std::vector<std::vector<std::shared_ptr<Job>>> &groups)
 {
-    const self_request &r = j.req;
-    auto m = mesh_for(s, SELF_REQ_SW2D, r.nX, r.N, r.L);
-    SELF_REQUIRE(m, SELF_ERR_OTHER, "request %lld: mesh set-up failed", j.id);
-    self_sw2d_desc d{r.N, m->nE, m->nEdges, r.precision, SELF_FUSED, r.g, r.dt, 0.0};
-    self_sw2d_t h = nullptr;
-    int rc = self_sw2d_create(&d, &h);
-    if (rc) return rc;
-    // the workers run their models side by side and allocate while others
-    // run: stages are launched from the host, not captured as graphs
-    rc = self_sw2d_set_graph(h, 0);
-    const size_t nv = (size_t)m->nE * (r.N + 1) * (r.N + 1);
-    std::vector<double> plv(nv, r.f0);
-    if (!rc) rc = self_sw2d_set_nodal(h, m->dS.data(), m->dP.data(), m->w.data(), m->w.data(), m->lW.data(),
-                                      m->lE.data(), m->lS.data(), m->lN.data());
-    if (!rc) rc = self_sw2d_set_geometry(h, m->edges.data(), m->dxds.data(), m->dxdp.data(), m->dyds.data(),
-                                         m->dydp.data(), m->J.data(), m->nHat.data(), m->nLen.data());
-    if (!rc) rc = self_sw2d_set_field(h, SELF_SW_BATHYMETRY, r.in1);
-    if (!rc) rc = self_sw2d_set_field(h, SELF_SW_PLANETARY_VORTICITY, plv.data());
-    if (!rc) rc = self_sw2d_calculate_bathymetry_gradient(h);
-    if (!rc) rc = self_sw2d_set_field(h, SELF_SW_SOLUTION, r.in0);
-    if (!rc) rc = self_sw2d_forward_step_rk3(h, 0.0, r.steps);
-    if (!rc) rc = self_sw2d_get_field(h, SELF_SW_SOLUTION, r.out);
-    self_sw2d_destroy(h);
+    std::vector<self_sw2d_t> hs;
+    int rc = SELF_OK;
+    for (auto &g : groups) {
+        self_sw2d_t h = nullptr;
+        rc = sw_group_model(s, g, &h);
+        if (rc) break;
+        hs.push_back(h);
+    }
+    const int steps = groups[0][0]->req.steps;
+    if (!rc) rc = hs.size() == 1 ? self_sw2d_forward_step_rk3(hs[0], 0.0, steps)
+                                 : self_sw2d_forward_step_rk3_multi(hs.data(), (int)hs.size(), steps, nullptr);
+    for (size_t k = 0; k < hs.size() && !rc; ++k) rc = sw_group_output(hs[k], groups[k]);
+    for (auto h : hs) self_sw2d_destroy(h);
     return rc;
 }
 
@@ -276,6 +299,7 @@ void worker_main(self_server_s *s)
     (void)hipSetDevice(s->device);
     for (;;) {
         std::vector<std::shared_ptr<Job>> batch;
+        std::vector<std::vector<std::shared_ptr<Job>>> groups;
         {
             std::unique_lock<std::mutex> g(s->mu);
             for (;;) {
@@ -296,38 +320,66 @@ void worker_main(self_server_s *s)
             size_t bytes = batch[0]->bytes;
             // continuous batching: queued shallow-water requests that can run
             // as members of the same ensemble join it while memory allows
-            for (auto it = s->queue.begin(); it != s->queue.end() && (int)batch.size() < kMaxMembers;) {
-                if (sw_batchable(*batch[0], **it) && s->reserved + bytes + (*it)->bytes <= s->budget) {
-                    bytes += (*it)->bytes;
-                    batch.push_back(*it);
-                    it = s->queue.erase(it);
-                } else {
-                    ++it;
+            auto pull_members = [&](std::vector<std::shared_ptr<Job>> &grp) {
+                for (auto it = s->queue.begin(); it != s->queue.end() && (int)grp.size() < kMaxMembers;) {
+                    if (sw_batchable(*grp[0], **it) && s->reserved + bytes + (*it)->bytes <= s->budget) {
+                        bytes += (*it)->bytes;
+                        grp.push_back(*it);
+                        it = s->queue.erase(it);
+                    } else {
+                        ++it;
+                    }
+                }
+            };
+            pull_members(batch);
+            groups.push_back(std::move(batch));
+            // more groups of the same precision and step count (other meshes
+            // or degrees) join the same persistent launches
+            const self_request &r0 = groups[0][0]->req;
+            if (sw_groupable(r0)) {
+                bool took = true;
+                while (took && (int)groups.size() < kMaxModels) {
+                    took = false;
+                    for (auto it = s->queue.begin(); it != s->queue.end(); ++it) {
+                        const self_request &c = (*it)->req;
+                        if (sw_groupable(c) && c.precision == r0.precision && c.steps == r0.steps &&
+                            s->reserved + bytes + (*it)->bytes <= s->budget) {
+                            std::vector<std::shared_ptr<Job>> grp{*it};
+                            bytes += (*it)->bytes;
+                            s->queue.erase(it);
+                            pull_members(grp);
+                            groups.push_back(std::move(grp));
+                            took = true;
+                            break;
+                        }
+                    }
                 }
             }
             s->reserved += bytes;
-            s->running += (int)batch.size();
+            for (auto &g : groups) s->running += (int)g.size();
         }
         const Clock::time_point start = Clock::now();
         int rc;
-        if (batch.size() > 1) rc = run_sw_batch(s, batch);
-        else rc = batch[0]->req.kind == SELF_REQ_SW2D ? run_sw(s, *batch[0]) : run_cgsem(s, *batch[0]);
+        if (groups[0][0]->req.kind == SELF_REQ_SW2D) rc = run_sw_groups(s, groups);
+        else rc = run_cgsem(s, *groups[0][0]);
         const Clock::time_point end = Clock::now();
         {
             std::lock_guard<std::mutex> g(s->mu);
-            for (auto &j : batch) {
-                j->start = start;
-                j->end = end;
-                j->res.status = rc;
-                j->res.queue_ms = ms_between(j->arrive, j->start);
-                j->res.run_ms = ms_between(j->start, j->end);
-                j->res.latency_ms = ms_between(j->arrive, j->end);
-                j->res.bytes = j->bytes;
-                j->res.members = (int)batch.size();
-                j->done = true;
-                s->reserved -= j->bytes;
-                s->running--;
-            }
+            for (auto &grp : groups)
+                for (auto &j : grp) {
+                    j->start = start;
+                    j->end = end;
+                    j->res.status = rc;
+                    j->res.queue_ms = ms_between(j->arrive, j->start);
+                    j->res.run_ms = ms_between(j->start, j->end);
+                    j->res.latency_ms = ms_between(j->arrive, j->end);
+                    j->res.bytes = j->bytes;
+                    j->res.members = (int)grp.size();
+                    j->res.models = (int)groups.size();
+                    j->done = true;
+                    s->reserved -= j->bytes;
+                    s->running--;
+                }
         }
         s->cvDone.notify_all();
         s->cvWork.notify_all();

@@ -174,6 +174,8 @@ def run_stream(server: Server, specs, rate=None, seed=DEFAULT_SEED):
                "mean_ms": float(lat.mean()), "max_ms": float(lat.max()),
                "sw_dof_stage_per_s": sum(s.dofs * 3 * s.steps for s, _, _ in out if s.kind == REQ_SW2D) / wall,
                "sw_mean_ensemble": float(np.mean([r.members for s, r, _ in out if s.kind == REQ_SW2D] or [0])),
+               "sw_mean_models_per_launch": float(np.mean([r.models for s, r, _ in out if s.kind == REQ_SW2D]
+                                                          or [0])),
                "arrival_rate_req_s": rate}
     del dof_work
     return out, summary

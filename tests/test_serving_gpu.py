@@ -127,6 +127,32 @@ def test_batched_shallow_water_requests_equal_alone(gpu_lib):
     alone.close()
 
 
+def test_grouped_models_of_other_meshes_and_degrees_equal_alone(gpu_lib):
+    """Queued shallow-water requests of one precision and step count on
+    different meshes and degrees run as several models in the same
+    persistent launches (each model an ensemble of the requests that share
+    its mesh); each ends bitwise where it ends alone.  A request with another
+    step count is never grouped with them."""
+    from self_amd import serving as S
+    specs = []
+    for i, (nX, N) in enumerate([(8, 3), (16, 5), (8, 7), (16, 3), (8, 3), (32, 5), (8, 5), (16, 5), (8, 7)]):
+        specs.append(S.make_inputs(S.Spec(S.REQ_SW2D, nX, N, 1300 + i, "fp64", steps=10)))
+    odd = S.make_inputs(S.Spec(S.REQ_SW2D, 8, 3, 1399, "fp64", steps=7))
+    srv = S.Server(workers=1)
+    mix, summary = S.run_stream(srv, specs + [odd])
+    srv.close()
+    assert summary["ok"]
+    assert max(r.models for _, r, _ in mix) > 1
+    assert max(r.members for _, r, _ in mix) > 1
+    assert mix[-1][1].models == 1 and mix[-1][1].members == 1
+    alone = S.Server(workers=1)
+    for spec, res, out in mix:
+        r1, o1 = alone.wait(alone.submit(spec))
+        assert r1.status == 0 and r1.models == 1 and r1.members == 1
+        assert np.array_equal(o1, out), (spec.nX, spec.N, res.members, res.models)
+    alone.close()
+
+
 @pytest.mark.parametrize("precision", ["fp64", "mixed"])
 def test_full_request_stream(gpu_lib, precision):
     """The 512-request stream of the serving workload (80 % shallow water on
