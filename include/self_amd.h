@@ -609,8 +609,18 @@ int self_cgsem2d_apply_preconditioner(self_cgsem2d_t h, const double *b, double 
  * elapsed_ms (nullable): device time of the solve. */
 int self_cgsem2d_solve(self_cgsem2d_t h, int method, int precond, double tol, int maxIters, int mInner,
                        int ortho, int *ioerr, int *iterations, double *resi, int resiLen, double *elapsed_ms);
-/* Mixed precision: FP32 PCG inner solves to innerTol, FP64 residual and
- * iterative refinement until |b - A x| <= tol |b - A x0|. */
+/* Batched CGSEM PCG (the reference's loop, CGsemElliptic_2D_Class.f90:
+ * 1114-1207, run for many problems at once; no counterpart for the
+ * batching): problems[0..nProblems) -- distinct models on one device, each
+ * with its own mesh, degree, flux coefficient, source, Dirichlet values
+ * and initial guess -- are solved together, every CG iteration being two
+ * launches for all problems still iterating.  precond 0 identity, 1 Jacobi.
+ * Each problem's solution, ioerr[p] and iterations[p] are bitwise those of
+ * self_cgsem2d_solve(problems[p], SELF_KRYLOV_CGSEM_PCG, precond, tol,
+ * maxIters, ...).  Work is queued on problems[0]'s stream; elapsed_ms
+ * (nullable): device time of the whole batch. */
+int self_cgsem2d_solve_batched(self_cgsem2d_t *problems, int nProblems, int precond, double tol, int maxIters,
+                               int *ioerr, int *iterations, double *elapsed_ms);
 /* Restarted FGMRES(mInner) (IterativeSolvers_Class.f90:656-822 control
  * flow) whose operator and preconditioner applications run in FP32 while
  * the basis, Gram-Schmidt (ortho 0 modified, 1 classical twice), Givens
@@ -620,6 +630,8 @@ int self_cgsem2d_solve(self_cgsem2d_t h, int method, int precond, double tol, in
  * loop it follows. */
 int self_cgsem2d_solve_gmres_mixed(self_cgsem2d_t h, int precond, double tol, int maxCycles, int mInner, int ortho,
                                    int *ioerr, int *iterations, double *relResidual, double *elapsed_ms);
+/* Mixed precision: FP32 PCG inner solves to innerTol, FP64 residual and
+ * iterative refinement until |b - A x| <= tol |b - A x0|. */
 int self_cgsem2d_solve_mixed(self_cgsem2d_t h, int precond, double tol, double innerTol, int maxInner,
                              int maxOuter, int *ioerr, int *outerIters, int *innerIters, double *relResidual,
                              double *elapsed_ms);

@@ -142,6 +142,8 @@ _SIGS = {
     "self_cgsem2d_residual": ([C.c_void_p, dp], C.c_int),
     "self_cgsem2d_solve": ([C.c_void_p, C.c_int, C.c_int, C.c_double, C.c_int, C.c_int, C.c_int, ip, ip, dp,
                             C.c_int, dp], C.c_int),
+    "self_cgsem2d_solve_batched": ([C.POINTER(C.c_void_p), C.c_int, C.c_int, C.c_double, C.c_int, ip, ip, dp],
+                                   C.c_int),
     "self_cgsem2d_set_preconditioner": ([C.c_void_p, dp, dp, C.c_int, C.c_double, C.c_int], C.c_int),
     "self_cgsem2d_apply_preconditioner": ([C.c_void_p, dp, dp, ip], C.c_int),
     "self_cgsem2d_solve_gmres_mixed": ([C.c_void_p, C.c_int, C.c_double, C.c_int, C.c_int, C.c_int,

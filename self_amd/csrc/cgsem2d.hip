@@ -114,58 +114,65 @@ __global__ __launch_bounds__(256) void k_cg_local(const T *__restrict__ x, const
     if (act) Lbuf[lid] = cg_divergence<N>(L.D, f1 + le * np, f2 + le * np, i, j);
 }
 
-// The folded CG direction step of iteration `it` (grid-stride over groups
-// of epb elements, at most kRedBlocks workgroups).  When `pending`, the end
-// of iteration it-1 comes first: every workgroup sums the update sweep's
-// partial rows 1 and 2 itself (same order, same bits), workgroup 0 records
-// it in the state, and all exit if the loop stops; otherwise beta is in
-// the state.  The gathered value is the search direction formed on the
-// fly, v = x (iteration 1) or x + beta vOld; every copy also stores it to
-// vNew[g] (the copies of a DOF store the same bits).  After the element
-// operator, v.Av is accumulated as the sum over elements of v_e . A_e v_e
-// (Dirichlet copies hold v = 0), one partial per workgroup in row 0: the
-// same number as the dot product of v with the assembled A v, so the
-// assembly can be fused with the vector update that follows.
+// LDS of one direction-step workgroup
 template <int N, typename T>
-__global__ __launch_bounds__(256) void k_cg_direction(const T *__restrict__ x, const T *__restrict__ vOld,
-                                                      T *__restrict__ vNew, const int *__restrict__ gid,
-                                                      const T *__restrict__ G3, const T *__restrict__ Dg,
-                                                      T *__restrict__ Lbuf, int nE, SolverState *st, RedSlot slot,
-                                                      int it, int pending)
+struct CgDirLds {
+    CgLDS<N, T> L;
+    T us[Shape<N>::epb * Shape<N>::np], f1[Shape<N>::epb * Shape<N>::np], f2[Shape<N>::epb * Shape<N>::np];
+    double lds4[4];
+    double sFold[2];
+    int sDone;
+};
+
+// The folded CG direction step of iteration `it`, as workgroup b of nB
+// (grid-stride over groups of epb elements, nB <= kRedBlocks).  When
+// `pending`, the end of iteration it-1 comes first: every workgroup sums
+// the update sweep's partial rows 1 and 2 itself (same order, same bits),
+// workgroup 0 records it in the state, and all exit if the loop stops;
+// otherwise beta is in the state.  The gathered value is the search
+// direction formed on the fly, v = x (iteration 1) or x + beta vOld; every
+// copy also stores it to vNew[g] (the copies of a DOF store the same bits).
+// After the element operator, v.Av is accumulated as the sum over elements
+// of v_e . A_e v_e (Dirichlet copies hold v = 0), one partial per
+// workgroup in row 0 at entry b: the same number as the dot product of v
+// with the assembled A v, so the assembly can be fused with the vector
+// update that follows.
+template <int N, typename T>
+__device__ __forceinline__ void cg_direction_wg(const T *__restrict__ x, const T *__restrict__ vOld,
+                                                T *__restrict__ vNew, const int *__restrict__ gid,
+                                                const T *__restrict__ G3, const T *__restrict__ Dg,
+                                                T *__restrict__ Lbuf, int nE, SolverState *st, const RedSlot &slot,
+                                                int it, int pending, int b, int nB, CgDirLds<N, T> &sm)
 {
     using S = Shape<N>;
     constexpr int n1 = S::n1, np = S::np, epb = S::epb;
-    __shared__ CgLDS<N, T> L;
-    __shared__ T us[epb * np], f1[epb * np], f2[epb * np];
-    __shared__ double lds4[4];
     T beta;
     if (pending) {
         // workgroup 0 may set `done` during this kernel, so each workgroup
         // reads it once (thread 0) and decides as a whole
-        __shared__ double sFold[2];
-        __shared__ int sDone;
         if (threadIdx.x < 128) {  // wavefront q sums row 1 + q
             const int q = threadIdx.x >> 6;
             const double Sq = wave_fold_sum(slot.partials + (1 + q) * kRedBlocks);
-            if ((threadIdx.x & 63) == 0) sFold[q] = Sq;
+            if ((threadIdx.x & 63) == 0) sm.sFold[q] = Sq;
         }
-        if (threadIdx.x == 0) sDone = st->done;
+        if (threadIdx.x == 0) sm.sDone = st->done;
         __syncthreads();
-        if (sDone) return;
-        if (blockIdx.x == 0 && threadIdx.x == 0) cg_end_apply(st, it - 1, sFold[0], sFold[1]);
-        if (cg_stop(st, sqrt(sFold[0]))) return;  // uniform over the workgroup
-        beta = (T)(sFold[1] / st->numP[(it - 1) & 1]);
+        if (sm.sDone) return;
+        if (b == 0 && threadIdx.x == 0) cg_end_apply(st, it - 1, sm.sFold[0], sm.sFold[1]);
+        if (cg_stop(st, sqrt(sm.sFold[0]))) return;  // uniform over the workgroup
+        beta = (T)(sm.sFold[1] / st->numP[(it - 1) & 1]);
     } else {
         if (st->done) return;  // nothing writes it during this kernel
         beta = (T)st->beta;
     }
-    for (int k = threadIdx.x; k < n1 * n1; k += blockDim.x) L.D[k] = Dg[k];
+    for (int k = threadIdx.x; k < n1 * n1; k += blockDim.x) sm.L.D[k] = Dg[k];
     const int t = threadIdx.x, le = t / np, nd = t - le * np;
     const int i = nd % n1, j = nd / n1;
     const size_t stride = (size_t)nE * np;
     const int nGroups = (nE + epb - 1) / epb;
+    T *us = sm.us, *f1 = sm.f1, *f2 = sm.f2;
     double acc = 0.0;
-    for (int grp = blockIdx.x; grp < nGroups; grp += gridDim.x) {
+    for (int grp = b; grp < nGroups; grp += nB) {
         const int e = grp * epb + le;
         const bool act = le < epb && e < nE;
         const size_t lid = (size_t)e * np + nd;
@@ -183,38 +190,52 @@ __global__ __launch_bounds__(256) void k_cg_direction(const T *__restrict__ x, c
             g22 = G3[2 * stride + lid];
         }
         __syncthreads();
-        if (act) cg_flux<N>(L.D, us + le * np, i, j, g11, g12, g22, f1 + le * np, f2 + le * np);
+        if (act) cg_flux<N>(sm.L.D, us + le * np, i, j, g11, g12, g22, f1 + le * np, f2 + le * np);
         __syncthreads();
         if (act) {
-            const T Lv = cg_divergence<N>(L.D, f1 + le * np, f2 + le * np, i, j);
+            const T Lv = cg_divergence<N>(sm.L.D, f1 + le * np, f2 + le * np, i, j);
             Lbuf[lid] = Lv;
             acc += (double)u * (double)Lv;
         }
     }
-    store_partial(slot, 0, acc, lds4);
+    store_partial_at(slot, 0, acc, sm.lds4, b);
 }
 
-// The rest of the folded CG iteration in one sweep over the global DOFs:
-// alpha = num / v.Av (each workgroup sums the direction step's nPart
-// partials itself), z = A v assembled from the copies in GlobalSum order and
-// used at once: u += alpha v, r -= alpha z, w = P r; r.r and r.w partials in
-// rows 1 and 2.  z is never stored.
+template <int N, typename T>
+__global__ __launch_bounds__(256) void k_cg_direction(const T *__restrict__ x, const T *__restrict__ vOld,
+                                                      T *__restrict__ vNew, const int *__restrict__ gid,
+                                                      const T *__restrict__ G3, const T *__restrict__ Dg,
+                                                      T *__restrict__ Lbuf, int nE, SolverState *st, RedSlot slot,
+                                                      int it, int pending)
+{
+    __shared__ CgDirLds<N, T> sm;
+    cg_direction_wg<N, T>(x, vOld, vNew, gid, G3, Dg, Lbuf, nE, st, slot, it, pending, blockIdx.x, gridDim.x, sm);
+}
+
+// The rest of the folded CG iteration in one sweep over the global DOFs, as
+// workgroup b of kFoldParts (kFoldThreads threads each): alpha = num / v.Av
+// (each workgroup sums the direction step's nPart partials itself), z = A v
+// assembled from the copies in GlobalSum order and used at once:
+// u += alpha v, r -= alpha z, w = P r; r.r and r.w partials in rows 1 and 2
+// at entry b.  z is never stored.  A workgroup b >= ceil(nG / kFoldThreads)
+// has no DOF: its partials are zero.
 template <typename T, bool PRE>
-__global__ __launch_bounds__(kFoldThreads) void k_cg_assemble_update(
-    const T *__restrict__ Lbuf, const int *__restrict__ cs, const int *__restrict__ ci, size_t nG, T *__restrict__ u,
-    const T *__restrict__ v, T *__restrict__ r, const T *__restrict__ dinv, T *__restrict__ w, SolverState *st,
-    RedSlot slot, int it, int nPart)
+__device__ __forceinline__ void cg_update_wg(const T *__restrict__ Lbuf, const int *__restrict__ cs,
+                                             const int *__restrict__ ci, size_t nG, T *__restrict__ u,
+                                             const T *__restrict__ v, T *__restrict__ r, const T *__restrict__ dinv,
+                                             T *__restrict__ w, SolverState *st, const RedSlot &slot, int it,
+                                             int nPart, int b)
 {
     if (st->done) return;
     const double vz = block_fold_sum<kFoldThreads>(slot.partials, nPart);
     const double a = st->numP[it & 1] / vz;
-    if (blockIdx.x == 0 && threadIdx.x == 0) {
+    if (b == 0 && threadIdx.x == 0) {
         st->vz = vz;
         st->alpha = a;
     }
     const T alpha = (T)a;
     double acc0 = 0.0, acc1 = 0.0;
-    for (size_t g = (size_t)blockIdx.x * blockDim.x + threadIdx.x; g < nG; g += (size_t)gridDim.x * blockDim.x) {
+    for (size_t g = (size_t)b * kFoldThreads + threadIdx.x; g < nG; g += (size_t)kFoldParts * kFoldThreads) {
         const int c0 = cs[g], c1 = cs[g + 1];
         T z = Lbuf[ci[c0]];
         for (int c = c0 + 1; c < c1; ++c) z = z + Lbuf[ci[c]];
@@ -229,7 +250,16 @@ __global__ __launch_bounds__(kFoldThreads) void k_cg_assemble_update(
         }
     }
     if constexpr (!PRE) acc1 = acc0;
-    store_partials2<kFoldThreads>(slot, 1, acc0, 2, acc1);
+    store_partials2_at<kFoldThreads>(slot, 1, acc0, 2, acc1, b);
+}
+
+template <typename T, bool PRE>
+__global__ __launch_bounds__(kFoldThreads) void k_cg_assemble_update(
+    const T *__restrict__ Lbuf, const int *__restrict__ cs, const int *__restrict__ ci, size_t nG, T *__restrict__ u,
+    const T *__restrict__ v, T *__restrict__ r, const T *__restrict__ dinv, T *__restrict__ w, SolverState *st,
+    RedSlot slot, int it, int nPart)
+{
+    cg_update_wg<T, PRE>(Lbuf, cs, ci, nG, u, v, r, dinv, w, st, slot, it, nPart, blockIdx.x);
 }
 
 // y[g] = sum over the copies of g (GlobalSum order); optional fused dot.
@@ -647,6 +677,105 @@ int device_geometry(CgsemOp &op)
     return SELF_OK;
 }
 
+// ------------------------------------------------------ batched CGSEM PCG
+// Many CGSEM problems (any mesh, degree, coefficient and source) solved
+// together: every CG iteration is two launches for all unfinished
+// problems.  Problem p owns nPart(p) = min(element groups, kRedBlocks)
+// direction workgroups and nUpd(p) = min(kFoldParts, ceil(nG / kFoldThreads))
+// update workgroups (the update workgroups of its single solve that have
+// DOFs; the others would only write zero partials, which stay zero in rows
+// 1 and 2 from the start).  A workgroup finds its problem and its index b
+// in that problem's grid from prefix sums over the problems still running,
+// and runs the single solve's workgroup body with that b: each problem's
+// arithmetic, reduction order included, is its single solve's, so the
+// results are bitwise those of self_cgsem2d_solve.
+struct CgBatchProb {
+    const int *gid, *cs, *ci;
+    const double *G3, *D, *dinv;  // dinv null: no preconditioner
+    double *Lbuf, *u, *r, *w, *vA, *vB;
+    SolverState *st;
+    RedSlot slot;
+    size_t nG;
+    int N, nE, nPart, nUpd;
+};
+
+union CgDirLdsAny {
+    CgDirLds<2, double> n2;
+    CgDirLds<3, double> n3;
+    CgDirLds<4, double> n4;
+    CgDirLds<5, double> n5;
+    CgDirLds<6, double> n6;
+    CgDirLds<7, double> n7;
+    CgDirLds<8, double> n8;
+    CgDirLds<9, double> n9;
+    CgDirLds<10, double> n10;
+    CgDirLds<11, double> n11;
+};
+
+// the k with start[k] <= wg < start[k + 1] (start[0] = 0, start[nAct] = grid)
+__device__ __forceinline__ int cgb_find(const int *start, int nAct, int wg)
+{
+    int lo = 0, hi = nAct;
+    while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (start[mid] <= wg) lo = mid;
+        else hi = mid;
+    }
+    return lo;
+}
+
+__global__ __launch_bounds__(256) void k_cgb_direction(const CgBatchProb *__restrict__ probs,
+                                                       const int *__restrict__ act, const int *__restrict__ start,
+                                                       int nAct, int it, int pending)
+{
+    __shared__ CgDirLdsAny sm;
+    const int k = cgb_find(start, nAct, blockIdx.x);
+    const CgBatchProb &P = probs[act[k]];
+    const int b = blockIdx.x - start[k];
+    const double *src = P.dinv ? P.w : P.r;
+    const double *vOld = (it & 1) ? P.vB : P.vA;
+    double *vNew = (it & 1) ? P.vA : P.vB;
+#define CGB_DIR(NN, F)                                                                                          \
+    case NN:                                                                                                    \
+        cg_direction_wg<NN, double>(src, vOld, vNew, P.gid, P.G3, P.D, P.Lbuf, P.nE, P.st, P.slot, it, pending, \
+                                    b, P.nPart, sm.F);                                                          \
+        break;
+    switch (P.N) {
+        CGB_DIR(2, n2) CGB_DIR(3, n3) CGB_DIR(4, n4) CGB_DIR(5, n5) CGB_DIR(6, n6)
+        CGB_DIR(7, n7) CGB_DIR(8, n8) CGB_DIR(9, n9) CGB_DIR(10, n10) CGB_DIR(11, n11)
+    default: break;
+    }
+#undef CGB_DIR
+}
+
+__global__ __launch_bounds__(kFoldThreads) void k_cgb_update(const CgBatchProb *__restrict__ probs,
+                                                             const int *__restrict__ act,
+                                                             const int *__restrict__ start, int nAct, int it)
+{
+    const int k = cgb_find(start, nAct, blockIdx.x);
+    const CgBatchProb &P = probs[act[k]];
+    const int b = blockIdx.x - start[k];
+    const double *v = (it & 1) ? P.vA : P.vB;
+    if (P.dinv)
+        cg_update_wg<double, true>(P.Lbuf, P.cs, P.ci, P.nG, P.u, v, P.r, P.dinv, P.w, P.st, P.slot, it, P.nPart, b);
+    else
+        cg_update_wg<double, false>(P.Lbuf, P.cs, P.ci, P.nG, P.u, v, P.r, P.dinv, P.w, P.st, P.slot, it, P.nPart,
+                                    b);
+}
+
+// one wavefront per running problem: the pending end of iteration it, then
+// its done flag for the host
+__global__ __launch_bounds__(64) void k_cgb_fold_end(const CgBatchProb *__restrict__ probs,
+                                                     const int *__restrict__ act, int it, int *__restrict__ flags)
+{
+    const CgBatchProb &P = probs[act[blockIdx.x]];
+    cg_fold_end_wave(P.st, P.slot, it);
+    if (threadIdx.x == 0) flags[blockIdx.x] = P.st->done;
+}
+
+constexpr int kCgbChunk = 32;        // iterations between reads of the done flags
+constexpr int kCgbMaxProblems = 4096;
+
 }  // namespace
 
 extern "C" {
@@ -934,6 +1063,128 @@ try {
     }
     *ioerr = res.ioerr;
     *iterations = res.iterations;
+    return SELF_OK;
+}
+SELF_CATCH_ALL
+
+int self_cgsem2d_solve_batched(self_cgsem2d_t *problems, int nProblems, int precond, double tol, int maxIters,
+                               int *ioerr, int *iterations, double *elapsed_ms)
+try {
+    SELF_REQUIRE(problems && ioerr && iterations, SELF_ERR_ARG, "null argument");
+    SELF_REQUIRE(nProblems >= 1 && nProblems <= kCgbMaxProblems, SELF_ERR_ARG, "nProblems = %d (1..%d)", nProblems,
+                 kCgbMaxProblems);
+    SELF_REQUIRE(maxIters >= 1 && tol > 0, SELF_ERR_ARG, "maxIters/tolerance");
+    SELF_REQUIRE(precond == 0 || precond == 1, SELF_ERR_ARG, "precond = %d (0 none, 1 Jacobi)", precond);
+    std::vector<self_cgsem2d_t> hs(problems, problems + nProblems);
+    for (int p = 0; p < nProblems; ++p) {
+        SELF_REQUIRE(hs[p], SELF_ERR_ARG, "problem %d is null", p);
+        SELF_REQUIRE(hs[p]->device == hs[0]->device, SELF_ERR_ARG, "problem %d is on another device", p);
+    }
+    {
+        std::vector<self_cgsem2d_t> sorted(hs);
+        std::sort(sorted.begin(), sorted.end());
+        SELF_REQUIRE(std::adjacent_find(sorted.begin(), sorted.end()) == sorted.end(), SELF_ERR_ARG,
+                     "a problem is listed twice");
+    }
+    SELF_HIP(hipSetDevice(hs[0]->device));
+    self_cgsem2d_s *h0 = hs[0];
+    const hipStream_t s = h0->op.stream;
+    for (auto h : hs) SELF_HIP(hipStreamSynchronize(h->op.stream));
+    // device tables: descriptors, then (per chunk) the running problems and
+    // the prefix sums of their direction and update workgroups, done flags
+    const size_t descBytes = sizeof(CgBatchProb) * nProblems;
+    const size_t tblInts = (size_t)3 * nProblems + 2 + nProblems;
+    char *tables = nullptr;
+    SELF_HIP(hipMalloc(&tables, descBytes + sizeof(int) * tblInts));
+    struct Release {
+        char *p;
+        ~Release() { (void)hipFree(p); }
+    } release{tables};
+    CgBatchProb *probsD = reinterpret_cast<CgBatchProb *>(tables);
+    int *tblD = reinterpret_cast<int *>(tables + descBytes);
+    int *flagsD = tblD + 3 * nProblems + 2;
+    SELF_HIP(hipEventRecord(h0->ev0, s));
+    std::vector<CgBatchProb> P(nProblems);
+    for (int p = 0; p < nProblems; ++p) {
+        self_cgsem2d_s *h = hs[p];
+        CgsemOp &op = h->op;
+        int rc = h->wk.ensure(op.n, 0, maxIters + 1, false);
+        if (!rc) rc = krylov_cg_begin(op, h->wk, precond == 1, kStopNormRel, tol, maxIters, op.xg, s);
+        if (rc) return rc;
+        // update partials of workgroups without DOFs: zero throughout
+        hipLaunchKernelGGL(k_fill_f64, dim3(8), dim3(256), 0, s, h->wk.slot.partials + kRedBlocks,
+                           (size_t)2 * kRedBlocks, 0.0);
+        CgBatchProb &q = P[p];
+        q.gid = op.gid;
+        q.cs = op.cs;
+        q.ci = op.ci;
+        q.G3 = op.G3;
+        q.D = op.D;
+        q.dinv = precond == 1 ? op.dinv : nullptr;
+        q.Lbuf = op.Lbuf;
+        q.u = h->wk.u;
+        q.r = h->wk.r;
+        q.w = h->wk.w;
+        q.vA = h->wk.v;
+        q.vB = h->wk.vB;
+        q.st = h->wk.st;
+        q.slot = h->wk.slot;
+        q.nG = op.nG;
+        q.N = op.N;
+        q.nE = (int)op.nE;
+        q.nPart = 0;
+        CG_DISPATCH_N(op.N, q.nPart = std::min(cg_grid<NN>(op.nE), kRedBlocks));
+        q.nUpd = (int)std::min<size_t>(kFoldParts, (op.nG + kFoldThreads - 1) / kFoldThreads);
+        SELF_REQUIRE(q.nPart > 0, SELF_ERR_ARG, "problem %d: N = %d", p, op.N);
+    }
+    SELF_HIP(hipGetLastError());
+    SELF_HIP(hipMemcpyAsync(probsD, P.data(), descBytes, hipMemcpyHostToDevice, s));
+    std::vector<int> act(nProblems), flags(nProblems);
+    for (int p = 0; p < nProblems; ++p) act[p] = p;
+    int nAct = nProblems;
+    for (int it0 = 1; it0 <= maxIters && nAct > 0; it0 += kCgbChunk) {
+        const int it1 = std::min(it0 + kCgbChunk - 1, maxIters);
+        // running list | direction prefix (nAct + 1) | update prefix (nAct + 1)
+        std::vector<int> tb((size_t)3 * nAct + 2);
+        int *dirS = tb.data() + nAct, *updS = dirS + nAct + 1;
+        dirS[0] = updS[0] = 0;
+        for (int k = 0; k < nAct; ++k) {
+            tb[k] = act[k];
+            dirS[k + 1] = dirS[k] + P[act[k]].nPart;
+            updS[k + 1] = updS[k] + P[act[k]].nUpd;
+        }
+        SELF_HIP(hipMemcpyAsync(tblD, tb.data(), sizeof(int) * tb.size(), hipMemcpyHostToDevice, s));
+        const int *actD = tblD, *dirD = tblD + nAct, *updD = dirD + nAct + 1;
+        for (int it = it0; it <= it1; ++it) {
+            hipLaunchKernelGGL(k_cgb_direction, dim3(dirS[nAct]), dim3(256), 0, s, (const CgBatchProb *)probsD, actD,
+                               dirD, nAct, it, (int)(it > it0));
+            hipLaunchKernelGGL(k_cgb_update, dim3(updS[nAct]), dim3(kFoldThreads), 0, s,
+                               (const CgBatchProb *)probsD, actD, updD, nAct, it);
+        }
+        hipLaunchKernelGGL(k_cgb_fold_end, dim3(nAct), dim3(64), 0, s, (const CgBatchProb *)probsD, actD, it1,
+                           flagsD);
+        SELF_HIP(hipGetLastError());
+        SELF_HIP(hipMemcpyAsync(flags.data(), flagsD, sizeof(int) * nAct, hipMemcpyDeviceToHost, s));
+        SELF_HIP(hipStreamSynchronize(s));
+        int m = 0;
+        for (int k = 0; k < nAct; ++k)
+            if (!flags[k]) act[m++] = act[k];
+        nAct = m;
+    }
+    for (int p = 0; p < nProblems; ++p) {
+        KrylovResult res;
+        const int rc = krylov_cg_finish(hs[p]->op, hs[p]->wk, kStopNormRel, tol, hs[p]->op.xg, res, s);
+        if (rc) return rc;
+        ioerr[p] = res.ioerr;
+        iterations[p] = res.iterations;
+    }
+    SELF_HIP(hipEventRecord(h0->ev1, s));
+    SELF_HIP(hipEventSynchronize(h0->ev1));
+    if (elapsed_ms) {
+        float ms = 0;
+        SELF_HIP(hipEventElapsedTime(&ms, h0->ev0, h0->ev1));
+        *elapsed_ms = ms;
+    }
     return SELF_OK;
 }
 SELF_CATCH_ALL

@@ -128,10 +128,7 @@ __global__ __launch_bounds__(kRedThreads) void k_update_urw(T *__restrict__ u, c
 // end of the last iteration of a chunk (one wavefront)
 __global__ __launch_bounds__(64) void k_cg_fold_end(SolverState *st, RedSlot slot, int it)
 {
-    if (st_done(st)) return;
-    const double S0 = wave_fold_sum(slot.partials + kRedBlocks);
-    const double S1 = wave_fold_sum(slot.partials + 2 * kRedBlocks);
-    if (threadIdx.x == 0) cg_end_apply(st, it, S0, S1);
+    cg_fold_end_wave(st, slot, it);
 }
 
 // ---------------------------------------------------------------- GMRES
@@ -532,9 +529,10 @@ int cg_chunk(DevOp &op, T *u, T *r, T *w, T *vA, T *vB, T *z, const T *dinv, siz
     return SELF_OK;
 }
 
-// CG / PCG of IterativeSolvers (stopRule sqrt-norm) and the CGSEM PCG (norm rule)
-int run_cg(DevOp &op, KrylovWork &wk, bool precond, int stopRule, double tol, int maxIters, double *sol,
-           KrylovResult &res, hipStream_t s)
+}  // namespace
+
+int krylov_cg_begin(DevOp &op, KrylovWork &wk, bool precond, int stopRule, double tol, int maxIters,
+                    const double *sol, hipStream_t s)
 {
     const size_t n = op.n;
     int rc = init_state(wk, tol, maxIters, stopRule, 0, s);
@@ -547,17 +545,17 @@ int run_cg(DevOp &op, KrylovWork &wk, bool precond, int stopRule, double tol, in
     // the update sweep of the iteration before
     SELF_RED_LAUNCH((k_precond_dot<double, false>), (k_precond_dot<double, true>), s, wk.r, dinv, wk.w, n, wk.st,
                        (int)kFinNum, wk.slot);
-    for (int it0 = 1; it0 <= maxIters; it0 += kChunk) {
-        if ((rc = cg_chunk<double>(op, wk.u, wk.r, wk.w, wk.v, wk.vB, wk.z, dinv, n, wk.st, wk.slot, it0,
-                                   std::min(it0 + kChunk - 1, maxIters), s)))
-            return rc;
-        SolverState h{};
-        if ((rc = read_state(wk, h, s))) return rc;
-        if (h.done) break;
-    }
+    SELF_HIP(hipGetLastError());
+    return SELF_OK;
+}
+
+int krylov_cg_finish(DevOp &op, KrylovWork &wk, int stopRule, double tol, double *sol, KrylovResult &res,
+                     hipStream_t s)
+{
     SolverState h{};
-    if ((rc = read_state(wk, h, s))) return rc;
-    SELF_HIP(hipMemcpyAsync(sol, wk.u, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
+    int rc = read_state(wk, h, s);
+    if (rc) return rc;
+    SELF_HIP(hipMemcpyAsync(sol, wk.u, sizeof(double) * op.n, hipMemcpyDeviceToDevice, s));
     res.r0 = h.r0;
     res.rNorm = h.rNorm;
     res.iterations = (h.done == 3) ? 0 : h.iter;
@@ -568,6 +566,26 @@ int run_cg(DevOp &op, KrylovWork &wk, bool precond, int stopRule, double tol, in
         if (h.rNorm / h.r0 > tol) res.ioerr = -1;  // CGsemElliptic_2D :1196-1200
     }
     return SELF_OK;
+}
+
+namespace {
+
+// CG / PCG of IterativeSolvers (stopRule sqrt-norm) and the CGSEM PCG (norm rule)
+int run_cg(DevOp &op, KrylovWork &wk, bool precond, int stopRule, double tol, int maxIters, double *sol,
+           KrylovResult &res, hipStream_t s)
+{
+    int rc = krylov_cg_begin(op, wk, precond, stopRule, tol, maxIters, sol, s);
+    if (rc) return rc;
+    const double *dinv = precond ? op.inv_diag() : nullptr;
+    for (int it0 = 1; it0 <= maxIters; it0 += kChunk) {
+        if ((rc = cg_chunk<double>(op, wk.u, wk.r, wk.w, wk.v, wk.vB, wk.z, dinv, op.n, wk.st, wk.slot, it0,
+                                   std::min(it0 + kChunk - 1, maxIters), s)))
+            return rc;
+        SolverState h{};
+        if ((rc = read_state(wk, h, s))) return rc;
+        if (h.done) break;
+    }
+    return krylov_cg_finish(op, wk, stopRule, tol, sol, res, s);
 }
 
 // CGSEM PCG (CGsemElliptic_2D_Class.f90:1114-1207) with the operator's own

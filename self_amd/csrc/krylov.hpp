@@ -150,6 +150,16 @@ __device__ inline void cg_end_apply(SolverState *st, int it, double S0, double S
     }
 }
 
+// The end of a chunk of folded CG iterations whose last is `it`: the
+// pending end of iteration it from partial rows 1 and 2 (one wavefront).
+__device__ __forceinline__ void cg_fold_end_wave(SolverState *st, const RedSlot &slot, int it)
+{
+    if (st->done) return;
+    const double S0 = wave_fold_sum(slot.partials + kRedBlocks);
+    const double S1 = wave_fold_sum(slot.partials + 2 * kRedBlocks);
+    if (threadIdx.x == 0) cg_end_apply(st, it, S0, S1);
+}
+
 // Operator plug-in (the deferred procedures of IterativeSolver).
 struct DevOp {
     size_t n = 0;                   // vector length the solver sees
@@ -247,6 +257,16 @@ int krylov_solve(DevOp &op, KrylovWork &wk, int method, int precond, double tol,
 // cycle is one step of FP64 iterative refinement with an FP32 inner solve.
 int krylov_solve_gmres_mixed(DevOp &op, KrylovWork &wk, int precond, double tol, int maxCycles, int m, int ortho,
                              double *sol, KrylovResult &res, hipStream_t s);
+
+// The CG / PCG of krylov_solve in parts, for drivers that interleave the
+// iterations of several solves (the batched CGSEM solve): the set-up
+// before iteration 1 (state, r = b - A x, u = x, w = P r and num = r.w;
+// the workspace must hold maxIters + 1 residuals) and, after the
+// iterations, the result and sol = u.  precond: Jacobi, else identity.
+int krylov_cg_begin(DevOp &op, KrylovWork &wk, bool precond, int stopRule, double tol, int maxIters,
+                    const double *sol, hipStream_t s);
+int krylov_cg_finish(DevOp &op, KrylovWork &wk, int stopRule, double tol, double *sol, KrylovResult &res,
+                     hipStream_t s);
 
 // Mixed precision: FP32 PCG inner solves, FP64 residual and iterative
 // refinement to |b - A x| / |b - A x0| <= tol.

@@ -44,17 +44,23 @@ __device__ __forceinline__ double block_sum(double v, double *lds4)
     return r;
 }
 
-// partial kernel: store this workgroup's sum of row k (every thread calls)
-__device__ __forceinline__ void store_partial(const RedSlot &slot, int k, double acc, double *lds4)
+// partial kernel: store this workgroup's sum of row k at entry idx (every
+// thread calls); store_partial: at the workgroup's index
+__device__ __forceinline__ void store_partial_at(const RedSlot &slot, int k, double acc, double *lds4, int idx)
 {
     const double p = block_sum(acc, lds4);
-    if (threadIdx.x == 0) slot.partials[(size_t)k * kRedBlocks + blockIdx.x] = p;
+    if (threadIdx.x == 0) slot.partials[(size_t)k * kRedBlocks + idx] = p;
+}
+
+__device__ __forceinline__ void store_partial(const RedSlot &slot, int k, double acc, double *lds4)
+{
+    store_partial_at(slot, k, acc, lds4, blockIdx.x);
 }
 
 // partial kernel with TPB-thread workgroups: rows k0 and k1 of this
-// workgroup (wavefront sums, then wavefronts in index order)
+// workgroup (wavefront sums, then wavefronts in index order) at entry idx
 template <int TPB>
-__device__ __forceinline__ void store_partials2(const RedSlot &slot, int k0, double a0, int k1, double a1)
+__device__ __forceinline__ void store_partials2_at(const RedSlot &slot, int k0, double a0, int k1, double a1, int idx)
 {
     constexpr int W = TPB / 64;
     __shared__ double l0[W], l1[W];
@@ -73,9 +79,15 @@ __device__ __forceinline__ void store_partials2(const RedSlot &slot, int k0, dou
             p0 += l0[q];
             p1 += l1[q];
         }
-        slot.partials[(size_t)k0 * kRedBlocks + blockIdx.x] = p0;
-        slot.partials[(size_t)k1 * kRedBlocks + blockIdx.x] = p1;
+        slot.partials[(size_t)k0 * kRedBlocks + idx] = p0;
+        slot.partials[(size_t)k1 * kRedBlocks + idx] = p1;
     }
+}
+
+template <int TPB>
+__device__ __forceinline__ void store_partials2(const RedSlot &slot, int k0, double a0, int k1, double a1)
+{
+    store_partials2_at<TPB>(slot, k0, a0, k1, a1, blockIdx.x);
 }
 
 template <int TPB>

@@ -260,3 +260,22 @@ class CGsemElliptic2D:
             self.Trash()
         except Exception:
             pass
+
+
+def SolveBatched(models, precond="jacobi", tolerance=1e-10, maxIters=20000):
+    """The CGSEM PCG (CGsemElliptic_2D_Class.f90:1114-1207) for many
+    CGsemElliptic2D models at once (any mesh sizes and degrees): every CG
+    iteration is two launches for all models still iterating.  Each model's
+    solution, ioerr and iteration count are bitwise those of its own
+    Solve("cgsem_pcg", precond, tolerance, maxIters).  Returns
+    (ioerr list, iterations list, elapsed_ms of the whole batch)."""
+    n = len(models)
+    if n == 0:
+        return [], [], 0.0
+    hs = (C.c_void_p * n)(*[m._h for m in models])
+    io = (C.c_int * n)()
+    it = (C.c_int * n)()
+    ms = C.c_double()
+    pc = {"none": 0, "jacobi": 1}[precond]
+    L.check(L.lib().self_cgsem2d_solve_batched(hs, n, pc, tolerance, maxIters, io, it, C.byref(ms)))
+    return list(io), list(it), ms.value

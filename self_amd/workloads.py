@@ -202,8 +202,13 @@ def cgsem_poisson_fields(x, y, seed=DEFAULT_SEED):
     on [0,1]^2, for -div(k grad u) = f."""
     k = 1.0 + 0.5 * np.sin(2 * math.pi * x) * np.sin(2 * math.pi * y)
     f = np.zeros_like(x)
+    sx, sy = {}, {}  # each sine evaluated once (same values as evaluating it per term)
     for a, m, n in cgsem_rhs_modes(seed):
-        f += a * np.sin(m * math.pi * x) * np.sin(n * math.pi * y)
+        if m not in sx:
+            sx[m] = np.sin(m * math.pi * x)
+        if n not in sy:
+            sy[n] = np.sin(n * math.pi * y)
+        f += a * sx[m] * sy[n]
     return k, f
 
 
@@ -223,6 +228,24 @@ def cgsem_helmholtz(nX=128, N=7, seed=DEFAULT_SEED, tolerance=1e-10, maxIters=20
     model.SetDirichletBoundaryCondition(np.zeros_like(k))
     model.SetSolution(np.zeros_like(k))
     return model, {"k": k, "f": f}
+
+
+def cgsem_batch(n=64, seed=DEFAULT_SEED, sizes=(16, 32, 64, 128), degrees=(3, 5, 7), tolerance=1e-10,
+                maxIters=20000):
+    """The batched CGSEM case: n problems, problem i on an nX x nX mesh
+    (nX seeded from sizes) of degree N (seeded from degrees), each the
+    variable-coefficient Poisson problem of cgsem_helmholtz with its own
+    seeded right-hand side (seed + 100 + i).  Returns (models, [(nX, N,
+    seed_i)])."""
+    rng = SplitMix64(seed + 41)
+    models, info = [], []
+    for i in range(n):
+        nX = sizes[rng.randint(0, len(sizes) - 1)]
+        N = degrees[rng.randint(0, len(degrees) - 1)]
+        m, _ = cgsem_helmholtz(nX, N, seed=seed + 100 + i, tolerance=tolerance, maxIters=maxIters)
+        models.append(m)
+        info.append((nX, N, seed + 100 + i))
+    return models, info
 
 
 def adv3d_gaussians_fields(x, y, z, seed=DEFAULT_SEED, n=3):
