@@ -392,7 +392,8 @@ def bench_sw2d_dipole(N=10, steps=200, warm=10):
             f.write(np.load(src)["mesh"].tobytes())
         for name, prec, form in (("fp64_plain", "fp64", "plain"), ("fp64_fused", "fp64", "fused"),
                                  ("mixed_fused", "mixed", "fused")):
-            m, info = sw2d_dipole_box(path, N, precision=prec, form=form)
+            # the mixed form unchecked, to time it and measure its error here
+            m, info = sw2d_dipole_box(path, N, precision=prec, form=form, mixedGuard=0)
             E = m.nElems
             m.ForwardStepRK3(0.0, warm)
             stage_ms = m.ForwardStepRK3Timed(0.0, steps) / steps / 3.0
@@ -423,9 +424,16 @@ def bench_sw2d_dipole(N=10, steps=200, warm=10):
                     r["members8_speedup_vs_single"] = 8 * stage_ms / ens_ms
             out[name] = r
             m.Trash()
+        # what the model does by default with precision "mixed": the guard
+        # compares the mixed and FP64 momentum tendencies and selects FP64
+        m, _ = sw2d_dipole_box(path, N, precision="mixed", form="fused")
+        err, switched = m.CheckMixed(m.mixedGuard)
+        out["mixed_guard"] = {"tendency_rel_l2_mixed_vs_fp64": err, "threshold": m.mixedGuard,
+                              "switched_to_fp64": switched}
+        m.Trash()
     a, b = sols["mixed_fused"], sols["fp64_fused"]
-    out["mixed_vs_fp64_rel_l2_UVH"] = [float(np.linalg.norm(a[:, f] - b[:, f]) / np.linalg.norm(b[:, f]))
-                                       for f in range(3)]
+    out["mixed_unchecked_vs_fp64_rel_l2_UVH"] = [float(np.linalg.norm(a[:, f] - b[:, f]) / np.linalg.norm(b[:, f]))
+                                                 for f in range(3)]
     out["steps_compared"] = warm + steps
     return out
 
@@ -585,6 +593,9 @@ def main():
     for name, prec, form in (("mixed_fused", "mixed", "fused"), ("fp64_fused", "fp64", "fused"),
                              ("fp64_plain", "fp64", "plain")):
         m, info = sw2d_geostrophic(nX, N, precision=prec, form=form, seed=SEED)
+        guard = None
+        if prec == "mixed":  # the model's first-step check, here before the timed region
+            m.mixedGuardResult = guard = m.CheckMixed(m.mixedGuard)
         if W:
             m.ForwardStepRK3(0.0, W)
         m.Synchronize()
@@ -595,6 +606,9 @@ def main():
         kt = m.KernelTimes(0.0, 20)
         results[name] = {"ms_per_step": ms / K, "wall_ms_per_step": 1e3 * wall / K,
                          "dof_stage_per_s": dof * 3 * K / (ms * 1e-3), "kernel_ms": kt, "dt": info["dt"]}
+        if guard:
+            results[name]["mixed_guard"] = {"tendency_rel_l2_mixed_vs_fp64": guard[0], "threshold": m.mixedGuard,
+                                            "switched_to_fp64": guard[1]}
         m.Trash()
         progress(f"{name}: {ms / K:.4f} ms per step")
 
@@ -665,7 +679,7 @@ def main():
                             "achieved_GBs": round(b_mixed / (mixed_stage_ms * 1e-3) / 1e9, 1),
                             "frac": round(b_mixed / (mixed_stage_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
                             "speedup_vs_fp64_fused": mixed["dof_stage_per_s"] / head["dof_stage_per_s"],
-                            "rel_l2_vs_fp64_UVH": acc},
+                            "rel_l2_vs_fp64_UVH": acc, "mixed_guard": mixed.get("mixed_guard")},
             "fp64_plain": {"dof_stage_per_s": results["fp64_plain"]["dof_stage_per_s"],
                            "ms_per_step": results["fp64_plain"]["ms_per_step"],
                            "kernel_ms_traces_edgeflux_mapped_rk": results["fp64_plain"]["kernel_ms"]},

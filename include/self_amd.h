@@ -254,6 +254,16 @@ int self_sw2d_edge_flux(self_sw2d_t h, double t);
 int self_sw2d_mapped_time_derivative(self_sw2d_t h, double t);
 /* GlobalTimeDerivative (:1222-1262): the three phases above. */
 int self_sw2d_global_time_derivative(self_sw2d_t h, double t);
+/* Guard of the mixed form (no counterpart: the reference computes in FP64
+ * only).  For a mixed single-state model: the momentum tendency (U, V) of
+ * the current state through one mixed fused stage, against the FP64
+ * GlobalTimeDerivative of the same state: relErr = |T_mixed - T_FP64|_2 /
+ * |T_FP64|_2.  Where the FP32 contractions lose the tendency to
+ * cancellation (deep, steep bathymetry, where g H^2/2 and the bathymetry
+ * source are many orders larger than their sum) relErr exceeds tol and the
+ * model switches to the FP64 fused form (*switched = 1).  The state is not
+ * changed; FP64 models return relErr = 0. */
+int self_sw2d_check_mixed(self_sw2d_t h, double tol, double *relErr, int *switched);
 /* ForwardStepRK3 (:1173-1218) applied nSteps times from tn. */
 int self_sw2d_forward_step_rk3(self_sw2d_t h, double tn, int nSteps);
 /* Same, bracketed by HIP events on the handle's stream; *elapsed_ms is the

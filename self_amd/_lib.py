@@ -116,6 +116,7 @@ _SIGS = {
     "self_sw2d_edge_flux": ([C.c_void_p, C.c_double], C.c_int),
     "self_sw2d_mapped_time_derivative": ([C.c_void_p, C.c_double], C.c_int),
     "self_sw2d_global_time_derivative": ([C.c_void_p, C.c_double], C.c_int),
+    "self_sw2d_check_mixed": ([C.c_void_p, C.c_double, dp, ip], C.c_int),
     "self_sw2d_forward_step_rk3": ([C.c_void_p, C.c_double, C.c_int], C.c_int),
     "self_sw2d_forward_step_rk3_timed": ([C.c_void_p, C.c_double, C.c_int, dp], C.c_int),
     "self_sw2d_calculate_bathymetry_gradient": ([C.c_void_p], C.c_int),

@@ -28,6 +28,7 @@
 // traces, metrics, normals and the RK accumulation stay FP64.
 
 #include <algorithm>
+#include <cmath>
 #include <cstring>
 #include <vector>
 
@@ -1158,6 +1159,35 @@ int steps_fused_graph(self_sw2d_s *h, int nPairs)
     return SELF_OK;
 }
 
+// Partial sums of the mixed-form guard over the momentum fields: per
+// workgroup, sum (G - T)^2 and sum T^2 (G the mixed stage's FP32 register
+// after a first stage, T the FP64 tendency), fixed order.
+__global__ __launch_bounds__(256) void k_sw_guard_sums(const float *__restrict__ gm, const double *__restrict__ t64,
+                                                       size_t n, double *__restrict__ part)
+{
+    __shared__ double sd[256], st[256];
+    double d2 = 0.0, t2 = 0.0;
+    for (size_t k = (size_t)blockIdx.x * 256 + threadIdx.x; k < n; k += (size_t)gridDim.x * 256) {
+        const double t = t64[k], d = (double)gm[k] - t;
+        d2 += d * d;
+        t2 += t * t;
+    }
+    sd[threadIdx.x] = d2;
+    st[threadIdx.x] = t2;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if (threadIdx.x < w) {
+            sd[threadIdx.x] += sd[threadIdx.x + w];
+            st[threadIdx.x] += st[threadIdx.x + w];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        part[2 * blockIdx.x] = sd[0];
+        part[2 * blockIdx.x + 1] = st[0];
+    }
+}
+
 int check_ready(self_sw2d_s *h)
 {
     SELF_REQUIRE(h, SELF_ERR_ARG, "null handle");
@@ -1532,6 +1562,78 @@ int self_sw2d_global_time_derivative(self_sw2d_t h, double t)
     int rc = check_ready(h);
     if (rc) return rc;
     return gtd_plain(h);
+}
+
+int self_sw2d_check_mixed(self_sw2d_t h, double tol, double *relErr, int *switched)
+{
+    int rc = check_ready(h);
+    if (rc) return rc;
+    SELF_REQUIRE(relErr && switched, SELF_ERR_ARG, "null argument");
+    *relErr = 0.0;
+    *switched = 0;
+    if (h->d.precision != SELF_MIXED) return SELF_OK;
+    SELF_REQUIRE(h->members == 1, SELF_ERR_STATE, "the mixed-form check runs on a single state (members = %d)",
+                 h->members);
+    // the FP64 tendency of the current state
+    if ((rc = gtd_plain(h))) return rc;
+    // one mixed fused stage from the same state into scratch buffers: with
+    // a = 0 its register holds the mixed tendency, rounded to FP32
+    const size_t nf = h->nE * h->np, nt = h->nE * 12 * h->n1;
+    constexpr int kParts = 256;
+    double *qS = nullptr, *trA = nullptr, *trB = nullptr, *part = nullptr;
+    float *gS = nullptr;
+    struct Scratch {
+        void *p[5];
+        ~Scratch()
+        {
+            for (void *x : p)
+                if (x) (void)hipFree(x);
+        }
+    } scratch{};
+    SELF_HIP(hipMalloc(&qS, sizeof(double) * 3 * nf));
+    scratch.p[0] = qS;
+    SELF_HIP(hipMalloc(&gS, sizeof(float) * 3 * nf));
+    scratch.p[1] = gS;
+    SELF_HIP(hipMalloc(&trA, sizeof(double) * nt));
+    scratch.p[2] = trA;
+    SELF_HIP(hipMalloc(&trB, sizeof(double) * nt));
+    scratch.p[3] = trB;
+    SELF_HIP(hipMalloc(&part, sizeof(double) * 2 * kParts));
+    scratch.p[4] = part;
+    SELF_HIP(hipMemcpyAsync(qS, h->q, sizeof(double) * 3 * nf, hipMemcpyDeviceToDevice, h->stream));
+    SELF_HIP(hipMemsetAsync(trA, 0, sizeof(double) * nt, h->stream));
+    SELF_HIP(hipMemsetAsync(trB, 0, sizeof(double) * nt, h->stream));
+    if ((rc = launch_traces_push(h, trA))) return rc;
+    SWKernelArgs A = make_args(h);
+    A.q = qS;
+    A.G32 = gS;
+    A.trIn = trA;
+    A.trOut = trB;
+    A.a = 0.0;
+    A.gdt = 0.0;
+    A.gcoef = 0.0;
+    if ((rc = launch_stage_fused(h, A))) return rc;
+    hipLaunchKernelGGL(k_sw_guard_sums, dim3(kParts), dim3(256), 0, h->stream, (const float *)gS,
+                       (const double *)h->tend, 2 * nf, part);
+    SELF_HIP(hipGetLastError());
+    std::vector<double> ph(2 * kParts);
+    SELF_HIP(hipMemcpyAsync(ph.data(), part, sizeof(double) * 2 * kParts, hipMemcpyDeviceToHost, h->stream));
+    SELF_HIP(hipStreamSynchronize(h->stream));
+    double d2 = 0.0, t2 = 0.0;
+    for (int k = 0; k < kParts; ++k) {
+        d2 += ph[2 * k];
+        t2 += ph[2 * k + 1];
+    }
+    *relErr = t2 > 0.0 ? std::sqrt(d2 / t2) : (d2 > 0.0 ? INFINITY : 0.0);
+    if (*relErr > tol) {
+        // the FP64 fused form: the FP64 geometry, bathymetry and Coriolis
+        // are set for every model; the FP64 register is allocated at creation
+        // (and again here if self_sw2d_set_members replaced it)
+        if (!h->G64 && (rc = dalloc(h, &h->G64, 3 * nf))) return rc;
+        h->d.precision = SELF_FP64;
+        *switched = 1;
+    }
+    return SELF_OK;
 }
 
 int self_sw2d_forward_step_rk3(self_sw2d_t h, double tn, int nSteps)

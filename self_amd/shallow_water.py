@@ -57,12 +57,21 @@ class ShallowWater:
 
     precision: "fp64" or "mixed"; form: "fused" (one kernel per RK stage) or
     "plain" (one kernel per reference phase).
+    mixedGuard: for precision "mixed", the first ForwardStepRK3 of a single
+    state first runs CheckMixed(mixedGuard): where the mixed stage's momentum
+    tendency differs from the FP64 one by more than this (relative L2), the
+    model continues in FP64.  0 or None: no check.
     """
+
+    MIXED_GUARD = 2e-3
 
     def __init__(self, params: SWParams, precision: str = "fp64", form: str = "fused",
                  quirk_dmatp_transposed: bool = False, mesh: QuadMesh | None = None,
-                 nodal: NodalStorage2D | None = None):
+                 nodal: NodalStorage2D | None = None, mixedGuard: float | None = MIXED_GUARD):
         self.params = params
+        self.mixedGuard = mixedGuard if precision == "mixed" else None
+        self.mixedGuardResult = None  # (relErr, switched) once the check has run
+        self.members = 1
         N = params.polyDeg
         self.nS = self.nP = N
         self.nEq = 3
@@ -185,11 +194,27 @@ class ShallowWater:
     def GlobalTimeDerivative(self, t=0.0):
         L.check(L.lib().self_sw2d_global_time_derivative(self._h, t))
 
+    def CheckMixed(self, tolerance=1e-5):
+        """Mixed form only: relative L2 difference of the momentum tendency of
+        the current state between the mixed stage and the FP64 path; above
+        tolerance the model switches to FP64.  Returns (relErr, switched)."""
+        e, s = C.c_double(), C.c_int()
+        L.check(L.lib().self_sw2d_check_mixed(self._h, tolerance, C.byref(e), C.byref(s)))
+        if s.value:
+            self.precision = "fp64"
+        return e.value, bool(s.value)
+
+    def _guard(self):
+        if self.mixedGuard and self.mixedGuardResult is None and self.members == 1:
+            self.mixedGuardResult = self.CheckMixed(self.mixedGuard)
+
     def ForwardStepRK3(self, tn=0.0, nSteps=1):
+        self._guard()
         L.check(L.lib().self_sw2d_forward_step_rk3(self._h, tn, nSteps))
 
     def ForwardStepRK3Timed(self, tn=0.0, nSteps=1):
         """Returns device milliseconds for nSteps steps (HIP events on the model's stream)."""
+        self._guard()
         ms = C.c_double()
         L.check(L.lib().self_sw2d_forward_step_rk3_timed(self._h, tn, nSteps, C.byref(ms)))
         return ms.value

@@ -88,16 +88,20 @@ def sw2d_cfl_dt(U, V, H, g, dx_elem, N, cfl=0.5):
 
 
 def sw2d_geostrophic(nX=64, N=7, precision="fp64", form="fused", L=1.0e6, seed=DEFAULT_SEED,
-                     cfl=0.5, g=9.81, f0=1.0e-4, quirk_dmatp_transposed=False, compute_gradient=True):
+                     cfl=0.5, g=9.81, f0=1.0e-4, quirk_dmatp_transposed=False, compute_gradient=True,
+                     mixedGuard="default"):
     """Build a ShallowWater model with the geostrophic case loaded.
 
     Returns (model, info) with info holding the host arrays that were set.
+    mixedGuard: ShallowWater's (the model's default unless given).
     """
     from .shallow_water import ShallowWater, SWParams
 
     p = SWParams(polyDeg=N, nXElem=nX, nYElem=nX, xScale=L, yScale=L, g=g, f0=f0,
                  periodicX=True, periodicY=True)
-    model = ShallowWater(p, precision=precision, form=form, quirk_dmatp_transposed=quirk_dmatp_transposed)
+    guard = ShallowWater.MIXED_GUARD if mixedGuard == "default" else mixedGuard
+    model = ShallowWater(p, precision=precision, form=form, quirk_dmatp_transposed=quirk_dmatp_transposed,
+                         mixedGuard=guard)
     m = model.mesh
     h, U, V, H = sw2d_geostrophic_fields(m.x, m.y, L=L, g=g, f0=f0, seed=seed)
     sol = np.stack([U, V, H], axis=1)
@@ -135,11 +139,12 @@ def sw2d_dipole_fields(x, y, g=9.81, f0=1.0e-4, vMax=1.75, x0=2.0e5, y0=2.05e5, 
 
 
 def sw2d_dipole_box(mesh_path, N=10, precision="fp64", form="fused", dt=0.1, g=9.81, f0=1.0e-4,
-                    scale=4.0e5):
+                    scale=4.0e5, mixedGuard="default"):
     """The shallow-water model on the dipole run's unstructured mesh file
     (448 elements, read with the SpecMesh reader and scaled by 4e5 as the
     run's parameters ask), degree N, its time step and state
-    (sw2d_dipole_fields).  Returns (model, info)."""
+    (sw2d_dipole_fields).  Returns (model, info).  mixedGuard: ShallowWater's
+    (the model's default unless given; 0 runs the mixed form unchecked)."""
     from .mesh import SpecMesh
     from .nodal import NodalStorage2D
     from .shallow_water import ShallowWater, SWParams
@@ -148,7 +153,8 @@ def sw2d_dipole_box(mesh_path, N=10, precision="fp64", form="fused", dt=0.1, g=9
     nodal = NodalStorage2D(N, L.GAUSS, L.DG)
     mesh = SpecMesh(mesh_path, N, nodal.nodes).ScaleTheMesh(scale, scale)
     p = SWParams(polyDeg=N, dt=dt, g=g, f0=f0)
-    model = ShallowWater(p, precision=precision, form=form, mesh=mesh, nodal=nodal)
+    guard = ShallowWater.MIXED_GUARD if mixedGuard == "default" else mixedGuard
+    model = ShallowWater(p, precision=precision, form=form, mesh=mesh, nodal=nodal, mixedGuard=guard)
     h, U, V, H = sw2d_dipole_fields(mesh.x, mesh.y, g=g, f0=f0)
     sol = np.stack([U, V, H], axis=1)
     bathy = np.stack([h, np.zeros_like(h), np.zeros_like(h)], axis=1)

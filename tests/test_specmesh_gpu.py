@@ -162,23 +162,58 @@ def test_dipole_box_N10_matches_oracle(gpu_lib, tmp_path, form):
 
 
 def test_dipole_box_N10_mixed_accuracy(gpu_lib, tmp_path):
-    """The mixed form on the dipole's 200 -> 2800 m shelf stays within 5e-5
-    of FP64 over 100 steps (measured 1.6e-5 in U): FP32 contractions of the
-    pressure flux g H^2 / 2 (~4e7) against the bathymetry source lose about
-    six digits of the ~1e-4 residual tendency.  This case is outside the
-    1e-6 gate the mixed form meets on the f-plane geostrophic workload
-    (shallow relief); FP64 is the form for it."""
+    """The mixed form run unchecked (mixedGuard=0) on the dipole's 200 -> 2800
+    m shelf stays within 5e-5 of FP64 over 100 steps (measured 1.6e-5 in U):
+    FP32 contractions of the pressure flux g H^2 / 2 (~4e7) against the
+    bathymetry source lose about six digits of the ~1e-4 residual tendency.
+    This case is outside the 1e-6 gate the mixed form meets on the f-plane
+    geostrophic workload (shallow relief); the mixed-form guard selects FP64
+    here (test_mixed_guard_selects_fp64_on_the_dipole_shelf)."""
     from self_amd.workloads import sw2d_dipole_box
     path = dipole_mesh_file(tmp_path)
     m64, _ = sw2d_dipole_box(path, 10, precision="fp64", form="fused")
-    mmx, _ = sw2d_dipole_box(path, 10, precision="mixed", form="fused")
+    mmx, _ = sw2d_dipole_box(path, 10, precision="mixed", form="fused", mixedGuard=0)
     m64.ForwardStepRK3(0.0, 100)
     mmx.ForwardStepRK3(0.0, 100)
+    assert mmx.precision == "mixed" and mmx.mixedGuardResult is None
     a, b = mmx.GetSolution(), m64.GetSolution()
     for f in range(3):
         assert rel_l2(a[:, f], b[:, f]) < 5e-5, (f, rel_l2(a[:, f], b[:, f]))
     m64.Trash()
     mmx.Trash()
+
+
+def test_mixed_guard_selects_fp64_on_the_dipole_shelf(gpu_lib, tmp_path):
+    """The mixed-form guard (CheckMixed, run by the first step): on the
+    dipole's shelf at N = 10 the mixed stage's momentum tendency is off by
+    about 4e-2 of the FP64 one, above the 2e-3 threshold, so the model
+    switches to FP64 and then steps bitwise as an FP64 model; on the
+    geostrophic workload (measured 4.7e-4 at 64^2 N = 7, which holds the
+    1e-6 gate over 500 steps) and on curved meshes (about 2e-7) it stays
+    mixed."""
+    from self_amd.workloads import sw2d_dipole_box, sw2d_geostrophic
+    path = dipole_mesh_file(tmp_path)
+    m64, _ = sw2d_dipole_box(path, 10, precision="fp64", form="fused")
+    mmx, _ = sw2d_dipole_box(path, 10, precision="mixed", form="fused")
+    m64.ForwardStepRK3(0.0, 10)
+    mmx.ForwardStepRK3(0.0, 10)
+    err, switched = mmx.mixedGuardResult
+    print(f"dipole N=10 guard {err:.3e} switched {switched}", flush=True)
+    assert switched and err > 1e-2 and mmx.precision == "fp64"
+    assert np.array_equal(mmx.GetSolution(), m64.GetSolution())
+    m64.Trash()
+    mmx.Trash()
+    geo, _ = sw2d_geostrophic(16, 7, precision="mixed")
+    geo.ForwardStepRK3(0.0, 2)
+    err, switched = geo.mixedGuardResult
+    print(f"geostrophic 16^2 N=7 guard {err:.3e}", flush=True)
+    assert not switched and 0.0 < err < 2e-3 and geo.precision == "mixed"
+    geo.Trash()
+    case = CASES[1]
+    mc, _, _ = setup(tmp_path, precision="mixed", **case)
+    err, switched = mc.CheckMixed()
+    assert not switched and 0.0 < err < 1e-5, err
+    mc.Trash()
 
 
 def test_spec_mesh_file_parameter(gpu_lib, tmp_path):
