@@ -595,6 +595,22 @@ int self_cgsem2d_set_solution(self_cgsem2d_t h, const double *u);
 int self_cgsem2d_get_solution(self_cgsem2d_t h, double *u);
 /* MatrixAction (:930-976) of a masked local array; Au returned masked */
 int self_cgsem2d_matrix_action(self_cgsem2d_t h, const double *u, double *Au);
+/* Mask (:350-430) in place on an element-local array: the owner copy of
+ * each global DOF keeps its value (edge: the primary element's copy;
+ * corner: the first element of the node's element list), other copies and
+ * Dirichlet nodes become 0. */
+int self_cgsem2d_mask(self_cgsem2d_t h, double *u);
+/* Local node -> index of its global DOF in the solver's vectors (0-based),
+ * -1 on Dirichlet nodes; element-local layout (no reference counterpart:
+ * the reference keeps masked local arrays). */
+int self_cgsem2d_dof_map(self_cgsem2d_t h, int *gid);
+/* The preconditioner's assembled N = 1 operator (MatrixAction of
+ * CGsemPreconditioner_2D, CGsemPreconditioner_2D_Class.f90:564-610, on the
+ * sub-mesh's free nodes, which are the solver's global DOFs): row g holds
+ * *width (column, value) pairs, unused slots (g, 0.0); cols / vals
+ * (nGlobal x width, row-major) may be null to ask for the width.  What
+ * ConstructPCMatrix (:750-851) writes out. */
+int self_cgsem2d_pc_matrix(self_cgsem2d_t h, int *width, int *cols, double *vals);
 /* Residual (:980-1068) of the current solution, masked local layout */
 int self_cgsem2d_residual(self_cgsem2d_t h, double *r);
 /* CGsemPreconditioner_2D % Build (src/cgsem/CGsemPreconditioner_2D_Class.f90

@@ -48,7 +48,7 @@ SW2D_SRC := ../examples/shallowwater/planewave/localmods/SWParamsClass \
             ../examples/shallowwater/planewave/localmods/ConservativeShallowWaterClass
 
 all: $(OUT)/ref_adv2d $(OUT)/ref_adv3d $(OUT)/ref_cgsem2d $(OUT)/ref_krylov $(OUT)/ref_nodal $(OUT)/ref_pickup \
-     $(OUT)/ref_specmesh $(OUT)/ref_sw2d
+     $(OUT)/ref_specmesh $(OUT)/ref_sw2d $(OUT)/ref_cgsem_paraboloid $(OUT)/dropin_cgsem_paraboloid
 
 # One recipe compiles the shared chain in order: each module needs the .mod
 # files of the ones before it.
@@ -78,6 +78,26 @@ $(eval $(call model,nodal,))
 $(eval $(call model,pickup,$(ADV2D_SRC)))
 $(eval $(call model,specmesh,))
 $(eval $(call model,sw2d,$(SW2D_SRC)))
+
+# The reference's CGSEM example driver (examples/cgsem-elliptic2d/paraboloid),
+# unmodified: against the reference's own modules (ref_cgsem_paraboloid, whose
+# output files are the golden files of the Fortran drop-in), and against the
+# drop-in modules of self_amd/fortran linked with libself_amd.so
+# (dropin_cgsem_paraboloid, run by the GPU tests; built after
+# `make -C self_amd/fortran`).
+CGSEM_DRIVER := ../examples/cgsem-elliptic2d/paraboloid/localmods/CGsemElliptic2D_Driver
+FDROP        := $(HERE)../self_amd/fortran/_build
+$(OUT)/ref_cgsem_paraboloid: $(OUT)/ref_cgsem2d $(REF)/$(CGSEM_DRIVER).f90
+	$(AMDFLANG) $(FFLAGS) -J$(OUT)/cgsem2d -I$(OUT) -c $(REF)/$(CGSEM_DRIVER).f90 -o $(OUT)/cgsem2d/paraboloid_driver.o
+	$(AMDFLANG) $(FFLAGS) -o $@.tmp $(COMMON_OBJ) $(addprefix $(OUT)/cgsem2d/,$(addsuffix .o,$(notdir $(CGSEM2D_SRC)))) \
+	  $(OUT)/cgsem2d/paraboloid_driver.o && mv $@.tmp $@
+
+$(OUT)/dropin_cgsem_paraboloid: $(REF)/$(CGSEM_DRIVER).f90 $(FDROP)/cgsem_dropin.o
+	@mkdir -p $(OUT)/dropin
+	$(AMDFLANG) $(FFLAGS) -J$(OUT)/dropin -I$(FDROP) -c $(REF)/$(CGSEM_DRIVER).f90 -o $(OUT)/dropin/paraboloid_driver.o
+	$(AMDFLANG) $(FFLAGS) -o $@.tmp $(OUT)/dropin/paraboloid_driver.o $(FDROP)/self_amd_cabi.o \
+	  $(FDROP)/shallowwater_amd.o $(FDROP)/sw_dropin.o $(FDROP)/cgsem_dropin.o \
+	  -L$(HERE)../self_amd -lself_amd -Wl,-rpath,'$$ORIGIN/../../self_amd' && mv $@.tmp $@
 
 clean:
 	rm -rf $(OUT)

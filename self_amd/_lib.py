@@ -145,6 +145,9 @@ _SIGS = {
                             C.c_int, dp], C.c_int),
     "self_cgsem2d_solve_batched": ([C.POINTER(C.c_void_p), C.c_int, C.c_int, C.c_double, C.c_int, ip, ip, dp],
                                    C.c_int),
+    "self_cgsem2d_mask": ([C.c_void_p, dp], C.c_int),
+    "self_cgsem2d_dof_map": ([C.c_void_p, ip], C.c_int),
+    "self_cgsem2d_pc_matrix": ([C.c_void_p, ip, ip, dp], C.c_int),
     "self_cgsem2d_set_preconditioner": ([C.c_void_p, dp, dp, C.c_int, C.c_double, C.c_int], C.c_int),
     "self_cgsem2d_apply_preconditioner": ([C.c_void_p, dp, dp, ip], C.c_int),
     "self_cgsem2d_solve_gmres_mixed": ([C.c_void_p, C.c_int, C.c_double, C.c_int, C.c_int, C.c_int,

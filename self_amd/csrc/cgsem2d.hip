@@ -1022,6 +1022,51 @@ try {
 }
 SELF_CATCH_ALL
 
+// Mask (:350-430) on an element-local array: only the owner copy of each
+// global DOF (edge: the primary element's; corner: the first element of the
+// node's list) keeps its value; other copies and Dirichlet nodes become 0.
+int self_cgsem2d_mask(self_cgsem2d_t h, double *u)
+try {
+    SELF_REQUIRE(h && u, SELF_ERR_ARG, "null argument");
+    const CgsemOp &op = h->op;
+    std::vector<char> keep(op.nLocal, 0);
+    for (size_t g = 0; g < op.nG; ++g) keep[op.ciH[op.csH[g]]] = 1;
+    for (size_t l = 0; l < op.nLocal; ++l)
+        if (!keep[l]) u[l] = 0.0;
+    return SELF_OK;
+}
+SELF_CATCH_ALL
+
+int self_cgsem2d_dof_map(self_cgsem2d_t h, int *gid)
+try {
+    SELF_REQUIRE(h && gid, SELF_ERR_ARG, "null argument");
+    std::memcpy(gid, h->op.gidH.data(), sizeof(int) * h->op.nLocal);
+    return SELF_OK;
+}
+SELF_CATCH_ALL
+
+int self_cgsem2d_pc_matrix(self_cgsem2d_t h, int *width, int *cols, double *vals)
+try {
+    SELF_REQUIRE(h && width, SELF_ERR_ARG, "null argument");
+    SELF_REQUIRE(h->op.pc, SELF_ERR_STATE, "no preconditioner set (self_cgsem2d_set_preconditioner)");
+    const FemPC &pc = *h->op.pc;
+    *width = pc.width;
+    if (!cols || !vals) return SELF_OK;
+    SELF_HIP(hipSetDevice(h->device));
+    const size_t ne = (size_t)pc.width * pc.n;
+    std::vector<int> c(ne);
+    std::vector<double> v(ne);
+    SELF_HIP(hipMemcpy(c.data(), pc.col, sizeof(int) * ne, hipMemcpyDeviceToHost));
+    SELF_HIP(hipMemcpy(v.data(), pc.val, sizeof(double) * ne, hipMemcpyDeviceToHost));
+    for (size_t g = 0; g < pc.n; ++g)  // device slot-major -> row-major
+        for (int q = 0; q < pc.width; ++q) {
+            cols[g * pc.width + q] = c[(size_t)q * pc.n + g];
+            vals[g * pc.width + q] = v[(size_t)q * pc.n + g];
+        }
+    return SELF_OK;
+}
+SELF_CATCH_ALL
+
 // Residual of the current solution, masked local layout.
 int self_cgsem2d_residual(self_cgsem2d_t h, double *r)
 try {

@@ -361,6 +361,41 @@ MODULE SELF_AMD_CABI
       REAL(C_DOUBLE), INTENT(out) :: resi(*)
       TYPE(C_PTR), VALUE :: elapsedMs
     END FUNCTION
+    INTEGER(C_INT) FUNCTION self_cgsem2d_matrix_action(h, u, Au) BIND(C, name="self_cgsem2d_matrix_action")
+      IMPORT :: C_INT, C_PTR, C_DOUBLE
+      TYPE(C_PTR), VALUE :: h
+      REAL(C_DOUBLE), INTENT(in) :: u(*)
+      REAL(C_DOUBLE), INTENT(out) :: Au(*)
+    END FUNCTION
+    INTEGER(C_INT) FUNCTION self_cgsem2d_mask(h, u) BIND(C, name="self_cgsem2d_mask")
+      IMPORT :: C_INT, C_PTR, C_DOUBLE
+      TYPE(C_PTR), VALUE :: h
+      REAL(C_DOUBLE), INTENT(inout) :: u(*)
+    END FUNCTION
+    INTEGER(C_INT) FUNCTION self_cgsem2d_dof_map(h, gid) BIND(C, name="self_cgsem2d_dof_map")
+      IMPORT :: C_INT, C_PTR
+      TYPE(C_PTR), VALUE :: h
+      INTEGER(C_INT), INTENT(out) :: gid(*)
+    END FUNCTION
+    INTEGER(C_INT) FUNCTION self_cgsem2d_global_dofs(h, nGlobal) BIND(C, name="self_cgsem2d_global_dofs")
+      IMPORT :: C_INT, C_PTR
+      TYPE(C_PTR), VALUE :: h
+      INTEGER(C_INT), INTENT(out) :: nGlobal
+    END FUNCTION
+    INTEGER(C_INT) FUNCTION self_cgsem2d_set_preconditioner(h, x, y, pcIterates, pcTolerance, coefficient) &
+                                                            BIND(C, name="self_cgsem2d_set_preconditioner")
+      IMPORT :: C_INT, C_PTR, C_DOUBLE
+      TYPE(C_PTR), VALUE :: h
+      REAL(C_DOUBLE), INTENT(in) :: x(*), y(*)
+      INTEGER(C_INT), VALUE :: pcIterates, coefficient
+      REAL(C_DOUBLE), VALUE :: pcTolerance
+    END FUNCTION
+    ! cols / vals: TYPE(C_PTR) so that a width query can pass C_NULL_PTR
+    INTEGER(C_INT) FUNCTION self_cgsem2d_pc_matrix(h, width, cols, vals) BIND(C, name="self_cgsem2d_pc_matrix")
+      IMPORT :: C_INT, C_PTR
+      TYPE(C_PTR), VALUE :: h, cols, vals
+      INTEGER(C_INT), INTENT(out) :: width
+    END FUNCTION
     INTEGER(C_INT) FUNCTION self_cgsem2d_solve_reliable(h, precond, tol, delta, maxIters, maxUpdates, ioerr, &
                                                         updates, iterations, relResidual, elapsedMs) &
                                                         BIND(C, name="self_cgsem2d_solve_reliable")

@@ -70,3 +70,25 @@ def test_reference_driver_compiles_unchanged_against_the_drop_in(tmp_path, examp
                                                              "-o", exe], capture_output=True, text=True, timeout=300)
     assert r.returncode == 0, r.stderr[-3000:]
     assert os.path.exists(exe)
+
+
+def test_reference_cgsem_driver_compiles_unchanged_against_the_drop_in(tmp_path):
+    """The reference's CGSEM example driver (examples/cgsem-elliptic2d/
+    paraboloid/localmods/CGsemElliptic2D_Driver.f90, read in place: Build,
+    Solve with a Dirichlet function, ConstructMatrix, the preconditioner's
+    ConstructPCMatrix, WriteTecplot, WriteResidual, Trash, and the mesh's
+    GetPositionAtNode) compiles and links, unmodified, against the drop-in
+    modules of self_amd/fortran/cgsem_dropin.f90 and libself_amd.so."""
+    src = "/root/reference/examples/cgsem-elliptic2d/paraboloid/localmods/CGsemElliptic2D_Driver.f90"
+    build = os.path.join(ROOT, "self_amd", "fortran", "_build")
+    fc = "/opt/rocm/bin/amdflang"
+    if not (os.path.exists(src) and os.path.exists(os.path.join(build, "cgsem_dropin.o")) and os.path.exists(fc)):
+        pytest.skip("reference tree, amdflang or the built drop-in modules absent")
+    obj = [os.path.join(build, f) for f in ("self_amd_cabi.o", "shallowwater_amd.o", "sw_dropin.o",
+                                            "cgsem_dropin.o")]
+    exe = str(tmp_path / "driver")
+    r = subprocess.run([fc, "-O2", "-I", build, "-J", str(tmp_path), src] + obj +
+                       ["-L" + os.path.join(ROOT, "self_amd"), "-lself_amd", "-o", exe],
+                       capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stderr[-3000:]
+    assert os.path.exists(exe)
