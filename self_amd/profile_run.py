@@ -8,6 +8,7 @@
     python -m self_amd.profile_run cgsem --nx 128 --precond 1
     python -m self_amd.profile_run cgsem --precision mixed --steps 2000
     python -m self_amd.profile_run hpe --steps 5
+    python -m self_amd.profile_run cgsem-batched --nx 64 [--iters 64]
 
 `sw2d-dipole` runs the shallow-water model on an ISM-V2 mesh file (the
 dipole run's box.mesh, scaled by 4e5) with the dipole state.
@@ -16,7 +17,9 @@ dipole run's box.mesh, scaled by 4e5) with the dipole state.
 WRITE_SIZE of a copy of known size).  Every other mode builds the named
 workload with its seeded inputs and runs the HIP path only.  For cgsem,
 --precision fp64 runs the reference's PCG loop and mixed the FP32 PCG with
-FP64 reliable updates, each for at most --steps iterations.
+FP64 reliable updates, each for at most --steps iterations.  cgsem-batched
+solves the --nx problems of the batched workload (meshes 16^2..128^2, N 3, 5,
+7; Jacobi-PCG to 1e-10) in one launch sequence.
 """
 from __future__ import annotations
 
@@ -29,7 +32,8 @@ from . import _lib as L
 
 def main(argv=None):
     ap = argparse.ArgumentParser(prog="python -m self_amd.profile_run")
-    ap.add_argument("mode", choices=["copy", "sw2d", "sw2d-dipole", "adv2d", "adv3d", "cgsem", "hpe"])
+    ap.add_argument("mode", choices=["copy", "sw2d", "sw2d-dipole", "adv2d", "adv3d", "cgsem", "hpe",
+                                     "cgsem-batched"])
     ap.add_argument("--mesh", default=None, help="ISM-V2 mesh file (sw2d-dipole)")
     ap.add_argument("--precision", default="mixed", choices=["fp64", "mixed"])
     ap.add_argument("--form", default="fused", choices=["fused", "plain"])
@@ -37,6 +41,7 @@ def main(argv=None):
     ap.add_argument("--N", type=int, default=7)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--precond", type=int, default=1)
+    ap.add_argument("--iters", type=int, default=20000, help="iteration limit (cgsem-batched)")
     ap.add_argument("--contraction", default="fp64", choices=["fp64", "fp32", "bf16x3"],
                     help="arithmetic of the 3-D volume contractions (adv3d)")
     a = ap.parse_args(argv)
@@ -79,6 +84,14 @@ def main(argv=None):
         m.ForwardStepRK3(0.0, a.steps)
         m.Synchronize()
         print(f"adv3d {a.contraction}: {a.steps} RK3 steps", flush=True)
+    elif a.mode == "cgsem-batched":
+        from .krylov import SolveBatched
+        from .workloads import cgsem_batch
+        models, _ = cgsem_batch(a.nx or 64)
+        io, its, ms = SolveBatched(models, precond="jacobi", tolerance=1e-10, maxIters=a.iters)
+        print(f"cgsem batched: {len(models)} problems, {ms:.1f} ms, iterations {min(its)}..{max(its)}", flush=True)
+        for m in models:
+            m.Trash()
     elif a.mode == "cgsem":
         from .workloads import cgsem_helmholtz
         m, _ = cgsem_helmholtz(nX=a.nx or 128, N=a.N)
