@@ -48,7 +48,8 @@ SW2D_SRC := ../examples/shallowwater/planewave/localmods/SWParamsClass \
             ../examples/shallowwater/planewave/localmods/ConservativeShallowWaterClass
 
 all: $(OUT)/ref_adv2d $(OUT)/ref_adv3d $(OUT)/ref_cgsem2d $(OUT)/ref_krylov $(OUT)/ref_nodal $(OUT)/ref_pickup \
-     $(OUT)/ref_specmesh $(OUT)/ref_sw2d $(OUT)/ref_cgsem_paraboloid $(OUT)/dropin_cgsem_paraboloid
+     $(OUT)/ref_specmesh $(OUT)/ref_sw2d $(OUT)/ref_cgsem_paraboloid $(OUT)/dropin_cgsem_paraboloid \
+     $(OUT)/ref_adv2d_ic $(OUT)/ref_adv2d_driver $(OUT)/dropin_adv2d_driver
 
 # One recipe compiles the shared chain in order: each module needs the .mod
 # files of the ones before it.
@@ -97,6 +98,28 @@ $(OUT)/dropin_cgsem_paraboloid: $(REF)/$(CGSEM_DRIVER).f90 $(FDROP)/cgsem_dropin
 	$(AMDFLANG) $(FFLAGS) -J$(OUT)/dropin -I$(FDROP) -c $(REF)/$(CGSEM_DRIVER).f90 -o $(OUT)/dropin/paraboloid_driver.o
 	$(AMDFLANG) $(FFLAGS) -o $@.tmp $(OUT)/dropin/paraboloid_driver.o $(FDROP)/self_amd_cabi.o \
 	  $(FDROP)/shallowwater_amd.o $(FDROP)/sw_dropin.o $(FDROP)/cgsem_dropin.o \
+	  -L$(HERE)../self_amd -lself_amd -Wl,-rpath,'$$ORIGIN/../../self_amd' && mv $@.tmp $@
+
+# The 2-D advection model's own programs (src/highend/advection-diffusion2d),
+# unmodified: its initial-condition program and its driver against the
+# reference's modules (their files are the golden files of the Fortran
+# drop-in), and the driver against the drop-in modules with libself_amd.so.
+ADV2D_IC  := highend/advection-diffusion2d/Advection_InitialConditions
+ADV2D_DRV := highend/advection-diffusion2d/Advection_Driver
+ADV2D_OBJ := $(addprefix $(OUT)/adv2d/,$(addsuffix .o,$(notdir $(ADV2D_SRC))))
+$(OUT)/ref_adv2d_ic: $(OUT)/ref_adv2d $(REF)/$(ADV2D_IC).f90
+	$(AMDFLANG) $(FFLAGS) -J$(OUT)/adv2d -I$(OUT) -c $(REF)/$(ADV2D_IC).f90 -o $(OUT)/adv2d/adv2d_ic.o
+	$(AMDFLANG) $(FFLAGS) -o $@.tmp $(COMMON_OBJ) $(ADV2D_OBJ) $(OUT)/adv2d/adv2d_ic.o && mv $@.tmp $@
+
+$(OUT)/ref_adv2d_driver: $(OUT)/ref_adv2d $(REF)/$(ADV2D_DRV).f90
+	$(AMDFLANG) $(FFLAGS) -J$(OUT)/adv2d -I$(OUT) -c $(REF)/$(ADV2D_DRV).f90 -o $(OUT)/adv2d/adv2d_driver.o
+	$(AMDFLANG) $(FFLAGS) -o $@.tmp $(COMMON_OBJ) $(ADV2D_OBJ) $(OUT)/adv2d/adv2d_driver.o && mv $@.tmp $@
+
+$(OUT)/dropin_adv2d_driver: $(REF)/$(ADV2D_DRV).f90 $(FDROP)/adv_dropin.o
+	@mkdir -p $(OUT)/dropin
+	$(AMDFLANG) $(FFLAGS) -J$(OUT)/dropin -I$(FDROP) -c $(REF)/$(ADV2D_DRV).f90 -o $(OUT)/dropin/adv2d_driver.o
+	$(AMDFLANG) $(FFLAGS) -o $@.tmp $(OUT)/dropin/adv2d_driver.o $(FDROP)/self_amd_cabi.o \
+	  $(FDROP)/shallowwater_amd.o $(FDROP)/sw_dropin.o $(FDROP)/adv_dropin.o \
 	  -L$(HERE)../self_amd -lself_amd -Wl,-rpath,'$$ORIGIN/../../self_amd' && mv $@.tmp $@
 
 clean:

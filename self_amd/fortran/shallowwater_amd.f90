@@ -28,6 +28,9 @@ MODULE ShallowWaterAMD_Class
     ! an ISM-V2 mesh file instead of the structured nXElem x nYElem mesh
     ! (SWParams % SpecMeshFile; blank = the structured mesh)
     CHARACTER(LEN=256) :: SpecMeshFile = ' '
+    ! 1: the reference's nodal storage, whose p-direction derivative matrix
+    ! is transposed (NodalStorage_2D); 0: the derivative matrix itself
+    INTEGER        :: quirkDMatP = 0
   END TYPE SWParamsAMD
 
   TYPE ShallowWaterAMD
@@ -80,7 +83,8 @@ CONTAINS
     myDGSEM % nP = N
     ALLOCATE(myDGSEM % nodes(0:N), myDGSEM % qWeight(0:N), dMatS(0:N,0:N), dMatP(0:N,0:N))
     ALLOCATE(lW(0:N), lE(0:N), lS(0:N), lN(0:N))
-    CALL self_check(self_nodal2d_build(N, SELF_GAUSS, SELF_DG, 0_C_INT, myDGSEM % nodes, myDGSEM % qWeight, &
+    CALL self_check(self_nodal2d_build(N, SELF_GAUSS, SELF_DG, INT(params % quirkDMatP, C_INT), myDGSEM % nodes, &
+                                       myDGSEM % qWeight, &
                                        dMatS, dMatP, lW, lE, lS, lN), 'self_nodal2d_build')
     IF (LEN_TRIM(params % SpecMeshFile) > 0) THEN
       ! BuildQuadMesh (:289-304): ReadSpecMeshFile, then ScaleTheMesh

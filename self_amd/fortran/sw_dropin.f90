@@ -17,7 +17,9 @@
 !                          CalculateBathymetryGradient, GlobalTimeDerivative,
 !                          ForwardStepRK3, CoarseToFine, WriteTecplot
 !                          (:1918-1970), WritePickup, ReadPickup.
-! The device model is ShallowWaterAMD (shallowwater_amd.f90) over the C ABI;
+! The device model is ShallowWaterAMD (shallowwater_amd.f90) over the C ABI,
+! with the nodal storage as the reference builds it (its p-direction
+! derivative matrix transposed, NodalStorage_2D);
 ! this layer reads the parameters, builds it, and writes the reference's
 ! output formats.  The reference's mesh is walled (no periodic boundaries).
 MODULE ModelPrecision
@@ -286,6 +288,7 @@ CONTAINS
     p % precision = SELF_FP64
     p % form = SELF_FUSED
     p % SpecMeshFile = TRIM(myDGSEM % params % SpecMeshFile)
+    p % quirkDMatP = 1  ! the reference's nodal storage, so that a driver reproduces its results
     CALL myDGSEM % model % Build(p)
 
     N = myDGSEM % params % polyDeg

@@ -22,6 +22,20 @@ def _tec(path, nvar):
     return v.reshape(-1, nvar)
 
 
+def numeric_rows(path, nvar):
+    """(rows, nvar) of every number in a list-directed text file, skipping
+    the lines that are not all numbers (titles, zone headers, wrapped
+    variable lists)."""
+    vals = []
+    for line in open(path).read().splitlines():
+        toks = line.split()
+        try:
+            vals.extend(float(t.replace("D", "E")) for t in toks)
+        except ValueError:
+            continue
+    return np.array(vals).reshape(-1, nvar)
+
+
 def read_run(d):
     """Every output of one driver run in directory d, as arrays."""
     out = {}

@@ -92,3 +92,24 @@ def test_reference_cgsem_driver_compiles_unchanged_against_the_drop_in(tmp_path)
                        capture_output=True, text=True, timeout=300)
     assert r.returncode == 0, r.stderr[-3000:]
     assert os.path.exists(exe)
+
+
+def test_reference_adv2d_driver_compiles_unchanged_against_the_drop_in(tmp_path):
+    """The reference's 2-D advection-diffusion driver (src/highend/
+    advection-diffusion2d/Advection_Driver.f90, read in place: Build with the
+    pickup at iterInit, GlobalTimeDerivative, WriteTecplot, ForwardStepRK3,
+    WritePickup, mesh % WriteTecplot, Trash) compiles and links, unmodified,
+    against the drop-in modules of self_amd/fortran/adv_dropin.f90 and
+    libself_amd.so."""
+    src = "/root/reference/src/highend/advection-diffusion2d/Advection_Driver.f90"
+    build = os.path.join(ROOT, "self_amd", "fortran", "_build")
+    fc = "/opt/rocm/bin/amdflang"
+    if not (os.path.exists(src) and os.path.exists(os.path.join(build, "adv_dropin.o")) and os.path.exists(fc)):
+        pytest.skip("reference tree, amdflang or the built drop-in modules absent")
+    obj = [os.path.join(build, f) for f in ("self_amd_cabi.o", "shallowwater_amd.o", "sw_dropin.o", "adv_dropin.o")]
+    exe = str(tmp_path / "driver")
+    r = subprocess.run([fc, "-O2", "-I", build, "-J", str(tmp_path), src] + obj +
+                       ["-L" + os.path.join(ROOT, "self_amd"), "-lself_amd", "-o", exe],
+                       capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stderr[-3000:]
+    assert os.path.exists(exe)

@@ -54,7 +54,7 @@ def _model():
     from self_amd.shallow_water import ShallowWater, SWParams
     p = SWParams(polyDeg=5, nXElem=6, nYElem=4, xScale=1.0e6, yScale=6.0e5, dt=20.0, g=9.81, f0=1e-4, betaY=1e-11,
                  periodicX=False, periodicY=False)
-    return ShallowWater(p)
+    return ShallowWater(p, quirk_dmatp_transposed=True)  # the reference's nodal storage, as the drop-in
 
 
 def _lagrange_matrix(nodes, z):
