@@ -1,4 +1,4 @@
-"""Writes tests/golden/ref_adv2d_driver.npz: a run of the reference's own 2-D
+"""Writes tests/golden/refdriver_adv2d.npz: a run of the reference's own 2-D
 advection-diffusion programs (src/highend/advection-diffusion2d/
 Advection_InitialConditions.f90 and Advection_Driver.f90, compiled
 unmodified against the reference's modules by oracle/ref.mk into
@@ -38,7 +38,7 @@ def main():
         for it in (0, 10, 20):
             out[f"tec{it}"] = numeric_rows(os.path.join(d, f"Advection.{it:010d}.tec"), 10)
         out["mesh"] = numeric_rows(os.path.join(d, "mesh.tec"), 7)
-    path = os.path.join(HERE, "ref_adv2d_driver.npz")
+    path = os.path.join(HERE, "refdriver_adv2d.npz")
     np.savez_compressed(path, **out)
     print(path, os.path.getsize(path), {k: v.shape for k, v in out.items()})
 

@@ -4,7 +4,7 @@ against the drop-in modules (self_amd/fortran/adv_dropin.f90 with
 sw_dropin.f90) by oracle/ref.mk into oracle/_ref/dropin_adv2d_driver, run on
 the GPU from the initial pickup the reference's own initial-condition
 program wrote, against the same driver compiled against the reference's
-modules (fixture tests/golden/ref_adv2d_driver.npz, made by
+modules (fixture tests/golden/refdriver_adv2d.npz, made by
 tests/golden/make_ref_adv2d_driver.py; 2 tracers, kappa 1e-3, N = 5 on a
 4 x 3 mesh scaled to 2 x 1.5, inflow boundaries tanh(t), 20 RK3 steps):
  * the final pickup: tracers to 1e-12 of their largest value, velocity,
@@ -27,7 +27,7 @@ pytestmark = pytest.mark.gpu
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 EXE = os.path.join(ROOT, "oracle", "_ref", "dropin_adv2d_driver")
-GOLDEN = os.path.join(ROOT, "tests", "golden", "ref_adv2d_driver.npz")
+GOLDEN = os.path.join(ROOT, "tests", "golden", "refdriver_adv2d.npz")
 
 
 def _cols(a, b):
