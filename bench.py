@@ -471,6 +471,13 @@ def bench_cgsem(nX=128, N=7, tol=1e-10):
                                              "true_rel_residual": float(np.linalg.norm(m.Residual())) / r0,
                                              "speedup_vs_pcg_fp64": out["pcg_fp64"]["ms"] / ms}
     m.SetSolution(zero)
+    # the reference's loop stops on its recursive residual (true residual
+    # above tol); the FP64 sequence with reliable updates stops on the true one
+    io, updates, its, relr, ms = m.SolveReliable(tolerance=tol, delta=1e-2, maxIters=40000, inner="fp64")
+    out["pcg_fp64_reliable_updates"] = {"ioerr": io, "updates": updates, "iterations": its, "ms": ms,
+                                        "iters_per_s": its / (ms * 1e-3), "rel_residual": relr,
+                                        "true_rel_residual": float(np.linalg.norm(m.Residual())) / r0}
+    m.SetSolution(zero)
     # the reference's own preconditioner (inner CG on the linear sub-mesh)
     # in its PCG loop, and as the right preconditioner of FGMRES(30) in FP64
     # and in the mixed form (FP32 operator and preconditioner, FP64 basis

@@ -670,6 +670,15 @@ int self_cgsem2d_solve_mixed(self_cgsem2d_t h, int precond, double tol, double i
 int self_cgsem2d_solve_reliable(self_cgsem2d_t h, int precond, double tol, double delta, int maxIters,
                                 int maxUpdates, int *ioerr, int *updates, int *iterations, double *relResidual,
                                 double *elapsed_ms);
+/* The same with an FP64 PCG sequence: the CGSEM PCG's iteration
+ * (CGsemElliptic_2D_Class.f90:1114-1207) whose residual is replaced by the
+ * true FP64 residual whenever it has dropped by delta, so that the solve
+ * stops on |b - A x| <= tol |b - A x0| itself, where the reference's loop
+ * stops on its recursively updated residual (which drifts from the true
+ * one by rounding). */
+int self_cgsem2d_solve_reliable_fp64(self_cgsem2d_t h, int precond, double tol, double delta, int maxIters,
+                                     int maxUpdates, int *ioerr, int *updates, int *iterations, double *relResidual,
+                                     double *elapsed_ms);
 
 /* --------------------------------------------------------------- serving
  * Request loop (no reference counterpart: the reference's drivers run one

@@ -156,6 +156,8 @@ _SIGS = {
                                  C.c_int),
     "self_cgsem2d_solve_reliable": ([C.c_void_p, C.c_int, C.c_double, C.c_double, C.c_int, C.c_int, ip, ip, ip, dp,
                                      dp], C.c_int),
+    "self_cgsem2d_solve_reliable_fp64": ([C.c_void_p, C.c_int, C.c_double, C.c_double, C.c_int, C.c_int, ip, ip, ip, dp,
+                                     dp], C.c_int),
     "self_adv2d_create": ([C.c_int, C.c_int, C.c_int, C.c_int, ip] + [dp] * 14 + [C.c_double, C.c_double,
                                                                           C.POINTER(C.c_void_p)], C.c_int),
     "self_adv2d_destroy": ([C.c_void_p], C.c_int),

@@ -1260,9 +1260,34 @@ try {
 }
 SELF_CATCH_ALL
 
+namespace {
+int solve_reliable_entry(self_cgsem2d_t h, int precond, double tol, double delta, int maxIters, int maxUpdates,
+                         int *ioerr, int *updates, int *iterations, double *relResidual, double *elapsed_ms,
+                         bool innerFP64);
+}
+
 int self_cgsem2d_solve_reliable(self_cgsem2d_t h, int precond, double tol, double delta, int maxIters,
                                 int maxUpdates, int *ioerr, int *updates, int *iterations, double *relResidual,
                                 double *elapsed_ms)
+{
+    return solve_reliable_entry(h, precond, tol, delta, maxIters, maxUpdates, ioerr, updates, iterations,
+                                relResidual, elapsed_ms, false);
+}
+
+int self_cgsem2d_solve_reliable_fp64(self_cgsem2d_t h, int precond, double tol, double delta, int maxIters,
+                                     int maxUpdates, int *ioerr, int *updates, int *iterations, double *relResidual,
+                                     double *elapsed_ms)
+{
+    return solve_reliable_entry(h, precond, tol, delta, maxIters, maxUpdates, ioerr, updates, iterations,
+                                relResidual, elapsed_ms, true);
+}
+
+}  // extern "C"
+
+namespace {
+int solve_reliable_entry(self_cgsem2d_t h, int precond, double tol, double delta, int maxIters, int maxUpdates,
+                         int *ioerr, int *updates, int *iterations, double *relResidual, double *elapsed_ms,
+                         bool innerFP64)
 try {
     SELF_REQUIRE(h && ioerr && updates && iterations, SELF_ERR_ARG, "null argument");
     SELF_REQUIRE(maxIters >= 1 && maxUpdates >= 1 && tol > 0.0, SELF_ERR_ARG,
@@ -1273,7 +1298,7 @@ try {
     int its = 0, ups = 0;
     SELF_HIP(hipEventRecord(h->ev0, op.stream));
     int rc = krylov_solve_reliable(op, h->wk, precond, tol, delta, maxIters, maxUpdates, op.xg, res, its, ups,
-                                   op.stream);
+                                   op.stream, innerFP64);
     if (rc) return rc;
     SELF_HIP(hipEventRecord(h->ev1, op.stream));
     SELF_HIP(hipEventSynchronize(h->ev1));
@@ -1290,4 +1315,4 @@ try {
 }
 SELF_CATCH_ALL
 
-}  // extern "C"
+}  // namespace

@@ -19,6 +19,7 @@
 
 #include <cmath>
 #include <cstring>
+#include <type_traits>
 #include <vector>
 
 #include "krylov.hpp"
@@ -338,8 +339,9 @@ __global__ __launch_bounds__(256) void k_convert(const A *__restrict__ a, B *__r
         b[g] = (B)((double)a[g] * scale);
 }
 
-__global__ __launch_bounds__(256) void k_add_scaled32(double *__restrict__ x, const float *__restrict__ e, size_t n,
-                                                      double scale)
+template <typename T>
+__global__ __launch_bounds__(256) void k_add_scaled(double *__restrict__ x, const T *__restrict__ e, size_t n,
+                                                    double scale)
 {
     for (size_t g = (size_t)blockIdx.x * blockDim.x + threadIdx.x; g < n; g += (size_t)gridDim.x * blockDim.x)
         x[g] = x[g] + scale * (double)e[g];
@@ -820,7 +822,7 @@ int krylov_solve_ir(DevOp &op, KrylovWork &wk, int precond, double tol, double i
         }
         if ((rc = read_state(wk, h, s))) return rc;
         innerTotal += h.iter;
-        hipLaunchKernelGGL(k_add_scaled32, dim3(gv), dim3(256), 0, s, x, wk.u32, n, rn);
+        hipLaunchKernelGGL(k_add_scaled<float>, dim3(gv), dim3(256), 0, s, x, (const float *)wk.u32, n, rn);
         SELF_HIP(hipGetLastError());
     }
     res.ioerr = -1;
@@ -829,34 +831,50 @@ int krylov_solve_ir(DevOp &op, KrylovWork &wk, int precond, double tol, double i
 }
 
 
-// Mixed precision with reliable updates (the solver of Clark et al.,
-// "Solving lattice QCD systems of equations using mixed precision solvers on
-// GPUs", 2010, applied to PCG): one FP32 PCG sequence on A e = r / |r0| that
-// is never restarted.  Whenever its recursive residual has dropped by
-// `delta` since the last update (or reaches the overall target), the FP32
-// iterate is added to the FP64 solution, the true residual is recomputed in
-// FP64 and replaces the FP32 one, and the iteration continues with the same
-// search direction (beta from the new residual).  Every vector update and
-// operator application runs in FP32; solution, residual and convergence
-// test are FP64.  The update and the decision to continue run on the
-// device (k_reliable_next), so an update costs the host no extra wait.
-// maxIters bounds the FP32 iterations, maxUpdates the refinements.
-// State: done 1 = FP32 segment reached its target, 2 = maxIters,
-// 4 = converged after an update, 5 = maxUpdates reached; l counts updates,
-// rc holds the last true residual norm.
-int krylov_solve_reliable(DevOp &op, KrylovWork &wk, int precond, double tol, double delta, int maxIters,
-                          int maxUpdates, double *x, KrylovResult &res, int &iters, int &updates, hipStream_t s)
+// Reliable updates (the solver of Clark et al., "Solving lattice QCD
+// systems of equations using mixed precision solvers on GPUs", 2010, applied
+// to PCG): one PCG sequence on A e = r / |r0| in the arithmetic T that is
+// never restarted.  Whenever its recursive residual has dropped by `delta`
+// since the last update (or reaches the overall target), the iterate is
+// added to the FP64 solution, the true residual is recomputed in FP64 and
+// replaces the recursive one, and the iteration continues with the same
+// search direction (beta from the new residual).  T = float: every vector
+// update and operator application runs in FP32 (the mixed solver); T =
+// double: an FP64 PCG whose stop test sees the true residual, not only the
+// recursively updated one.  Solution, residual and convergence test are
+// FP64.  The update and the decision to continue run on the device
+// (k_reliable_next), so an update costs the host no extra wait.  maxIters
+// bounds the inner iterations, maxUpdates the refinements.
+// State: done 1 = segment reached its target, 2 = maxIters, 4 = converged
+// after an update, 5 = maxUpdates reached; l counts updates, rc holds the
+// last true residual norm.
+template <typename T>
+int solve_reliable(DevOp &op, KrylovWork &wk, int precond, double tol, double delta, int maxIters, int maxUpdates,
+                   double *x, KrylovResult &res, int &iters, int &updates, hipStream_t s)
 {
-    if (!op.has_f32()) { set_error("operator has no FP32 form"); return SELF_ERR_STATE; }
+    constexpr bool F32 = std::is_same<T, float>::value;
+    if (F32 && !op.has_f32()) { set_error("operator has no FP32 form"); return SELF_ERR_STATE; }
     if (!(delta > 0.0 && delta < 1.0)) { set_error("reliable-update factor %g outside (0, 1)", delta); return SELF_ERR_ARG; }
     const size_t n = op.n;
-    int rc = wk.ensure(n, 0, maxIters + 1, true);
+    // FP64 inner sequence: its residual needs a buffer of its own beside the
+    // true residual wk.r -- wk.z, which operators that gather their input
+    // never use, else the first GMRES basis vector
+    const bool own = !F32 && !op.fuses_direction();
+    int rc = wk.ensure(n, own ? 1 : 0, maxIters + 1, F32);
     if (rc) return rc;
+    T *uI, *rI, *wI, *vA, *vB, *zI;
+    const T *dinv;
+    if constexpr (F32) {
+        uI = wk.u32; rI = wk.r32; wI = wk.w32; vA = wk.v32; vB = wk.vB32; zI = wk.z32;
+        dinv = precond ? op.inv_diag32() : nullptr;
+    } else {
+        uI = wk.u; rI = own ? wk.V : wk.z; wI = wk.w; vA = wk.v; vB = wk.vB; zI = wk.z;
+        dinv = precond ? op.inv_diag() : nullptr;
+    }
     const int gv = vgrid(n);
-    const float *dinv = precond ? op.inv_diag32() : nullptr;
     iters = 0;
     updates = 0;
-    // FP64 residual of the initial guess; it fixes the FP32 scaling
+    // FP64 residual of the initial guess; it fixes the scaling
     if ((rc = init_state(wk, tol, 1, kStopNormRel, 0, s))) return rc;
     if ((rc = op.residual(x, wk.r, wk.st, kFinBnorm, wk.slot, s))) return rc;
     SolverState h{};
@@ -867,25 +885,23 @@ int krylov_solve_reliable(DevOp &op, KrylovWork &wk, int precond, double tol, do
     res.ioerr = 0;
     res.iterations = 0;
     if (r0 == 0.0) return SELF_OK;
-    hipLaunchKernelGGL((k_convert<double, float>), dim3(gv), dim3(256), 0, s, wk.r, wk.r32, n, 1.0 / scale);
-    hipLaunchKernelGGL(k_fill<float>, dim3(gv), dim3(256), 0, s, wk.u32, n, 0.0f);
-    // FP32 state: iteration 1, stop test relative to |r32| = 1
+    hipLaunchKernelGGL((k_convert<double, T>), dim3(gv), dim3(256), 0, s, wk.r, rI, n, 1.0 / scale);
+    hipLaunchKernelGGL(k_fill<T>, dim3(gv), dim3(256), 0, s, uI, n, (T)0);
+    // inner state: iteration 1, stop test relative to |r / r0| = 1
     if ((rc = init_state(wk, std::max(delta, tol), maxIters, kStopNormRel, 0, s))) return rc;
-    SELF_RED_LAUNCH((k_dot<float, false>), (k_dot<float, true>), s, wk.r32, wk.r32, n, wk.st, (int)kFinR0Plain,
-                       wk.slot);
-    SELF_RED_LAUNCH((k_precond_dot<float, false>), (k_precond_dot<float, true>), s, wk.r32, dinv, wk.w32, n, wk.st,
+    SELF_RED_LAUNCH((k_dot<T, false>), (k_dot<T, true>), s, (const T *)rI, (const T *)rI, n, wk.st,
+                       (int)kFinR0Plain, wk.slot);
+    SELF_RED_LAUNCH((k_precond_dot<T, false>), (k_precond_dot<T, true>), s, (const T *)rI, dinv, wI, n, wk.st,
                        (int)kFinNum, wk.slot);
     // chunks sized from the last segment's length, so that the host sees a
     // segment's end soon after it happens (iterations launched after it
     // are no-ops)
     int it0 = 1, seg0 = 1, chunk = kChunk;
     for (;;) {
-        // FP32 iterations until the state reports a stop
+        // inner iterations until the state reports a stop
         for (;;) {
             const int it1 = std::min(it0 + chunk - 1, maxIters);
-            if ((rc = cg_chunk<float>(op, wk.u32, wk.r32, wk.w32, wk.v32, wk.vB32, wk.z32, dinv, n, wk.st, wk.slot,
-                                      it0, it1, s)))
-                return rc;
+            if ((rc = cg_chunk<T>(op, uI, rI, wI, vA, vB, zI, dinv, n, wk.st, wk.slot, it0, it1, s))) return rc;
             if ((rc = read_state(wk, h, s))) return rc;
             if (h.done) break;
             it0 = it1 + 1;
@@ -895,19 +911,19 @@ int krylov_solve_reliable(DevOp &op, KrylovWork &wk, int precond, double tol, do
         seg0 = h.iter + 1;
         // done 1 or 2: reliable update, x += scale e and r = b - A x in FP64
         hipLaunchKernelGGL(k_reliable_begin, dim3(1), dim3(64), 0, s, wk.st);
-        hipLaunchKernelGGL(k_add_scaled32, dim3(gv), dim3(256), 0, s, x, wk.u32, n, scale);
+        hipLaunchKernelGGL(k_add_scaled<T>, dim3(gv), dim3(256), 0, s, x, (const T *)uI, n, scale);
         if ((rc = op.residual(x, wk.r, wk.st, kFinBnorm, wk.slot, s))) return rc;
         hipLaunchKernelGGL(k_reliable_next, dim3(1), dim3(64), 0, s, wk.st, r0, scale, tol, delta, maxUpdates);
-        if (h.done == 2) {  // FP32 iterations exhausted: this update is the last
+        if (h.done == 2) {  // inner iterations exhausted: this update is the last
             if ((rc = read_state(wk, h, s))) return rc;
             break;
         }
         // continue from the true residual with the same direction (no-ops
         // once k_reliable_next has stopped the solve)
-        hipLaunchKernelGGL((k_convert<double, float>), dim3(gv), dim3(256), 0, s, wk.r, wk.r32, n, 1.0 / scale);
-        hipLaunchKernelGGL(k_fill<float>, dim3(gv), dim3(256), 0, s, wk.u32, n, 0.0f);
-        SELF_RED_LAUNCH((k_precond_dot<float, false>), (k_precond_dot<float, true>), s, wk.r32, dinv, wk.w32, n,
-                           wk.st, (int)kFinNum, wk.slot);
+        hipLaunchKernelGGL((k_convert<double, T>), dim3(gv), dim3(256), 0, s, wk.r, rI, n, 1.0 / scale);
+        hipLaunchKernelGGL(k_fill<T>, dim3(gv), dim3(256), 0, s, uI, n, (T)0);
+        SELF_RED_LAUNCH((k_precond_dot<T, false>), (k_precond_dot<T, true>), s, (const T *)rI, dinv, wI, n, wk.st,
+                           (int)kFinNum, wk.slot);
         it0 = h.iter + 1;
         if (it0 > maxIters) {  // k_reliable_next stops the solve (done 5)
             if ((rc = read_state(wk, h, s))) return rc;
@@ -921,6 +937,16 @@ int krylov_solve_reliable(DevOp &op, KrylovWork &wk, int precond, double tol, do
     res.rNorm = h.l > 0 ? h.rc : r0;
     res.ioerr = (h.done == 4 || res.rNorm <= tol * r0) ? 0 : -1;
     return SELF_OK;
+}
+
+int krylov_solve_reliable(DevOp &op, KrylovWork &wk, int precond, double tol, double delta, int maxIters,
+                          int maxUpdates, double *x, KrylovResult &res, int &iters, int &updates, hipStream_t s,
+                          bool innerFP64)
+{
+    return innerFP64 ? solve_reliable<double>(op, wk, precond, tol, delta, maxIters, maxUpdates, x, res, iters,
+                                              updates, s)
+                     : solve_reliable<float>(op, wk, precond, tol, delta, maxIters, maxUpdates, x, res, iters,
+                                             updates, s);
 }
 
 }  // namespace self

@@ -273,11 +273,12 @@ int krylov_cg_finish(DevOp &op, KrylovWork &wk, int stopRule, double tol, double
 int krylov_solve_ir(DevOp &op, KrylovWork &wk, int precond, double tol, double innerTol, int maxInner,
                     int maxOuter, double *sol, KrylovResult &res, int &innerTotal, hipStream_t s);
 
-// Mixed precision with reliable updates: one FP32 PCG sequence, never
-// restarted; the FP32 iterate is folded into the FP64 solution and the
-// residual recomputed in FP64 whenever the FP32 residual has dropped by
-// delta.  res.iterations counts the updates, iters the FP32 iterations.
+// Reliable updates: one PCG sequence (FP32, or FP64 with innerFP64), never
+// restarted; its iterate is folded into the FP64 solution and the residual
+// recomputed in FP64 whenever the recursive residual has dropped by delta.
+// res.iterations counts the updates, iters the inner iterations.
 int krylov_solve_reliable(DevOp &op, KrylovWork &wk, int precond, double tol, double delta, int maxIters,
-                          int maxUpdates, double *sol, KrylovResult &res, int &iters, int &updates, hipStream_t s);
+                          int maxUpdates, double *sol, KrylovResult &res, int &iters, int &updates, hipStream_t s,
+                          bool innerFP64 = false);
 
 }  // namespace self

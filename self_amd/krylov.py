@@ -68,17 +68,6 @@ class LinOp:
         L.check(L.lib().self_linop_elliptic1d_unmask(self._h, L.ptr(L.f64(sol)), L.ptr(u), L.ptr(x)))
         return x, u
 
-    def SolveReliable(self, tolerance=1e-10, delta=0.2, maxIters=40000, maxUpdates=400, precond="jacobi"):
-        """FP32 PCG with reliable updates (one never-restarted FP32 sequence,
-        FP64 solution and residual refreshed whenever the FP32 residual has
-        dropped by delta): (ioerr, updates, iterations, relResidual, ms)."""
-        io, up, it, rel, ms = C.c_int(), C.c_int(), C.c_int(), C.c_double(), C.c_double()
-        pc = {"none": 0, "jacobi": 1}[precond]
-        L.check(L.lib().self_cgsem2d_solve_reliable(self._h, pc, tolerance, delta, maxIters, maxUpdates,
-                                                    C.byref(io), C.byref(up), C.byref(it), C.byref(rel),
-                                                    C.byref(ms)))
-        return io.value, up.value, it.value, rel.value, ms.value
-
     def Trash(self):
         if getattr(self, "_h", None):
             L.check(L.lib().self_linop_destroy(self._h))
@@ -239,15 +228,18 @@ class CGsemElliptic2D:
                                                  C.byref(ms)))
         return io.value, outer.value, inner.value, rel.value, ms.value
 
-    def SolveReliable(self, tolerance=1e-10, delta=0.2, maxIters=40000, maxUpdates=400, precond="jacobi"):
-        """FP32 PCG with reliable updates (one never-restarted FP32 sequence,
-        FP64 solution and residual refreshed whenever the FP32 residual has
-        dropped by delta): (ioerr, updates, iterations, relResidual, ms)."""
+    def SolveReliable(self, tolerance=1e-10, delta=0.2, maxIters=40000, maxUpdates=400, precond="jacobi",
+                      inner="fp32"):
+        """PCG with reliable updates (one never-restarted sequence, FP64
+        solution and residual refreshed whenever the recursive residual has
+        dropped by delta): inner "fp32" runs the sequence in FP32, "fp64" in
+        FP64 (then the solve stops on the true residual).  Returns (ioerr,
+        updates, iterations, relResidual, ms)."""
         io, up, it, rel, ms = C.c_int(), C.c_int(), C.c_int(), C.c_double(), C.c_double()
         pc = {"none": 0, "jacobi": 1}[precond]
-        L.check(L.lib().self_cgsem2d_solve_reliable(self._h, pc, tolerance, delta, maxIters, maxUpdates,
-                                                    C.byref(io), C.byref(up), C.byref(it), C.byref(rel),
-                                                    C.byref(ms)))
+        fn = {"fp32": L.lib().self_cgsem2d_solve_reliable, "fp64": L.lib().self_cgsem2d_solve_reliable_fp64}[inner]
+        L.check(fn(self._h, pc, tolerance, delta, maxIters, maxUpdates, C.byref(io), C.byref(up), C.byref(it),
+                   C.byref(rel), C.byref(ms)))
         return io.value, up.value, it.value, rel.value, ms.value
 
     def Trash(self):

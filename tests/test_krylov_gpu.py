@@ -247,3 +247,31 @@ def test_cgsem_reliable_updates_reach_the_fp64_solution(gpu_lib):
     with pytest.raises(L.SelfError):
         g.SolveReliable(delta=1.0)
     g.Trash()
+
+
+def test_cgsem_fp64_reliable_updates_stop_on_the_true_residual(gpu_lib):
+    """The FP64 form of the reliable-update solve (an FP64 PCG sequence whose
+    residual is replaced by the true FP64 residual whenever it has dropped
+    by delta): the true residual, recomputed through the model's Residual,
+    meets the tolerance; the solution is the FP64 Jacobi-PCG solution to
+    1e-10; the sequence needs about as many iterations as the reference's
+    loop (it is never restarted); a repeat solve is bitwise the same."""
+    _, g, shape = _cgsem_pair(N=5, nX=8, nY=6, dirichlet=0.2)
+    io_o, it_o, _ = g.Solve("cgsem_pcg", precond="jacobi", tolerance=1e-12, maxIters=4000)
+    ref = g.GetSolution()
+    g.SetSolution(np.zeros(shape))
+    r0 = np.linalg.norm(g.Residual())
+    io, updates, its, rel, _ = g.SolveReliable(tolerance=1e-12, delta=1e-2, inner="fp64")
+    true = np.linalg.norm(g.Residual()) / r0
+    print(f"reference loop: {it_o} iterations; FP64 reliable: {its} iterations, {updates} updates, "
+          f"true residual {true:.3e}", flush=True)
+    assert io == 0 and updates >= 2
+    assert true <= 1e-12 and abs(true - rel) <= 1e-3 * rel
+    sol = g.GetSolution()
+    assert rel_l2(sol.ravel(), ref.ravel()) < 1e-10
+    assert its <= 1.1 * it_o + 10, (its, it_o)
+    g.SetSolution(np.zeros(shape))
+    io2, updates2, its2, rel2, _ = g.SolveReliable(tolerance=1e-12, delta=1e-2, inner="fp64")
+    assert (io2, updates2, its2, rel2) == (io, updates, its, rel)
+    assert np.array_equal(g.GetSolution(), sol)
+    g.Trash()
