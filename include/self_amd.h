@@ -184,8 +184,6 @@ int self_coarse_to_fine_2d(int nS, int nP, size_t nElems, const double *Ts, cons
                            const double *fOld, double *fNew);
 int self_coarse_to_fine_2d_dev(int nS, int nP, size_t nElems, const double *Ts, const double *Tp, int nNewS,
                                int nNewP, const double *fOld, double *fNew, void *stream);
-/* CoarseToFine_3D (src/interp/Lagrange_3D_Class.f90:545-604): (0:N)^3 ->
- * (0:nNew)^3 with T (0:nNew,0:N) in each direction. */
 /* No counterpart in the reference: self_coarse_to_fine_2d for callers that
  * cannot allocate device memory (code under HIP graph capture, output loops
  * of a serving process).  Same arguments and results (host arrays, bitwise
@@ -196,6 +194,8 @@ int self_coarse_to_fine_2d_dev(int nS, int nP, size_t nElems, const double *Ts, 
  * the array and are serialized inside the process. */
 int self_coarse_to_fine_2d_noalloc(int nS, int nP, size_t nElems, const double *Ts, const double *Tp, int nNewS,
                                    int nNewP, const double *fOld, double *fNew);
+/* CoarseToFine_3D (src/interp/Lagrange_3D_Class.f90:545-604): (0:N)^3 ->
+ * (0:nNew)^3 with T (0:nNew,0:N) in each direction. */
 int self_coarse_to_fine_3d(int N, size_t nElems, const double *T, int nNew, const double *fOld, double *fNew);
 int self_coarse_to_fine_3d_dev(int N, size_t nElems, const double *T, int nNew, const double *fOld, double *fNew,
                                void *stream);

@@ -49,7 +49,8 @@ SW2D_SRC := ../examples/shallowwater/planewave/localmods/SWParamsClass \
 
 all: $(OUT)/ref_adv2d $(OUT)/ref_adv3d $(OUT)/ref_cgsem2d $(OUT)/ref_krylov $(OUT)/ref_nodal $(OUT)/ref_pickup \
      $(OUT)/ref_specmesh $(OUT)/ref_sw2d $(OUT)/ref_cgsem_paraboloid $(OUT)/dropin_cgsem_paraboloid \
-     $(OUT)/ref_adv2d_ic $(OUT)/ref_adv2d_driver $(OUT)/dropin_adv2d_driver
+     $(OUT)/ref_adv2d_ic $(OUT)/ref_adv2d_driver $(OUT)/dropin_adv2d_driver \
+     $(OUT)/ref_adv3d_ic $(OUT)/ref_adv3d_driver $(OUT)/dropin_adv3d_ic $(OUT)/dropin_adv3d_driver
 
 # One recipe compiles the shared chain in order: each module needs the .mod
 # files of the ones before it.
@@ -121,6 +122,28 @@ $(OUT)/dropin_adv2d_driver: $(REF)/$(ADV2D_DRV).f90 $(FDROP)/adv_dropin.o
 	$(AMDFLANG) $(FFLAGS) -o $@.tmp $(OUT)/dropin/adv2d_driver.o $(FDROP)/self_amd_cabi.o \
 	  $(FDROP)/shallowwater_amd.o $(FDROP)/sw_dropin.o $(FDROP)/adv_dropin.o \
 	  -L$(HERE)../self_amd -lself_amd -Wl,-rpath,'$$ORIGIN/../../self_amd' && mv $@.tmp $@
+
+# The 3-D advection model's own programs (src/highend/advection3d), the same
+# way: against the reference's modules, and against the 3-D drop-in modules
+# (self_amd/fortran/_build3d, which reads the shared ones from _build).
+ADV3D_OBJ := $(addprefix $(OUT)/adv3d/,$(addsuffix .o,$(notdir $(ADV3D_SRC))))
+FDROP3    := $(HERE)../self_amd/fortran/_build3d
+DROP3_OBJ := $(FDROP)/self_amd_cabi.o $(FDROP)/shallowwater_amd.o $(FDROP)/sw_dropin.o $(FDROP3)/adv3d_dropin.o
+# $(call adv3d_prog,short name,program file under highend/advection3d)
+define adv3d_prog
+$(OUT)/ref_adv3d_$(1): $(OUT)/ref_adv3d $(REF)/highend/advection3d/$(2).f90
+	$(AMDFLANG) $(FFLAGS) -J$(OUT)/adv3d -I$(OUT) -c $(REF)/highend/advection3d/$(2).f90 -o $(OUT)/adv3d/prog_$(1).o
+	$(AMDFLANG) $(FFLAGS) -o $$@.tmp $(COMMON_OBJ) $(ADV3D_OBJ) $(OUT)/adv3d/prog_$(1).o && mv $$@.tmp $$@
+
+$(OUT)/dropin_adv3d_$(1): $(REF)/highend/advection3d/$(2).f90 $(FDROP3)/adv3d_dropin.o
+	@mkdir -p $(OUT)/dropin3d
+	$(AMDFLANG) $(FFLAGS) -J$(OUT)/dropin3d -I$(FDROP3) -I$(FDROP) -c $(REF)/highend/advection3d/$(2).f90 \
+	  -o $(OUT)/dropin3d/prog_$(1).o
+	$(AMDFLANG) $(FFLAGS) -o $$@.tmp $(OUT)/dropin3d/prog_$(1).o $(DROP3_OBJ) \
+	  -L$(HERE)../self_amd -lself_amd -Wl,-rpath,'$$$$ORIGIN/../../self_amd' && mv $$@.tmp $$@
+endef
+$(eval $(call adv3d_prog,ic,Advection3D_InitialConditions))
+$(eval $(call adv3d_prog,driver,Advection3D_Driver))
 
 clean:
 	rm -rf $(OUT)

@@ -104,6 +104,85 @@ MODULE SELF_AMD_CABI
       REAL(C_DOUBLE), INTENT(out) :: x(*), y(*), dxds(*), dxdp(*), dyds(*), dydp(*), J(*), nHat(*), nLen(*)
     END FUNCTION
 
+    ! ------------------------------------------------------------ 3-D set-up and advection
+    INTEGER(C_INT) FUNCTION self_coarse_to_fine_3d(N, nElems, T, nNew, fOld, fNew) BIND(C, name="self_coarse_to_fine_3d")
+      IMPORT :: C_INT, C_DOUBLE, C_SIZE_T
+      INTEGER(C_INT), VALUE :: N, nNew
+      INTEGER(C_SIZE_T), VALUE :: nElems
+      REAL(C_DOUBLE) :: T(*), fOld(*), fNew(*)
+    END FUNCTION
+    INTEGER(C_INT) FUNCTION self_nodal3d_build(N, quadType, form, nodes, qWeight, dMat, lagLeft, lagRight) &
+                                               BIND(C, name="self_nodal3d_build")
+      IMPORT :: C_INT, C_DOUBLE
+      INTEGER(C_INT), VALUE :: N, quadType, form
+      REAL(C_DOUBLE), INTENT(out) :: nodes(*), qWeight(*), dMat(*), lagLeft(*), lagRight(*)
+    END FUNCTION
+    INTEGER(C_INT) FUNCTION self_rolloff_filter1d(N, nCutoff, nodes, w, F1) BIND(C, name="self_rolloff_filter1d")
+      IMPORT :: C_INT, C_DOUBLE
+      INTEGER(C_INT), VALUE :: N, nCutoff
+      REAL(C_DOUBLE), INTENT(in) :: nodes(*), w(*)
+      REAL(C_DOUBLE), INTENT(out) :: F1(*)
+    END FUNCTION
+    INTEGER(C_INT) FUNCTION self_hexmesh_counts(nX, nY, nZ, periodicX, periodicY, periodicZ, nElems, nFaces) &
+                                                BIND(C, name="self_hexmesh_counts")
+      IMPORT :: C_INT
+      INTEGER(C_INT), VALUE :: nX, nY, nZ, periodicX, periodicY, periodicZ
+      INTEGER(C_INT), INTENT(out) :: nElems, nFaces
+    END FUNCTION
+    INTEGER(C_INT) FUNCTION self_hexmesh_structured(nX, nY, nZ, N, periodicX, periodicY, periodicZ, xScale, yScale, &
+                                                    zScale, warp, warpSign, nodes, faces, x, y, z, cov, J, nHat, &
+                                                    nLen, bpos) BIND(C, name="self_hexmesh_structured")
+      IMPORT :: C_INT, C_DOUBLE
+      INTEGER(C_INT), VALUE :: nX, nY, nZ, N, periodicX, periodicY, periodicZ
+      REAL(C_DOUBLE), VALUE :: xScale, yScale, zScale, warp
+      INTEGER(C_INT), INTENT(in) :: warpSign(*)
+      REAL(C_DOUBLE), INTENT(in) :: nodes(*)
+      INTEGER(C_INT), INTENT(out) :: faces(*)
+      REAL(C_DOUBLE), INTENT(out) :: x(*), y(*), z(*), cov(*), J(*), nHat(*), nLen(*), bpos(*)
+    END FUNCTION
+    INTEGER(C_INT) FUNCTION self_adv3d_create(N, nTracers, nElems, nFaces, faces, cov, J, nHat, nLen, bpos, dMat, &
+                                              qWeight, lagLeft, lagRight, filter1d, useFilter, dt, h) &
+                                              BIND(C, name="self_adv3d_create")
+      IMPORT :: C_INT, C_DOUBLE, C_PTR
+      INTEGER(C_INT), VALUE :: N, nTracers, nElems, nFaces, useFilter
+      INTEGER(C_INT), INTENT(in) :: faces(*)
+      REAL(C_DOUBLE), INTENT(in) :: cov(*), J(*), nHat(*), nLen(*), bpos(*), dMat(*), qWeight(*), lagLeft(*), &
+                                    lagRight(*), filter1d(*)
+      REAL(C_DOUBLE), VALUE :: dt
+      TYPE(C_PTR), INTENT(out) :: h
+    END FUNCTION
+    INTEGER(C_INT) FUNCTION self_adv3d_destroy(h) BIND(C, name="self_adv3d_destroy")
+      IMPORT :: C_INT, C_PTR
+      TYPE(C_PTR), VALUE :: h
+    END FUNCTION
+    INTEGER(C_INT) FUNCTION self_adv3d_set_field(h, field, dat) BIND(C, name="self_adv3d_set_field")
+      IMPORT :: C_INT, C_PTR, C_DOUBLE
+      TYPE(C_PTR), VALUE :: h
+      INTEGER(C_INT), VALUE :: field
+      REAL(C_DOUBLE), INTENT(in) :: dat(*)
+    END FUNCTION
+    INTEGER(C_INT) FUNCTION self_adv3d_get_field(h, field, dat) BIND(C, name="self_adv3d_get_field")
+      IMPORT :: C_INT, C_PTR, C_DOUBLE
+      TYPE(C_PTR), VALUE :: h
+      INTEGER(C_INT), VALUE :: field
+      REAL(C_DOUBLE), INTENT(out) :: dat(*)
+    END FUNCTION
+    INTEGER(C_INT) FUNCTION self_adv3d_global_time_derivative(h, t, stage) &
+                                                             BIND(C, name="self_adv3d_global_time_derivative")
+      IMPORT :: C_INT, C_PTR, C_DOUBLE
+      TYPE(C_PTR), VALUE :: h
+      REAL(C_DOUBLE), VALUE :: t
+      INTEGER(C_INT), VALUE :: stage
+    END FUNCTION
+    INTEGER(C_INT) FUNCTION self_adv3d_forward_step_rk3(h, tn, nSteps, elapsedMs) &
+                                                       BIND(C, name="self_adv3d_forward_step_rk3")
+      IMPORT :: C_INT, C_PTR, C_DOUBLE
+      TYPE(C_PTR), VALUE :: h
+      REAL(C_DOUBLE), VALUE :: tn
+      INTEGER(C_INT), VALUE :: nSteps
+      TYPE(C_PTR), VALUE :: elapsedMs
+    END FUNCTION
+
     ! ------------------------------------------------------------ shallow water 2-D
     INTEGER(C_INT) FUNCTION self_sw2d_create(desc, h) BIND(C, name="self_sw2d_create")
       IMPORT :: C_INT, C_PTR, self_sw2d_desc

@@ -113,3 +113,30 @@ def test_reference_adv2d_driver_compiles_unchanged_against_the_drop_in(tmp_path)
                        capture_output=True, text=True, timeout=300)
     assert r.returncode == 0, r.stderr[-3000:]
     assert os.path.exists(exe)
+
+
+@pytest.mark.parametrize("program", ["Advection3D_Driver", "Advection3D_InitialConditions"])
+def test_reference_adv3d_programs_compile_unchanged_against_the_drop_in(tmp_path, program):
+    """The reference's 3-D advection programs (src/highend/advection3d/
+    Advection3D_Driver.f90 and Advection3D_InitialConditions.f90, read in
+    place: Build with and without the pickup, the clocks of Timing,
+    dgStorage % GetNumberOfNodes, mesh % GetPositionAtNode, SetSolution,
+    SetVelocity, SetRelaxationField/Factor, GlobalTimeDerivative(t, m),
+    WriteTecplot, ForwardStepRK3, WritePickup, mesh % WriteTecplot, Trash)
+    compile and link, unmodified, against the drop-in modules of
+    self_amd/fortran/adv3d_dropin.f90 (module directory _build3d, shared
+    modules from _build) and libself_amd.so."""
+    src = f"/root/reference/src/highend/advection3d/{program}.f90"
+    build = os.path.join(ROOT, "self_amd", "fortran", "_build")
+    build3 = os.path.join(ROOT, "self_amd", "fortran", "_build3d")
+    fc = "/opt/rocm/bin/amdflang"
+    if not (os.path.exists(src) and os.path.exists(os.path.join(build3, "adv3d_dropin.o")) and os.path.exists(fc)):
+        pytest.skip("reference tree, amdflang or the built drop-in modules absent")
+    obj = [os.path.join(build, f) for f in ("self_amd_cabi.o", "shallowwater_amd.o", "sw_dropin.o")]
+    obj.append(os.path.join(build3, "adv3d_dropin.o"))
+    exe = str(tmp_path / "prog")
+    r = subprocess.run([fc, "-O2", "-I", build3, "-I", build, "-J", str(tmp_path), src] + obj +
+                       ["-L" + os.path.join(ROOT, "self_amd"), "-lself_amd", "-o", exe],
+                       capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stderr[-3000:]
+    assert os.path.exists(exe)
