@@ -137,6 +137,34 @@ struct CgDirLds {
 // workgroup in row 0 at entry b: the same number as the dot product of v
 // with the assembled A v, so the assembly can be fused with the vector
 // update that follows.
+// A node's DOF index and geometry for one group of a direction step.  The
+// loads are unconditional at a clamped index (node 0 of element 0 for a
+// thread without a node, whose values are then discarded), so that no branch
+// separates them and they stay in flight together.
+template <typename T>
+struct CgNodeLoads {
+    int g;
+    T g11, g12, g22;
+};
+
+template <int N, typename T>
+__device__ __forceinline__ CgNodeLoads<T> cg_node_loads(const int *__restrict__ gid, const T *__restrict__ G3,
+                                                        int nE, size_t stride, int grp, int le, int nd)
+{
+    constexpr int np = Shape<N>::np, epb = Shape<N>::epb;
+    const int e = grp * epb + le;
+    const bool act = grp >= 0 && le < epb && e < nE;
+    const size_t lid = act ? (size_t)e * np + nd : 0;
+    CgNodeLoads<T> o;
+    const int g = gid[lid];
+    const T a = G3[lid], bb = G3[stride + lid], c = G3[2 * stride + lid];
+    o.g = act ? g : -1;
+    o.g11 = act ? a : (T)0;
+    o.g12 = act ? bb : (T)0;
+    o.g22 = act ? c : (T)0;
+    return o;
+}
+
 template <int N, typename T>
 __device__ __forceinline__ void cg_direction_wg(const T *__restrict__ x, const T *__restrict__ vOld,
                                                 T *__restrict__ vNew, const int *__restrict__ gid,
@@ -146,6 +174,16 @@ __device__ __forceinline__ void cg_direction_wg(const T *__restrict__ x, const T
 {
     using S = Shape<N>;
     constexpr int n1 = S::n1, np = S::np, epb = S::epb;
+    static_assert(n1 * n1 <= 256, "one thread per entry of the nodal matrix");
+    const int t = threadIdx.x, le = t / np, nd = t - le * np;
+    const int i = nd % n1, j = nd / n1;
+    const size_t stride = (size_t)nE * np;
+    const int nGroups = (nE + epb - 1) / epb;
+    // What does not depend on beta is loaded first, so that its latency
+    // overlaps the fold of the previous iteration's sums: the nodal matrix
+    // and the first group's DOF indices and geometry.
+    const T dk = Dg[t < n1 * n1 ? t : 0];
+    CgNodeLoads<T> cur = cg_node_loads<N, T>(gid, G3, nE, stride, b < nGroups ? b : -1, le, nd);
     T beta;
     if (pending) {
         // workgroup 0 may set `done` during this kernel, so each workgroup
@@ -165,38 +203,38 @@ __device__ __forceinline__ void cg_direction_wg(const T *__restrict__ x, const T
         if (st->done) return;  // nothing writes it during this kernel
         beta = (T)st->beta;
     }
-    for (int k = threadIdx.x; k < n1 * n1; k += blockDim.x) sm.L.D[k] = Dg[k];
-    const int t = threadIdx.x, le = t / np, nd = t - le * np;
-    const int i = nd % n1, j = nd / n1;
-    const size_t stride = (size_t)nE * np;
-    const int nGroups = (nE + epb - 1) / epb;
+    if (t < n1 * n1) sm.L.D[t] = dk;
+    // iteration 1 has no previous direction: it reads x twice instead
+    const T *__restrict__ vPrev = it == 1 ? x : vOld;
     T *us = sm.us, *f1 = sm.f1, *f2 = sm.f2;
     double acc = 0.0;
     for (int grp = b; grp < nGroups; grp += nB) {
         const int e = grp * epb + le;
         const bool act = le < epb && e < nE;
         const size_t lid = (size_t)e * np + nd;
-        T u = 0, g11 = 0, g12 = 0, g22 = 0;
-        if (act) {
-            const int g = gid[lid];
-            if (g >= 0) {
-                const T wv = x[g];
-                u = (it == 1) ? wv : wv + beta * vOld[g];
-                vNew[g] = u;
-            }
-            us[le * np + nd] = u;
-            g11 = G3[lid];
-            g12 = G3[stride + lid];
-            g22 = G3[2 * stride + lid];
+        const int g = cur.g;
+        const int gc = g >= 0 ? g : 0;  // a masked node reads DOF 0 and keeps u = 0
+        T wv = x[gc], vo = vPrev[gc];
+        // an empty statement that takes the gathered values, so the next
+        // group's loads below are issued after them and stay in flight
+        // while this group computes (3 % per iteration at 128x128, N = 7)
+        asm volatile("" : "+v"(wv), "+v"(vo));
+        const CgNodeLoads<T> nxt = cg_node_loads<N, T>(gid, G3, nE, stride, grp + nB < nGroups ? grp + nB : -1, le, nd);
+        T u = 0;
+        if (g >= 0) {
+            u = (it == 1) ? wv : wv + beta * vo;
+            vNew[g] = u;
         }
+        if (act) us[le * np + nd] = u;
         __syncthreads();
-        if (act) cg_flux<N>(sm.L.D, us + le * np, i, j, g11, g12, g22, f1 + le * np, f2 + le * np);
+        if (act) cg_flux<N>(sm.L.D, us + le * np, i, j, cur.g11, cur.g12, cur.g22, f1 + le * np, f2 + le * np);
         __syncthreads();
         if (act) {
             const T Lv = cg_divergence<N>(sm.L.D, f1 + le * np, f2 + le * np, i, j);
             Lbuf[lid] = Lv;
             acc += (double)u * (double)Lv;
         }
+        cur = nxt;
     }
     store_partial_at(slot, 0, acc, sm.lds4, b);
 }
@@ -212,13 +250,61 @@ __global__ __launch_bounds__(256) void k_cg_direction(const T *__restrict__ x, c
     cg_direction_wg<N, T>(x, vOld, vNew, gid, G3, Dg, Lbuf, nE, st, slot, it, pending, blockIdx.x, gridDim.x, sm);
 }
 
+// One DOF's own values and copy range for an update sweep (nG >= 1, so the
+// clamped index g is valid): independent loads, issued together.
+template <typename T>
+struct CgDofLoads {
+    int c0, c1;
+    T u, v, r, d;
+};
+
+template <typename T, bool PRE>
+__device__ __forceinline__ CgDofLoads<T> cg_dof_loads(const int *__restrict__ cs, const T *__restrict__ u,
+                                                      const T *__restrict__ v, const T *__restrict__ r,
+                                                      const T *__restrict__ dinv, size_t g)
+{
+    CgDofLoads<T> o;
+    o.c0 = cs[g];
+    o.c1 = cs[g + 1];
+    o.u = u[g];
+    o.v = v[g];
+    o.r = r[g];
+    o.d = (T)0;
+    if constexpr (PRE) o.d = dinv[g];
+    return o;
+}
+
+// Sum of the copies c0 .. c1-1 of one DOF in GlobalSum order (c1 > c0): the
+// first four are loaded together at indices clamped to the DOF's range (a
+// loop would wait for each copy before loading the next), any further ones
+// after them; the additions are the loop's, in the loop's order.
+template <typename T>
+__device__ __forceinline__ T cg_copies_sum(const T *__restrict__ Lbuf, const int *__restrict__ ci, int c0, int c1)
+{
+    const int last = c1 - 1;
+    const int i0 = ci[c0], i1 = ci[min(c0 + 1, last)], i2 = ci[min(c0 + 2, last)], i3 = ci[min(c0 + 3, last)];
+    T l0 = Lbuf[i0], l1 = Lbuf[i1], l2 = Lbuf[i2], l3 = Lbuf[i3];
+    // an empty statement that takes the values: without it the compiler
+    // moves each load under the test that uses it, one after the other
+    asm volatile("" : "+v"(l0), "+v"(l1), "+v"(l2), "+v"(l3));
+    const int n = c1 - c0;
+    T z = l0;
+    if (n > 1) z = z + l1;
+    if (n > 2) z = z + l2;
+    if (n > 3) z = z + l3;
+    for (int c = c0 + 4; c < c1; ++c) z = z + Lbuf[ci[c]];
+    return z;
+}
+
 // The rest of the folded CG iteration in one sweep over the global DOFs, as
 // workgroup b of kFoldParts (kFoldThreads threads each): alpha = num / v.Av
 // (each workgroup sums the direction step's nPart partials itself), z = A v
 // assembled from the copies in GlobalSum order and used at once:
 // u += alpha v, r -= alpha z, w = P r; r.r and r.w partials in rows 1 and 2
 // at entry b.  z is never stored.  A workgroup b >= ceil(nG / kFoldThreads)
-// has no DOF: its partials are zero.
+// has no DOF: its partials are zero.  A DOF's own values and copy range do
+// not depend on alpha, so the first DOF's are loaded before the state check
+// and the fold, and each next DOF's while the current one is summed.
 template <typename T, bool PRE>
 __device__ __forceinline__ void cg_update_wg(const T *__restrict__ Lbuf, const int *__restrict__ cs,
                                              const int *__restrict__ ci, size_t nG, T *__restrict__ u,
@@ -226,6 +312,9 @@ __device__ __forceinline__ void cg_update_wg(const T *__restrict__ Lbuf, const i
                                              T *__restrict__ w, SolverState *st, const RedSlot &slot, int it,
                                              int nPart, int b)
 {
+    const size_t step = (size_t)kFoldParts * kFoldThreads;
+    size_t g = (size_t)b * kFoldThreads + threadIdx.x;
+    CgDofLoads<T> cur = cg_dof_loads<T, PRE>(cs, u, v, r, dinv, g < nG ? g : nG - 1);
     if (st->done) return;
     const double vz = block_fold_sum<kFoldThreads>(slot.partials, nPart);
     const double a = st->numP[it & 1] / vz;
@@ -235,19 +324,20 @@ __device__ __forceinline__ void cg_update_wg(const T *__restrict__ Lbuf, const i
     }
     const T alpha = (T)a;
     double acc0 = 0.0, acc1 = 0.0;
-    for (size_t g = (size_t)b * kFoldThreads + threadIdx.x; g < nG; g += (size_t)kFoldParts * kFoldThreads) {
-        const int c0 = cs[g], c1 = cs[g + 1];
-        T z = Lbuf[ci[c0]];
-        for (int c = c0 + 1; c < c1; ++c) z = z + Lbuf[ci[c]];
-        u[g] = u[g] + alpha * v[g];
-        const T rv = r[g] - alpha * z;
+    for (; g < nG; g += step) {
+        const size_t gn = g + step;
+        const CgDofLoads<T> nxt = cg_dof_loads<T, PRE>(cs, u, v, r, dinv, gn < nG ? gn : nG - 1);
+        const T z = cg_copies_sum(Lbuf, ci, cur.c0, cur.c1);
+        u[g] = cur.u + alpha * cur.v;
+        const T rv = cur.r - alpha * z;
         r[g] = rv;
         acc0 += (double)rv * (double)rv;
         if constexpr (PRE) {
-            const T wv = dinv[g] * rv;
+            const T wv = cur.d * rv;
             w[g] = wv;
             acc1 += (double)rv * (double)wv;
         }
+        cur = nxt;
     }
     if constexpr (!PRE) acc1 = acc0;
     store_partials2_at<kFoldThreads>(slot, 1, acc0, 2, acc1, b);
@@ -562,6 +652,8 @@ int build_maps(CgsemOp &op, int nEdges, int nNodes, const int *edges, const int 
         if (d >= 0 && cpList[cpStart[d]] == (int)l) number[d] = next++;
     }
     op.nG = (size_t)next;
+    // interior nodes are never Dirichlet nodes, so N >= 2 leaves at least one
+    SELF_REQUIRE(op.nG >= 1, SELF_ERR_ARG, "no global degree of freedom");
     op.gidH.assign(nE * np, -1);
     for (size_t l = 0; l < nE * (size_t)np; ++l)
         if (dofKey[l] >= 0) op.gidH[l] = number[dofKey[l]];

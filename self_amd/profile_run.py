@@ -12,8 +12,9 @@
 
 `sw2d-dipole` runs the shallow-water model on an ISM-V2 mesh file (the
 dipole run's box.mesh, scaled by 4e5) with the dipole state.
-`copy` runs the STREAM-style copy kernel on 64 MiB (21 launches) and 1 GiB
-(11 launches): the calibration for hardware byte counters (FETCH_SIZE /
+`copy` runs the STREAM-style copy kernel on 24 MiB (read and written: about
+the shallow-water stage's 49.5 MB of traffic at 64x64, N = 7; 21
+launches), 64 MiB (21 launches) and 1 GiB (11 launches): the calibration for hardware byte counters (FETCH_SIZE /
 WRITE_SIZE of a copy of known size).  Every other mode builds the named
 workload with its seeded inputs and runs the HIP path only.  For cgsem,
 --precision fp64 runs the reference's PCG loop and mixed the FP32 PCG with
@@ -48,7 +49,7 @@ def main(argv=None):
     L.check(L.lib().self_init(0))
     if a.mode == "copy":
         g, ms = C.c_double(), C.c_double()
-        for size, iters in ((64 << 20, 20), (1 << 30, 10)):
+        for size, iters in ((24 << 20, 20), (64 << 20, 20), (1 << 30, 10)):
             L.check(L.lib().self_stream_copy(size, iters, C.byref(g), C.byref(ms)))
             print(f"copy {size} B: {g.value:.1f} GB/s, {ms.value * 1e3:.2f} us per copy", flush=True)
     elif a.mode == "sw2d":
