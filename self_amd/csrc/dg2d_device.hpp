@@ -213,6 +213,69 @@ __device__ __forceinline__ void stage_nodal(NodalLDS<N> &L, const double *__rest
     }
 }
 
+// Nodal constants in two steps around a kernel's element loads: nodal_fetch
+// issues this thread's loads of the nodal buffer (at clamped indices, so no
+// branch separates them) before the element data, nodal_put writes them to
+// LDS after it.  The counter of outstanding loads is in order, so writing
+// them then waits for these loads only; staged after the element loads (as
+// stage_nodal does) the write waits for every one of them, and no phase can
+// start before the last element load has arrived.  LANES threads (the
+// workgroup or one wavefront) share the copy, lane = 0 .. LANES-1.
+template <int N, bool F32, int LANES>
+struct NodalRegs {
+    static constexpr int n1 = N + 1, nd = 2 * n1 * n1 + 8 * n1;
+    static constexpr int K = (nd + LANES - 1) / LANES;
+    static constexpr int KF = F32 ? (n1 * n1 + LANES - 1) / LANES : 1;
+    double v[K];
+    float s[KF], p[KF];
+};
+
+template <int N, bool F32, int LANES>
+__device__ __forceinline__ void nodal_fetch(NodalRegs<N, F32, LANES> &R, const double *__restrict__ g,
+                                            const float *__restrict__ dS32, const float *__restrict__ dP32, int lane)
+{
+    using Rg = NodalRegs<N, F32, LANES>;
+    constexpr int n1 = Rg::n1, nd = Rg::nd;
+    #pragma unroll
+    for (int q = 0; q < Rg::K; ++q) {
+        const int k = min(lane + q * LANES, nd - 1);
+        const bool w = (k >= 2 * n1 * n1) && (k < 2 * n1 * n1 + 2 * n1);
+        const bool r = k >= 2 * n1 * n1 + 6 * n1;
+        R.v[q] = g[w ? k + 6 * n1 : (r ? k + 2 * n1 : k)];
+    }
+    if constexpr (F32) {
+        #pragma unroll
+        for (int q = 0; q < Rg::KF; ++q) {
+            const int k = min(lane + q * LANES, n1 * n1 - 1);
+            R.s[q] = dS32[k];
+            R.p[q] = dP32[k];
+        }
+    }
+}
+
+template <int N, bool F32, int LANES>
+__device__ __forceinline__ void nodal_put(NodalLDS<N> &L, const NodalRegs<N, F32, LANES> &R, int lane)
+{
+    using Rg = NodalRegs<N, F32, LANES>;
+    constexpr int n1 = Rg::n1, nd = Rg::nd;
+    double *dst = L.dS;  // members up to rsP are contiguous doubles
+    #pragma unroll
+    for (int q = 0; q < Rg::K; ++q) {
+        const int k = lane + q * LANES;
+        if (k < nd) dst[k] = R.v[q];
+    }
+    if constexpr (F32) {
+        #pragma unroll
+        for (int q = 0; q < Rg::KF; ++q) {
+            const int k = lane + q * LANES;
+            if (k < n1 * n1) {
+                L.dS32[k] = R.s[q];
+                L.dP32[k] = R.p[q];
+            }
+        }
+    }
+}
+
 // Orders a wavefront's LDS writes before its following LDS reads of other
 // lanes' data: the compiler keeps the order (fences, wave barrier) and the
 // wavefront waits for its outstanding LDS operations (s_waitcnt lgkmcnt(0),

@@ -441,12 +441,26 @@ __device__ __forceinline__ void side_normal(const SWKernelArgs &A, size_t e, int
 }
 
 // Loads of one node (element e, node nd), and for nd < 4 (N+1) of that
-// element's side node nd (side nd / (N+1), node nd mod (N+1)).
+// element's side node nd (side nd / (N+1), node nd mod (N+1)).  Every
+// thread loads from an address that is valid whatever its node: node 0 of
+// element 0 for a thread without a node (act false: it computes nothing
+// from them), side node nd mod 4 (N+1) for a thread without a side node
+// (its side values are never read).  No branch depends on the thread, so
+// the loads stay in flight together and each phase waits only for the loads
+// it uses: behind a thread-dependent branch the compiler waits for all
+// outstanding loads where the branch joins, and no phase could start before
+// the element's last load had arrived.  Optional fields (RK register, depth
+// for the drag, relaxation) stay under uniform tests.
 template <int N, bool MIXED>
 __device__ __forceinline__ void sw_load(const SWKernelArgs &A, int e, int nd, bool act, SWPre &P)
 {
     constexpr int n1 = N + 1, np = n1 * n1;
-    const size_t mstride = (size_t)A.nE * np, base = (size_t)e * np + nd;
+    const size_t mstride = (size_t)A.nE * np;
+    const size_t base = act ? (size_t)e * np + nd : 0;
+    const int ec = act ? e : 0;
+    const int ns = nd % (4 * n1);
+    const int sd0 = ns / n1, k0 = ns - sd0 * n1;
+    const size_t es = (size_t)ec * 4 + sd0;
     P.Jv = 1.0; P.hRaw = 1.0;
     #pragma unroll
     for (int k = 0; k < 3; ++k) {
@@ -457,28 +471,25 @@ __device__ __forceinline__ void sw_load(const SWKernelArgs &A, int e, int nd, bo
     #pragma unroll
     for (int k = 0; k < 5; ++k) { P.metD[k] = 0; P.metM[k] = 0; }
     P.hx = 0; P.hy = 0; P.vort = 0; P.hx32 = 0; P.hy32 = 0; P.vort32 = 0;
-    P.info0 = make_int2(0, 0);
-    if (!act) return;
-    if (nd < 4 * n1) {  // this thread's side node (all of them for N >= 3)
-        const int sd0 = nd / n1, k0 = nd - sd0 * n1;
-        const size_t es = (size_t)e * 4 + sd0;
-        P.info0 = A.side[es];
-        if constexpr (MIXED) {
-            #pragma unroll
-            for (int c = 0; c < 3; ++c) { P.fnM[c] = A.fnM[es * 3 + c]; P.fnD[c] = A.fnD[(es * 3 + c) * n1 + k0]; }
-        } else {
-            const size_t nb = es * 3 * n1 + k0;
-            P.fn0[0] = A.fnrm[nb]; P.fn0[1] = A.fnrm[nb + n1]; P.fn0[2] = A.fnrm[nb + 2 * n1];
-        }
-        const double *tr = A.trIn + es * 3 * n1 + k0;
-        P.oth0[0] = tr[0]; P.oth0[1] = tr[n1]; P.oth0[2] = tr[2 * n1];
+    P.info0 = A.side[es];
+    if constexpr (MIXED) {
+        #pragma unroll
+        for (int c = 0; c < 3; ++c) { P.fnM[c] = A.fnM[es * 3 + c]; P.fnD[c] = A.fnD[(es * 3 + c) * n1 + k0]; }
+    } else {
+        const size_t nb = es * 3 * n1 + k0;
+        P.fn0[0] = A.fnrm[nb]; P.fn0[1] = A.fnrm[nb + n1]; P.fn0[2] = A.fnrm[nb + 2 * n1];
     }
+    const double *tr = A.trIn + es * 3 * n1 + k0;
+    P.oth0[0] = tr[0]; P.oth0[1] = tr[n1]; P.oth0[2] = tr[2 * n1];
     #pragma unroll
     for (int f = 0; f < 3; ++f) P.s[f] = A.q[f * mstride + base];
     if constexpr (MIXED) {
         #pragma unroll
-        for (int k = 0; k < 5; ++k) { P.metM[k] = A.metM[(size_t)e * 5 + k]; P.metD[k] = A.metD[k * mstride + base]; }
+        for (int k = 0; k < 5; ++k) { P.metM[k] = A.metM[(size_t)ec * 5 + k]; P.metD[k] = A.metD[k * mstride + base]; }
         P.hx32 = A.bathy32[base]; P.hy32 = A.bathy32[mstride + base]; P.vort32 = A.plv32[base];
+        // the optional fields after the others (their uniform tests would
+        // otherwise let the compiler issue them first)
+        asm volatile("" ::: "memory");
         if (A.a != 0.0) {
             #pragma unroll
             for (int f = 0; f < 3; ++f) P.g32[f] = A.G32[f * mstride + base];
@@ -488,6 +499,7 @@ __device__ __forceinline__ void sw_load(const SWKernelArgs &A, int e, int nd, bo
         for (int k = 0; k < 4; ++k) P.met[k] = A.metric[k * mstride + base];
         P.Jv = A.metric[4 * mstride + base];
         P.hx = A.bathy[mstride + base]; P.hy = A.bathy[2 * mstride + base]; P.vort = A.plv[base];
+        asm volatile("" ::: "memory");
         if (A.a != 0.0) {
             #pragma unroll
             for (int f = 0; f < 3; ++f) P.gold[f] = A.G64[f * mstride + base];
@@ -692,6 +704,10 @@ __device__ __forceinline__ void sw_stage_group(const SWKernelArgs &A0, int wg, S
     }
     NodalLDS<N> &L = U.Lw[WL ? threadIdx.x / 64 : 0];
     const int t = threadIdx.x, le = t / np, nd = t - le * np;
+    constexpr int kLanes = WL ? 64 : 256;
+    const int lane = WL ? threadIdx.x % 64 : threadIdx.x;
+    NodalRegs<N, MIXED, kLanes> R;
+    nodal_fetch<N, MIXED, kLanes>(R, A.dS, A.dS32, A.dP32, lane);
     SWPre P[G];
     int e[G];
     bool act[G];
@@ -701,12 +717,8 @@ __device__ __forceinline__ void sw_stage_group(const SWKernelArgs &A0, int wg, S
         act[g] = le < epb && e[g] < A.nE;
         sw_load<N, MIXED>(A, e[g], nd, act[g], P[g]);
     }
-    if constexpr (WL) {
-        stage_nodal<N, MIXED>(L, A.dS, A.dS32, A.dP32, threadIdx.x % 64, 64);
-        wave_sync();
-    } else {
-        stage_nodal<N, MIXED>(L, A.dS, A.dS32, A.dP32);
-    }
+    nodal_put<N, MIXED, kLanes>(L, R, lane);
+    if constexpr (WL) wave_sync();
     const int ls = le < epb ? le : 0;
     double *qsE = U.qs + ls * 3 * np, *sCE = U.sC + ls * 3 * np, *pCE = U.pC + ls * 3 * np;
     double *bfE = U.bfs + ls * 12 * n1;
