@@ -668,14 +668,14 @@ __device__ __forceinline__ void sw_process(const SWKernelArgs &A, const NodalLDS
 // LDS of one fused stage workgroup.  WL: each element lies inside one
 // wavefront, so the phases synchronize per wavefront and the wavefronts of
 // a workgroup never wait for each other (one that has its data computes
-// while the others still load).  Measured at N = 7: mixed 10.45 -> 10.32 us
-// per stage, FP64 11.29 -> 11.63 us, so the FP64 form keeps workgroup
-// barriers.
+// while the others still load); each wavefront stages its own copy of the
+// nodal constants.  Measured at N = 7 (64x64, A/B on one box): FP64 11.67
+// -> 11.31 us per stage against workgroup barriers.
 template <int N, bool MIXED, int G>
 struct SwStageLds {
     using S = Shape<N>;
     static constexpr int np = S::np, epb = S::epb, n1 = S::n1;
-    static constexpr bool WL = MIXED && (G == 1) && (64 % np == 0);
+    static constexpr bool WL = (G == 1) && (64 % np == 0);
     NodalLDS<N> Lw[WL ? 4 : 1];
     double qs[epb * 3 * np], sC[epb * 3 * np], pC[epb * 3 * np], bfs[epb * 12 * n1];
     float sD[MIXED ? epb * 3 * np : 1], pD[MIXED ? epb * 3 * np : 1];
