@@ -79,6 +79,49 @@ __device__ __forceinline__ void stage_nodal3(Nodal3<N> &L, const double *__restr
     }
 }
 
+// The same constants in two steps around a kernel's element loads:
+// nodal3_fetch issues this thread's loads (clamped indices, no branch)
+// before them, nodal3_put writes LDS after them, so the write waits for these
+// loads only (the counter of outstanding loads is in order) and not for the
+// element loads.  LANES threads, lane = 0 .. LANES-1.
+template <int N, int LANES>
+struct Nodal3Regs {
+    static constexpr int n1 = N + 1, nv = 2 * n1 * n1 + 3 * n1;  // D, F1, then w, lL, lR
+    static constexpr int K = (nv + LANES - 1) / LANES;
+    double v[K];
+};
+
+template <int N, int LANES>
+__device__ __forceinline__ void nodal3_fetch(Nodal3Regs<N, LANES> &R, const double *__restrict__ g, int lane)
+{
+    using Rg = Nodal3Regs<N, LANES>;
+    constexpr int n1 = Rg::n1, n2 = n1 * n1;
+    #pragma unroll
+    for (int q = 0; q < Rg::K; ++q) {
+        const int k = min(lane + q * LANES, Rg::nv - 1);
+        // buffer: D (n2), w, lL, lR (n1 each), F1 (n2); here D, F1, w, lL, lR
+        const int src = k < n2 ? k : (k < 2 * n2 ? k + 2 * n1 + n1 : k - n2);
+        R.v[q] = g[src];
+    }
+}
+
+template <int N, int LANES>
+__device__ __forceinline__ void nodal3_put(Nodal3<N> &L, const Nodal3Regs<N, LANES> &R, int lane)
+{
+    using Rg = Nodal3Regs<N, LANES>;
+    constexpr int n1 = Rg::n1, n2 = n1 * n1;
+    #pragma unroll
+    for (int q = 0; q < Rg::K; ++q) {
+        const int k = lane + q * LANES;
+        const double v = R.v[q];
+        if (k < n2) L.D[k] = v;
+        else if (k < 2 * n2) L.F1[k - n2] = v;
+        else if (k < 2 * n2 + n1) { L.w[k - 2 * n2] = v; L.iw[k - 2 * n2] = 1.0 / v; }
+        else if (k < 2 * n2 + 2 * n1) L.lL[k - 2 * n2 - n1] = v;
+        else if (k < Rg::nv) L.lR[k - 2 * n2 - 2 * n1] = v;
+    }
+}
+
 // trace of one element's nodal field q (LDS, i fastest) at side s, face node (a, b)
 template <int N>
 __device__ __forceinline__ double trace3(const Nodal3<N> &L, const double *q, int s, int a, int b)
@@ -117,14 +160,42 @@ __device__ __forceinline__ double adv3_external(int bc, int f, const double *bp,
     return 0.0;
 }
 
+// The filter's stored energies and limiters of one element (ec) for the
+// filter of `stage` (0: none), loaded by every thread of the element (one
+// address per wavefront) before the filter starts, so the thread that
+// updates them does not wait for them between the filter's barriers.
+// Only this element's workgroup writes them, after reading.
+template <int NQ>
+struct Adv3FilterPre {
+    double E1[NQ], E2[NQ], lim[NQ], limPrev[NQ];
+};
+
+template <int NQ>
+__device__ __forceinline__ void adv3_filter_prefetch(const Adv3Args &A, int ec, int stage, Adv3FilterPre<NQ> &P)
+{
+    #pragma unroll
+    for (int f = 0; f < NQ; ++f) { P.E1[f] = 0.0; P.E2[f] = 0.0; P.lim[f] = 0.0; P.limPrev[f] = 0.0; }
+    if (stage == 0) return;
+    #pragma unroll
+    for (int f = 0; f < NQ; ++f) {
+        const size_t ei = (size_t)f * A.nE + ec;
+        const size_t li = ((size_t)(stage - 1) * NQ + f) * A.nE + ec;
+        P.E1[f] = A.E1[ei];
+        P.E2[f] = A.E2[ei];
+        P.lim[f] = A.lim[li];
+        if (stage > 1) P.limPrev[f] = A.lim[li - (size_t)NQ * A.nE];
+    }
+}
+
 // DoTheAdaptiveFiltering for the element group's state in cs (LDS) at the
 // given stage; t1, t2, t3 are LDS scratch of the size of cs; red holds
 // 2 n2 per element.  Replaces cs (and the register copy c) when the
-// limiter asks for it.  Every thread of the block must call it.
+// limiter asks for it.  Every thread of the block must call it.  pre: the
+// element's stored energies and limiters loaded in advance (or null).
 template <int N, int NQ>
 __device__ void adaptive_filter(const Adv3Args &A, const Nodal3<N> &L, double *cs, double *t1, double *t2,
                                 double *t3, double *red, int *flag, int le, int nd, int e, bool act, double J,
-                                double (&c)[NQ], int stage)
+                                double (&c)[NQ], int stage, const Adv3FilterPre<NQ> *pre = nullptr)
 {
     using S = Shape3<N>;
     constexpr int n1 = S::n1, n2 = S::n2, np = S::np;
@@ -176,12 +247,13 @@ __device__ void adaptive_filter(const Adv3Args &A, const Nodal3<N> &L, double *c
                 E2 = E2 + red[le * 2 * n2 + n2 + q];
             }
             const size_t ei = (size_t)f * A.nE + e;
-            const double dE1 = E1 - A.E1[ei], dE2 = E2 - A.E2[ei], xi = E2 / E1;
+            const double E1o = pre ? pre->E1[f] : A.E1[ei], E2o = pre ? pre->E2[f] : A.E2[ei];
+            const double dE1 = E1 - E1o, dE2 = E2 - E2o, xi = E2 / E1;
             const size_t li = ((size_t)(stage - 1) * NQ + f) * A.nE + e;
-            double lim = A.lim[li];
+            double lim = pre ? pre->lim[f] : A.lim[li];
             if (dE2 > 0.0 && fabs(dE1) / dE2 > 1.0) lim = xi;
             else if (dE2 < 0.0) lim = xi;
-            else if (stage > 1) lim = A.lim[li - (size_t)NQ * A.nE];
+            else if (stage > 1) lim = pre ? pre->limPrev[f] : A.lim[li - (size_t)NQ * A.nE];
             A.lim[li] = lim;
             A.E1[ei] = E1;
             A.E2[ei] = E2;
@@ -271,7 +343,7 @@ __global__ __launch_bounds__(Shape3<N>::threads) void k_adv3d_prologue(Adv3Args 
 // matrix-core contractions (mfma.hpp).  State, register, geometry, face
 // fluxes, lift, Jacobian and the filter are FP64 in every form.
 template <int N, int NQ, int PREC>
-__global__ __launch_bounds__(Shape3<N>::threads) void k_adv3d_stage(Adv3Args A)
+__global__ __launch_bounds__(Shape3<N>::threads, 4) void k_adv3d_stage(Adv3Args A)
 {
     using S = Shape3<N>;
     constexpr int n1 = S::n1, n2 = S::n2, np = S::np, epb = S::epb;
@@ -284,7 +356,66 @@ __global__ __launch_bounds__(Shape3<N>::threads) void k_adv3d_stage(Adv3Args A)
     __shared__ float Tm[nT];
     __shared__ float Df[64];
     __shared__ double rs[n1];
-    stage_nodal3<N>(L, A.nodal);
+    constexpr int nF = epb * 6 * n2;                       // face nodes of the element group
+    constexpr int FQ = (nF + S::threads - 1) / S::threads;  // per thread
+    const int t = threadIdx.x, le = t / np, nd = t - le * np;
+    const int e = blockIdx.x * epb + le;
+    const bool act = le < epb && e < A.nE;
+    const int i = nd % n1, j = (nd / n1) % n1, k = nd / n2;
+    const size_t ms = (size_t)A.nE * np, base = act ? (size_t)e * np + nd : 0;
+    const int ec = act ? e : 0;
+    // Loads first, in the order they are used, with no thread-dependent
+    // branch around them (a thread without a node or face node loads from a
+    // clamped valid address and ignores the value): the nodal constants,
+    // the face data, the element's state and geometry, the neighbours'
+    // traces (addressed by the face data), the filter's energies.
+    Nodal3Regs<N, S::threads> R;
+    nodal3_fetch<N, S::threads>(R, A.nodal, t);
+    int2 finfo[FQ];
+    double fuN[FQ], flen[FQ], fnbr[FQ][NQ];
+    #pragma unroll
+    for (int q = 0; q < FQ; ++q) {
+        const int w = t + q * S::threads;
+        const int wc = w < nF ? w : 0;
+        const int l2 = wc / (6 * n2), rem = wc - l2 * 6 * n2, sd = rem / n2, node = rem - sd * n2;
+        const int e2 = blockIdx.x * epb + l2, e2c = e2 < A.nE ? e2 : 0;
+        finfo[q] = A.side[(size_t)e2c * 6 + sd];
+        const size_t fb = (((size_t)e2c * 6 + sd) * 2) * n2 + node;
+        fuN[q] = A.fdat[fb];
+        flen[q] = A.fdat[fb + n2];
+    }
+    double c[NQ], gold[NQ], rl[NQ], rf[NQ];
+    const double J = A.geo[9 * ms + base];
+    // X/Y/ZFlux (:1624-1672) projected on the cofactor rows, with the
+    // static velocity folded into the rows once
+    const double as = A.acoef[base], ap = A.acoef[ms + base], aq = A.acoef[2 * ms + base];
+    #pragma unroll
+    for (int f = 0; f < NQ; ++f) c[f] = A.c[f * ms + base];
+    #pragma unroll
+    for (int f = 0; f < NQ; ++f) { gold[f] = 0.0; rl[f] = 0.0; rf[f] = 0.0; }
+    asm volatile("" ::: "memory");  // the optional fields after the others
+    if (A.mode == 0 && A.a != 0.0) {
+        #pragma unroll
+        for (int f = 0; f < NQ; ++f) gold[f] = A.G[f * ms + base];
+    }
+    if (A.hasRelax) {
+        #pragma unroll
+        for (int f = 0; f < NQ; ++f) { rl[f] = A.relax[f * ms + base]; rf[f] = A.rFac[f * ms + base]; }
+    }
+    #pragma unroll
+    for (int q = 0; q < FQ; ++q) {
+        const int w = t + q * S::threads;
+        const int wc = w < nF ? w : 0;
+        const int rem = wc % (6 * n2), node = rem % n2;
+        const int2 info = finfo[q];
+        int na = node % n1, nb = node / n1;
+        if (info.x >= 0) s3_map<N>(info.y, node % n1, node / n1, na, nb);
+        // a boundary face node reads element 0's slot and uses its external state instead
+        const size_t nbase = info.x >= 0 ? ((size_t)info.x * 6 + s3_nbr_side(info.y)) * NQ : 0;
+        #pragma unroll
+        for (int f = 0; f < NQ; ++f) fnbr[q][f] = A.trIn[(nbase + f) * n2 + na + n1 * nb];
+    }
+    nodal3_put<N, S::threads>(L, R, t);
     if (PREC) {
         for (int q = threadIdx.x; q < 64; q += blockDim.x) {
             const int r = q >> 3, m = q & 7;
@@ -296,20 +427,8 @@ __global__ __launch_bounds__(Shape3<N>::threads) void k_adv3d_stage(Adv3Args A)
             rs[r] = acc;
         }
     }
-    const int t = threadIdx.x, le = t / np, nd = t - le * np;
-    const int e = blockIdx.x * epb + le;
-    const bool act = le < epb && e < A.nE;
-    const int i = nd % n1, j = (nd / n1) % n1, k = nd / n2;
-    const size_t ms = (size_t)A.nE * np, base = (size_t)e * np + nd;
-    double c[NQ], gold[NQ], J = 1.0;
     if (act) {
-        J = A.geo[9 * ms + base];
-        // X/Y/ZFlux (:1624-1672) projected on the cofactor rows, with the
-        // static velocity folded into the rows once
-        const double as = A.acoef[base], ap = A.acoef[ms + base], aq = A.acoef[2 * ms + base];
         for (int f = 0; f < NQ; ++f) {
-            c[f] = A.c[f * ms + base];
-            gold[f] = (A.mode == 0 && A.a != 0.0) ? A.G[f * ms + base] : 0.0;
             const int o = (le * NQ + f) * np + nd;
             cs[o] = c[f];
             Fs[o] = as * c[f];
@@ -328,16 +447,16 @@ __global__ __launch_bounds__(Shape3<N>::threads) void k_adv3d_stage(Adv3Args A)
         }
     }
     // face fluxes (EdgeFlux :1237-1341 with RiemannSolver :1595-1620)
-    for (int w = t; w < epb * 6 * n2; w += blockDim.x) {
+    #pragma unroll
+    for (int q = 0; q < FQ; ++q) {
+        const int w = t + q * S::threads;
+        if (w >= nF) continue;
         const int l2 = w / (6 * n2), rem = w - l2 * 6 * n2, sd = rem / n2, node = rem - sd * n2;
         const int e2 = blockIdx.x * epb + l2;
         if (e2 >= A.nE) continue;
         const int a = node % n1, b = node / n1;
-        const int2 info = A.side[(size_t)e2 * 6 + sd];
-        const size_t fb = (((size_t)e2 * 6 + sd) * 2) * n2 + node;
-        const double uN = A.fdat[fb], len = A.fdat[fb + n2];
-        int na = a, nb = b;
-        if (info.x >= 0) s3_map<N>(info.y, a, b, na, nb);
+        const int2 info = finfo[q];
+        const double uN = fuN[q], len = flen[q];
         for (int f = 0; f < NQ; ++f) {
             const double own = trace3<N>(L, cs + (l2 * NQ + f) * np, sd, a, b);
             double in, ex, sign = 1.0;
@@ -345,8 +464,7 @@ __global__ __launch_bounds__(Shape3<N>::threads) void k_adv3d_stage(Adv3Args A)
                 in = own;
                 ex = adv3_external(info.x, f, A.bpos + (size_t)s3_slot(info.y) * 3 * n2, n2, node, A.t);
             } else {
-                const double nbr =
-                    A.trIn[(((size_t)info.x * 6 + s3_nbr_side(info.y)) * NQ + f) * n2 + na + n1 * nb];
+                const double nbr = fnbr[q][f];
                 if (s3_primary(info.y)) { in = own; ex = nbr; }
                 else { in = nbr; ex = own; sign = -1.0; }
             }
@@ -382,7 +500,7 @@ __global__ __launch_bounds__(Shape3<N>::threads) void k_adv3d_stage(Adv3Args A)
             tp = tp + (bN * L.lR[j] + bS * L.lL[j]) * L.iw[j];
             tq = tq + (bT * L.lR[k] + bB * L.lL[k]) * L.iw[k];
             double tend = ((-ts - tp) - tq) / J;
-            if (A.hasRelax) tend = tend + (A.relax[f * ms + base] - c[f]) * A.rFac[f * ms + base];
+            if (A.hasRelax) tend = tend + (rl[f] - c[f]) * rf[f];
             if (A.mode == 1) {
                 A.tend[f * ms + base] = tend;
             } else {
@@ -393,11 +511,16 @@ __global__ __launch_bounds__(Shape3<N>::threads) void k_adv3d_stage(Adv3Args A)
         }
     }
     if (A.mode == 1) return;
+    // the filter's stored energies: in flight during the next barriers and
+    // the filter's first passes
+    Adv3FilterPre<NQ> fp;
+    adv3_filter_prefetch<NQ>(A, ec, A.filterNext, fp);
     __syncthreads();  // all reads of cs / Fs / Fp / Fq for the divergence are done
     if (act)
         for (int f = 0; f < NQ; ++f) cs[(le * NQ + f) * np + nd] = c[f];
     __syncthreads();
-    if (A.filterNext) adaptive_filter<N, NQ>(A, L, cs, Fs, Fp, Fq, red, flag, le, nd, e, act, J, c, A.filterNext);
+    if (A.filterNext)
+        adaptive_filter<N, NQ>(A, L, cs, Fs, Fp, Fq, red, flag, le, nd, e, act, J, c, A.filterNext, &fp);
     if (act)
         for (int f = 0; f < NQ; ++f) A.c[f * ms + base] = c[f];
     write_traces<N, NQ>(A, L, cs);
