@@ -117,7 +117,7 @@ struct HpeArgs {
 
 // ---------------------------------------------------------------- columns
 template <int N>
-__global__ __launch_bounds__(HShape<N>::threads) void k_hpe_columns(HpeArgs A)
+__global__ __launch_bounds__(HShape<N>::threads, 4) void k_hpe_columns(HpeArgs A)
 {
     using S = HShape<N>;
     constexpr int n1 = S::n1, n2 = S::n2, np = S::np;
@@ -132,13 +132,33 @@ __global__ __launch_bounds__(HShape<N>::threads) void k_hpe_columns(HpeArgs A)
     const double *u = A.sol, *v = A.sol + nEp, *b = A.sol + 2 * nEp;
     double *w = A.sol + 3 * nEp, *p = A.sol + 4 * nEp;
     for (int t = tid; t < n2; t += blockDim.x) base[t] = 0.0;
+    // Each layer's loads are issued one layer ahead (unconditionally, at
+    // clamped addresses for threads without a node or face node), so they
+    // are in flight while the layer before is integrated: the column walk
+    // is otherwise one memory round trip per layer and sweep.
+    const size_t tn = act ? (size_t)tid : 0, thn = (size_t)(act ? i + n1 * j : 0);
+    constexpr int FQ = (6 * n2 + S::threads - 1) / S::threads;  // face nodes per thread
     // ---- DiagnoseP (top to bottom) and the face values of b and p
+    double bNx, pNx = 0.0, hNx;
+    {
+        const size_t e0 = (size_t)hE + (size_t)A.nH * (A.nL - 1);
+        bNx = b[e0 * np + tn];
+        if (!A.diagnoseP) pNx = p[e0 * np + tn];
+        hNx = A.height[e0 * n2 + thn];
+    }
     for (int layer = A.nL - 1; layer >= 0; --layer) {
         const size_t e = (size_t)hE + (size_t)A.nH * layer;
+        const double bCur = bNx, pCur = pNx, hCur = hNx;
+        if (layer > 0) {
+            const size_t en = e - (size_t)A.nH;
+            bNx = b[en * np + tn];
+            if (!A.diagnoseP) pNx = p[en * np + tn];
+            hNx = A.height[en * n2 + thn];
+        }
         __syncthreads();
         if (act) {
-            s0[tid] = b[e * np + tid];
-            if (!A.diagnoseP) s1[tid] = p[e * np + tid];
+            s0[tid] = bCur;
+            if (!A.diagnoseP) s1[tid] = pCur;
         }
         __syncthreads();
         if (A.diagnoseP) {
@@ -146,7 +166,7 @@ __global__ __launch_bounds__(HShape<N>::threads) void k_hpe_columns(HpeArgs A)
                 double acc = 0.0;
                 #pragma unroll
                 for (int m = 0; m < n1; ++m) acc = __builtin_fma(L.Idn[k * n1 + m], s0[i + n1 * (j + n1 * m)], acc);
-                s2[tid] = acc * A.height[e * n2 + i + n1 * j];
+                s2[tid] = acc * hCur;
             }
             __syncthreads();
             if (tid < n2) {
@@ -171,29 +191,59 @@ __global__ __launch_bounds__(HShape<N>::threads) void k_hpe_columns(HpeArgs A)
         }
     }
     // ---- CalculateDivU, CalculateBottomVerticalVelocity, DiagnoseW (bottom to top)
+    // one layer's node values (u, v, the six cofactors that meet them, J,
+    // the column height) and this thread's face node normal (nx, ny, nz, |n|)
+    struct Layer {
+        double u, v, g0, g1, g3, g4, g6, g7, J, h, nx[FQ], ny[FQ], nz[FQ], len[FQ];
+    };
+    const auto loadLayer = [&](size_t e) {
+        Layer Y;
+        const size_t g = e * np + tn;
+        Y.u = u[g];
+        Y.v = v[g];
+        Y.g0 = A.geo[0 * nEp + g];
+        Y.g1 = A.geo[1 * nEp + g];
+        Y.g3 = A.geo[3 * nEp + g];
+        Y.g4 = A.geo[4 * nEp + g];
+        Y.g6 = A.geo[6 * nEp + g];
+        Y.g7 = A.geo[7 * nEp + g];
+        Y.J = A.geo[9 * nEp + g];
+        Y.h = A.height[e * n2 + thn];
+        #pragma unroll
+        for (int q = 0; q < FQ; ++q) {
+            const int t = tid + q * S::threads, tf = t < 6 * n2 ? t : 0;
+            const int sdn = tf / n2, node = tf - sdn * n2;
+            const double *nr = A.nrm + ((e * 6 + sdn) * 4) * n2 + node;
+            Y.nx[q] = nr[0];
+            Y.ny[q] = nr[n2];
+            Y.nz[q] = nr[2 * n2];
+            Y.len[q] = nr[3 * n2];
+        }
+        return Y;
+    };
+    Layer Ynx = loadLayer((size_t)hE);
     for (int layer = 0; layer < A.nL; ++layer) {
         const size_t e = (size_t)hE + (size_t)A.nH * layer;
+        const Layer Y = Ynx;
+        if (layer + 1 < A.nL) Ynx = loadLayer(e + (size_t)A.nH);
         __syncthreads();
-        double Jv = 1.0;
         if (act) {
-            const size_t g = e * np + tid;
-            const double uu = u[g], vv = v[g];
-            s0[tid] = uu;
-            s1[tid] = vv;
-            s2[tid] = A.geo[0 * nEp + g] * uu + A.geo[1 * nEp + g] * vv;
-            s3[tid] = A.geo[3 * nEp + g] * uu + A.geo[4 * nEp + g] * vv;
-            s4[tid] = A.geo[6 * nEp + g] * uu + A.geo[7 * nEp + g] * vv;
-            Jv = A.geo[9 * nEp + g];
+            s0[tid] = Y.u;
+            s1[tid] = Y.v;
+            s2[tid] = Y.g0 * Y.u + Y.g1 * Y.v;
+            s3[tid] = Y.g3 * Y.u + Y.g4 * Y.v;
+            s4[tid] = Y.g6 * Y.u + Y.g7 * Y.v;
         }
         __syncthreads();
-        for (int t = tid; t < 6 * n2; t += blockDim.x) {
+        #pragma unroll
+        for (int q = 0; q < FQ; ++q) {
+            const int t = tid + q * S::threads;
+            if (t >= 6 * n2) continue;
             const int sdn = t / n2, node = t - sdn * n2, a = node % n1, bb = node / n1;
             const double ub = face_trace3<N>(L.lL, L.lR, s0, sdn, a, bb);
             const double vb = face_trace3<N>(L.lL, L.lR, s1, sdn, a, bb);
-            const double *nr = A.nrm + ((e * 6 + sdn) * 4) * n2 + node;
-            const double nx = nr[0], ny = nr[n2], nz = nr[2 * n2], len = nr[3 * n2];
-            fF[sdn * n2 + node] = (ub * nx + vb * ny) * len;
-            if (layer == 0 && sdn == 4) base[node] = -(ub * nx + vb * ny) / nz;
+            fF[sdn * n2 + node] = (ub * Y.nx[q] + vb * Y.ny[q]) * Y.len[q];
+            if (layer == 0 && sdn == 4) base[node] = -(ub * Y.nx[q] + vb * Y.ny[q]) / Y.nz[q];
         }
         __syncthreads();
         if (act) {
@@ -207,7 +257,7 @@ __global__ __launch_bounds__(HShape<N>::threads) void k_hpe_columns(HpeArgs A)
             ds += (fF[1 * n2 + j + n1 * k] * L.lR[i] + fF[3 * n2 + j + n1 * k] * L.lL[i]) * L.iw[i];
             dp += (fF[2 * n2 + i + n1 * k] * L.lR[j] + fF[0 * n2 + i + n1 * k] * L.lL[j]) * L.iw[j];
             dq += (fF[5 * n2 + i + n1 * j] * L.lR[k] + fF[4 * n2 + i + n1 * j] * L.lL[k]) * L.iw[k];
-            const double div = ((ds + dp) + dq) / Jv;
+            const double div = ((ds + dp) + dq) / Y.J;
             sd[tid] = div;
             if (A.divU) A.divU[e * np + tid] = div;
         }
@@ -216,7 +266,7 @@ __global__ __launch_bounds__(HShape<N>::threads) void k_hpe_columns(HpeArgs A)
             double acc = 0.0;
             #pragma unroll
             for (int m = 0; m < n1; ++m) acc = __builtin_fma(L.Iup[k * n1 + m], sd[i + n1 * (j + n1 * m)], acc);
-            s2[tid] = acc * A.height[e * n2 + i + n1 * j];
+            s2[tid] = acc * Y.h;
         }
         __syncthreads();
         if (tid < n2) {
