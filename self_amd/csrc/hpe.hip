@@ -107,6 +107,48 @@ __device__ __forceinline__ void stage_hnodal(HNodal<N> &L, const double *__restr
     }
 }
 
+// The same constants in two steps around a kernel's element loads (fetch
+// before them, LDS after them: the write then waits for these loads only,
+// the counter of outstanding loads being in order).  LANES threads.
+template <int N, int LANES>
+struct HNodalRegs {
+    static constexpr int n1 = N + 1, n11 = n1 * n1, nv = 3 * n11 + 3 * n1;  // D, Idn, Iup, w, lL, lR
+    static constexpr int K = (nv + LANES - 1) / LANES;
+    double v[K];
+};
+
+template <int N, int LANES>
+__device__ __forceinline__ void hnodal_fetch(HNodalRegs<N, LANES> &R, const double *__restrict__ g, int lane)
+{
+    using Rg = HNodalRegs<N, LANES>;
+    constexpr int n1 = Rg::n1, n11 = Rg::n11;
+    #pragma unroll
+    for (int q = 0; q < Rg::K; ++q) {
+        const int k = min(lane + q * LANES, Rg::nv - 1);
+        // buffer: D, w, lL, lR, Idn, Iup
+        const int src = k < n11 ? k : (k < 3 * n11 ? k + 3 * n1 : k - 2 * n11);
+        R.v[q] = g[src];
+    }
+}
+
+template <int N, int LANES>
+__device__ __forceinline__ void hnodal_put(HNodal<N> &L, const HNodalRegs<N, LANES> &R, int lane)
+{
+    using Rg = HNodalRegs<N, LANES>;
+    constexpr int n1 = Rg::n1, n11 = Rg::n11;
+    #pragma unroll
+    for (int q = 0; q < Rg::K; ++q) {
+        const int k = lane + q * LANES;
+        const double v = R.v[q];
+        if (k < n11) L.D[k] = v;
+        else if (k < 2 * n11) L.Idn[k - n11] = v;
+        else if (k < 3 * n11) L.Iup[k - 2 * n11] = v;
+        else if (k < 3 * n11 + n1) L.iw[k - 3 * n11] = 1.0 / v;
+        else if (k < 3 * n11 + 2 * n1) L.lL[k - 3 * n11 - n1] = v;
+        else if (k < Rg::nv) L.lR[k - 3 * n11 - 2 * n1] = v;
+    }
+}
+
 struct HpeArgs {
     int nE, nH, nL, diagnoseP, mode, first;
     double a, gdt, dt;
@@ -313,42 +355,79 @@ __global__ __launch_bounds__(HShape<N>::threads) void k_hpe_stage(HpeArgs A)
     __shared__ double fl[9 * np];   // [dir s,p,q][eq u,v,b][node]
     __shared__ double su[np], sv[np];
     __shared__ double F[6 * 3 * n2];  // [side][eq][node]
-    stage_hnodal<N>(L, A.nodal);
+    constexpr int FQ = (6 * n2 + S::threads - 1) / S::threads;  // face nodes per thread
     const size_t e = blockIdx.x;
     const int tid = threadIdx.x;
     const bool act = tid < np;
     const int i = tid % n1, j = (tid / n1) % n1, k = tid / n2;
-    const size_t NE = (size_t)A.nE, nEp = NE * np, g = e * np + tid;
-    double Jv = 1.0, uu = 0.0, vv = 0.0, bb0 = 0.0;
+    const size_t NE = (size_t)A.nE, nEp = NE * np, g = e * np + (act ? tid : 0);
+    // Loads first, with no thread-dependent branch around them (threads
+    // without a node or face node load at clamped addresses and ignore the
+    // values): nodal constants, face data, node values, the RK register,
+    // then the neighbours' traces addressed by the face data.
+    HNodalRegs<N, S::threads> R;
+    hnodal_fetch<N, S::threads>(R, A.nodal, tid);
+    int2 finfo[FQ];
+    double fown[FQ][3], fdat[FQ][6], fnbr[FQ][3];
+    #pragma unroll
+    for (int q = 0; q < FQ; ++q) {
+        const int t = tid + q * S::threads, tf = t < 6 * n2 ? t : 0;
+        const int sdn = tf / n2, node = tf - sdn * n2;
+        finfo[q] = A.side[e * 6 + sdn];
+        const double *own = A.tr + ((e * 6 + sdn) * 3) * n2 + node;
+        fown[q][0] = own[0]; fown[q][1] = own[n2]; fown[q][2] = own[2 * n2];
+        const double *fd = A.fdat + ((e * 6 + sdn) * 6) * n2 + node;
+        #pragma unroll
+        for (int c = 0; c < 6; ++c) fdat[q][c] = fd[c * n2];
+    }
+    const double pv = A.sol[4 * nEp + g], Bw = A.Bvol[g] * A.sol[3 * nEp + g];
+    const double uu = A.sol[g], vv = A.sol[nEp + g], bb0 = A.sol[2 * nEp + g];
+    double geo[9];
+    #pragma unroll
+    for (int c = 0; c < 9; ++c) geo[c] = A.geo[c * nEp + g];
+    const double Jv = A.geo[9 * nEp + g];
+    double Gold[3] = {0.0, 0.0, 0.0};
+    asm volatile("" ::: "memory");  // the register after the others
+    if ((A.mode == 0 && !A.first) || A.mode == 2) {
+        #pragma unroll
+        for (int f = 0; f < 3; ++f) Gold[f] = A.G[(size_t)f * nEp + g];
+    }
+    #pragma unroll
+    for (int q = 0; q < FQ; ++q) {
+        const int t = tid + q * S::threads, tf = t < 6 * n2 ? t : 0;
+        const int node = tf % n2, a = node % n1, bq = node / n1;
+        const int2 info = finfo[q];
+        int na = a, nb = bq;
+        if (info.x >= 0) s3_map<N>(info.y, a, bq, na, nb);
+        // a boundary face node reads element 0's slot and ignores it
+        const size_t nbase = info.x >= 0 ? ((size_t)info.x * 6 + s3_nbr_side(info.y)) * 3 : 0;
+        const double *nbr = A.tr + nbase * n2 + na + n1 * nb;
+        fnbr[q][0] = nbr[0]; fnbr[q][1] = nbr[n2]; fnbr[q][2] = nbr[2 * n2];
+    }
+    hnodal_put<N, S::threads>(L, R, tid);
     if (act) {
-        const double pv = A.sol[4 * nEp + g], Bw = A.Bvol[g] * A.sol[3 * nEp + g];
-        uu = A.sol[g];
-        vv = A.sol[nEp + g];
-        bb0 = A.sol[2 * nEp + g];
         su[tid] = uu;
         sv[tid] = vv;
         #pragma unroll
         for (int d = 0; d < 3; ++d) {
-            fl[(3 * d + 0) * np + tid] = A.geo[(3 * d + 0) * nEp + g] * pv;
-            fl[(3 * d + 1) * np + tid] = A.geo[(3 * d + 1) * nEp + g] * pv;
-            fl[(3 * d + 2) * np + tid] = A.geo[(3 * d + 2) * nEp + g] * Bw;
+            fl[(3 * d + 0) * np + tid] = geo[3 * d + 0] * pv;
+            fl[(3 * d + 1) * np + tid] = geo[3 * d + 1] * pv;
+            fl[(3 * d + 2) * np + tid] = geo[3 * d + 2] * Bw;
         }
-        Jv = A.geo[9 * nEp + g];
     }
     __syncthreads();
-    for (int t = tid; t < 6 * n2; t += blockDim.x) {
+    #pragma unroll
+    for (int q = 0; q < FQ; ++q) {
+        const int t = tid + q * S::threads;
+        if (t >= 6 * n2) continue;
         const int sdn = t / n2, node = t - sdn * n2, a = node % n1, bq = node / n1;
-        const int2 info = A.side[e * 6 + sdn];
-        const double *own = A.tr + ((e * 6 + sdn) * 3) * n2 + node;
-        const double bO = own[0], wO = own[n2], pO = own[2 * n2];
-        const double *fd = A.fdat + ((e * 6 + sdn) * 6) * n2 + node;
-        const double nx = fd[0], ny = fd[n2], nz = fd[2 * n2], len = fd[3 * n2], Bf = fd[4 * n2], Wf = fd[5 * n2];
+        const int2 info = finfo[q];
+        const double bO = fown[q][0], wO = fown[q][1], pO = fown[q][2];
+        const double nx = fdat[q][0], ny = fdat[q][1], nz = fdat[q][2], len = fdat[q][3], Bf = fdat[q][4],
+                     Wf = fdat[q][5];
         double f0, f1, f2, sgn = 1.0;
         if (info.x >= 0) {
-            int na, nb;
-            s3_map<N>(info.y, a, bq, na, nb);
-            const double *nbr = A.tr + (((size_t)info.x * 6 + s3_nbr_side(info.y)) * 3) * n2 + na + n1 * nb;
-            const double bN = nbr[0], wN = nbr[n2], pN = nbr[2 * n2];
+            const double bN = fnbr[q][0], wN = fnbr[q][1], pN = fnbr[q][2];
             if (s3_primary(info.y)) {
                 hpe_riemann(bO, wO, pO, bN, wN, pN, nx, ny, nz, Bf, Wf, f0, f1, f2);
             } else {
@@ -394,11 +473,11 @@ __global__ __launch_bounds__(HShape<N>::threads) void k_hpe_stage(HpeArgs A)
         if (A.mode == 1) {  // the time derivative alone
             A.tend[gi] = tendv[f];
         } else if (A.mode == 0) {  // RK3 stage
-            const double Gn = A.first ? tendv[f] : A.a * A.G[gi] + tendv[f];
+            const double Gn = A.first ? tendv[f] : A.a * Gold[f] + tendv[f];
             A.G[gi] = Gn;
             A.sol[gi] = sol0[f] + A.gdt * Gn;
         } else {  // AB2 step, G the previous tendency
-            const double Gn = 0.5 * (3.0 * tendv[f] - A.G[gi]);
+            const double Gn = 0.5 * (3.0 * tendv[f] - Gold[f]);
             A.sol[gi] = sol0[f] + A.dt * Gn;
             A.G[gi] = tendv[f];
         }
