@@ -13,6 +13,7 @@ namespace {
 
 constexpr int kPcThreads = 256;
 constexpr int kMaxWidth = 32;  // ELL slots per row (a node shared by up to 8 elements of the linear mesh)
+constexpr int kPcChunk = 8;    // ELL slots of a row loaded together
 
 __device__ __forceinline__ bool pc_skip(const PcState *st, const SolverState *outer)
 {
@@ -91,16 +92,43 @@ __global__ __launch_bounds__(kPcThreads) void k_pc_spmv(const T *__restrict__ r,
     if (pc_skip(st, outer)) return;
     const bool first = st->iter == 1;
     const T beta = (T)st->beta;
+    // iteration 1 has no previous direction: it reads r in its place (and
+    // ignores it), so the same loads serve every iteration
+    const T *__restrict__ vPrev = first ? r : vOld;
     double acc = 0.0;
     for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
-        const T vn = first ? r[i] : __builtin_fma(beta, vOld[i], r[i]);
-        vNew[i] = vn;
+        T ri = r[i], vi = vPrev[i];
         T zi = (T)0;
-        for (int k = 0; k < width; ++k) {
-            const int c = col[(size_t)k * n + i];
-            const T vc = first ? r[c] : __builtin_fma(beta, vOld[c], r[c]);
-            zi = __builtin_fma(val[(size_t)k * n + i], vc, zi);
+        // the row's slots kPcChunk at a time: their column indices and values
+        // first, then the gathered entries, all in flight together, then the
+        // multiply-adds in slot order (a loop that loads a slot's column and
+        // then its entry waits twice per slot); slots past the width repeat
+        // the last one and are not added
+        for (int k0 = 0; k0 < width; k0 += kPcChunk) {
+            int cc[kPcChunk];
+            T vv[kPcChunk], rc[kPcChunk], vc[kPcChunk];
+            #pragma unroll
+            for (int u = 0; u < kPcChunk; ++u) {
+                const size_t kk = (size_t)min(k0 + u, width - 1) * n + i;
+                cc[u] = col[kk];
+                vv[u] = val[kk];
+            }
+            #pragma unroll
+            for (int u = 0; u < kPcChunk; ++u) {
+                rc[u] = r[cc[u]];
+                vc[u] = vPrev[cc[u]];
+            }
+            #pragma unroll
+            for (int u = 0; u < kPcChunk; ++u) asm volatile("" : "+v"(rc[u]), "+v"(vc[u]));
+            #pragma unroll
+            for (int u = 0; u < kPcChunk; ++u) {
+                const T x = first ? rc[u] : __builtin_fma(beta, vc[u], rc[u]);
+                if (k0 + u < width) zi = __builtin_fma(vv[u], x, zi);
+            }
         }
+        asm volatile("" : "+v"(ri), "+v"(vi));
+        const T vn = first ? ri : __builtin_fma(beta, vi, ri);
+        vNew[i] = vn;
         z[i] = zi;
         acc += (double)vn * (double)zi;
     }
