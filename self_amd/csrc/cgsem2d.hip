@@ -91,22 +91,27 @@ __global__ __launch_bounds__(256) void k_cg_local(const T *__restrict__ x, const
     if (st && (st->done || (!RESID && st->innerExit))) return;
     __shared__ CgLDS<N, T> L;
     __shared__ T us[epb * np], f1[epb * np], f2[epb * np];
-    for (int k = threadIdx.x; k < n1 * n1; k += blockDim.x) L.D[k] = Dg[k];
+    static_assert(n1 * n1 <= 256, "one thread per entry of the nodal matrix");
+    // every load at a clamped valid address, no thread-dependent branch
+    // around them: the nodal matrix, the node's DOF index and geometry, then
+    // the gathered value (a loaded value behind a branch is waited for at
+    // the join, one load after the other)
+    const T dk = Dg[threadIdx.x < n1 * n1 ? threadIdx.x : 0];
     const int t = threadIdx.x, le = t / np, nd = t - le * np;
     const int e = blockIdx.x * epb + le;
     const bool act = le < epb && e < nE;
     const int i = nd % n1, j = nd / n1;
-    const size_t lid = (size_t)e * np + nd, stride = (size_t)nE * np;
-    T g11 = 0, g12 = 0, g22 = 0;
+    const size_t lid = act ? (size_t)e * np + nd : 0, stride = (size_t)nE * np;
+    const int g = gid[lid];
+    const T g11 = G3[lid], g12 = G3[stride + lid], g22 = G3[2 * stride + lid];
+    T xv = x[g >= 0 ? g : 0];
+    asm volatile("" : "+v"(xv));
+    if (threadIdx.x < n1 * n1) L.D[threadIdx.x] = dk;
     if (act) {
-        const int g = gid[lid];
         T u;
-        if (g >= 0) u = x[g];
+        if (g >= 0) u = xv;
         else u = RESID ? (T)uD[lid] : (T)0;
         us[le * np + nd] = u;
-        g11 = G3[lid];
-        g12 = G3[stride + lid];
-        g22 = G3[2 * stride + lid];
     }
     __syncthreads();
     if (act) cg_flux<N>(L.D, us + le * np, i, j, g11, g12, g22, f1 + le * np, f2 + le * np);
@@ -371,8 +376,7 @@ __global__ __launch_bounds__(TPB) void k_cg_assemble(const T *__restrict__ Lbuf,
     double acc = 0.0;
     for (size_t g = (size_t)blockIdx.x * blockDim.x + threadIdx.x; g < nG; g += (size_t)gridDim.x * blockDim.x) {
         const int c0 = cs[g], c1 = cs[g + 1];
-        T sum = Lbuf[ci[c0]];
-        for (int c = c0 + 1; c < c1; ++c) sum = sum + Lbuf[ci[c]];
+        const T sum = cg_copies_sum(Lbuf, ci, c0, c1);
         T out;
         if (MODE == 1) {
             double r = bl[ci[c0]] - (double)sum;
