@@ -68,6 +68,7 @@ __global__ __launch_bounds__(kPcThreads) void k_pc_final_start(const double *par
         st->r0 = sqrt(S);
         st->rNorm = st->r0;
         st->num = S;  // num = r.w with w = r (:716-717)
+        st->numP[1] = S;
         st->den = 0.0;
         st->beta = 0.0;
         st->iter = 1;
@@ -82,16 +83,40 @@ __global__ __launch_bounds__(kPcThreads) void k_pc_final_start(const double *par
 // v.z; the direction of neighbouring rows is formed on the fly from the
 // same inputs with the same fused multiply-add, so it equals what their
 // own rows store.
+// The end of iteration it-1 (rNorm from the update's r.r partials, the
+// stop test rNorm/r0 < TOL, num and c = num/den, :728-732) comes first:
+// every workgroup sums the partials itself in one fixed order, so all take
+// the same decision from the same bits; workgroup 0 records it.
 template <typename T>
 __global__ __launch_bounds__(kPcThreads) void k_pc_spmv(const T *__restrict__ r, const T *__restrict__ vOld,
                                                         T *__restrict__ vNew, T *__restrict__ z,
                                                         const int *__restrict__ col, const T *__restrict__ val,
-                                                        int width, size_t n, double *partials, const PcState *st,
-                                                        const SolverState *outer)
+                                                        int width, size_t n, double *partials, PcState *st,
+                                                        const SolverState *outer, int it)
 {
-    if (pc_skip(st, outer)) return;
-    const bool first = st->iter == 1;
-    const T beta = (T)st->beta;
+    if (pc_skip(st, outer)) return;  // decided by an earlier launch: the same for every workgroup
+    const bool first = it == 1;
+    T beta = (T)0;
+    if (!first) {
+        __shared__ double sS;
+        const double S0 = pc_final_sum(partials + kPcParts);
+        if (threadIdx.x == 0) sS = S0;
+        __syncthreads();
+        const double S = sS, rNorm = sqrt(S), den = st->numP[(it - 1) & 1];
+        const bool stop = rNorm / st->r0 < st->tol || it - 1 >= st->maxIters;
+        if (blockIdx.x == 0 && threadIdx.x == 0) {
+            st->rNorm = rNorm;
+            st->den = den;
+            st->num = S;
+            st->numP[it & 1] = S;
+            st->beta = S / den;
+            st->iterations = it - 1;
+            if (stop) st->done = 1;
+            else st->iter = it;
+        }
+        if (stop) return;
+        beta = (T)(S / den);
+    }
     // iteration 1 has no previous direction: it reads r in its place (and
     // ignores it), so the same loads serve every iteration
     const T *__restrict__ vPrev = first ? r : vOld;
@@ -136,23 +161,22 @@ __global__ __launch_bounds__(kPcThreads) void k_pc_spmv(const T *__restrict__ r,
     if (threadIdx.x == 0) partials[blockIdx.x] = p;
 }
 
-__global__ __launch_bounds__(kPcThreads) void k_pc_final_alpha(const double *partials, PcState *st,
-                                                               const SolverState *outer)
-{
-    if (pc_skip(st, outer)) return;
-    const double S = pc_final_sum(partials);
-    if (threadIdx.x == 0) st->alpha = st->num / S;  // a = num / (v.z) (:723)
-}
-
-// u += a v, r -= a z, |r|^2 (:724-727)
+// a = num / (v.z) (:723), each workgroup summing the product step's
+// partials itself; u += a v, r -= a z, |r|^2 (:724-727)
 template <typename T>
 __global__ __launch_bounds__(kPcThreads) void k_pc_update(T *__restrict__ x, const T *__restrict__ v,
                                                           T *__restrict__ r, const T *__restrict__ z, size_t n,
-                                                          double *partials, const PcState *st,
-                                                          const SolverState *outer)
+                                                          double *partials, PcState *st,
+                                                          const SolverState *outer, int it)
 {
     if (pc_skip(st, outer)) return;
-    const T a = (T)st->alpha;
+    __shared__ double sS;
+    const double S0 = pc_final_sum(partials);
+    if (threadIdx.x == 0) sS = S0;
+    __syncthreads();
+    const double alpha = st->numP[it & 1] / sS;
+    if (blockIdx.x == 0 && threadIdx.x == 0) st->alpha = alpha;
+    const T a = (T)alpha;
     double acc = 0.0;
     for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
         x[i] = x[i] + a * v[i];
@@ -161,24 +185,24 @@ __global__ __launch_bounds__(kPcThreads) void k_pc_update(T *__restrict__ x, con
         acc += (double)rv * (double)rv;
     }
     const double p = pc_block_sum(acc);
-    if (threadIdx.x == 0) partials[blockIdx.x] = p;
+    if (threadIdx.x == 0) partials[kPcParts + blockIdx.x] = p;
 }
 
-// rNorm, the stop test rNorm/r0 < TOL (:728-732) and the next iteration's
-// num = r.r and c = num/den
+// the end of the last iteration issued (it = maxIters): rNorm and the
+// count; the loop ends there whatever the stop test says (:728-732)
 __global__ __launch_bounds__(kPcThreads) void k_pc_final_rnorm(const double *partials, PcState *st,
-                                                               const SolverState *outer)
+                                                               const SolverState *outer, int it)
 {
     if (pc_skip(st, outer)) return;
-    const double S = pc_final_sum(partials);
+    const double S = pc_final_sum(partials + kPcParts);
     if (threadIdx.x == 0) {
+        const double den = st->numP[it & 1];
         st->rNorm = sqrt(S);
-        st->den = st->num;
+        st->den = den;
         st->num = S;
-        st->beta = S / st->den;
-        st->iterations = st->iter;
-        if (st->rNorm / st->r0 < st->tol || st->iter >= st->maxIters) st->done = 1;
-        else st->iter += 1;
+        st->beta = S / den;
+        st->iterations = it;
+        st->done = 1;
     }
 }
 
@@ -207,12 +231,13 @@ int pc_apply(FemPC &pc, const T *b, T *x, T *r, T *z, T *v0, T *v1, const T *val
         T *vNew = v[it & 1];
         const T *vOld = v[(it + 1) & 1];
         hipLaunchKernelGGL(k_pc_spmv<T>, g, t, 0, s, (const T *)r, vOld, vNew, z, (const int *)pc.col, val,
-                           pc.width, n, pc.partials, (const PcState *)pc.st, outer);
-        hipLaunchKernelGGL(k_pc_final_alpha, dim3(1), t, 0, s, (const double *)pc.partials, pc.st, outer);
-        hipLaunchKernelGGL(k_pc_update<T>, g, t, 0, s, x, (const T *)vNew, r, (const T *)z, n, pc.partials,
-                           (const PcState *)pc.st, outer);
-        hipLaunchKernelGGL(k_pc_final_rnorm, dim3(1), t, 0, s, (const double *)pc.partials, pc.st, outer);
+                           pc.width, n, pc.partials, pc.st, outer, it);
+        hipLaunchKernelGGL(k_pc_update<T>, g, t, 0, s, x, (const T *)vNew, r, (const T *)z, n, pc.partials, pc.st,
+                           outer, it);
     }
+    if (pc.maxIters > 0)
+        hipLaunchKernelGGL(k_pc_final_rnorm, dim3(1), t, 0, s, (const double *)pc.partials, pc.st, outer,
+                           pc.maxIters);
     hipLaunchKernelGGL(k_pc_finish<T>, g, t, 0, s, x, (const T *)r, n, (const PcState *)pc.st, outer);
     SELF_HIP(hipGetLastError());
     return SELF_OK;
@@ -324,7 +349,7 @@ int FemPC::build(int N, size_t nE, const double *x, const double *y, const doubl
     if ((rc = dev(&col, ne)) || (rc = dev(&val, ne)) || (rc = dev(&val32, ne)) || (rc = dev(&r, nG)) ||
         (rc = dev(&z, nG)) || (rc = dev(&v[0], nG)) || (rc = dev(&v[1], nG)) || (rc = dev(&r32, nG)) ||
         (rc = dev(&z32, nG)) || (rc = dev(&v32[0], nG)) || (rc = dev(&v32[1], nG)) ||
-        (rc = dev(&partials, kPcParts)) || (rc = dev(&st, 1)))
+        (rc = dev(&partials, 2 * kPcParts)) || (rc = dev(&st, 1)))
         return rc;
     SELF_HIP(hipMemcpy(col, ellc.data(), sizeof(int) * ne, hipMemcpyHostToDevice));
     SELF_HIP(hipMemcpy(val, ellv.data(), sizeof(double) * ne, hipMemcpyHostToDevice));

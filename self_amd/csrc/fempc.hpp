@@ -13,9 +13,11 @@
 // wavefront's row reads coalesce); the element matrices are the reference's
 // FluxDivergence at N = 1 (two-point Gauss-Lobatto rule, straight-sided
 // bilinear sub-elements).  The inner CG keeps its scalars on the device and
-// runs as four launches per iteration; every kernel returns at once when the
-// inner CG has converged or the outer solver is done, so the host issues
-// pcIterates iterations without reading anything back.
+// runs as two launches per iteration (each workgroup sums the previous
+// launch's partials itself, in one fixed order, so all hold the same bits);
+// every kernel returns at once when the inner CG has converged or the outer
+// solver is done, so the host issues pcIterates iterations without reading
+// anything back.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -30,6 +32,7 @@ struct SolverState;
 
 struct PcState {
     double num, den, alpha, beta, r0, rNorm, tol;
+    double numP[2];  // r.r at the start of iteration it, in slot it & 1
     int iter, done, maxIters, iterations;
 };
 
@@ -45,7 +48,7 @@ struct FemPC {
     float *val32 = nullptr;
     double *r = nullptr, *z = nullptr, *v[2] = {nullptr, nullptr};
     float *r32 = nullptr, *z32 = nullptr, *v32[2] = {nullptr, nullptr};
-    double *partials = nullptr;  // [kPcParts]
+    double *partials = nullptr;  // [2][kPcParts]: product step (v.z), update (r.r)
     PcState *st = nullptr;
     size_t bytes = 0;
     long long applications = 0, innerTotal = 0;  // host counters (set by readback only)
