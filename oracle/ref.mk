@@ -10,7 +10,8 @@
 # shallow-water class of the planewave example behind ref_sw2d.f90.  All outputs
 # (objects, .mod files, binaries) go to oracle/_ref/.
 #
-#   make -f oracle/ref.mk   # -> oracle/_ref/ref_{adv2d,adv3d,cgsem2d,krylov,nodal,pickup,specmesh,sw2d}
+#   make -f oracle/ref.mk   # -> oracle/_ref/ref_{adv2d,adv3d,cgsem2d,krylov,nodal,pickup,specmesh,sw2d}, the
+#                           # reference's example programs (ref_*) and the same programs on the drop-in (dropin_*)
 #
 # No OpenMP, as in the reference's own build/makefile (OPT=-O3, no -fopenmp):
 # its !$OMP directives stay comments (Advection3DClass.f90 has some that
@@ -50,7 +51,8 @@ SW2D_SRC := ../examples/shallowwater/planewave/localmods/SWParamsClass \
 all: $(OUT)/ref_adv2d $(OUT)/ref_adv3d $(OUT)/ref_cgsem2d $(OUT)/ref_krylov $(OUT)/ref_nodal $(OUT)/ref_pickup \
      $(OUT)/ref_specmesh $(OUT)/ref_sw2d $(OUT)/ref_cgsem_paraboloid $(OUT)/dropin_cgsem_paraboloid \
      $(OUT)/ref_adv2d_ic $(OUT)/ref_adv2d_driver $(OUT)/dropin_adv2d_driver \
-     $(OUT)/ref_adv3d_ic $(OUT)/ref_adv3d_driver $(OUT)/dropin_adv3d_ic $(OUT)/dropin_adv3d_driver
+     $(OUT)/ref_adv3d_ic $(OUT)/ref_adv3d_driver $(OUT)/dropin_adv3d_ic $(OUT)/dropin_adv3d_driver \
+     $(OUT)/dropin_elliptic1d_driver
 
 # One recipe compiles the shared chain in order: each module needs the .mod
 # files of the ones before it.
@@ -144,6 +146,22 @@ $(OUT)/dropin_adv3d_$(1): $(REF)/highend/advection3d/$(2).f90 $(FDROP3)/adv3d_dr
 endef
 $(eval $(call adv3d_prog,ic,Advection3D_InitialConditions))
 $(eval $(call adv3d_prog,driver,Advection3D_Driver))
+
+# The 1-D elliptic example's driver (examples/elliptic1d/parabola/localmods/
+# Elliptic1D_Driver.f90), unmodified, against the drop-in modules
+# (self_amd/fortran/elliptic1d_dropin.f90) with libself_amd.so.  The
+# example's own Elliptic1D_Class does not compile against the reference's
+# IterativeSolvers_Class (flang: its CopyTo / CopyFrom overrides, :90-91,
+# are incompatible with the deferred interfaces), so its golden files are
+# the outputs the example ships under run/.
+ELL1D_DIR := ../examples/elliptic1d/parabola/localmods
+$(OUT)/dropin_elliptic1d_driver: $(REF)/$(ELL1D_DIR)/Elliptic1D_Driver.f90 $(FDROP)/elliptic1d_dropin.o
+	@mkdir -p $(OUT)/dropin
+	$(AMDFLANG) $(FFLAGS) -J$(OUT)/dropin -I$(FDROP) -c $(REF)/$(ELL1D_DIR)/Elliptic1D_Driver.f90 \
+	  -o $(OUT)/dropin/elliptic1d_driver.o
+	$(AMDFLANG) $(FFLAGS) -o $@.tmp $(OUT)/dropin/elliptic1d_driver.o $(FDROP)/self_amd_cabi.o \
+	  $(FDROP)/shallowwater_amd.o $(FDROP)/sw_dropin.o $(FDROP)/elliptic1d_dropin.o \
+	  -L$(HERE)../self_amd -lself_amd -Wl,-rpath,'$$ORIGIN/../../self_amd' && mv $@.tmp $@
 
 clean:
 	rm -rf $(OUT)

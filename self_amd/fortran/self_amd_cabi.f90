@@ -378,6 +378,19 @@ MODULE SELF_AMD_CABI
       REAL(C_DOUBLE), INTENT(in) :: A(*), b(*)
       TYPE(C_PTR), INTENT(out) :: op
     END FUNCTION
+    INTEGER(C_INT) FUNCTION self_linop_elliptic1d_create(N, nElems, xScale, op) &
+        BIND(C, name="self_linop_elliptic1d_create")
+      IMPORT :: C_INT, C_PTR, C_DOUBLE
+      INTEGER(C_INT), VALUE :: N, nElems
+      REAL(C_DOUBLE), VALUE :: xScale
+      TYPE(C_PTR), INTENT(out) :: op
+    END FUNCTION
+    INTEGER(C_INT) FUNCTION self_linop_elliptic1d_unmask(op, sol, u, x) BIND(C, name="self_linop_elliptic1d_unmask")
+      IMPORT :: C_INT, C_PTR, C_DOUBLE
+      TYPE(C_PTR), VALUE :: op
+      REAL(C_DOUBLE), INTENT(in) :: sol(*)
+      REAL(C_DOUBLE), INTENT(out) :: u(*), x(*)
+    END FUNCTION
     INTEGER(C_INT) FUNCTION self_linop_destroy(op) BIND(C, name="self_linop_destroy")
       IMPORT :: C_INT, C_PTR
       TYPE(C_PTR), VALUE :: op

@@ -140,3 +140,24 @@ def test_reference_adv3d_programs_compile_unchanged_against_the_drop_in(tmp_path
                        capture_output=True, text=True, timeout=300)
     assert r.returncode == 0, r.stderr[-3000:]
     assert os.path.exists(exe)
+
+
+def test_reference_elliptic1d_driver_compiles_unchanged_against_the_drop_in(tmp_path):
+    """The reference's 1-D elliptic driver (examples/elliptic1d/parabola/
+    localmods/Elliptic1D_Driver.f90, read in place: Build, Solve(ioerr),
+    WriteTecplot, WriteResidual, Trash of TYPE Elliptic1D) compiles and
+    links, unmodified, against the drop-in modules of
+    self_amd/fortran/elliptic1d_dropin.f90 and libself_amd.so."""
+    src = "/root/reference/examples/elliptic1d/parabola/localmods/Elliptic1D_Driver.f90"
+    build = os.path.join(ROOT, "self_amd", "fortran", "_build")
+    fc = "/opt/rocm/bin/amdflang"
+    if not (os.path.exists(src) and os.path.exists(os.path.join(build, "elliptic1d_dropin.o")) and os.path.exists(fc)):
+        pytest.skip("reference tree, amdflang or the built drop-in modules absent")
+    obj = [os.path.join(build, f) for f in ("self_amd_cabi.o", "shallowwater_amd.o", "sw_dropin.o",
+                                            "elliptic1d_dropin.o")]
+    exe = str(tmp_path / "driver")
+    r = subprocess.run([fc, "-O2", "-I", build, "-J", str(tmp_path), src] + obj +
+                       ["-L" + os.path.join(ROOT, "self_amd"), "-lself_amd", "-o", exe],
+                       capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stderr[-3000:]
+    assert os.path.exists(exe)
