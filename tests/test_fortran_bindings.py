@@ -115,10 +115,15 @@ def test_reference_adv2d_driver_compiles_unchanged_against_the_drop_in(tmp_path)
     assert os.path.exists(exe)
 
 
+ADV3D_DIRS = ["src/highend/advection3d", "examples/advection3d/spot-advection/localmods"]
+
+
+@pytest.mark.parametrize("where", ADV3D_DIRS)
 @pytest.mark.parametrize("program", ["Advection3D_Driver", "Advection3D_InitialConditions"])
-def test_reference_adv3d_programs_compile_unchanged_against_the_drop_in(tmp_path, program):
-    """The reference's 3-D advection programs (src/highend/advection3d/
-    Advection3D_Driver.f90 and Advection3D_InitialConditions.f90, read in
+def test_reference_adv3d_programs_compile_unchanged_against_the_drop_in(tmp_path, where, program):
+    """The reference's 3-D advection programs (Advection3D_Driver.f90 and
+    Advection3D_InitialConditions.f90 of src/highend/advection3d and of the
+    spot-advection example, read in
     place: Build with and without the pickup, the clocks of Timing,
     dgStorage % GetNumberOfNodes, mesh % GetPositionAtNode, SetSolution,
     SetVelocity, SetRelaxationField/Factor, GlobalTimeDerivative(t, m),
@@ -126,7 +131,7 @@ def test_reference_adv3d_programs_compile_unchanged_against_the_drop_in(tmp_path
     compile and link, unmodified, against the drop-in modules of
     self_amd/fortran/adv3d_dropin.f90 (module directory _build3d, shared
     modules from _build) and libself_amd.so."""
-    src = f"/root/reference/src/highend/advection3d/{program}.f90"
+    src = f"/root/reference/{where}/{program}.f90"
     build = os.path.join(ROOT, "self_amd", "fortran", "_build")
     build3 = os.path.join(ROOT, "self_amd", "fortran", "_build3d")
     fc = "/opt/rocm/bin/amdflang"
@@ -142,13 +147,14 @@ def test_reference_adv3d_programs_compile_unchanged_against_the_drop_in(tmp_path
     assert os.path.exists(exe)
 
 
-def test_reference_elliptic1d_driver_compiles_unchanged_against_the_drop_in(tmp_path):
-    """The reference's 1-D elliptic driver (examples/elliptic1d/parabola/
-    localmods/Elliptic1D_Driver.f90, read in place: Build, Solve(ioerr),
+@pytest.mark.parametrize("where", ["examples/elliptic1d/parabola/localmods", "src/cgsem"])
+def test_reference_elliptic1d_driver_compiles_unchanged_against_the_drop_in(tmp_path, where):
+    """The reference's 1-D elliptic driver (Elliptic1D_Driver.f90 of the
+    parabola example and of src/cgsem, read in place: Build, Solve(ioerr),
     WriteTecplot, WriteResidual, Trash of TYPE Elliptic1D) compiles and
     links, unmodified, against the drop-in modules of
     self_amd/fortran/elliptic1d_dropin.f90 and libself_amd.so."""
-    src = "/root/reference/examples/elliptic1d/parabola/localmods/Elliptic1D_Driver.f90"
+    src = f"/root/reference/{where}/Elliptic1D_Driver.f90"
     build = os.path.join(ROOT, "self_amd", "fortran", "_build")
     fc = "/opt/rocm/bin/amdflang"
     if not (os.path.exists(src) and os.path.exists(os.path.join(build, "elliptic1d_dropin.o")) and os.path.exists(fc)):
